@@ -1,0 +1,510 @@
+// C ABI implementation: octree, context, cases, fields, ghost fill.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+
+#include "kernels.h"
+#include "runtime.h"
+
+using namespace ffvhc;
+
+namespace ffvhc {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+bool field_ok(ffvhc_ctx* ctx, int f, int prec) {
+  return f >= 0 && f < (int)ctx->fields.size() && ctx->fields[f].used && (prec == 0 || ctx->fields[f].prec == prec);
+}
+
+std::vector<int> selected_cases(ffvhc_ctx* ctx) {
+  std::vector<int> ids;
+  if (!ctx->sel.empty()) {
+    for (int c : ctx->sel)
+      if (c >= 0 && c < (int)ctx->cases.size() && ctx->cases[c].used) ids.push_back(c);
+    return ids;
+  }
+  for (int c = 0; c < (int)ctx->cases.size(); c++)
+    if (ctx->cases[c].used) ids.push_back(c);
+  return ids;
+}
+
+int refresh_active(ffvhc_ctx* ctx) {
+  if (!ctx->dirty) return 0;
+  ctx->hactive.clear();
+  ctx->multilevel = false;
+  for (int c : selected_cases(ctx)) {
+    const CaseRec& r = ctx->cases[c];
+    for (int s = 0; s < r.nblk; s++) ctx->hactive.push_back(r.slot_begin + s);
+    if (r.max_level > 0) ctx->multilevel = true;
+  }
+  ctx->nactive = (int)ctx->hactive.size();
+  if (ctx->nactive)
+    HIPCHK(hipMemcpyAsync(ctx->dactive, ctx->hactive.data(), sizeof(int) * ctx->nactive, hipMemcpyHostToDevice,
+                          ctx->sm));
+  HIPCHK(hipStreamSynchronize(ctx->sm));
+  ctx->dirty = false;
+  return 0;
+}
+
+int alloc_field(ffvhc_ctx* ctx, int prec, int* field) {
+  if (prec != 8 && prec != 4) return fail(-1, "precision must be 8 or 4");
+  int id = -1;
+  for (int i = 0; i < (int)ctx->fields.size(); i++)
+    if (!ctx->fields[i].used) {
+      id = i;
+      break;
+    }
+  if (id < 0) {
+    id = (int)ctx->fields.size();
+    ctx->fields.emplace_back();
+  }
+  FieldRec& f = ctx->fields[id];
+  f = FieldRec();
+  f.prec = prec;
+  size_t nd = (size_t)ctx->cap * NC * prec, nh = (size_t)ctx->cap * NH * prec;
+  HIPCHK(hipMalloc(&f.d[0], nd));
+  HIPCHK(hipMalloc(&f.h[0], nh));
+  HIPCHK(hipMemsetAsync(f.d[0], 0, nd, ctx->sm));
+  HIPCHK(hipMemsetAsync(f.h[0], 0, nh, ctx->sm));
+  ctx->bytes += (long long)(nd + nh);
+  f.used = true;
+  *field = id;
+  return 0;
+}
+
+int ensure_second_buffer(ffvhc_ctx* ctx, int field) {
+  FieldRec& f = ctx->fields[field];
+  if (f.d[1]) return 0;
+  size_t nd = (size_t)ctx->cap * NC * f.prec, nh = (size_t)ctx->cap * NH * f.prec;
+  HIPCHK(hipMalloc(&f.d[1], nd));
+  HIPCHK(hipMalloc(&f.h[1], nh));
+  HIPCHK(hipMemsetAsync(f.d[1], 0, nd, ctx->sm));
+  HIPCHK(hipMemsetAsync(f.h[1], 0, nh, ctx->sm));
+  ctx->bytes += (long long)(nd + nh);
+  return 0;
+}
+
+}  // namespace ffvhc
+
+extern "C" {
+
+const char* ffvhc_last_error(void) { return g_err.c_str(); }
+
+// ------------------------------------------------------------------------------------- tree
+
+int ffvhc_tree_build(const ffvhc_tree_spec* spec, ffvhc_tree** out) {
+  if (!spec || !out) return fail(-1, "null argument");
+  if (spec->nregions < 0 || spec->nregions > FFVHC_MAX_REGIONS) return fail(-1, "bad region count");
+  OctreeSpec s;
+  for (int d = 0; d < 3; d++) {
+    s.root[d] = spec->root[d];
+    s.origin[d] = spec->origin[d];
+    s.periodic[d] = spec->periodic[d];
+  }
+  s.root_length = spec->root_length;
+  s.n = spec->n;
+  s.min_level = spec->min_level;
+  s.margin = spec->margin;
+  s.ordering = spec->ordering;
+  for (int r = 0; r < spec->nregions; r++) {
+    const ffvhc_region& g = spec->regions[r];
+    Region q{};
+    if (g.kind < 0 || g.kind > 2) return fail(-1, "bad region kind");
+    q.kind = (Region::Kind)g.kind;
+    for (int d = 0; d < 3; d++) {
+      q.a[d] = g.a[d];
+      q.b[d] = g.b[d];
+    }
+    q.r = g.r;
+    q.dr = g.dr;
+    q.level = g.level;
+    s.regions.push_back(q);
+  }
+  try {
+    auto* t = new ffvhc_tree;
+    t->tree = new Octree(s);
+    *out = t;
+  } catch (const std::exception& e) {
+    return fail(-1, e.what());
+  }
+  return 0;
+}
+
+int ffvhc_tree_num_blocks(const ffvhc_tree* t) { return t ? (int)t->tree->blocks().size() : -1; }
+int ffvhc_tree_max_level(const ffvhc_tree* t) { return t ? t->tree->max_level() : -1; }
+
+int ffvhc_tree_table(const ffvhc_tree* t, int* level, double* org, double* dx, int* ftype, int* nbr, int* sub) {
+  if (!t) return fail(-1, "null tree");
+  const auto& B = t->tree->blocks();
+  for (size_t i = 0; i < B.size(); i++) {
+    if (level) level[i] = B[i].level;
+    if (dx) dx[i] = B[i].dx;
+    for (int d = 0; d < 3; d++)
+      if (org) org[3 * i + d] = B[i].org[d];
+    for (int f = 0; f < 6; f++) {
+      if (ftype) ftype[6 * i + f] = B[i].ftype[f];
+      if (sub) sub[6 * i + f] = B[i].sub[f];
+      for (int s = 0; s < 4; s++)
+        if (nbr) nbr[24 * i + 4 * f + s] = B[i].nbr[f][s];
+    }
+  }
+  return 0;
+}
+
+void ffvhc_tree_free(ffvhc_tree* t) {
+  if (!t) return;
+  delete t->tree;
+  delete t;
+}
+
+// ---------------------------------------------------------------------------------- context
+
+int ffvhc_ctx_create(int device, long long max_blocks, ffvhc_ctx** out) {
+  if (!out || max_blocks <= 0) return fail(-1, "bad arguments");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(-1, "no such device");
+  HIPCHK(hipSetDevice(device));
+  std::unique_ptr<ffvhc_ctx> c(new ffvhc_ctx);
+  c->device = device;
+  c->cap = max_blocks;
+  HIPCHK(hipStreamCreateWithFlags(&c->sm, hipStreamNonBlocking));
+  c->hblk.assign((size_t)max_blocks, DevBlock{});
+  c->hcase.assign(MAX_CASES, DevCase{});
+  c->hst.assign(MAX_CASES, SolveState{});
+  HIPCHK(hipMalloc(&c->dblk, sizeof(DevBlock) * max_blocks));
+  HIPCHK(hipMalloc(&c->dcase, sizeof(DevCase) * MAX_CASES));
+  HIPCHK(hipMalloc(&c->dactive, sizeof(int) * max_blocks));
+  HIPCHK(hipMalloc(&c->dst, sizeof(SolveState) * MAX_CASES));
+  HIPCHK(hipMalloc(&c->dpart, sizeof(double) * 4 * max_blocks));
+  HIPCHK(hipMalloc(&c->dcut, sizeof(float) * 6 * NC * max_blocks));
+  HIPCHK(hipMalloc(&c->dflag, sizeof(uint32_t) * NC * max_blocks));
+  HIPCHK(hipMalloc(&c->dcid, sizeof(uint8_t) * 6 * NC * max_blocks));
+  HIPCHK(hipMemset(c->dst, 0, sizeof(SolveState) * MAX_CASES));
+  c->bytes = (long long)(sizeof(DevBlock) + sizeof(int) + 4 * sizeof(double) + (6 * 4 + 4 + 6) * NC) * max_blocks;
+  c->free_slots.push_back({0, max_blocks});
+  *out = c.release();
+  return 0;
+}
+
+void ffvhc_ctx_destroy(ffvhc_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->sm);
+  for (auto& f : c->fields)
+    for (int b = 0; b < 2; b++) {
+      if (f.d[b]) (void)hipFree(f.d[b]);
+      if (f.h[b]) (void)hipFree(f.h[b]);
+    }
+  (void)hipFree(c->dblk);
+  (void)hipFree(c->dcase);
+  (void)hipFree(c->dactive);
+  (void)hipFree(c->dst);
+  (void)hipFree(c->dpart);
+  (void)hipFree(c->dcut);
+  (void)hipFree(c->dflag);
+  (void)hipFree(c->dcid);
+  (void)hipStreamDestroy(c->sm);
+  delete c;
+}
+
+int ffvhc_ctx_sync(ffvhc_ctx* c) {
+  if (!c) return fail(-1, "null context");
+  HIPCHK(hipStreamSynchronize(c->sm));
+  return 0;
+}
+
+long long ffvhc_ctx_device_bytes(const ffvhc_ctx* c) { return c ? c->bytes : -1; }
+
+int ffvhc_ctx_select(ffvhc_ctx* c, const int* ids, int n) {
+  if (!c) return fail(-1, "null context");
+  c->sel.clear();
+  for (int i = 0; i < n; i++) {
+    if (ids[i] < 0 || ids[i] >= (int)c->cases.size() || !c->cases[ids[i]].used) return fail(-4, "no such case");
+    c->sel.push_back(ids[i]);
+  }
+  c->dirty = true;
+  return refresh_active(c);
+}
+
+// ------------------------------------------------------------------------------------ cases
+
+static int upload_case_meta(ffvhc_ctx* c, int id) {
+  const CaseRec& r = c->cases[id];
+  DevCase& d = c->hcase[id];
+  d.slot_begin = r.slot_begin;
+  d.nblk = r.nblk;
+  d.rho = r.params.rho;
+  d.mu = r.params.mu;
+  d.dt = r.params.dt;
+  d.gx = r.params.gx;
+  d.gy = r.params.gy;
+  d.gz = r.params.gz;
+  d.betag = r.params.betag;
+  d.tr = r.params.tr;
+  for (int g = 0; g < NUM_BC; g++)
+    for (int f = 0; f < 6; f++) {
+      d.bct[g][f] = r.params.bct[g][f];
+      d.bcv[g][f] = r.params.bcv[g][f];
+    }
+  HIPCHK(hipMemcpyAsync(c->dcase + id, &d, sizeof(DevCase), hipMemcpyHostToDevice, c->sm));
+  HIPCHK(hipStreamSynchronize(c->sm));
+  return 0;
+}
+
+int ffvhc_case_add(ffvhc_ctx* c, const ffvhc_tree* t, const ffvhc_case_params* p, int g, const double* cut,
+                   const int* cid, const int* pid, int* case_id) {
+  if (!c || !t || !p || !case_id) return fail(-1, "null argument");
+  if (t->tree->spec().n != NB) return fail(-1, "device blocks are 16^3 cells");
+  if ((cut || cid || pid) && g < 1) return fail(-1, "geometry needs g >= 1");
+  const auto& B = t->tree->blocks();
+  const int64_t nb = (int64_t)B.size();
+  // first-fit contiguous slot range
+  int64_t begin = -1;
+  for (auto& iv : c->free_slots)
+    if (iv.second - iv.first >= nb) {
+      begin = iv.first;
+      iv.first += nb;
+      break;
+    }
+  if (begin < 0) return fail(-3, "not enough free block slots");
+  c->free_slots.erase(std::remove_if(c->free_slots.begin(), c->free_slots.end(),
+                                     [](const std::pair<int64_t, int64_t>& iv) { return iv.first >= iv.second; }),
+                      c->free_slots.end());
+  int id = -1;
+  for (int i = 0; i < (int)c->cases.size(); i++)
+    if (!c->cases[i].used) {
+      id = i;
+      break;
+    }
+  if (id < 0) {
+    if ((int)c->cases.size() >= MAX_CASES) return fail(-3, "too many cases");
+    id = (int)c->cases.size();
+    c->cases.emplace_back();
+  }
+  CaseRec& r = c->cases[id];
+  r.used = true;
+  r.slot_begin = (int)begin;
+  r.nblk = (int)nb;
+  r.max_level = t->tree->max_level();
+  r.ncells = nb * NC;
+  r.params = *p;
+  // block table with global slot numbers
+  std::vector<float> hcut;
+  std::vector<uint32_t> hflag;
+  std::vector<uint8_t> hcidv;
+  const int np = NB + 2 * g;
+  const size_t npad = (size_t)np * np * np;
+  hcut.assign((size_t)6 * NC * nb, 1.0f);
+  hflag.assign((size_t)NC * nb, 1u << 6);
+  hcidv.assign((size_t)6 * NC * nb, 0);
+  for (int64_t i = 0; i < nb; i++) {
+    DevBlock& d = c->hblk[begin + i];
+    d = DevBlock{};
+    d.case_id = id;
+    d.level = B[i].level;
+    d.dx = B[i].dx;
+    for (int q = 0; q < 3; q++) d.org[q] = B[i].org[q];
+    for (int f = 0; f < 6; f++) {
+      d.ftype[f] = B[i].ftype[f];
+      d.sub[f] = B[i].sub[f];
+      for (int s = 0; s < 4; s++) d.nbr[f][s] = B[i].nbr[f][s] < 0 ? -1 : (int)(begin + B[i].nbr[f][s]);
+    }
+    int irregular = 0;
+    if (cut || cid || pid) {
+      for (int k = 0; k < NB; k++)
+        for (int j = 0; j < NB; j++)
+          for (int ii = 0; ii < NB; ii++) {
+            size_t m = (size_t)i * npad + (size_t)(ii + g) + np * ((size_t)(j + g) + np * (size_t)(k + g));
+            size_t q = (size_t)i * NC + ii + NB * (j + NB * k);
+            uint32_t fl = 0;
+            for (int f = 0; f < 6; f++) {
+              int id6 = cid ? cid[(size_t)f * nb * npad + m] : 0;
+              if (id6 != 0) {
+                fl |= 1u << f;
+                if (id6 < 0 || id6 > 255) return fail(-1, "cut ids must be in 0..255");
+              }
+              hcidv[(size_t)f * NC * nb + q] = (uint8_t)id6;
+              float cv = cut ? (float)cut[(size_t)f * nb * npad + m] : 1.0f;
+              if (cut && (double)cv != cut[(size_t)f * nb * npad + m])
+                return fail(-1, "cut distances must be float32 values (Cutlib CutPos32Array)");
+              hcut[(size_t)f * NC * nb + q] = cv;
+            }
+            if (!pid || pid[m] == 1)
+              fl |= 1u << 6;
+            hflag[q] = fl;
+            if (fl != (1u << 6)) irregular = 1;
+          }
+    }
+    d.flags = irregular;
+  }
+  // geometry is stored per global slot: [f][slot][cell] for cuts/cids, [slot][cell] for flags
+  HIPCHK(hipMemcpyAsync(c->dblk + begin, c->hblk.data() + begin, sizeof(DevBlock) * nb, hipMemcpyHostToDevice, c->sm));
+  for (int f = 0; f < 6; f++) {
+    HIPCHK(hipMemcpyAsync(c->dcut + (size_t)f * NC * c->cap + (size_t)begin * NC, hcut.data() + (size_t)f * NC * nb,
+                          sizeof(float) * NC * nb, hipMemcpyHostToDevice, c->sm));
+    HIPCHK(hipMemcpyAsync(c->dcid + (size_t)f * NC * c->cap + (size_t)begin * NC, hcidv.data() + (size_t)f * NC * nb,
+                          NC * nb, hipMemcpyHostToDevice, c->sm));
+  }
+  HIPCHK(hipMemcpyAsync(c->dflag + (size_t)begin * NC, hflag.data(), sizeof(uint32_t) * NC * nb,
+                        hipMemcpyHostToDevice, c->sm));
+  int rc = upload_case_meta(c, id);
+  if (rc) return rc;
+  c->dirty = true;
+  *case_id = id;
+  return refresh_active(c);
+}
+
+int ffvhc_case_remove(ffvhc_ctx* c, int id) {
+  if (!c || id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
+  CaseRec& r = c->cases[id];
+  c->free_slots.push_back({r.slot_begin, (int64_t)r.slot_begin + r.nblk});
+  std::sort(c->free_slots.begin(), c->free_slots.end());
+  std::vector<std::pair<int64_t, int64_t>> merged;
+  for (auto& iv : c->free_slots) {
+    if (!merged.empty() && merged.back().second == iv.first) merged.back().second = iv.second;
+    else merged.push_back(iv);
+  }
+  c->free_slots = merged;
+  r = CaseRec();
+  c->sel.erase(std::remove(c->sel.begin(), c->sel.end(), id), c->sel.end());
+  c->dirty = true;
+  return refresh_active(c);
+}
+
+int ffvhc_case_num_blocks(const ffvhc_ctx* c, int id) {
+  if (!c || id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return -1;
+  return c->cases[id].nblk;
+}
+
+int ffvhc_case_set_bc(ffvhc_ctx* c, int id, int group, int face, int type, double value) {
+  if (!c || id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
+  if (group < 0 || group >= NUM_BC || face < 0 || face > 5) return fail(-1, "bad group/face");
+  c->cases[id].params.bct[group][face] = type;
+  c->cases[id].params.bcv[group][face] = value;
+  return upload_case_meta(c, id);
+}
+
+// ----------------------------------------------------------------------------------- fields
+
+int ffvhc_field_alloc(ffvhc_ctx* c, int prec, int* field) {
+  if (!c || !field) return fail(-1, "null argument");
+  return alloc_field(c, prec, field);
+}
+
+int ffvhc_field_free(ffvhc_ctx* c, int f) {
+  if (!c || !field_ok(c, f, 0)) return fail(-4, "no such field");
+  HIPCHK(hipStreamSynchronize(c->sm));
+  FieldRec& r = c->fields[f];
+  for (int b = 0; b < 2; b++) {
+    if (r.d[b]) (void)hipFree(r.d[b]);
+    if (r.h[b]) (void)hipFree(r.h[b]);
+  }
+  r = FieldRec();
+  return 0;
+}
+
+}  // extern "C"
+
+template <class T>
+static void pack(const T* pad, int g, T* in, T* halo, int64_t nb) {
+  const int np = NB + 2 * g;
+  const size_t npad = (size_t)np * np * np;
+  for (int64_t b = 0; b < nb; b++) {
+    const T* P = pad + b * npad;
+    auto at = [&](int i, int j, int k) { return P[(size_t)(i + g) + np * ((size_t)(j + g) + np * (size_t)(k + g))]; };
+    for (int k = 0; k < NB; k++)
+      for (int j = 0; j < NB; j++)
+        for (int i = 0; i < NB; i++) in[b * NC + i + NB * (j + NB * k)] = at(i, j, k);
+    for (int f = 0; f < 6; f++)
+      for (int bb = 0; bb < NB; bb++)
+        for (int a = 0; a < NB; a++) {
+          int i, j, k;
+          face_cell(f, a, bb, -1, i, j, k);
+          halo[b * NH + halo_index(f, a, bb)] = at(i, j, k);
+        }
+  }
+}
+
+template <class T>
+static void unpack(const T* in, const T* halo, int g, T* pad, int64_t nb) {
+  const int np = NB + 2 * g;
+  const size_t npad = (size_t)np * np * np;
+  for (int64_t b = 0; b < nb; b++) {
+    T* P = pad + b * npad;
+    auto at = [&](int i, int j, int k) -> T& {
+      return P[(size_t)(i + g) + np * ((size_t)(j + g) + np * (size_t)(k + g))];
+    };
+    for (int k = 0; k < NB; k++)
+      for (int j = 0; j < NB; j++)
+        for (int i = 0; i < NB; i++) at(i, j, k) = in[b * NC + i + NB * (j + NB * k)];
+    for (int f = 0; f < 6; f++)
+      for (int bb = 0; bb < NB; bb++)
+        for (int a = 0; a < NB; a++) {
+          int i, j, k;
+          face_cell(f, a, bb, -1, i, j, k);
+          at(i, j, k) = halo[b * NH + halo_index(f, a, bb)];
+        }
+  }
+}
+
+extern "C" {
+
+int ffvhc_field_upload(ffvhc_ctx* c, int f, int id, int g, const void* padded) {
+  if (!c || !field_ok(c, f, 0) || !padded || g < 1) return fail(-1, "bad arguments");
+  if (id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
+  const CaseRec& r = c->cases[id];
+  FieldRec& F = c->fields[f];
+  const int64_t nb = r.nblk;
+  std::vector<char> in((size_t)nb * NC * F.prec), halo((size_t)nb * NH * F.prec);
+  if (F.prec == 8) pack((const double*)padded, g, (double*)in.data(), (double*)halo.data(), nb);
+  else pack((const float*)padded, g, (float*)in.data(), (float*)halo.data(), nb);
+  HIPCHK(hipMemcpyAsync((char*)F.d[0] + (size_t)r.slot_begin * NC * F.prec, in.data(), in.size(),
+                        hipMemcpyHostToDevice, c->sm));
+  HIPCHK(hipMemcpyAsync((char*)F.h[0] + (size_t)r.slot_begin * NH * F.prec, halo.data(), halo.size(),
+                        hipMemcpyHostToDevice, c->sm));
+  HIPCHK(hipStreamSynchronize(c->sm));
+  return 0;
+}
+
+int ffvhc_field_download(ffvhc_ctx* c, int f, int id, int g, void* padded) {
+  if (!c || !field_ok(c, f, 0) || !padded || g < 1) return fail(-1, "bad arguments");
+  if (id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
+  const CaseRec& r = c->cases[id];
+  FieldRec& F = c->fields[f];
+  const int64_t nb = r.nblk;
+  std::vector<char> in((size_t)nb * NC * F.prec), halo((size_t)nb * NH * F.prec);
+  HIPCHK(hipMemcpyAsync(in.data(), (char*)F.d[0] + (size_t)r.slot_begin * NC * F.prec, in.size(),
+                        hipMemcpyDeviceToHost, c->sm));
+  HIPCHK(hipMemcpyAsync(halo.data(), (char*)F.h[0] + (size_t)r.slot_begin * NH * F.prec, halo.size(),
+                        hipMemcpyDeviceToHost, c->sm));
+  HIPCHK(hipStreamSynchronize(c->sm));
+  if (F.prec == 8) unpack((const double*)in.data(), (const double*)halo.data(), g, (double*)padded, nb);
+  else unpack((const float*)in.data(), (const float*)halo.data(), g, (float*)padded, nb);
+  return 0;
+}
+
+int ffvhc_ghost_fill(ffvhc_ctx* c, int f, int bcg, int updater) {
+  if (!c || !field_ok(c, f, 0)) return fail(-1, "bad field");
+  if (bcg < 0 || bcg >= NUM_BC || (updater != 1 && updater != 5)) return fail(-1, "bad bc group / updater");
+  int rc = refresh_active(c);
+  if (rc) return rc;
+  FieldRec& F = c->fields[f];
+  if (F.prec == 8)
+    launch_ghost_pull<double>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, (const double*)F.d[0],
+                              (double*)F.h[0], bcg, updater, c->multilevel);
+  else
+    launch_ghost_pull<float>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, (const float*)F.d[0],
+                             (float*)F.h[0], bcg, updater, c->multilevel);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+// Host-side launchers of the HIP kernels (all asynchronous on the given stream).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace ffvhc {
+
+template <class T>
+void launch_ghost_pull(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                       const T* x, T* h, int bcg, int updater, bool multilevel);
+
+template <class T>
+void launch_rbgs_iter(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                      SolveState* st, Coef7<T> A, const T* b, const T* xin, T* xout, const T* hin, T* hout,
+                      double* part, size_t nslots, int n, int bcg);
+
+template <class T>
+void launch_select(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const SolveState* st, T* x0,
+                   const T* x1, T* h0, const T* h1);
+
+template <class T>
+void launch_fill(hipStream_t sm, const int* active, int nact, T* x, T* h, T v);
+template <class T>
+void launch_copy(hipStream_t sm, const int* active, int nact, T* y, T* hy, const T* x, const T* hx);
+
+void launch_solve_init(hipStream_t sm, SolveState* st, int ncase, double omega, double eps, int countMax);
+
+}  // namespace ffvhc

@@ -1,0 +1,413 @@
+// Core kernels: ghost-layer pull fill, fused RBGS iteration (residual + red/black sweep +
+// halo push + per-case convergence), result selection, BLAS-1 helpers.
+#include <cfloat>
+
+#include "device.h"
+#include "ghost.h"
+#include "kernels.h"
+
+namespace ffvhc {
+
+// ------------------------------------------------------------------------------------------
+// Per-case deterministic reductions: every workgroup writes its block's partial sum(s); the
+// last workgroup of the case (ticket) adds the partials in slot order with a fixed tree.
+// ------------------------------------------------------------------------------------------
+
+template <int NT>
+__device__ inline double block_sum(double v, double* red) {
+  // fixed-pattern wave reduction, then a fixed-order pass over the waves
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NT / 64; q++) s += red[q];
+  return s;  // valid in thread 0
+}
+
+template <int NT>
+__device__ inline double case_sum(const double* part, int begin, int count, double* red) {
+  double v = 0.0;
+  const volatile double* p = part;
+  for (int q = threadIdx.x; q < count; q += NT) v += p[begin + q];
+  // tree in shared memory with fixed pairing
+  __shared__ double tree[NT];
+  tree[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = NT / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) tree[threadIdx.x] = tree[threadIdx.x] + tree[threadIdx.x + h];
+    __syncthreads();
+  }
+  double r = tree[0];
+  __syncthreads();
+  (void)red;
+  return r;
+}
+
+// returns true in every thread of the last workgroup of case c
+__device__ inline bool last_of_case(SolveState* st, int c, int nblk, int* flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned int prev = atomicAdd(&st[c].ticket, 1u);
+    *flag = (prev == (unsigned int)(nblk - 1));
+  }
+  __syncthreads();
+  bool last = *flag;
+  if (last) __threadfence();
+  return last;
+}
+
+__device__ inline void converged(SolveState& s, double rr, int count) {
+  // FFVILS.h:24-47 IsConverged
+  s.rr = rr;
+  double bb = s.bb;
+  double residual = (fabs(bb) > FLT_MIN) ? sqrt(fabs(rr) / fabs(bb)) : sqrt(fabs(rr));
+  s.residual = residual;
+  s.count = count;
+  if (residual <= s.eps) {
+    s.done = 1;
+  } else if (isnan(residual) || isinf(residual)) {
+    s.done = 1;
+    s.status = 1;
+  } else if (count >= s.countMax) {
+    s.done = 1;
+    s.count = s.countMax + 1;  // loop exhausted: for(count=1; count<=countMax; ++count)
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Pull ghost fill (LocalScalar3D::ImposeBoundaryCondition, LocalScalar3D.h:266-296).
+// phase 0: outer BC, same level, F2C;  phase 1: C2F (updater 5 reads the coarse ghost that
+// phase 0 wrote).  One workgroup per (block, face), one thread per face cell.
+// ------------------------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void ghost_pull_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                          const DevCase* __restrict__ cs, const T* __restrict__ x,
+                                                          T* __restrict__ h, int bcg, int updater, int phase) {
+  const int slot = active[blockIdx.x / 6];
+  const int f = blockIdx.x % 6;
+  const DevBlock& B = blk[slot];
+  const int ft = B.ftype[f];
+  if ((phase == 0) == (ft == FACE_COARSER)) return;
+  const int p = threadIdx.x, a = p & (NB - 1), b = p >> 4;
+  const T* me = x + (size_t)slot * NC;
+  auto own = [&](int i, int j, int k) { return me[i + NB * (j + NB * k)]; };
+  int i, j, k;
+  T* hp = h + (size_t)slot * NH + halo_index(f, a, b);
+  T v;
+  if (ft == FACE_OUTER) {
+    const DevCase& C = cs[B.case_id];
+    const int type = C.bct[bcg][f];
+    if (type < 0) return;
+    T xc = (T)(type == BCT_NEUMANN ? C.bcv[bcg][f] * B.dx : C.bcv[bcg][f]);
+    face_cell(f, a, b, 0, i, j, k);
+    T x_adj = own(i, j, k);
+    face_cell(f ^ 1, a, b, 0, i, j, k);
+    T x_far = own(i, j, k);
+    bool pp = !(f & 1) && B.ftype[f ^ 1] == FACE_OUTER && C.bct[bcg][f ^ 1] == BCT_PERIODIC;
+    v = outer_value<T>(f, type, xc, x_adj, x_far, *hp, pp);
+  } else if (ft == FACE_SAME) {
+    const T* nb = x + (size_t)B.nbr[f][0] * NC;
+    face_cell(f ^ 1, a, b, 0, i, j, k);
+    v = nb[i + NB * (j + NB * k)];
+  } else if (ft == FACE_FINER) {
+    int oa = a / (NB / 2), ob = b / (NB / 2);
+    int s = ((f >> 1) == 1) ? (ob + 2 * oa) : (oa + 2 * ob);
+    const T* fb = x + (size_t)B.nbr[f][s] * NC;
+    auto fine = [&](int ii, int jj, int kk) { return fb[ii + NB * (jj + NB * kk)]; };
+    v = f2c_mean<T>(f, a, b, fine, updater == 5);
+  } else {  // FACE_COARSER
+    const int sub = B.sub[f];
+    const int cslot = B.nbr[f][0];
+    const T* cb = x + (size_t)cslot * NC;
+    auto coarse = [&](int ii, int jj, int kk) { return cb[ii + NB * (jj + NB * kk)]; };
+    if (updater == 5) {
+      int ca, cbb;
+      c2f_coarse_ab(f, sub, a, b, ca, cbb);
+      face_cell(f ^ 1, ca, cbb, 0, i, j, k);
+      T c_in = coarse(i, j, k);
+      T c_gh = h[(size_t)cslot * NH + halo_index(f ^ 1, ca, cbb)];
+      T dp = (c_in - c_gh) * (T)0.5;
+      face_cell(f, a, b, 0, i, j, k);
+      v = own(i, j, k) + dp;
+    } else {
+      v = c2f_trilinear<T>(f, sub, a, b, coarse);
+    }
+  }
+  *hp = v;
+}
+
+// ------------------------------------------------------------------------------------------
+// Fused RBGS iteration n (FFVILS::RBGS, FFVILS.h:547-594 with RBGS_Smoother 104-162 and
+// rbgs_smoother bils.f90:57-108):
+//   * r = b - A x^{n-1} over the block (CalcR2 of the previous iterate; iteration 1 instead
+//     accumulates b.b, the DOT before the loop),
+//   * red then black SOR on block-local parity with the ghost layer frozen,
+//   * x^n to the other ping-pong buffer, its ghost layer pushed into the neighbours' next halo
+//     (same-level copy, F2C mean, C2F trilinear: Scalar3DUpdater1 semantics) and the outer BC.
+// The last workgroup of each case finishes the residual of x^{n-1} and sets `done`; later
+// launches of a finished case return at once, so the device never waits for the host.
+// ------------------------------------------------------------------------------------------
+constexpr int RB_NT = 512;
+constexpr int LP = NB + 2;  // LDS pitch
+__device__ inline int lds(int i, int j, int k) { return (i + 1) + LP * ((j + 1) + LP * (k + 1)); }
+
+template <class T>
+__global__ __launch_bounds__(RB_NT) void rbgs_iter_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                          const DevCase* __restrict__ cs, SolveState* __restrict__ st,
+                                                          Coef7<T> A,
+                                                          const T* __restrict__ bv, const T* __restrict__ xin,
+                                                          T* __restrict__ xout, const T* __restrict__ hin,
+                                                          T* __restrict__ hout, double* __restrict__ part,
+                                                          size_t nslots, int n, int bcg) {
+  __shared__ T s[LP * LP * LP];
+  __shared__ double red[RB_NT / 64];
+  __shared__ int flag;
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const int c = B.case_id;
+  if (st[c].done) return;
+  const int t = threadIdx.x;
+  const int tx = t & 15, ty = (t >> 4) & 15, tz = t >> 8;
+  const size_t base = (size_t)slot * NC;
+  const size_t hbase = (size_t)slot * NH;
+#pragma unroll
+  for (int kk = 0; kk < 8; kk++) {
+    const int k = tz * 8 + kk;
+    s[lds(tx, ty, k)] = xin[base + tx + NB * (ty + NB * k)];
+  }
+  for (int q = t; q < NH; q += RB_NT) {
+    int f = q >> 8, p = q & 255, i, j, k;
+    face_cell(f, p & 15, p >> 4, -1, i, j, k);
+    s[lds(i, j, k)] = hin[hbase + q];
+  }
+  __syncthreads();
+  const bool sweep = n <= st[c].countMax;
+  const T omega = (T)st[c].omega;
+  const int par = (tx + ty) & 1;
+  double rr = 0.0, bb = 0.0;
+  T xr[4], sA[4][7], sb[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int k = tz * 8 + 2 * m + h;
+      const bool isred = (h == par);
+      const size_t g = base + tx + NB * (ty + NB * k);
+      const int li = lds(tx, ty, k);
+      T a[7];
+#pragma unroll
+      for (int q = 0; q < 7; q++) a[q] = A.p[q][g];
+      const T b0 = bv[g];
+      const T xp = s[li];
+      const T rs = b0 - a[1] * s[li - 1] - a[2] * s[li + 1] - a[3] * s[li - LP] - a[4] * s[li + LP] -
+                   a[5] * s[li - LP * LP] - a[6] * s[li + LP * LP];
+      const T r = rs - a[0] * xp;
+      rr += (double)r * (double)r;
+      if (n == 1) bb += (double)b0 * (double)b0;
+      if (isred) {
+        xr[m] = xp - omega * (xp - rs / a[0]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 7; q++) sA[m][q] = a[q];
+        sb[m] = b0;
+      }
+    }
+  }
+  if (sweep) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 4; m++) s[lds(tx, ty, tz * 8 + 2 * m + par)] = xr[m];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const int k = tz * 8 + 2 * m + (1 - par);
+      const int li = lds(tx, ty, k);
+      const T xp = s[li];
+      const T rs = sb[m] - sA[m][1] * s[li - 1] - sA[m][2] * s[li + 1] - sA[m][3] * s[li - LP] -
+                   sA[m][4] * s[li + LP] - sA[m][5] * s[li - LP * LP] - sA[m][6] * s[li + LP * LP];
+      s[li] = xp - omega * (xp - rs / sA[m][0]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 8; kk++) {
+      const int k = tz * 8 + kk;
+      xout[base + tx + NB * (ty + NB * k)] = s[lds(tx, ty, k)];
+    }
+    // halo push
+    auto L = [&](int i, int j, int k) { return s[lds(i, j, k)]; };
+    const DevCase& C = cs[c];
+    for (int f = 0; f < 6; f++) {
+      const int ft = B.ftype[f];
+      if (ft == FACE_OUTER) {
+        const int type = C.bct[bcg][f];
+        if (type < 0) continue;
+        T xc = (T)(type == BCT_NEUMANN ? C.bcv[bcg][f] * B.dx : C.bcv[bcg][f]);
+        bool pp = !(f & 1) && B.ftype[f ^ 1] == FACE_OUTER && C.bct[bcg][f ^ 1] == BCT_PERIODIC;
+        for (int p = t; p < NF; p += RB_NT) {
+          int a = p & 15, b = p >> 4, i, j, k;
+          face_cell(f, a, b, 0, i, j, k);
+          T xa = L(i, j, k);
+          face_cell(f ^ 1, a, b, 0, i, j, k);
+          T xf = L(i, j, k);
+          hout[hbase + f * NF + p] = outer_value<T>(f, type, xc, xa, xf, hin[hbase + f * NF + p], pp);
+        }
+      } else if (ft == FACE_SAME) {
+        const size_t nb = (size_t)B.nbr[f][0] * NH + (f ^ 1) * NF;
+        for (int p = t; p < NF; p += RB_NT) {
+          int i, j, k;
+          face_cell(f, p & 15, p >> 4, 0, i, j, k);
+          hout[nb + p] = L(i, j, k);
+        }
+      } else if (ft == FACE_COARSER) {
+        const size_t cb = (size_t)B.nbr[f][0] * NH;
+        int oa, ob;
+        sub_origins_ab(f, B.sub[f], oa, ob);
+        for (int p = t; p < NF / 4; p += RB_NT) {
+          int ca = (p & 7) + 8 * oa, cbb = (p >> 3) + 8 * ob;
+          hout[cb + halo_index(f ^ 1, ca, cbb)] = f2c_mean<T>(f ^ 1, ca, cbb, L, false);
+        }
+      } else {
+        for (int q = t; q < 4 * NF; q += RB_NT) {
+          const int sub = q >> 8, p = q & 255;
+          const size_t fb = (size_t)B.nbr[f][sub] * NH;
+          hout[fb + halo_index(f ^ 1, p & 15, p >> 4)] = c2f_trilinear<T>(f ^ 1, sub, p & 15, p >> 4, L);
+        }
+      }
+    }
+  }
+  // residual of x^{n-1} (and b.b on the first launch)
+  double rsum = block_sum<RB_NT>(rr, red);
+  double bsum = (n == 1) ? block_sum<RB_NT>(bb, red) : 0.0;
+  if (t == 0) {
+    part[slot] = rsum;
+    part[nslots + slot] = bsum;
+  }
+  const DevCase& C = cs[c];
+  if (!last_of_case(st, c, C.nblk, &flag)) return;
+  double tr = case_sum<RB_NT>(part, C.slot_begin, C.nblk, red);
+  double tb = (n == 1) ? case_sum<RB_NT>(part + nslots, C.slot_begin, C.nblk, red) : 0.0;
+  if (t == 0) {
+    SolveState& S = st[c];
+    S.ticket = 0;
+    S.iter = n;
+    if (n == 1) {
+      S.bb = tb;
+      if (S.countMax < 1) S.done = 1;
+    } else {
+      converged(S, tr, n - 1);
+      S.parity = (n - 1) & 1;
+    }
+  }
+}
+
+// Copy buffer 1 into buffer 0 for every case whose result landed in buffer 1.
+template <class T>
+__global__ __launch_bounds__(256) void select_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                     const SolveState* __restrict__ st, T* __restrict__ x0,
+                                                     const T* __restrict__ x1, T* __restrict__ h0,
+                                                     const T* __restrict__ h1) {
+  const int slot = active[blockIdx.x];
+  if (st[blk[slot].case_id].parity == 0) return;
+  const size_t base = (size_t)slot * NC, hb = (size_t)slot * NH;
+  for (int q = threadIdx.x; q < NC; q += 256) x0[base + q] = x1[base + q];
+  for (int q = threadIdx.x; q < NH; q += 256) h0[hb + q] = h1[hb + q];
+}
+
+__global__ void solve_init_kernel(SolveState* st, int ncase, double omega, double eps, int countMax) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncase) return;
+  SolveState s = {};
+  s.omega = omega;
+  s.eps = eps;
+  s.countMax = countMax;
+  s.count = 0;
+  s.rr0 = 1.0;
+  st[c] = s;
+}
+
+// ------------------------------------------------------------------------------------------
+// host launchers
+// ------------------------------------------------------------------------------------------
+template <class T>
+void launch_ghost_pull(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                       const T* x, T* h, int bcg, int updater, bool multilevel) {
+  if (nact == 0) return;
+  hipLaunchKernelGGL(ghost_pull_kernel<T>, dim3(nact * 6), dim3(256), 0, sm, active, blk, cs, x, h, bcg, updater, 0);
+  if (multilevel)
+    hipLaunchKernelGGL(ghost_pull_kernel<T>, dim3(nact * 6), dim3(256), 0, sm, active, blk, cs, x, h, bcg, updater, 1);
+}
+
+template <class T>
+void launch_rbgs_iter(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                      SolveState* st, Coef7<T> A, const T* b, const T* xin, T* xout, const T* hin,
+                      T* hout, double* part, size_t nslots, int n, int bcg) {
+  if (nact == 0) return;
+  hipLaunchKernelGGL(rbgs_iter_kernel<T>, dim3(nact), dim3(RB_NT), 0, sm, active, blk, cs, st, A, b, xin,
+                     xout, hin, hout, part, nslots, n, bcg);
+}
+
+template <class T>
+void launch_select(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const SolveState* st, T* x0,
+                   const T* x1, T* h0, const T* h1) {
+  if (nact == 0) return;
+  hipLaunchKernelGGL(select_kernel<T>, dim3(nact), dim3(256), 0, sm, active, blk, st, x0, x1, h0, h1);
+}
+
+void launch_solve_init(hipStream_t sm, SolveState* st, int ncase, double omega, double eps, int countMax) {
+  hipLaunchKernelGGL(solve_init_kernel, dim3((ncase + 63) / 64), dim3(64), 0, sm, st, ncase, omega, eps, countMax);
+}
+
+template void launch_ghost_pull<double>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, const double*,
+                                        double*, int, int, bool);
+template void launch_ghost_pull<float>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, const float*,
+                                       float*, int, int, bool);
+template void launch_rbgs_iter<double>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,
+                                       Coef7<double>, const double*, const double*, double*, const double*, double*,
+                                       double*, size_t, int, int);
+template void launch_rbgs_iter<float>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,
+                                      Coef7<float>, const float*, const float*, float*, const float*, float*, double*,
+                                      size_t, int, int);
+template void launch_select<double>(hipStream_t, const int*, int, const DevBlock*, const SolveState*, double*,
+                                    const double*, double*, const double*);
+template void launch_select<float>(hipStream_t, const int*, int, const DevBlock*, const SolveState*, float*,
+                                   const float*, float*, const float*);
+
+}  // namespace ffvhc
+
+namespace ffvhc {
+
+// FFVILS::Fill / Copy over the selected blocks (interior and ghost layer)
+template <class T>
+__global__ __launch_bounds__(256) void fill_kernel(const int* __restrict__ active, T* __restrict__ x, T* __restrict__ h,
+                                                   T v) {
+  const int slot = active[blockIdx.x];
+  for (int q = threadIdx.x; q < NC; q += 256) x[(size_t)slot * NC + q] = v;
+  for (int q = threadIdx.x; q < NH; q += 256) h[(size_t)slot * NH + q] = v;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void copy_kernel(const int* __restrict__ active, T* __restrict__ y, T* __restrict__ hy,
+                                                   const T* __restrict__ x, const T* __restrict__ hx) {
+  const int slot = active[blockIdx.x];
+  for (int q = threadIdx.x; q < NC; q += 256) y[(size_t)slot * NC + q] = x[(size_t)slot * NC + q];
+  for (int q = threadIdx.x; q < NH; q += 256) hy[(size_t)slot * NH + q] = hx[(size_t)slot * NH + q];
+}
+
+template <class T>
+void launch_fill(hipStream_t sm, const int* active, int nact, T* x, T* h, T v) {
+  if (nact) hipLaunchKernelGGL(fill_kernel<T>, dim3(nact), dim3(256), 0, sm, active, x, h, v);
+}
+template <class T>
+void launch_copy(hipStream_t sm, const int* active, int nact, T* y, T* hy, const T* x, const T* hx) {
+  if (nact) hipLaunchKernelGGL(copy_kernel<T>, dim3(nact), dim3(256), 0, sm, active, y, hy, x, hx);
+}
+template void launch_fill<double>(hipStream_t, const int*, int, double*, double*, double);
+template void launch_fill<float>(hipStream_t, const int*, int, float*, float*, float);
+template void launch_copy<double>(hipStream_t, const int*, int, double*, double*, const double*, const double*);
+template void launch_copy<float>(hipStream_t, const int*, int, float*, float*, const float*, const float*);
+
+}  // namespace ffvhc

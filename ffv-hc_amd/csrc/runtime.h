@@ -1,0 +1,87 @@
+// Host-side runtime state behind the C ABI: one context per GPU holding every resident case's
+// blocks in device slots, the fields allocated over all slots, and solver scratch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ffvhc.h"
+#include "device.h"
+#include "octree.h"
+
+struct ffvhc_tree {
+  ffvhc::Octree* tree;
+};
+
+namespace ffvhc {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return ffvhc::fail(-2, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct FieldRec {
+  bool used = false;
+  int prec = 8;          // bytes per value
+  void* d[2] = {nullptr, nullptr};  // interior, ping-pong
+  void* h[2] = {nullptr, nullptr};  // halo, ping-pong
+};
+
+struct CaseRec {
+  bool used = false;
+  int slot_begin = 0;
+  int nblk = 0;
+  int max_level = 0;
+  int64_t ncells = 0;
+  ffvhc_case_params params{};
+};
+
+}  // namespace ffvhc
+
+struct ffvhc_ctx {
+  int device = 0;
+  hipStream_t sm = nullptr;
+  int64_t cap = 0;  // slots
+  // block table
+  std::vector<ffvhc::DevBlock> hblk;
+  ffvhc::DevBlock* dblk = nullptr;
+  std::vector<ffvhc::DevCase> hcase;
+  ffvhc::DevCase* dcase = nullptr;
+  std::vector<ffvhc::CaseRec> cases;
+  // geometry per slot: cut distances (float, exact: Cutlib stores float32), cell flags
+  // (bits 0-5 face d cut, bit 6 fluid), cut ids (uint8 per face, force kernel only)
+  float* dcut = nullptr;
+  uint32_t* dflag = nullptr;
+  uint8_t* dcid = nullptr;
+  // active block list (selected cases)
+  std::vector<int> sel;       // selected case ids (empty = all)
+  std::vector<int> hactive;
+  int* dactive = nullptr;
+  int nactive = 0;
+  bool multilevel = false;
+  bool dirty = true;
+  // solver state
+  ffvhc::SolveState* dst = nullptr;
+  std::vector<ffvhc::SolveState> hst;
+  double* dpart = nullptr;  // 4 * cap partial sums
+  std::vector<ffvhc::FieldRec> fields;
+  std::vector<int> scratch;  // internally allocated fields
+  long long bytes = 0;
+  // free slot intervals [begin, end)
+  std::vector<std::pair<int64_t, int64_t>> free_slots;
+};
+
+namespace ffvhc {
+int refresh_active(ffvhc_ctx* ctx);
+std::vector<int> selected_cases(ffvhc_ctx* ctx);
+int ensure_second_buffer(ffvhc_ctx* ctx, int field);
+int alloc_field(ffvhc_ctx* ctx, int prec, int* field);
+bool field_ok(ffvhc_ctx* ctx, int f, int prec);
+}  // namespace ffvhc

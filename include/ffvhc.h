@@ -1,0 +1,163 @@
+/*
+ * ffvhc.h -- C ABI of the MI355X-native FFV-HC fractional-step hot path.
+ *
+ * Everything is plain C: opaque handles, plain pointers and sizes, int status codes.
+ * Device memory is owned by the context; hosts hold handles.  Status codes:
+ *   0 ok, -1 invalid argument, -2 HIP error, -3 out of capacity, -4 not found,
+ *   1 NaN/Inf residual (the reference calls Exit(0) there, FFVILS.h:36-43),
+ *   2 CFL guard tripped (Solver.cpp:2665-2675).
+ * ffvhc_last_error() describes the last non-zero status of the calling thread.
+ *
+ * Field arrays crossing the boundary use the reference's per-block layout: for each block in
+ * block order, a padded (n+2g)^3 array, x fastest (LocalScalar3D.h:232-235, Solver.cpp:588).
+ * The device keeps the interior and the first ghost layer (the only one a 7-point stencil
+ * reads); download fills those cells and leaves the rest of the caller's array untouched.
+ */
+#ifndef FFVHC_H
+#define FFVHC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFVHC_MAX_REGIONS 32
+
+/* ------------------------------------------------------------------ block octree (host) */
+
+/* Refinement region; kind 0 ball, 1 spherical shell (SphereDivider2.h:75-155), 2 box
+ * (BoundingBox lists, SphereDivider3.h:72-112). */
+typedef struct ffvhc_region {
+  int kind;
+  double a[3];
+  double b[3];
+  double r;
+  double dr;
+  int level;
+} ffvhc_region;
+
+/* DomainInfo/RootBlock + Tree of the reference configuration (FFVConfig.cpp:36-127). */
+typedef struct ffvhc_tree_spec {
+  int root[3];          /* RootBlock/Grid */
+  double origin[3];     /* RootBlock/Origin */
+  double root_length;   /* RootBlock/Length */
+  int periodic[3];      /* RootBlock/PeriodicX..Z */
+  int n;                /* LeafBlock/NumberOfCells (16 on the device) */
+  int min_level;        /* Tree/MinLevel */
+  double margin;        /* search-region margin, in cells of the finest level (NumberOfMarginalCells) */
+  int ordering;         /* Tuning/BlockOrdering: 0 Z, 1 Hilbert */
+  int nregions;
+  ffvhc_region regions[FFVHC_MAX_REGIONS];
+} ffvhc_tree_spec;
+
+typedef struct ffvhc_tree ffvhc_tree;
+
+/* Replaces BCMOctree construction + Partition + makeNeighborInfo (Solver.cpp:306-387). */
+int ffvhc_tree_build(const ffvhc_tree_spec* spec, ffvhc_tree** out);
+int ffvhc_tree_num_blocks(const ffvhc_tree* tree);
+int ffvhc_tree_max_level(const ffvhc_tree* tree);
+/* Block table, block order: level[nb], org[3nb], dx[nb], ftype[6nb] (0 outer, 1 same level,
+ * 2 coarser neighbour, 3 finer neighbours), nbr[24nb], sub[6nb]; any pointer may be NULL. */
+int ffvhc_tree_table(const ffvhc_tree* tree, int* level, double* org, double* dx, int* ftype, int* nbr, int* sub);
+void ffvhc_tree_free(ffvhc_tree* tree);
+
+/* ------------------------------------------------------------------ geometry (host) */
+
+/* Sphere as cut cells: replaces Cutlib's CalcCutInfo (Solver.cpp:530-682, cut positions stored
+ * as float32) with an analytic sphere, then ModifyCut1/2 (bstl_cutoff, zero-cut detection,
+ * hole filling v1/v2/v3, Solver.cpp:684-1046) and the phase flood fill from the seed point
+ * (FillCellFlag/InitPhase, Solver.cpp:1182-1287, 1404-1537). */
+typedef struct ffvhc_sphere_spec {
+  double center[3];
+  double radius;
+  int cut_id;           /* boundary id of the sphere surface (a phase boundary) */
+  double cutoff;        /* ShapeApproximation/Cutoff */
+  int hole_filling[3];  /* GridGeneration/HoleFilling, HoleFilling2, HoleFilling3 */
+  double seed[3];       /* ApplicationControl/Filling/Origin (a fluid point) */
+} ffvhc_sphere_spec;
+
+/* Outputs per block, padded with g ghost layers: c[6][nb][npad] (real), cid[6][nb][npad],
+ * pid[nb][npad] with npad = (n+2g)^3.  counts = {fluid cells, solid cells, hole faces v1,
+ * hole faces v2}.  Any output pointer may be NULL except pid. */
+int ffvhc_geometry_sphere(const ffvhc_tree* tree, const ffvhc_sphere_spec* sphere, int g, double* c, int* cid,
+                          int* pid, long long* counts);
+
+/* ------------------------------------------------------------------ device context */
+
+typedef struct ffvhc_ctx ffvhc_ctx;
+
+const char* ffvhc_last_error(void);
+
+/* One context per GPU; max_blocks bounds the blocks of all resident cases (every field is
+ * allocated for max_blocks blocks of 16^3 cells). */
+int ffvhc_ctx_create(int device, long long max_blocks, ffvhc_ctx** out);
+void ffvhc_ctx_destroy(ffvhc_ctx* ctx);
+int ffvhc_ctx_sync(ffvhc_ctx* ctx);
+/* bytes of device memory the context holds */
+long long ffvhc_ctx_device_bytes(const ffvhc_ctx* ctx);
+
+/* Physical parameters and outer BCs of one flow case.  BC groups: 0 UX, 1 UY, 2 UZ, 3 P,
+ * 4 Neumann-0 fields (boundaryTypeNULL, Solver.cpp:2024-2034), 5 zero Dirichlet.  Types
+ * -1 none, 0 Dirichlet, 1 Neumann (value per unit length, scaled by dx as BlockScalar3D.h:74-76),
+ * 9 periodic. */
+typedef struct ffvhc_case_params {
+  double rho, mu, dt;
+  double gx, gy, gz, betag, tr;
+  int bct[6][6];
+  double bcv[6][6];
+} ffvhc_case_params;
+
+/* Add a case: its blocks get device slots; geometry arrays as from ffvhc_geometry_sphere
+ * (g ghost layers) or NULL for an all-fluid, cut-free grid. */
+int ffvhc_case_add(ffvhc_ctx* ctx, const ffvhc_tree* tree, const ffvhc_case_params* params, int g, const double* c,
+                   const int* cid, const int* pid, int* case_id);
+int ffvhc_case_remove(ffvhc_ctx* ctx, int case_id);
+int ffvhc_case_num_blocks(const ffvhc_ctx* ctx, int case_id);
+/* LocalScalar3D::ResetBoundaryConditionValue (LocalScalar3D.h:325-335) */
+int ffvhc_case_set_bc(ffvhc_ctx* ctx, int case_id, int group, int face, int type, double value);
+/* Restrict subsequent calls to these cases (NULL / 0 = every resident case). */
+int ffvhc_ctx_select(ffvhc_ctx* ctx, const int* case_ids, int ncases);
+
+/* ------------------------------------------------------------------ fields */
+
+/* Field storage for every slot: precision 8 (FP64) or 4 (FP32); nbuf 1 or 2 (ping-pong). */
+int ffvhc_field_alloc(ffvhc_ctx* ctx, int precision, int* field);
+int ffvhc_field_free(ffvhc_ctx* ctx, int field);
+int ffvhc_field_upload(ffvhc_ctx* ctx, int field, int case_id, int g, const void* padded);
+int ffvhc_field_download(ffvhc_ctx* ctx, int field, int case_id, int g, void* padded);
+/* FFVILS::Fill (FFVILS.h:425-445): every cell and ghost = value */
+int ffvhc_field_fill(ffvhc_ctx* ctx, int field, double value);
+/* FFVILS::Copy (FFVILS.h:447-466) */
+int ffvhc_field_copy(ffvhc_ctx* ctx, int dst, int src);
+
+/* LocalScalar3D::ImposeBoundaryCondition (LocalScalar3D.h:266-296): ghost update then outer BC.
+ * updater 1: Scalar3DUpdater1 (pressure, default fields); 5: Scalar3DUpdater5 (velocity). */
+int ffvhc_ghost_fill(ffvhc_ctx* ctx, int field, int bc_group, int updater);
+
+/* ------------------------------------------------------------------ linear solvers */
+
+/* A = 7 fields {Ap, Aw, Ae, As, An, Ab, At}; counts/residuals are per selected case, in the
+ * order of ffvhc_ctx_select (or case id order).  Return 1 if any case hit a NaN/Inf residual. */
+
+/* FFVILS::RBGS (FFVILS.h:547-594) */
+int ffvhc_rbgs(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countMax, double eps,
+               int* count, double* residual);
+/* FFVILS::RBGS_WithFixedAverage (FFVILS.h:596-659): mean over Ncells removed every iteration */
+int ffvhc_rbgs_fixed_average(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countMax,
+                             double eps, int* count, double* residual);
+/* FFVILS::CG (FFVILS.h:661-791): Jacobi-preconditioned CG, countPre damped-Jacobi sweeps */
+int ffvhc_cg(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countPre, int countMax,
+             double eps, int* count, double* residual);
+/* FFVILS::Jacobi (FFVILS.h:492-545) */
+int ffvhc_jacobi(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countMax, double eps,
+                 int* count, double* residual);
+
+/* Mixed precision: inner sweeps / matvecs in FP32 wrapped in FP64 defect correction to the same
+ * relative residual eps (FP64 residual, dots and state).  method 0 RBGS, 1 Jacobi-PCG.
+ * inner_eps: relative reduction per correction.  outer[c] = corrections taken. */
+int ffvhc_solve_mixed(ffvhc_ctx* ctx, int method, int x, const int* A, int b, int bc_group, double omega,
+                      int countMax, double eps, double inner_eps, int* count, double* residual, int* outer);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,204 @@
+"""GPU parity of the ghost fill and the fused RBGS iteration against the oracle (FP64).
+
+Bit-exact: the device kernels evaluate the same expressions in the same order with no FMA
+contraction; only reduction order differs (residual norms, checked to 1e-12 relative).
+Compared region: interior cells and the first ghost layer of each face (what the device keeps).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import ffvhc
+from oracle import oracle as O
+from helpers import G, N, OracleGrid, cell_centres, interior, live_mask
+
+pytestmark = pytest.mark.gpu
+
+LIVE = live_mask()
+
+
+def trees():
+    return {
+        "flat": ffvhc.Tree(root=(3, 2, 2), root_length=1.0),
+        "two_level": ffvhc.Tree(root=(3, 3, 2), root_length=1.0,
+                                regions=[dict(kind="ball", a=(1.4, 1.6, 1.0), r=0.45, level=1)]),
+        "three_level": ffvhc.Tree(root=(2, 2, 2), root_length=1.0,
+                                  regions=[dict(kind="ball", a=(0.9, 1.1, 1.05), r=0.2, level=2)]),
+    }
+
+
+BC = {ffvhc.BC_UX: [(0, 1.0), (1, 0.0), (1, 0.0), (1, 0.25), (0, -0.5), (1, 0.0)],
+      ffvhc.BC_P: [(1, 0.0), (0, 0.0), (1, 0.0), (1, 0.0), (1, 0.0), (1, 0.0)]}
+
+
+def assert_live_equal(got, want, what):
+    bad = np.flatnonzero((got != want)[:, LIVE])
+    assert bad.size == 0, "%s: %d live cells differ, max |d| %.3e" % (
+        what, bad.size, np.max(np.abs((got - want)[:, LIVE])))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = ffvhc.Context(max_blocks=4096)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("name", ["flat", "two_level", "three_level"])
+@pytest.mark.parametrize("updater,group", [(1, ffvhc.BC_P), (5, ffvhc.BC_UX)])
+def test_ghost_fill_bit_exact(ctx, name, updater, group):
+    tree = trees()[name]
+    params = ffvhc.case_params(bc=BC)
+    cid = ctx.add_case(tree, params)
+    try:
+        og = OracleGrid(tree)
+        rng = np.random.default_rng(7)
+        f = rng.uniform(-1, 1, og.shape)
+        fld = ctx.field(8)
+        ctx.upload(fld, cid, f)
+        ctx.ghost_fill(fld, group, updater)
+        got = ctx.download(fld, cid, out=f.copy())
+        bct = [params.bct[group][q] for q in range(6)]
+        bcv = [params.bcv[group][q] for q in range(6)]
+        og.ghost_fill(f, updater, bct, bcv)
+        assert_live_equal(got, f, "ghost fill %s u%d" % (name, updater))
+        ctx.free(fld)
+    finally:
+        ctx.remove_case(cid)
+
+
+def make_system(og, rng, bct, bcv):
+    A = [og.zeros() for _ in range(7)]
+    for q in range(1, 7):
+        A[q][:] = rng.uniform(0.6, 1.4, og.shape)
+    A[0][:] = -(sum(A[q] for q in range(1, 7))) - rng.uniform(0.0, 0.2, og.shape)
+    b = rng.uniform(-1, 1, og.shape)
+    og.L.orc_matrix_bc(og.ref(), O.parr(A), O.ptr(b), O.cint6(bct), O.cdbl6(bcv))
+    return A, b
+
+
+def run_oracle_rbgs(og, x, A, b, bct, bcv, omega, cmax, eps):
+    cnt, res = ctypes.c_int(), ctypes.c_double()
+    og.L.orc_rbgs(og.ref(), O.ptr(x), O.parr(A), O.ptr(b), O.cint6(bct), O.cdbl6(bcv), ctypes.c_double(omega), cmax,
+                  ctypes.c_double(eps), ctypes.byref(cnt), ctypes.byref(res))
+    return cnt.value, res.value
+
+
+def upload_system(ctx, cid, A, b, x):
+    fa = [ctx.field(8) for _ in range(7)]
+    for q in range(7):
+        ctx.upload(fa[q], cid, A[q])
+    fb, fx = ctx.field(8), ctx.field(8)
+    ctx.upload(fb, cid, b)
+    ctx.upload(fx, cid, x)
+    return fa, fb, fx
+
+
+@pytest.mark.parametrize("name", ["flat", "two_level", "three_level"])
+def test_rbgs_fixed_count_bit_exact(ctx, name):
+    tree = trees()[name]
+    params = ffvhc.case_params(bc=BC)
+    cid = ctx.add_case(tree, params)
+    try:
+        og = OracleGrid(tree)
+        rng = np.random.default_rng(3)
+        bct = [params.bct[ffvhc.BC_P][q] for q in range(6)]
+        bcv = [params.bcv[ffvhc.BC_P][q] for q in range(6)]
+        A, b = make_system(og, rng, bct, bcv)
+        x = rng.uniform(-1, 1, og.shape)
+        og.ghost_fill(x, 1, bct, bcv)
+        fa, fb, fx = upload_system(ctx, cid, A, b, x)
+        cnt, res, rc = ctx.rbgs(fx, fa, fb, ffvhc.BC_P, 1.2, 9, 0.0, ncases=1)
+        got = ctx.download(fx, cid, out=x.copy())
+        c2, r2 = run_oracle_rbgs(og, x, A, b, bct, bcv, 1.2, 9, 0.0)
+        assert rc == 0 and cnt[0] == c2 == 10
+        assert abs(res[0] - r2) <= 1e-12 * abs(r2)
+        assert_live_equal(got, x, "rbgs %s" % name)
+        for f in fa + [fb, fx]:
+            ctx.free(f)
+    finally:
+        ctx.remove_case(cid)
+
+
+def test_rbgs_converges_like_oracle(ctx):
+    tree = trees()["two_level"]
+    params = ffvhc.case_params(bc=BC)
+    cid = ctx.add_case(tree, params)
+    try:
+        og = OracleGrid(tree)
+        rng = np.random.default_rng(4)
+        bct = [params.bct[ffvhc.BC_P][q] for q in range(6)]
+        bcv = [params.bcv[ffvhc.BC_P][q] for q in range(6)]
+        A, b = make_system(og, rng, bct, bcv)
+        x = og.zeros()
+        fa, fb, fx = upload_system(ctx, cid, A, b, x)
+        cnt, res, rc = ctx.rbgs(fx, fa, fb, ffvhc.BC_P, 1.2, 500, 1e-6, ncases=1)
+        got = ctx.download(fx, cid, out=x.copy())
+        c2, r2 = run_oracle_rbgs(og, x, A, b, bct, bcv, 1.2, 500, 1e-6)
+        assert cnt[0] == c2 and c2 < 500
+        assert res[0] <= 1e-6
+        assert np.allclose(got[:, LIVE], x[:, LIVE], rtol=0, atol=1e-12 * np.max(np.abs(x)))
+        for f in fa + [fb, fx]:
+            ctx.free(f)
+    finally:
+        ctx.remove_case(cid)
+
+
+def test_two_cases_in_one_launch_equal_alone(ctx):
+    """Batched solve of two cases of different sizes: each result is bit-identical to its solo run."""
+    ta, tb_ = trees()["two_level"], trees()["flat"]
+    params = ffvhc.case_params(bc=BC)
+    bct = [params.bct[ffvhc.BC_P][q] for q in range(6)]
+    bcv = [params.bcv[ffvhc.BC_P][q] for q in range(6)]
+    results = {}
+    for mode in ("alone", "mix"):
+        ids = []
+        fields = []
+        for k, tr in enumerate((ta, tb_)):
+            if mode == "alone" and k == 1:
+                continue
+            ids.append(ctx.add_case(tr, params))
+        if mode == "alone":
+            ids.append(None)
+        systems = []
+        for k, tr in enumerate((ta, tb_)):
+            og = OracleGrid(tr)
+            rng = np.random.default_rng(20 + k)
+            A, b = make_system(og, rng, bct, bcv)
+            systems.append((og, A, b))
+        fa = [ctx.field(8) for _ in range(7)]
+        fb, fx = ctx.field(8), ctx.field(8)
+        fields = fa + [fb, fx]
+        outs = []
+        if mode == "mix":
+            for k in range(2):
+                og, A, b = systems[k]
+                for q in range(7):
+                    ctx.upload(fa[q], ids[k], A[q])
+                ctx.upload(fb, ids[k], b)
+                ctx.upload(fx, ids[k], og.zeros())
+            cnt, res, rc = ctx.rbgs(fx, fa, fb, ffvhc.BC_P, 1.1, 300, 1e-5, ncases=2)
+            outs = [(ctx.download(fx, ids[k]), cnt[k], res[k]) for k in range(2)]
+            for i in ids:
+                ctx.remove_case(i)
+        else:
+            ctx.remove_case(ids[0])
+            for k in range(2):
+                i = ctx.add_case((ta, tb_)[k], params)
+                og, A, b = systems[k]
+                for q in range(7):
+                    ctx.upload(fa[q], i, A[q])
+                ctx.upload(fb, i, b)
+                ctx.upload(fx, i, og.zeros())
+                cnt, res, rc = ctx.rbgs(fx, fa, fb, ffvhc.BC_P, 1.1, 300, 1e-5, ncases=1)
+                outs.append((ctx.download(fx, i), cnt[0], res[0]))
+                ctx.remove_case(i)
+        results[mode] = outs
+        for f in fields:
+            ctx.free(f)
+    for k in range(2):
+        xa, ca, ra = results["alone"][k]
+        xm, cm, rm = results["mix"][k]
+        assert ca == cm and ra == rm
+        assert np.array_equal(xa[:, LIVE], xm[:, LIVE])
