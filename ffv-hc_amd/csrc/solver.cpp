@@ -67,6 +67,40 @@ int rbgs_run(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, in
   return 0;
 }
 
+template <class T>
+int rbgs_fixed_run(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int nsweep, double* ms) {
+  int rc = ensure_second_buffer(c, x);
+  if (rc) return rc;
+  FieldRec& X = c->fields[x];
+  launch_solve_init(c->sm, c->dst, (int)c->cases.size(), omega, -1.0, nsweep);
+  Coef7<T> K = coef<T>(c, A);
+  const T* bv = (const T*)c->fields[b].d[0];
+  T* xs[2] = {(T*)X.d[0], (T*)X.d[1]};
+  T* hs[2] = {(T*)X.h[0], (T*)X.h[1]};
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->sm));
+  for (int n = 1; n <= nsweep; n++)
+    launch_rbgs_iter<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, bv, xs[(n - 1) & 1], xs[n & 1],
+                        hs[(n - 1) & 1], hs[n & 1], c->dpart, (size_t)c->cap, n, bcg);
+  HIPCHK(hipEventRecord(e1, c->sm));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // the result of sweep nsweep is in buffer (nsweep & 1)
+  if (nsweep & 1) {
+    launch_copy<T>(c->sm, c->dactive, c->nactive, xs[0], hs[0], xs[1], hs[1]);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(c->sm));
+  return 0;
+}
+
 int report(ffvhc_ctx* c, int* count, double* residual) {
   const std::vector<int> ids = selected_cases(c);
   HIPCHK(hipStreamSynchronize(c->sm));
@@ -133,6 +167,18 @@ int ffvhc_rbgs(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, 
                  : rbgs_run<float>(c, x, A, b, bcg, omega, countMax, eps);
   if (rc) return rc;
   return report(c, count, residual);
+}
+
+int ffvhc_rbgs_fixed(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int nsweep, double* ms) {
+  if (!c) return fail(-1, "null context");
+  int prec = field_ok(c, x, 0) ? c->fields[x].prec : 8;
+  int rc = check_args(c, x, A, b, prec);
+  if (rc) return rc;
+  if (bcg < 0 || bcg >= NUM_BC || nsweep < 1) return fail(-1, "bad bc group / sweep count");
+  rc = refresh_active(c);
+  if (rc) return rc;
+  return prec == 8 ? rbgs_fixed_run<double>(c, x, A, b, bcg, omega, nsweep, ms)
+                   : rbgs_fixed_run<float>(c, x, A, b, bcg, omega, nsweep, ms);
 }
 
 }  // extern "C"

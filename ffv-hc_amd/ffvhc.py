@@ -72,6 +72,7 @@ _sig = {
     "ffvhc_field_copy": (c_int, [_vp, c_int, c_int]),
     "ffvhc_ghost_fill": (c_int, [_vp, c_int, c_int, c_int]),
     "ffvhc_rbgs": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
+    "ffvhc_rbgs_fixed": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, _P(c_double)]),
     "ffvhc_rbgs_fixed_average": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
     "ffvhc_cg": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_int, c_double, _vp, _vp]),
     "ffvhc_jacobi": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
@@ -286,6 +287,13 @@ class Context:
         if rc < 0:
             check(rc, "ffvhc_rbgs")
         return cnt, res, rc
+
+    def rbgs_fixed(self, x, A, b, group, omega, nsweep):
+        """nsweep fused RBGS iterations; returns the device time in ms."""
+        ms = c_double()
+        a = np.asarray(A, np.int32)
+        check(_L.ffvhc_rbgs_fixed(self.h, x, _ptr(a), b, group, omega, nsweep, ctypes.byref(ms)), "ffvhc_rbgs_fixed")
+        return ms.value
 
     def rbgs_fixed_average(self, x, A, b, group, omega, count_max, eps, ncases=None):
         nc = ncases or self._nsel()

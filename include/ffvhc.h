@@ -141,6 +141,9 @@ int ffvhc_ghost_fill(ffvhc_ctx* ctx, int field, int bc_group, int updater);
 /* FFVILS::RBGS (FFVILS.h:547-594) */
 int ffvhc_rbgs(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countMax, double eps,
                int* count, double* residual);
+/* Exactly nsweep fused RBGS iterations (FFVILS::RBGS with eps < 0, countMax = nsweep) on the
+ * context stream, bracketed by HIP events; *ms = device time of the sweeps (timing entry). */
+int ffvhc_rbgs_fixed(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int nsweep, double* ms);
 /* FFVILS::RBGS_WithFixedAverage (FFVILS.h:596-659): mean over Ncells removed every iteration */
 int ffvhc_rbgs_fixed_average(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countMax,
                              double eps, int* count, double* residual);
