@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--omega", type=float, default=1.2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--profile-sweeps", type=int, default=0,
+                    help="only run this many fused FP64 and FP32 RBGS iterations (for rocprofv3) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: every box in this run has one GPU; multi-GPU runs are not part of this build "
@@ -136,6 +138,13 @@ def main():
     A, b, bct = poisson_system(tree)
     ctx, cid, f64, f32 = setup(tree, A, b, bct)
     import ffvhc
+    if args.profile_sweeps:
+        for f in (f64, f32):
+            ctx.fill(f["x"], 0.0)
+            ms = ctx.rbgs_fixed(f["x"], f["A"], f["b"], ffvhc.BC_P, args.omega, args.profile_sweeps)
+            print("profile: %d sweeps, %.2f us/sweep" % (args.profile_sweeps, ms * 1e3 / args.profile_sweeps))
+        ctx.close()
+        return
 
     # kernel time of the fused RBGS iteration (event-timed back-to-back launches)
     ctx.fill(f64["x"], 0.0)

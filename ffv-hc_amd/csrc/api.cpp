@@ -79,6 +79,19 @@ int alloc_field(ffvhc_ctx* ctx, int prec, int* field) {
   return 0;
 }
 
+int scratch_field(ffvhc_ctx* ctx, int prec, int role, int* field) {
+  const int key = prec * 100 + role;
+  for (auto& kv : ctx->scratch_ids)
+    if (kv.first == key) {
+      *field = kv.second;
+      return 0;
+    }
+  int rc = alloc_field(ctx, prec, field);
+  if (rc) return rc;
+  ctx->scratch_ids.push_back({key, *field});
+  return 0;
+}
+
 int ensure_second_buffer(ffvhc_ctx* ctx, int field) {
   FieldRec& f = ctx->fields[field];
   if (f.d[1]) return 0;
@@ -179,15 +192,18 @@ int ffvhc_ctx_create(int device, long long max_blocks, ffvhc_ctx** out) {
   c->hblk.assign((size_t)max_blocks, DevBlock{});
   c->hcase.assign(MAX_CASES, DevCase{});
   c->hst.assign(MAX_CASES, SolveState{});
+  c->hst2.assign(MAX_CASES, SolveState{});
   HIPCHK(hipMalloc(&c->dblk, sizeof(DevBlock) * max_blocks));
   HIPCHK(hipMalloc(&c->dcase, sizeof(DevCase) * MAX_CASES));
   HIPCHK(hipMalloc(&c->dactive, sizeof(int) * max_blocks));
   HIPCHK(hipMalloc(&c->dst, sizeof(SolveState) * MAX_CASES));
+  HIPCHK(hipMalloc(&c->dst2, sizeof(SolveState) * MAX_CASES));
   HIPCHK(hipMalloc(&c->dpart, sizeof(double) * 4 * max_blocks));
   HIPCHK(hipMalloc(&c->dcut, sizeof(float) * 6 * NC * max_blocks));
   HIPCHK(hipMalloc(&c->dflag, sizeof(uint32_t) * NC * max_blocks));
   HIPCHK(hipMalloc(&c->dcid, sizeof(uint8_t) * 6 * NC * max_blocks));
   HIPCHK(hipMemset(c->dst, 0, sizeof(SolveState) * MAX_CASES));
+  HIPCHK(hipMemset(c->dst2, 0, sizeof(SolveState) * MAX_CASES));
   c->bytes = (long long)(sizeof(DevBlock) + sizeof(int) + 4 * sizeof(double) + (6 * 4 + 4 + 6) * NC) * max_blocks;
   c->free_slots.push_back({0, max_blocks});
   *out = c.release();
@@ -207,6 +223,7 @@ void ffvhc_ctx_destroy(ffvhc_ctx* c) {
   (void)hipFree(c->dcase);
   (void)hipFree(c->dactive);
   (void)hipFree(c->dst);
+  (void)hipFree(c->dst2);
   (void)hipFree(c->dpart);
   (void)hipFree(c->dcut);
   (void)hipFree(c->dflag);
@@ -249,11 +266,15 @@ static int upload_case_meta(ffvhc_ctx* c, int id) {
   d.gz = r.params.gz;
   d.betag = r.params.betag;
   d.tr = r.params.tr;
-  for (int g = 0; g < NUM_BC; g++)
+  for (int g = 0; g < NUM_USER_BC; g++)
     for (int f = 0; f < 6; f++) {
       d.bct[g][f] = r.params.bct[g][f];
       d.bcv[g][f] = r.params.bcv[g][f];
     }
+  for (int f = 0; f < 6; f++) {
+    d.bct[BC_HOM][f] = r.params.bct[BC_P][f];
+    d.bcv[BC_HOM][f] = 0.0;
+  }
   HIPCHK(hipMemcpyAsync(c->dcase + id, &d, sizeof(DevCase), hipMemcpyHostToDevice, c->sm));
   HIPCHK(hipStreamSynchronize(c->sm));
   return 0;
@@ -386,7 +407,7 @@ int ffvhc_case_num_blocks(const ffvhc_ctx* c, int id) {
 
 int ffvhc_case_set_bc(ffvhc_ctx* c, int id, int group, int face, int type, double value) {
   if (!c || id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
-  if (group < 0 || group >= NUM_BC || face < 0 || face > 5) return fail(-1, "bad group/face");
+  if (group < 0 || group >= NUM_USER_BC || face < 0 || face > 5) return fail(-1, "bad group/face");
   c->cases[id].params.bct[group][face] = type;
   c->cases[id].params.bcv[group][face] = value;
   return upload_case_meta(c, id);

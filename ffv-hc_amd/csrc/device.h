@@ -23,7 +23,9 @@ constexpr int MAX_CASES = 1024;
 
 // Outer-BC groups carried by each case: the velocity components, pressure and the
 // boundaryTypeNULL fields (Neumann 0: A, b, face velocities, scratch, Solver.cpp:2024-2034).
-enum BcGroup { BC_UX = 0, BC_UY = 1, BC_UZ = 2, BC_P = 3, BC_NULL = 4, BC_ZERO = 5, NUM_BC = 6 };
+// BC_HOM carries the pressure BC types with zero values: the homogeneous BC of a correction.
+enum BcGroup { BC_UX = 0, BC_UY = 1, BC_UZ = 2, BC_P = 3, BC_NULL = 4, BC_ZERO = 5, BC_HOM = 6, NUM_BC = 7 };
+constexpr int NUM_USER_BC = 6;
 enum BcType { BCT_NONE = -1, BCT_DIRICHLET = 0, BCT_NEUMANN = 1, BCT_PERIODIC = 9 };
 
 struct DevBlock {

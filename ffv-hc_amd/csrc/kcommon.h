@@ -1,0 +1,134 @@
+// Device helpers shared by the solver kernels: deterministic per-case reductions, the
+// convergence test and the halo push epilogue.
+#pragma once
+
+#include <cfloat>
+
+#include "device.h"
+#include "ghost.h"
+
+namespace ffvhc {
+
+// ------------------------------------------------------------------------------------------
+// Per-case deterministic reductions: every workgroup writes its block's partial sum(s); the
+// last workgroup of the case (ticket) adds the partials in slot order with a fixed tree.
+// ------------------------------------------------------------------------------------------
+
+template <int NT>
+__device__ inline double block_sum(double v, double* red) {
+  // fixed-pattern wave reduction, then a fixed-order pass over the waves
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NT / 64; q++) s += red[q];
+  return s;  // valid in thread 0
+}
+
+template <int NT>
+__device__ inline double case_sum(const double* part, int begin, int count, double* red) {
+  double v = 0.0;
+  const volatile double* p = part;
+  for (int q = threadIdx.x; q < count; q += NT) v += p[begin + q];
+  // tree in shared memory with fixed pairing
+  __shared__ double tree[NT];
+  tree[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = NT / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) tree[threadIdx.x] = tree[threadIdx.x] + tree[threadIdx.x + h];
+    __syncthreads();
+  }
+  double r = tree[0];
+  __syncthreads();
+  (void)red;
+  return r;
+}
+
+// returns true in every thread of the last workgroup of case c
+__device__ inline bool last_of_case(SolveState* st, int c, int nblk, int* flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned int prev = atomicAdd(&st[c].ticket, 1u);
+    *flag = (prev == (unsigned int)(nblk - 1));
+  }
+  __syncthreads();
+  bool last = *flag;
+  if (last) __threadfence();
+  return last;
+}
+
+__device__ inline void converged(SolveState& s, double rr, int count) {
+  // FFVILS.h:24-47 IsConverged
+  s.rr = rr;
+  double bb = s.bb;
+  double residual = (fabs(bb) > FLT_MIN) ? sqrt(fabs(rr) / fabs(bb)) : sqrt(fabs(rr));
+  s.residual = residual;
+  s.count = count;
+  if (residual <= s.eps) {
+    s.done = 1;
+  } else if (isnan(residual) || isinf(residual)) {
+    s.done = 1;
+    s.status = 1;
+  } else if (count >= s.countMax) {
+    s.done = 1;
+    s.count = s.countMax + 1;  // loop exhausted: for(count=1; count<=countMax; ++count)
+  }
+}
+
+constexpr int LP = NB + 2;  // LDS pitch
+__device__ inline int lds(int i, int j, int k) { return (i + 1) + LP * ((j + 1) + LP * (k + 1)); }
+
+// Push this block's ghost-layer contributions into the next halo buffer: its own outer faces
+// (bcx.f90), the facing layer of same-level neighbours, F2C means into a coarser neighbour's
+// quarter face and C2F trilinear values into each finer neighbour (Scalar3DUpdater1 semantics).
+// `L(i,j,k)` reads the block's new interior (LDS).
+template <class T, int NT, class Acc>
+__device__ inline void push_halo(const DevBlock& B, const DevCase& C, int bcg, int slot, const Acc& L,
+                                 const T* __restrict__ hin, T* __restrict__ hout) {
+  const int t = threadIdx.x;
+  const size_t hbase = (size_t)slot * NH;
+  for (int f = 0; f < 6; f++) {
+    const int ft = B.ftype[f];
+    if (ft == FACE_OUTER) {
+      const int type = C.bct[bcg][f];
+      if (type < 0) continue;
+      T xc = (T)(type == BCT_NEUMANN ? C.bcv[bcg][f] * B.dx : C.bcv[bcg][f]);
+      bool pp = !(f & 1) && B.ftype[f ^ 1] == FACE_OUTER && C.bct[bcg][f ^ 1] == BCT_PERIODIC;
+      for (int p = t; p < NF; p += NT) {
+        int a = p & 15, b = p >> 4, i, j, k;
+        face_cell(f, a, b, 0, i, j, k);
+        T xa = L(i, j, k);
+        face_cell(f ^ 1, a, b, 0, i, j, k);
+        T xf = L(i, j, k);
+        hout[hbase + f * NF + p] = outer_value<T>(f, type, xc, xa, xf, hin[hbase + f * NF + p], pp);
+      }
+    } else if (ft == FACE_SAME) {
+      const size_t nb = (size_t)B.nbr[f][0] * NH + (f ^ 1) * NF;
+      for (int p = t; p < NF; p += NT) {
+        int i, j, k;
+        face_cell(f, p & 15, p >> 4, 0, i, j, k);
+        hout[nb + p] = L(i, j, k);
+      }
+    } else if (ft == FACE_COARSER) {
+      const size_t cb = (size_t)B.nbr[f][0] * NH;
+      int oa, ob;
+      sub_origins_ab(f, B.sub[f], oa, ob);
+      for (int p = t; p < NF / 4; p += NT) {
+        int ca = (p & 7) + 8 * oa, cbb = (p >> 3) + 8 * ob;
+        hout[cb + halo_index(f ^ 1, ca, cbb)] = f2c_mean<T>(f ^ 1, ca, cbb, L, false);
+      }
+    } else {
+      for (int q = t; q < 4 * NF; q += NT) {
+        const int sub = q >> 8, p = q & 255;
+        const size_t fb = (size_t)B.nbr[f][sub] * NH;
+        hout[fb + halo_index(f ^ 1, p & 15, p >> 4)] = c2f_trilinear<T>(f ^ 1, sub, p & 15, p >> 4, L);
+      }
+    }
+  }
+}
+
+}  // namespace ffvhc

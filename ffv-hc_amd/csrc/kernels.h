@@ -25,6 +25,26 @@ void launch_fill(hipStream_t sm, const int* active, int nact, T* x, T* h, T v);
 template <class T>
 void launch_copy(hipStream_t sm, const int* active, int nact, T* y, T* hy, const T* x, const T* hx);
 
-void launch_solve_init(hipStream_t sm, SolveState* st, int ncase, double omega, double eps, int countMax);
+template <class T>
+void launch_cg_init(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs, SolveState* st,
+                    Coef7<T> A, const T* b, const T* x, const T* hx, T* r, double* part, size_t nslots);
+template <class T>
+void launch_cg_p(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                 const SolveState* st, Coef7<T> A, const T* r, T* p, T* hp, int count, int bcg);
+template <class T>
+void launch_cg_q(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs, SolveState* st,
+                 Coef7<T> A, const T* p, const T* hp, T* q, double* part);
+template <class T>
+void launch_cg_xr(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs, SolveState* st,
+                  Coef7<T> A, T* x, T* r, const T* p, const T* q, double* part, size_t nslots, int count);
+void launch_defect(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                   SolveState* st, Coef7<double> A, const double* b, const double* x, const double* hx, float* r32,
+                   double* part, size_t nslots, int outer);
+void launch_add_correction(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const SolveState* st,
+                           double* x, const float* e);
+void launch_convert(hipStream_t sm, const int* active, int nact, float* y, const double* x);
+
+void launch_solve_init(hipStream_t sm, SolveState* st, int ncase, double omega, double eps, int countMax,
+                       const SolveState* outer = nullptr);
 
 }  // namespace ffvhc

@@ -1,0 +1,361 @@
+// Jacobi-preconditioned CG (FFVILS::CG, FFVILS.h:661-791) as three fused kernels per iteration,
+// with every scalar (alpha, beta, the convergence test) computed on the device by the last
+// workgroup of each case.  The preconditioner is the reference's Jacobi_PreConditioner with
+// countPreConditioner = 1 from z = 0 (FFVILS.h:468-490, bils.f90:5-55), i.e.
+// z = 0 - omega*(0 - (r - A_nb*0 ...)/Ap), evaluated literally so the rounding matches.
+//
+//   init : r = b - A x (calc_r), b.b, z, r.z                                      [FFVILS.h:679-712]
+//   P    : p = z (first iteration, Copy) or z + beta p (XPAY); push p's ghost layer [716-731]
+//   Q    : q = A p (calc_ax), q.p -> alpha                                        [733-749]
+//   XR   : x += alpha p, r -= alpha q, r.r -> IsConverged; next z, r.z -> beta     [751-787]
+#include "kcommon.h"
+#include "kernels.h"
+
+namespace ffvhc {
+
+constexpr int CG_NT = 256;
+
+template <class T>
+__device__ inline T jacobi_z(T r, T ap, const T* an, T omega) {
+  // jacobi_smoother with x0 = 0 everywhere, one sweep
+  const T x0 = (T)0;
+  T rp = r - an[0] * x0 - an[1] * x0 - an[2] * x0 - an[3] * x0 - an[4] * x0 - an[5] * x0;
+  T xt = rp / ap;
+  return x0 - omega * (x0 - xt);
+}
+
+template <class T>
+__global__ __launch_bounds__(CG_NT) void cg_init_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                        const DevCase* __restrict__ cs, SolveState* __restrict__ st,
+                                                        Coef7<T> A, const T* __restrict__ bv, const T* __restrict__ x,
+                                                        const T* __restrict__ hx, T* __restrict__ r,
+                                                        double* __restrict__ part, size_t nslots) {
+  __shared__ T s[LP * LP * LP];
+  __shared__ double red[CG_NT / 64];
+  __shared__ int flag;
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const int c = B.case_id;
+  if (st[c].done) return;
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  for (int k = 0; k < NB; k++) s[lds(tx, ty, k)] = x[base + tx + NB * (ty + NB * k)];
+  for (int q = t; q < NH; q += CG_NT) {
+    int f = q >> 8, p = q & 255, i, j, kk;
+    face_cell(f, p & 15, p >> 4, -1, i, j, kk);
+    s[lds(i, j, kk)] = hx[(size_t)slot * NH + q];
+  }
+  __syncthreads();
+  const T omega = (T)st[c].omega;
+  double bb = 0.0, rz = 0.0;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = base + tx + NB * (ty + NB * k);
+    const int li = lds(tx, ty, k);
+    T a[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) a[q] = A.p[q][g];
+    const T b0 = bv[g];
+    const T rr = b0 - a[1] * s[li - 1] - a[2] * s[li + 1] - a[3] * s[li - LP] - a[4] * s[li + LP] -
+                 a[5] * s[li - LP * LP] - a[6] * s[li + LP * LP] - a[0] * s[li];
+    r[g] = rr;
+    const T z = jacobi_z<T>(rr, a[0], a + 1, omega);
+    bb += (double)b0 * (double)b0;
+    rz += (double)rr * (double)z;
+  }
+  double sb = block_sum<CG_NT>(bb, red);
+  double sz = block_sum<CG_NT>(rz, red);
+  if (t == 0) {
+    part[slot] = sb;
+    part[nslots + slot] = sz;
+  }
+  const DevCase& C = cs[c];
+  if (!last_of_case(st, c, C.nblk, &flag)) return;
+  double tb = case_sum<CG_NT>(part, C.slot_begin, C.nblk, red);
+  double tz = case_sum<CG_NT>(part + nslots, C.slot_begin, C.nblk, red);
+  if (t == 0) {
+    SolveState& S = st[c];
+    S.ticket = 0;
+    S.bb = tb;
+    S.rr0 = 1.0;
+    if (fabs(tb) < FLT_MIN) {  // FFVILS.h:683-687
+      S.residual = 0.0;
+      S.count = 0;
+      S.done = 1;
+    } else if (fabs(tz) < FLT_MIN) {  // FFVILS.h:718-722
+      S.residual = tz;
+      S.count = 0;
+      S.done = 1;
+    } else {
+      S.rr1 = tz;
+      S.beta = tz / S.rr0;
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(CG_NT) void cg_p_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                     const DevCase* __restrict__ cs, const SolveState* __restrict__ st,
+                                                     Coef7<T> A, const T* __restrict__ r, T* __restrict__ p,
+                                                     T* __restrict__ hp, int count, int bcg) {
+  __shared__ T s[LP * LP * LP];
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const int c = B.case_id;
+  if (st[c].done) return;
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  const T omega = (T)st[c].omega;
+  const T beta = (T)st[c].beta;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = base + tx + NB * (ty + NB * k);
+    T an[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) an[q] = A.p[q + 1][g];
+    const T z = jacobi_z<T>(r[g], A.p[0][g], an, omega);
+    const T pn = (count == 1) ? z : z + beta * p[g];
+    p[g] = pn;
+    s[lds(tx, ty, k)] = pn;
+  }
+  __syncthreads();
+  push_halo<T, CG_NT>(B, cs[c], bcg, slot, [&](int i, int j, int k) { return s[lds(i, j, k)]; }, hp, hp);
+}
+
+template <class T>
+__global__ __launch_bounds__(CG_NT) void cg_q_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                     const DevCase* __restrict__ cs, SolveState* __restrict__ st,
+                                                     Coef7<T> A, const T* __restrict__ p, const T* __restrict__ hp,
+                                                     T* __restrict__ q, double* __restrict__ part) {
+  __shared__ T s[LP * LP * LP];
+  __shared__ double red[CG_NT / 64];
+  __shared__ int flag;
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const int c = B.case_id;
+  if (st[c].done) return;
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  for (int k = 0; k < NB; k++) s[lds(tx, ty, k)] = p[base + tx + NB * (ty + NB * k)];
+  for (int w = t; w < NH; w += CG_NT) {
+    int f = w >> 8, pp = w & 255, i, j, kk;
+    face_cell(f, pp & 15, pp >> 4, -1, i, j, kk);
+    s[lds(i, j, kk)] = hp[(size_t)slot * NH + w];
+  }
+  __syncthreads();
+  double qp = 0.0;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = base + tx + NB * (ty + NB * k);
+    const int li = lds(tx, ty, k);
+    T a[7];
+#pragma unroll
+    for (int w = 0; w < 7; w++) a[w] = A.p[w][g];
+    const T pv = s[li];
+    const T qv = a[1] * s[li - 1] + a[2] * s[li + 1] + a[3] * s[li - LP] + a[4] * s[li + LP] +
+                 a[5] * s[li - LP * LP] + a[6] * s[li + LP * LP] + a[0] * pv;
+    q[g] = qv;
+    qp += (double)qv * (double)pv;
+  }
+  double sq = block_sum<CG_NT>(qp, red);
+  if (t == 0) part[slot] = sq;
+  const DevCase& C = cs[c];
+  if (!last_of_case(st, c, C.nblk, &flag)) return;
+  double tq = case_sum<CG_NT>(part, C.slot_begin, C.nblk, red);
+  if (t == 0) {
+    SolveState& S = st[c];
+    S.ticket = 0;
+    S.alpha = S.rr1 / tq;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(CG_NT) void cg_xr_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                      const DevCase* __restrict__ cs, SolveState* __restrict__ st,
+                                                      Coef7<T> A, T* __restrict__ x, T* __restrict__ r,
+                                                      const T* __restrict__ p, const T* __restrict__ q,
+                                                      double* __restrict__ part, size_t nslots, int count) {
+  __shared__ double red[CG_NT / 64];
+  __shared__ int flag;
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const int c = B.case_id;
+  if (st[c].done) return;
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  const T alpha = (T)st[c].alpha;
+  const T nalpha = -alpha;
+  const T omega = (T)st[c].omega;
+  double rr = 0.0, rz = 0.0;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = base + tx + NB * (ty + NB * k);
+    x[g] = x[g] + alpha * p[g];
+    const T rn = r[g] + nalpha * q[g];
+    r[g] = rn;
+    T an[6];
+#pragma unroll
+    for (int w = 0; w < 6; w++) an[w] = A.p[w + 1][g];
+    const T z = jacobi_z<T>(rn, A.p[0][g], an, omega);
+    rr += (double)rn * (double)rn;
+    rz += (double)rn * (double)z;
+  }
+  double sr = block_sum<CG_NT>(rr, red);
+  double sz = block_sum<CG_NT>(rz, red);
+  if (t == 0) {
+    part[slot] = sr;
+    part[nslots + slot] = sz;
+  }
+  const DevCase& C = cs[c];
+  if (!last_of_case(st, c, C.nblk, &flag)) return;
+  double tr = case_sum<CG_NT>(part, C.slot_begin, C.nblk, red);
+  double tz = case_sum<CG_NT>(part + nslots, C.slot_begin, C.nblk, red);
+  if (t == 0) {
+    SolveState& S = st[c];
+    S.ticket = 0;
+    S.iter = count;
+    converged(S, tr, count);
+    if (!S.done) {
+      S.rr0 = S.rr1;
+      if (fabs(tz) < FLT_MIN) {
+        S.residual = tz;
+        S.count = 0;
+        S.done = 1;
+      } else {
+        S.rr1 = tz;
+        S.beta = tz / S.rr0;
+      }
+    }
+  }
+}
+
+template <class T>
+void launch_cg_init(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs, SolveState* st,
+                    Coef7<T> A, const T* b, const T* x, const T* hx, T* r, double* part, size_t nslots) {
+  if (nact) hipLaunchKernelGGL(cg_init_kernel<T>, dim3(nact), dim3(CG_NT), 0, sm, active, blk, cs, st, A, b, x, hx, r,
+                               part, nslots);
+}
+template <class T>
+void launch_cg_p(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                 const SolveState* st, Coef7<T> A, const T* r, T* p, T* hp, int count, int bcg) {
+  if (nact) hipLaunchKernelGGL(cg_p_kernel<T>, dim3(nact), dim3(CG_NT), 0, sm, active, blk, cs, st, A, r, p, hp,
+                               count, bcg);
+}
+template <class T>
+void launch_cg_q(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs, SolveState* st,
+                 Coef7<T> A, const T* p, const T* hp, T* q, double* part) {
+  if (nact) hipLaunchKernelGGL(cg_q_kernel<T>, dim3(nact), dim3(CG_NT), 0, sm, active, blk, cs, st, A, p, hp, q, part);
+}
+template <class T>
+void launch_cg_xr(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs, SolveState* st,
+                  Coef7<T> A, T* x, T* r, const T* p, const T* q, double* part, size_t nslots, int count) {
+  if (nact) hipLaunchKernelGGL(cg_xr_kernel<T>, dim3(nact), dim3(CG_NT), 0, sm, active, blk, cs, st, A, x, r, p, q,
+                               part, nslots, count);
+}
+
+#define INST(T)                                                                                                   \
+  template void launch_cg_init<T>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,      \
+                                  Coef7<T>, const T*, const T*, const T*, T*, double*, size_t);                   \
+  template void launch_cg_p<T>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, const SolveState*,   \
+                               Coef7<T>, const T*, T*, T*, int, int);                                              \
+  template void launch_cg_q<T>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,         \
+                               Coef7<T>, const T*, const T*, T*, double*);                                         \
+  template void launch_cg_xr<T>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,        \
+                                Coef7<T>, T*, T*, const T*, const T*, double*, size_t, int);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace ffvhc
+
+namespace ffvhc {
+
+// ------------------------------------------------------------------------------------------
+// FP64 defect correction around an FP32 inner solve.
+//   defect: r = b - A x in FP64 (calc_r order, x with its ghost layer), r.r and (first pass) b.b,
+//           IsConverged on the FP64 residual; r is handed to the inner solve as FP32.
+//   update: x += e (FP32 correction widened to FP64) over the interior.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(CG_NT) void defect_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                       const DevCase* __restrict__ cs, SolveState* __restrict__ st,
+                                                       Coef7<double> A, const double* __restrict__ bv,
+                                                       const double* __restrict__ x, const double* __restrict__ hx,
+                                                       float* __restrict__ r32, double* __restrict__ part,
+                                                       size_t nslots, int outer) {
+  __shared__ double s[LP * LP * LP];
+  __shared__ double red[CG_NT / 64];
+  __shared__ int flag;
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const int c = B.case_id;
+  if (st[c].done) return;
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  for (int k = 0; k < NB; k++) s[lds(tx, ty, k)] = x[base + tx + NB * (ty + NB * k)];
+  for (int q = t; q < NH; q += CG_NT) {
+    int f = q >> 8, p = q & 255, i, j, kk;
+    face_cell(f, p & 15, p >> 4, -1, i, j, kk);
+    s[lds(i, j, kk)] = hx[(size_t)slot * NH + q];
+  }
+  __syncthreads();
+  double rr = 0.0, bb = 0.0;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = base + tx + NB * (ty + NB * k);
+    const int li = lds(tx, ty, k);
+    double a[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) a[q] = A.p[q][g];
+    const double b0 = bv[g];
+    const double r = b0 - a[1] * s[li - 1] - a[2] * s[li + 1] - a[3] * s[li - LP] - a[4] * s[li + LP] -
+                     a[5] * s[li - LP * LP] - a[6] * s[li + LP * LP] - a[0] * s[li];
+    r32[g] = (float)r;
+    rr += r * r;
+    if (outer == 0) bb += b0 * b0;
+  }
+  double sr = block_sum<CG_NT>(rr, red);
+  double sb = outer == 0 ? block_sum<CG_NT>(bb, red) : 0.0;
+  if (t == 0) {
+    part[slot] = sr;
+    part[nslots + slot] = sb;
+  }
+  const DevCase& C = cs[c];
+  if (!last_of_case(st, c, C.nblk, &flag)) return;
+  double tr = case_sum<CG_NT>(part, C.slot_begin, C.nblk, red);
+  double tb = outer == 0 ? case_sum<CG_NT>(part + nslots, C.slot_begin, C.nblk, red) : 0.0;
+  if (t == 0) {
+    SolveState& S = st[c];
+    S.ticket = 0;
+    if (outer == 0) S.bb = tb;
+    S.iter = outer;
+    converged(S, tr, outer);
+  }
+}
+
+__global__ __launch_bounds__(256) void add_correction_kernel(const int* __restrict__ active,
+                                                             const DevBlock* __restrict__ blk,
+                                                             const SolveState* __restrict__ st,
+                                                             double* __restrict__ x, const float* __restrict__ e) {
+  const int slot = active[blockIdx.x];
+  if (st[blk[slot].case_id].done) return;
+  const size_t base = (size_t)slot * NC;
+  for (int q = threadIdx.x; q < NC; q += 256) x[base + q] = x[base + q] + (double)e[base + q];
+}
+
+__global__ __launch_bounds__(256) void convert_kernel(const int* __restrict__ active, float* __restrict__ y,
+                                                      const double* __restrict__ x) {
+  const int slot = active[blockIdx.x];
+  const size_t base = (size_t)slot * NC;
+  for (int q = threadIdx.x; q < NC; q += 256) y[base + q] = (float)x[base + q];
+}
+
+void launch_defect(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                   SolveState* st, Coef7<double> A, const double* b, const double* x, const double* hx, float* r32,
+                   double* part, size_t nslots, int outer) {
+  if (nact) hipLaunchKernelGGL(defect_kernel, dim3(nact), dim3(CG_NT), 0, sm, active, blk, cs, st, A, b, x, hx, r32,
+                               part, nslots, outer);
+}
+void launch_add_correction(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const SolveState* st,
+                           double* x, const float* e) {
+  if (nact) hipLaunchKernelGGL(add_correction_kernel, dim3(nact), dim3(256), 0, sm, active, blk, st, x, e);
+}
+void launch_convert(hipStream_t sm, const int* active, int nact, float* y, const double* x) {
+  if (nact) hipLaunchKernelGGL(convert_kernel, dim3(nact), dim3(256), 0, sm, active, y, x);
+}
+
+}  // namespace ffvhc

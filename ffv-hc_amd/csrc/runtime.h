@@ -68,11 +68,14 @@ struct ffvhc_ctx {
   bool multilevel = false;
   bool dirty = true;
   // solver state
-  ffvhc::SolveState* dst = nullptr;
+  ffvhc::SolveState* dst = nullptr;   // inner / single-level solve state per case
+  ffvhc::SolveState* dst2 = nullptr;  // outer (defect correction) state per case
   std::vector<ffvhc::SolveState> hst;
+  std::vector<ffvhc::SolveState> hst2;
+  // internally allocated scratch fields, keyed by precision * 100 + role
+  std::vector<std::pair<int, int>> scratch_ids;
   double* dpart = nullptr;  // 4 * cap partial sums
   std::vector<ffvhc::FieldRec> fields;
-  std::vector<int> scratch;  // internally allocated fields
   long long bytes = 0;
   // free slot intervals [begin, end)
   std::vector<std::pair<int64_t, int64_t>> free_slots;
@@ -83,5 +86,6 @@ int refresh_active(ffvhc_ctx* ctx);
 std::vector<int> selected_cases(ffvhc_ctx* ctx);
 int ensure_second_buffer(ffvhc_ctx* ctx, int field);
 int alloc_field(ffvhc_ctx* ctx, int prec, int* field);
+int scratch_field(ffvhc_ctx* ctx, int prec, int role, int* field);
 bool field_ok(ffvhc_ctx* ctx, int f, int prec);
 }  // namespace ffvhc

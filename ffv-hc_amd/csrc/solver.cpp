@@ -1,4 +1,5 @@
-// Linear solvers behind the C ABI (FFVILS.h): RBGS with the fused device-resident iteration.
+// Linear solvers behind the C ABI (FFVILS.h): fused device-resident RBGS and Jacobi-PCG, and the
+// FP32-inner / FP64-defect-correction mixed-precision drivers.
 #include <algorithm>
 #include <cstring>
 
@@ -14,6 +15,9 @@ namespace {
 constexpr int kBatchMin = 8;
 constexpr int kBatchMax = 256;
 
+// scratch roles
+enum { R_R = 1, R_P = 2, R_Q = 3, R_E = 4, R_RHS = 5, R_A = 10 };
+
 template <class T>
 Coef7<T> coef(ffvhc_ctx* c, const int* A) {
   Coef7<T> k;
@@ -21,12 +25,13 @@ Coef7<T> coef(ffvhc_ctx* c, const int* A) {
   return k;
 }
 
-int read_state(ffvhc_ctx* c, const std::vector<int>& ids, bool* all_done) {
-  HIPCHK(hipMemcpyAsync(c->hst.data(), c->dst, sizeof(SolveState) * c->cases.size(), hipMemcpyDeviceToHost, c->sm));
+int read_state(ffvhc_ctx* c, SolveState* dst, std::vector<SolveState>& hst, const std::vector<int>& ids,
+               bool* all_done) {
+  HIPCHK(hipMemcpyAsync(hst.data(), dst, sizeof(SolveState) * c->cases.size(), hipMemcpyDeviceToHost, c->sm));
   HIPCHK(hipStreamSynchronize(c->sm));
   *all_done = true;
   for (int id : ids)
-    if (!c->hst[id].done) *all_done = false;
+    if (!hst[id].done) *all_done = false;
   return 0;
 }
 
@@ -38,15 +43,12 @@ int check_args(ffvhc_ctx* c, int x, const int* A, int b, int prec) {
   return 0;
 }
 
+// RBGS on field x (ping-pong), matrix K, rhs bv; state in st. Result left in buffer 0.
 template <class T>
-int rbgs_run(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countMax, double eps) {
-  int rc = ensure_second_buffer(c, x);
-  if (rc) return rc;
-  FieldRec& X = c->fields[x];
+int rbgs_core(ffvhc_ctx* c, FieldRec& X, Coef7<T> K, const T* bv, int bcg, double omega, int countMax, double eps,
+              SolveState* st, std::vector<SolveState>& hst, const SolveState* outer, int* launched) {
   const std::vector<int> ids = selected_cases(c);
-  launch_solve_init(c->sm, c->dst, (int)c->cases.size(), omega, eps, countMax);
-  Coef7<T> K = coef<T>(c, A);
-  const T* bv = (const T*)c->fields[b].d[0];
+  launch_solve_init(c->sm, st, (int)c->cases.size(), omega, eps, countMax, outer);
   T* xs[2] = {(T*)X.d[0], (T*)X.d[1]};
   T* hs[2] = {(T*)X.h[0], (T*)X.h[1]};
   int n = 1, batch = kBatchMin;
@@ -54,16 +56,77 @@ int rbgs_run(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, in
   while (!done) {
     int last = std::min(n + batch - 1, countMax + 1);
     for (; n <= last; n++)
-      launch_rbgs_iter<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, bv, xs[(n - 1) & 1], xs[n & 1],
+      launch_rbgs_iter<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, xs[(n - 1) & 1], xs[n & 1],
                           hs[(n - 1) & 1], hs[n & 1], c->dpart, (size_t)c->cap, n, bcg);
     HIPCHK(hipGetLastError());
-    rc = read_state(c, ids, &done);
+    int rc = read_state(c, st, hst, ids, &done);
     if (rc) return rc;
     if (n > countMax + 1) done = true;
     batch = std::min(batch * 2, kBatchMax);
   }
-  launch_select<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dst, xs[0], xs[1], hs[0], hs[1]);
+  if (launched) *launched = n - 1;
+  launch_select<T>(c->sm, c->dactive, c->nactive, c->dblk, st, xs[0], xs[1], hs[0], hs[1]);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Jacobi-PCG (PreCount = 1) on x, matrix K, rhs bv; scratch r,p,q of the same precision.
+template <class T>
+int cg_core(ffvhc_ctx* c, FieldRec& X, Coef7<T> K, const T* bv, int bcg, double omega, int countMax, double eps,
+            SolveState* st, std::vector<SolveState>& hst, const SolveState* outer, int* launched) {
+  const std::vector<int> ids = selected_cases(c);
+  const int prec = (int)sizeof(T);
+  int fr, fp, fq;
+  int rc = scratch_field(c, prec, R_R, &fr);
+  if (!rc) rc = scratch_field(c, prec, R_P, &fp);
+  if (!rc) rc = scratch_field(c, prec, R_Q, &fq);
+  if (rc) return rc;
+  FieldRec &Rf = c->fields[fr], &Pf = c->fields[fp], &Qf = c->fields[fq];
+  launch_solve_init(c->sm, st, (int)c->cases.size(), omega, eps, countMax, outer);
+  launch_cg_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, (const T*)X.d[0],
+                    (const T*)X.h[0], (T*)Rf.d[0], c->dpart, (size_t)c->cap);
+  int n = 1, batch = kBatchMin;
+  bool done = false;
+  while (!done) {
+    int last = std::min(n + batch - 1, countMax);
+    for (; n <= last; n++) {
+      launch_cg_p<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, (const T*)Rf.d[0], (T*)Pf.d[0],
+                     (T*)Pf.h[0], n, BC_NULL);
+      launch_cg_q<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, (const T*)Pf.d[0], (const T*)Pf.h[0],
+                     (T*)Qf.d[0], c->dpart);
+      launch_cg_xr<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, (T*)X.d[0], (T*)Rf.d[0],
+                      (const T*)Pf.d[0], (const T*)Qf.d[0], c->dpart, (size_t)c->cap, n);
+    }
+    HIPCHK(hipGetLastError());
+    rc = read_state(c, st, hst, ids, &done);
+    if (rc) return rc;
+    if (n > countMax) done = true;
+    batch = std::min(batch * 2, kBatchMax);
+  }
+  if (launched) *launched = n - 1;
+  // FFVILS.h:789 plsx->ImposeBoundaryCondition after the loop
+  launch_ghost_pull<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, (const T*)X.d[0], (T*)X.h[0], bcg, 1,
+                       c->multilevel);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int report(ffvhc_ctx* c, const std::vector<SolveState>& hst, int* count, double* residual) {
+  const std::vector<int> ids = selected_cases(c);
+  int status = 0;
+  for (size_t i = 0; i < ids.size(); i++) {
+    const SolveState& s = hst[ids[i]];
+    if (count) count[i] = s.count;
+    if (residual) residual[i] = s.residual;
+    if (s.status) status = 1;
+  }
+  if (status) set_error("residual is NaN or Inf");
+  return status;
+}
+
+int sync_state(ffvhc_ctx* c, SolveState* dst, std::vector<SolveState>& hst) {
+  HIPCHK(hipStreamSynchronize(c->sm));
+  HIPCHK(hipMemcpy(hst.data(), dst, sizeof(SolveState) * c->cases.size(), hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -92,8 +155,7 @@ int rbgs_fixed_run(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double ome
   if (ms) *ms = t;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  // the result of sweep nsweep is in buffer (nsweep & 1)
-  if (nsweep & 1) {
+  if (nsweep & 1) {  // the result of sweep nsweep is in buffer (nsweep & 1)
     launch_copy<T>(c->sm, c->dactive, c->nactive, xs[0], hs[0], xs[1], hs[1]);
     HIPCHK(hipGetLastError());
   }
@@ -101,19 +163,13 @@ int rbgs_fixed_run(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double ome
   return 0;
 }
 
-int report(ffvhc_ctx* c, int* count, double* residual) {
-  const std::vector<int> ids = selected_cases(c);
-  HIPCHK(hipStreamSynchronize(c->sm));
-  HIPCHK(hipMemcpy(c->hst.data(), c->dst, sizeof(SolveState) * c->cases.size(), hipMemcpyDeviceToHost));
-  int status = 0;
-  for (size_t i = 0; i < ids.size(); i++) {
-    const SolveState& s = c->hst[ids[i]];
-    if (count) count[i] = s.count;
-    if (residual) residual[i] = s.residual;
-    if (s.status) status = 1;
-  }
-  if (status) set_error("residual is NaN or Inf");
-  return status;
+int prologue(ffvhc_ctx* c, int x, const int* A, int b, int bcg, int* prec) {
+  if (!c) return fail(-1, "null context");
+  *prec = field_ok(c, x, 0) ? c->fields[x].prec : 8;
+  int rc = check_args(c, x, A, b, *prec);
+  if (rc) return rc;
+  if (bcg < 0 || bcg >= NUM_BC) return fail(-1, "bad bc group");
+  return refresh_active(c);
 }
 
 }  // namespace
@@ -148,37 +204,134 @@ int ffvhc_field_copy(ffvhc_ctx* c, int dst, int src) {
 
 int ffvhc_rbgs(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countMax, double eps, int* count,
                double* residual) {
-  if (!c) return fail(-1, "null context");
-  int prec = field_ok(c, x, 0) ? c->fields[x].prec : 8;
-  int rc = check_args(c, x, A, b, prec);
+  int prec = 8;
+  int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
-  if (bcg < 0 || bcg >= NUM_BC) return fail(-1, "bad bc group");
-  rc = refresh_active(c);
-  if (rc) return rc;
-  if (countMax < 1) {
-    // for(count=1; count<=countMax; ...) never runs: count = 1, residual untouched
+  if (countMax < 1) {  // for(count=1; count<=countMax; ...) never runs
     const std::vector<int> ids = selected_cases(c);
-    for (size_t i = 0; i < ids.size(); i++) {
+    for (size_t i = 0; i < ids.size(); i++)
       if (count) count[i] = 1;
-    }
     return 0;
   }
-  rc = prec == 8 ? rbgs_run<double>(c, x, A, b, bcg, omega, countMax, eps)
-                 : rbgs_run<float>(c, x, A, b, bcg, omega, countMax, eps);
+  rc = ensure_second_buffer(c, x);
   if (rc) return rc;
-  return report(c, count, residual);
+  if (prec == 8)
+    rc = rbgs_core<double>(c, c->fields[x], coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega, countMax,
+                           eps, c->dst, c->hst, nullptr, nullptr);
+  else
+    rc = rbgs_core<float>(c, c->fields[x], coef<float>(c, A), (const float*)c->fields[b].d[0], bcg, omega, countMax,
+                          eps, c->dst, c->hst, nullptr, nullptr);
+  if (rc) return rc;
+  rc = sync_state(c, c->dst, c->hst);
+  if (rc) return rc;
+  return report(c, c->hst, count, residual);
 }
 
 int ffvhc_rbgs_fixed(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int nsweep, double* ms) {
-  if (!c) return fail(-1, "null context");
-  int prec = field_ok(c, x, 0) ? c->fields[x].prec : 8;
-  int rc = check_args(c, x, A, b, prec);
+  int prec = 8;
+  int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
-  if (bcg < 0 || bcg >= NUM_BC || nsweep < 1) return fail(-1, "bad bc group / sweep count");
-  rc = refresh_active(c);
-  if (rc) return rc;
+  if (nsweep < 1) return fail(-1, "nsweep must be >= 1");
   return prec == 8 ? rbgs_fixed_run<double>(c, x, A, b, bcg, omega, nsweep, ms)
                    : rbgs_fixed_run<float>(c, x, A, b, bcg, omega, nsweep, ms);
+}
+
+int ffvhc_cg(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countPre, int countMax, double eps,
+             int* count, double* residual) {
+  int prec = 8;
+  int rc = prologue(c, x, A, b, bcg, &prec);
+  if (rc) return rc;
+  if (countPre != 1)
+    return fail(-1, "the device CG implements countPreConditioner = 1 (z = omega r / Ap); see DESIGN.md");
+  if (countMax < 1) {
+    const std::vector<int> ids = selected_cases(c);
+    for (size_t i = 0; i < ids.size(); i++)
+      if (count) count[i] = 1;
+    return 0;
+  }
+  if (prec == 8)
+    rc = cg_core<double>(c, c->fields[x], coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega, countMax,
+                         eps, c->dst, c->hst, nullptr, nullptr);
+  else
+    rc = cg_core<float>(c, c->fields[x], coef<float>(c, A), (const float*)c->fields[b].d[0], bcg, omega, countMax, eps,
+                        c->dst, c->hst, nullptr, nullptr);
+  if (rc) return rc;
+  rc = sync_state(c, c->dst, c->hst);
+  if (rc) return rc;
+  return report(c, c->hst, count, residual);
+}
+
+// FP32 inner solve in FP64 defect correction.  count = inner iterations summed over the
+// corrections, residual = final FP64 relative residual, outer = corrections applied.
+int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int bcg, double omega, int countMax,
+                      double eps, double inner_eps, int* count, double* residual, int* outer) {
+  int prec = 8;
+  int rc = prologue(c, x, A, b, bcg, &prec);
+  if (rc) return rc;
+  if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
+  if (method != 0 && method != 1) return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG");
+  const std::vector<int> ids = selected_cases(c);
+  int fa[7], fr, fe;
+  for (int q = 0; q < 7; q++) {
+    rc = scratch_field(c, 4, R_A + q, &fa[q]);
+    if (rc) return rc;
+  }
+  rc = scratch_field(c, 4, R_RHS, &fr);
+  if (!rc) rc = scratch_field(c, 4, R_E, &fe);
+  if (rc) return rc;
+  if (method == 0) {
+    rc = ensure_second_buffer(c, fe);
+    if (rc) return rc;
+  }
+  // FP32 copy of the matrix
+  for (int q = 0; q < 7; q++)
+    launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
+                   (const double*)c->fields[A[q]].d[0]);
+  Coef7<double> K64 = coef<double>(c, A);
+  Coef7<float> K32 = coef<float>(c, fa);
+  FieldRec &X = c->fields[x], &E = c->fields[fe], &Rf = c->fields[fr];
+  std::vector<int> inner_total(c->cases.size(), 0), outers(c->cases.size(), -1);
+  const int max_outer = 64;
+  launch_solve_init(c->sm, c->dst2, (int)c->cases.size(), omega, eps, max_outer);
+  for (int o = 0;; o++) {
+    launch_defect(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst2, K64, (const double*)c->fields[b].d[0],
+                  (const double*)X.d[0], (const double*)X.h[0], (float*)Rf.d[0], c->dpart, (size_t)c->cap, o);
+    HIPCHK(hipGetLastError());
+    bool done = false;
+    rc = read_state(c, c->dst2, c->hst2, ids, &done);
+    if (rc) return rc;
+    for (int id : ids)
+      if (c->hst2[id].done && outers[id] < 0) outers[id] = o;
+    if (done || o >= max_outer) break;
+    launch_fill<float>(c->sm, c->dactive, c->nactive, (float*)E.d[0], (float*)E.h[0], 0.0f);
+    if (method == 0)
+      rc = rbgs_core<float>(c, E, K32, (const float*)Rf.d[0], BC_HOM, omega, countMax, inner_eps, c->dst, c->hst,
+                            c->dst2, nullptr);
+    else
+      rc = cg_core<float>(c, E, K32, (const float*)Rf.d[0], BC_HOM, omega, countMax, inner_eps, c->dst, c->hst,
+                          c->dst2, nullptr);
+    if (rc) return rc;
+    rc = sync_state(c, c->dst, c->hst);
+    if (rc) return rc;
+    for (int id : ids)
+      if (!c->hst2[id].done) inner_total[id] += std::max(c->hst[id].count, 0);
+    launch_add_correction(c->sm, c->dactive, c->nactive, c->dblk, c->dst2, (double*)X.d[0], (const float*)E.d[0]);
+    launch_ghost_pull<double>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, (const double*)X.d[0],
+                              (double*)X.h[0], bcg, 1, c->multilevel);
+    HIPCHK(hipGetLastError());
+  }
+  rc = sync_state(c, c->dst2, c->hst2);
+  if (rc) return rc;
+  int status = 0;
+  for (size_t i = 0; i < ids.size(); i++) {
+    const SolveState& s = c->hst2[ids[i]];
+    if (count) count[i] = inner_total[ids[i]];
+    if (residual) residual[i] = s.residual;
+    if (outer) outer[i] = outers[ids[i]] < 0 ? max_outer : outers[ids[i]];
+    if (s.status) status = 1;
+  }
+  if (status) set_error("residual is NaN or Inf");
+  return status;
 }
 
 }  // extern "C"

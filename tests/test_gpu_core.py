@@ -202,3 +202,90 @@ def test_two_cases_in_one_launch_equal_alone(ctx):
         xm, cm, rm = results["mix"][k]
         assert ca == cm and ra == rm
         assert np.array_equal(xa[:, LIVE], xm[:, LIVE])
+
+
+def run_oracle_cg(og, x, A, b, bct, bcv, omega, cmax, eps):
+    cnt, res = ctypes.c_int(), ctypes.c_double()
+    scratch = [og.zeros() for _ in range(5)]
+    og.L.orc_cg(og.ref(), O.ptr(x), O.parr(A), O.ptr(b), *[O.ptr(s) for s in scratch], O.cint6(bct), O.cdbl6(bcv),
+                ctypes.c_double(omega), 1, cmax, ctypes.c_double(eps), ctypes.byref(cnt), ctypes.byref(res))
+    return cnt.value, res.value
+
+
+def spd_system(og, rng, bct, bcv):
+    """Symmetric 7-point system (uniform bands per block), outer BC folded; single level keeps it SPD."""
+    A = [og.zeros() for _ in range(7)]
+    for q in range(1, 7):
+        A[q][:] = 1.0
+    A[0][:] = -6.0 - rng.uniform(0.0, 0.05, og.shape)
+    b = rng.uniform(-1, 1, og.shape)
+    og.L.orc_matrix_bc(og.ref(), O.parr(A), O.ptr(b), O.cint6(bct), O.cdbl6(bcv))
+    return A, b
+
+
+@pytest.mark.parametrize("name", ["flat", "two_level"])
+def test_cg_fixed_count_matches_oracle(ctx, name):
+    tree = trees()[name]
+    params = ffvhc.case_params(bc=BC)
+    cid = ctx.add_case(tree, params)
+    try:
+        og = OracleGrid(tree)
+        rng = np.random.default_rng(8)
+        bct = [params.bct[ffvhc.BC_P][q] for q in range(6)]
+        bcv = [params.bcv[ffvhc.BC_P][q] for q in range(6)]
+        A, b = spd_system(og, rng, bct, bcv)
+        x = og.zeros()
+        fa, fb, fx = upload_system(ctx, cid, A, b, x)
+        cnt, res, rc = ctx.cg(fx, fa, fb, ffvhc.BC_P, 1.0, 1, 15, 0.0, ncases=1)
+        got = ctx.download(fx, cid, out=x.copy())
+        c2, r2 = run_oracle_cg(og, x, A, b, bct, bcv, 1.0, 15, 0.0)
+        assert rc == 0 and cnt[0] == c2 == 16
+        assert abs(res[0] - r2) <= 1e-9 * abs(r2)
+        scale = np.max(np.abs(x[:, LIVE]))
+        assert np.max(np.abs(got[:, LIVE] - x[:, LIVE])) <= 1e-12 * scale
+        for f in fa + [fb, fx]:
+            ctx.free(f)
+    finally:
+        ctx.remove_case(cid)
+
+
+def true_residual(og, A, x, b):
+    rr = og.L.orc_grid_r2(og.ref(), O.parr(A), O.ptr(x), O.ptr(b))
+    bb = og.L.orc_grid_dot(og.ref(), O.ptr(b), O.ptr(b))
+    return np.sqrt(rr / bb)
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_mixed_precision_reaches_fp64_residual(ctx, method):
+    tree = trees()["two_level"]
+    params = ffvhc.case_params(bc=BC)
+    cid = ctx.add_case(tree, params)
+    try:
+        og = OracleGrid(tree)
+        rng = np.random.default_rng(9)
+        bct = [params.bct[ffvhc.BC_P][q] for q in range(6)]
+        bcv = [params.bcv[ffvhc.BC_P][q] for q in range(6)]
+        A, b = spd_system(og, rng, bct, bcv) if method == 1 else make_system(og, rng, bct, bcv)
+        x = og.zeros()
+        fa, fb, fx = upload_system(ctx, cid, A, b, x)
+        eps = 1e-10
+        cnt, res, outer, rc = ctx.solve_mixed(method, fx, fa, fb, ffvhc.BC_P, 1.0 if method else 1.2, 100000, eps,
+                                              1e-4, ncases=1)
+        assert rc == 0 and res[0] <= eps and outer[0] >= 2
+        got = ctx.download(fx, cid, out=x.copy())
+        # the FP64 residual the device reports is the true one (computed by the oracle on the result)
+        tr = true_residual(og, A, got, b)
+        assert tr <= eps * 1.0001 and abs(tr - res[0]) <= 1e-3 * res[0] + 1e-16
+        # and it is the FP64 solution: compare with the FP64 solver at the same tolerance
+        ctx.upload(fx, cid, og.zeros())
+        if method == 1:
+            c64, r64, _ = ctx.cg(fx, fa, fb, ffvhc.BC_P, 1.0, 1, 100000, eps, ncases=1)
+        else:
+            c64, r64, _ = ctx.rbgs(fx, fa, fb, ffvhc.BC_P, 1.2, 100000, eps, ncases=1)
+        ref = ctx.download(fx, cid, out=x.copy())
+        scale = np.max(np.abs(ref[:, LIVE]))
+        assert np.max(np.abs(got[:, LIVE] - ref[:, LIVE])) <= 1e-7 * scale
+        for f in fa + [fb, fx]:
+            ctx.free(f)
+    finally:
+        ctx.remove_case(cid)
