@@ -7,8 +7,9 @@ cells (bcut.f90:608-832), outer faces Neumann except X+ Dirichlet p = 0 (see DES
 all-Neumann 2-level operator is singular and not symmetric, so no solver reaches 1e-10 on it);
 seeded uniform[-1,1] RHS (PCG64 seed 0) with the mean removed.
 
-A step is one Poisson solve from x = 0 to relative residual ||b-Ax||/||b|| <= 1e-10 (FFVILS.h:24-47).
-value = cells * steps / seconds of the timed steps (Mcell-steps/s), inputs resident in HBM.
+A step is one Poisson solve from x = 0 to relative residual ||b-Ax||/||b|| <= 1e-10 (FFVILS.h:24-47)
+with Jacobi-preconditioned CG (FFVILS.h:661-791).  value = cells * steps / seconds for the mixed path
+(FP32 inner PCG, FP64 defect correction, FP64 residual); the plain FP64 PCG is timed beside it.
 
 Prints one JSON line.  --gpus N > 1 exits: every box here has one GPU.
 """
@@ -26,7 +27,6 @@ sys.path.insert(0, ROOT)
 
 N, G = 16, 2
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL = {"rbgs64": 92.0, "rbgs32": 46.0}  # algorithmic bytes / cell / iteration
 
 
 def build_c2(margin=1.0):
@@ -85,116 +85,156 @@ def setup(tree, A, b, bct):
     return ctx, cid, f64, f32
 
 
-def cpu_baseline(tree, A, b, bct, iters_per_solve, budget_s=15.0):
-    """Oracle (C restatement, OpenMP over blocks) RBGS sweeps on the same system, host cores."""
-    import ctypes
+def oracle_grid(tree):
     from oracle import oracle as O
     tb = tree.table()
     tab = dict(tb)
     tab.update(g=G, ftype=tb["ftype"].ravel(), nbr=tb["nbr"].ravel(), sub=tb["sub"].ravel(), org=tb["org"].ravel())
-    grid = O.make_grid(tab)
+    return O.make_grid(tab), tab
+
+
+def cpu_baseline(tree, A, b, bct, iters_per_solve, budget_s=15.0):
+    """Oracle (C restatement of FFVILS::CG, OpenMP over blocks) on the same system, host cores:
+    a bounded number of PCG iterations, scaled by the iterations the FP64 solve needs."""
+    import ctypes
+    from oracle import oracle as O
+    grid, _tab = oracle_grid(tree)
     L = O.lib()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     threads = min(threads, 16)
     L.orc_set_threads(threads)
-    x = np.zeros_like(b)
+    scratch = [np.zeros_like(b) for _ in range(6)]
     cnt, res = ctypes.c_int(), ctypes.c_double()
-    args = (ctypes.byref(grid), O.ptr(x), O.parr(A), O.ptr(b), O.cint6(bct), O.cdbl6([0.0] * 6),
-            ctypes.c_double(1.2))
-    L.orc_rbgs(*args, 2, ctypes.c_double(-1.0), ctypes.byref(cnt), ctypes.byref(res))
-    sweeps, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        L.orc_rbgs(*args, 5, ctypes.c_double(-1.0), ctypes.byref(cnt), ctypes.byref(res))
-        sweeps += 5
+
+    def run(k):
+        x = scratch[5]
+        x[:] = 0.0
+        L.orc_cg(ctypes.byref(grid), O.ptr(x), O.parr(A), O.ptr(b), *[O.ptr(q) for q in scratch[:5]],
+                 O.cint6(bct), O.cdbl6([0.0] * 6), ctypes.c_double(1.0), 1, k, ctypes.c_double(-1.0),
+                 ctypes.byref(cnt), ctypes.byref(res))
+
+    run(2)
+    k, t0 = 4, time.perf_counter()
+    run(k)
     dt = time.perf_counter() - t0
+    while dt < budget_s / 4 and k < iters_per_solve:
+        k *= 2
+        t0 = time.perf_counter()
+        run(k)
+        dt = time.perf_counter() - t0
     cells = tree.nblk * N ** 3
-    per_sweep = dt / sweeps
-    return {"value": cells / (per_sweep * iters_per_solve) / 1e6, "unit": "Mcell-steps/s", "cores": threads,
+    per_it = dt / k
+    return {"value": cells / (per_it * iters_per_solve) / 1e6, "unit": "Mcell-steps/s", "cores": threads,
             "kind": "port",
-            "sample": "%d RBGS sweeps (+residual) of the same %d-block system in %.1f s on %d threads "
-                      "(%.2f Mcell-sweeps/s), scaled by the %d sweeps the FP64 solve needs"
-                      % (sweeps, tree.nblk, dt, threads, cells * sweeps / dt / 1e6, iters_per_solve)}
+            "sample": "%d Jacobi-PCG iterations of the same %d-block system in %.2f s on %d threads, scaled by "
+                      "the %d iterations the FP64 solve to 1e-10 needs" % (k, tree.nblk, dt, threads,
+                                                                           iters_per_solve)}
+
+
+# algorithmic bytes per cell per launch of the fused kernels (interior arrays; halo: 6x256 ghosts
+# per 4096 cells read or written once = 1.5 words)
+def kernel_bytes(w):
+    return {"cg_p": (4 + 1.5) * w, "cg_q": (9 + 1.5) * w, "cg_xr": 7 * w, "rbgs": (10 + 3.0) * w}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--eps", type=float, default=1e-10)
-    ap.add_argument("--count-max", type=int, default=400000)
-    ap.add_argument("--omega", type=float, default=1.2)
+    ap.add_argument("--inner-eps", type=float, default=1e-5)
+    ap.add_argument("--count-max", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--profile-sweeps", type=int, default=0,
-                    help="only run this many fused FP64 and FP32 RBGS iterations (for rocprofv3) and exit")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: timing loops and one solve per precision")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: every box in this run has one GPU; multi-GPU runs are not part of this build "
               "(the sharding design is in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    import ffvhc
     tree = build_c2()
     cells = tree.nblk * N ** 3
     A, b, bct = poisson_system(tree)
     ctx, cid, f64, f32 = setup(tree, A, b, bct)
-    import ffvhc
-    if args.profile_sweeps:
-        for f in (f64, f32):
-            ctx.fill(f["x"], 0.0)
-            ms = ctx.rbgs_fixed(f["x"], f["A"], f["b"], ffvhc.BC_P, args.omega, args.profile_sweeps)
-            print("profile: %d sweeps, %.2f us/sweep" % (args.profile_sweeps, ms * 1e3 / args.profile_sweeps))
+    P = ffvhc.BC_P
+
+    # per-kernel device times (HIP events on the library's stream)
+    nit = 20 if args.profile else 100
+    kt = {}
+    for tag, f, w in (("64", f64, 8), ("32", f32, 4)):
+        ctx.fill(f["x"], 0.0)
+        ctx.cg_timing(f["x"], f["A"], f["b"], 1.0, 5)
+        ms = ctx.cg_timing(f["x"], f["A"], f["b"], 1.0, nit) / nit
+        ctx.fill(f["x"], 0.0)
+        ctx.rbgs_fixed(f["x"], f["A"], f["b"], P, 1.2, 5)
+        mr = ctx.rbgs_fixed(f["x"], f["A"], f["b"], P, 1.2, nit) / nit
+        by = kernel_bytes(w)
+        kt[tag] = {name: {"us": round(t * 1e3, 2), "GBs": round(by[name] * cells / (t * 1e-3) / 1e9, 1)}
+                   for name, t in (("cg_p", ms[0]), ("cg_q", ms[1]), ("cg_xr", ms[2]), ("rbgs", mr))}
+
+    def solve64():
+        ctx.fill(f64["x"], 0.0)
+        cnt, res, rc = ctx.cg(f64["x"], f64["A"], f64["b"], P, 1.0, 1, args.count_max, args.eps, ncases=1)
+        return int(cnt[0]), float(res[0]), 1
+
+    def solve_mixed():
+        ctx.fill(f64["x"], 0.0)
+        cnt, res, outer, rc = ctx.solve_mixed(1, f64["x"], f64["A"], f64["b"], P, 1.0, args.count_max, args.eps,
+                                              args.inner_eps, ncases=1)
+        return int(cnt[0]), float(res[0]), int(outer[0])
+
+    timings = {}
+    steps = 1 if args.profile else args.steps
+    warm = 0 if args.profile else args.warmup
+    for tag, fn in (("fp64", solve64), ("mixed", solve_mixed)):
+        for _ in range(warm):
+            fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        st = [fn() for _ in range(steps)]
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        timings[tag] = {"seconds": dt, "iters": st[-1][0], "residual": st[-1][1], "outer": st[-1][2],
+                        "mcell_steps_s": cells * steps / dt / 1e6, "ms_per_step": dt / steps * 1e3}
+    if args.profile:
+        print(json.dumps({"profile": True, "kernels": kt, "timings": timings}))
         ctx.close()
         return
-
-    # kernel time of the fused RBGS iteration (event-timed back-to-back launches)
-    ctx.fill(f64["x"], 0.0)
-    ctx.rbgs_fixed(f64["x"], f64["A"], f64["b"], ffvhc.BC_P, args.omega, 20)
-    ms64 = ctx.rbgs_fixed(f64["x"], f64["A"], f64["b"], ffvhc.BC_P, args.omega, 200) / 200
-    ctx.fill(f32["x"], 0.0)
-    ctx.rbgs_fixed(f32["x"], f32["A"], f32["b"], ffvhc.BC_P, args.omega, 20)
-    ms32 = ctx.rbgs_fixed(f32["x"], f32["A"], f32["b"], ffvhc.BC_P, args.omega, 200) / 200
-
-    def solve():
-        ctx.fill(f64["x"], 0.0)
-        cnt, res, rc = ctx.rbgs(f64["x"], f64["A"], f64["b"], ffvhc.BC_P, args.omega, args.count_max, args.eps,
-                                ncases=1)
-        return int(cnt[0]), float(res[0])
-
-    for _ in range(args.warmup):
-        solve()
-    ctx.sync()
-    t0 = time.perf_counter()
-    stats = [solve() for _ in range(args.steps)]
-    ctx.sync()
-    dt = time.perf_counter() - t0
-    iters = stats[-1][0]
-    value = cells * args.steps / dt / 1e6
-    achieved = BYTES_PER_CELL["rbgs64"] * cells / (ms64 * 1e-3) / 1e9
+    mix, d64 = timings["mixed"], timings["fp64"]
+    # dominant kernel of the headline (mixed) path: the FP32 matvec of the inner PCG
+    dom = kt["32"]["cg_q"]
     out = {
         "metric": "Mcell-steps/s mixed vs FP64 at equal residual; p50/p99 served-case latency",
-        "value": round(value, 4),
+        "value": round(mix["mcell_steps_s"], 3),
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "ms_per_step": round(mix["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 inner / f64 defect correction",
         "data": "synthetic (seeded uniform[-1,1] RHS, mean removed)",
-        "config": {"workload": "standalone Poisson, 2-level octree, roots 8^3 x 16^3 cells, ball r=L/4 level 1",
-                   "blocks": tree.nblk, "cells": cells, "solver": "RBGS omega=%.2f" % args.omega,
-                   "eps": args.eps, "iterations_per_step": iters, "final_residual": stats[-1][1]},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "rbgs_iter_kernel<double>", "kernel_us": round(ms64 * 1e3, 2),
-                     "bytes_per_cell": BYTES_PER_CELL["rbgs64"]},
-        "fp32_sweep": {"kernel_us": round(ms32 * 1e3, 2),
-                       "achieved_GBs": round(BYTES_PER_CELL["rbgs32"] * cells / (ms32 * 1e-3) / 1e9, 1)},
+        "config": {"workload": "standalone Poisson, 2-level octree: roots 8^3 of 16^3-cell blocks, level 1 in a "
+                               "ball of radius L/4 (1-cell margin)",
+                   "blocks": tree.nblk, "cells": cells, "solver": "Jacobi-PCG (PreCount 1, omega 1)",
+                   "eps": args.eps, "inner_eps": args.inner_eps},
+        "fp64": {"value": round(d64["mcell_steps_s"], 3), "ms_per_step": round(d64["ms_per_step"], 3),
+                 "iterations": d64["iters"], "residual": d64["residual"]},
+        "mixed": {"iterations_inner": mix["iters"], "corrections": mix["outer"], "residual": mix["residual"]},
+        "speedup_mixed_vs_fp64": round(mix["mcell_steps_s"] / d64["mcell_steps_s"], 3),
+        "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "cg_q_kernel<float>", "kernel_us": dom["us"],
+                     "bytes_per_cell": kernel_bytes(4)["cg_q"]},
+        "kernels": kt,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(tree, A, b, bct, iters, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(tree, A, b, bct, d64["iters"], args.cpu_seconds)
     ctx.close()
     print(json.dumps(out))
 

@@ -1,8 +1,9 @@
 // Jacobi-preconditioned CG (FFVILS::CG, FFVILS.h:661-791) as three fused kernels per iteration,
 // with every scalar (alpha, beta, the convergence test) computed on the device by the last
 // workgroup of each case.  The preconditioner is the reference's Jacobi_PreConditioner with
-// countPreConditioner = 1 from z = 0 (FFVILS.h:468-490, bils.f90:5-55), i.e.
-// z = 0 - omega*(0 - (r - A_nb*0 ...)/Ap), evaluated literally so the rounding matches.
+// countPreConditioner = 1 from z = 0 (FFVILS.h:468-490, bils.f90:5-55):
+// z = 0 - omega*(0 - (r - A_nb*0 - ...)/Ap), which equals omega*(r/Ap) exactly in IEEE arithmetic
+// (every A_nb*0 is a signed zero, x - (+-0) = x, 0 - (0 - y) = y), so only Ap is read.
 //
 //   init : r = b - A x (calc_r), b.b, z, r.z                                      [FFVILS.h:679-712]
 //   P    : p = z (first iteration, Copy) or z + beta p (XPAY); push p's ghost layer [716-731]
@@ -16,12 +17,8 @@ namespace ffvhc {
 constexpr int CG_NT = 256;
 
 template <class T>
-__device__ inline T jacobi_z(T r, T ap, const T* an, T omega) {
-  // jacobi_smoother with x0 = 0 everywhere, one sweep
-  const T x0 = (T)0;
-  T rp = r - an[0] * x0 - an[1] * x0 - an[2] * x0 - an[3] * x0 - an[4] * x0 - an[5] * x0;
-  T xt = rp / ap;
-  return x0 - omega * (x0 - xt);
+__device__ inline T jacobi_z(T r, T ap, T omega) {
+  return omega * (r / ap);
 }
 
 template <class T>
@@ -58,7 +55,7 @@ __global__ __launch_bounds__(CG_NT) void cg_init_kernel(const int* __restrict__ 
     const T rr = b0 - a[1] * s[li - 1] - a[2] * s[li + 1] - a[3] * s[li - LP] - a[4] * s[li + LP] -
                  a[5] * s[li - LP * LP] - a[6] * s[li + LP * LP] - a[0] * s[li];
     r[g] = rr;
-    const T z = jacobi_z<T>(rr, a[0], a + 1, omega);
+    const T z = jacobi_z<T>(rr, a[0], omega);
     bb += (double)b0 * (double)b0;
     rz += (double)rr * (double)z;
   }
@@ -108,10 +105,7 @@ __global__ __launch_bounds__(CG_NT) void cg_p_kernel(const int* __restrict__ act
   const T beta = (T)st[c].beta;
   for (int k = 0; k < NB; k++) {
     const size_t g = base + tx + NB * (ty + NB * k);
-    T an[6];
-#pragma unroll
-    for (int q = 0; q < 6; q++) an[q] = A.p[q + 1][g];
-    const T z = jacobi_z<T>(r[g], A.p[0][g], an, omega);
+    const T z = jacobi_z<T>(r[g], A.p[0][g], omega);
     const T pn = (count == 1) ? z : z + beta * p[g];
     p[g] = pn;
     s[lds(tx, ty, k)] = pn;
@@ -189,10 +183,7 @@ __global__ __launch_bounds__(CG_NT) void cg_xr_kernel(const int* __restrict__ ac
     x[g] = x[g] + alpha * p[g];
     const T rn = r[g] + nalpha * q[g];
     r[g] = rn;
-    T an[6];
-#pragma unroll
-    for (int w = 0; w < 6; w++) an[w] = A.p[w + 1][g];
-    const T z = jacobi_z<T>(rn, A.p[0][g], an, omega);
+    const T z = jacobi_z<T>(rn, A.p[0][g], omega);
     rr += (double)rn * (double)rn;
     rz += (double)rn * (double)z;
   }

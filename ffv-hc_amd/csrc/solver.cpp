@@ -163,6 +163,49 @@ int rbgs_fixed_run(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double ome
   return 0;
 }
 
+// niter PCG iterations with events between the three kernels: ms[0..2] = summed device time
+// of the P, Q and XR kernels.
+template <class T>
+int cg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int niter, double* ms) {
+  const int prec = (int)sizeof(T);
+  int fr, fp, fq;
+  int rc = scratch_field(c, prec, R_R, &fr);
+  if (!rc) rc = scratch_field(c, prec, R_P, &fp);
+  if (!rc) rc = scratch_field(c, prec, R_Q, &fq);
+  if (rc) return rc;
+  FieldRec &X = c->fields[x], &Rf = c->fields[fr], &Pf = c->fields[fp], &Qf = c->fields[fq];
+  Coef7<T> K = coef<T>(c, A);
+  launch_solve_init(c->sm, c->dst, (int)c->cases.size(), omega, -1.0, niter);
+  launch_cg_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, (const T*)c->fields[b].d[0],
+                    (const T*)X.d[0], (const T*)X.h[0], (T*)Rf.d[0], c->dpart, (size_t)c->cap);
+  std::vector<hipEvent_t> ev(4 * niter);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  for (int n = 1; n <= niter; n++) {
+    hipEvent_t* e = &ev[4 * (n - 1)];
+    HIPCHK(hipEventRecord(e[0], c->sm));
+    launch_cg_p<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, (const T*)Rf.d[0], (T*)Pf.d[0],
+                   (T*)Pf.h[0], n, BC_NULL);
+    HIPCHK(hipEventRecord(e[1], c->sm));
+    launch_cg_q<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, (const T*)Pf.d[0], (const T*)Pf.h[0],
+                   (T*)Qf.d[0], c->dpart);
+    HIPCHK(hipEventRecord(e[2], c->sm));
+    launch_cg_xr<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, (T*)X.d[0], (T*)Rf.d[0],
+                    (const T*)Pf.d[0], (const T*)Qf.d[0], c->dpart, (size_t)c->cap, n);
+    HIPCHK(hipEventRecord(e[3], c->sm));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->sm));
+  ms[0] = ms[1] = ms[2] = 0.0;
+  for (int n = 0; n < niter; n++)
+    for (int k = 0; k < 3; k++) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, ev[4 * n + k], ev[4 * n + k + 1]));
+      ms[k] += t;
+    }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return 0;
+}
+
 int prologue(ffvhc_ctx* c, int x, const int* A, int b, int bcg, int* prec) {
   if (!c) return fail(-1, "null context");
   *prec = field_ok(c, x, 0) ? c->fields[x].prec : 8;
@@ -259,6 +302,15 @@ int ffvhc_cg(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, in
   rc = sync_state(c, c->dst, c->hst);
   if (rc) return rc;
   return report(c, c->hst, count, residual);
+}
+
+int ffvhc_cg_timing(ffvhc_ctx* c, int x, const int* A, int b, double omega, int niter, double* ms3) {
+  int prec = 8;
+  int rc = prologue(c, x, A, b, BC_NULL, &prec);
+  if (rc) return rc;
+  if (niter < 1 || !ms3) return fail(-1, "niter >= 1 and ms[3] required");
+  return prec == 8 ? cg_timing_run<double>(c, x, A, b, omega, niter, ms3)
+                   : cg_timing_run<float>(c, x, A, b, omega, niter, ms3);
 }
 
 // FP32 inner solve in FP64 defect correction.  count = inner iterations summed over the

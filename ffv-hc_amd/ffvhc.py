@@ -75,6 +75,7 @@ _sig = {
     "ffvhc_rbgs_fixed": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, _P(c_double)]),
     "ffvhc_rbgs_fixed_average": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
     "ffvhc_cg": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_int, c_double, _vp, _vp]),
+    "ffvhc_cg_timing": (c_int, [_vp, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_jacobi": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
     "ffvhc_solve_mixed": (c_int, [_vp, c_int, c_int, _vp, c_int, c_int, c_double, c_int, c_double, c_double, _vp,
                                   _vp, _vp]),
@@ -314,6 +315,13 @@ class Context:
         if rc < 0:
             check(rc, "ffvhc_cg")
         return cnt, res, rc
+
+    def cg_timing(self, x, A, b, omega, niter):
+        """Device ms of the three PCG kernels summed over niter iterations."""
+        ms = np.zeros(3)
+        a = np.asarray(A, np.int32)
+        check(_L.ffvhc_cg_timing(self.h, x, _ptr(a), b, omega, niter, _ptr(ms)), "ffvhc_cg_timing")
+        return ms
 
     def solve_mixed(self, method, x, A, b, group, omega, count_max, eps, inner_eps, ncases=None):
         nc = ncases or self._nsel()

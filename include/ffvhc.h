@@ -150,6 +150,9 @@ int ffvhc_rbgs_fixed_average(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_
 /* FFVILS::CG (FFVILS.h:661-791): Jacobi-preconditioned CG, countPre damped-Jacobi sweeps */
 int ffvhc_cg(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countPre, int countMax,
              double eps, int* count, double* residual);
+/* niter PCG iterations (no convergence test) with HIP events between the three fused kernels;
+ * ms[0..2] = total device time of the p-update, matvec and x/r-update kernels (timing entry). */
+int ffvhc_cg_timing(ffvhc_ctx* ctx, int x, const int* A, int b, double omega, int niter, double* ms);
 /* FFVILS::Jacobi (FFVILS.h:492-545) */
 int ffvhc_jacobi(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countMax, double eps,
                  int* count, double* residual);
