@@ -8,8 +8,9 @@ all-Neumann 2-level operator is singular and not symmetric, so no solver reaches
 seeded uniform[-1,1] RHS (PCG64 seed 0) with the mean removed.
 
 A step is one Poisson solve from x = 0 to relative residual ||b-Ax||/||b|| <= 1e-10 (FFVILS.h:24-47)
-with Jacobi-preconditioned CG (FFVILS.h:661-791).  value = cells * steps / seconds for the mixed path
-(FP32 inner PCG, FP64 defect correction, FP64 residual); the plain FP64 PCG is timed beside it.
+with Jacobi right-preconditioned BiCGSTAB (FFVILS.h:793-974) -- the coarse/fine interpolation makes the
+octree operator non-symmetric, where Jacobi-PCG stagnates.  value = cells * steps / seconds for the mixed
+path (FP32 inner BiCGSTAB, FP64 defect correction, FP64 residual); plain FP64 BiCGSTAB is timed beside it.
 
 Prints one JSON line.  --gpus N > 1 exits: every box here has one GPU.
 """
@@ -94,8 +95,8 @@ def oracle_grid(tree):
 
 
 def cpu_baseline(tree, A, b, bct, iters_per_solve, budget_s=15.0):
-    """Oracle (C restatement of FFVILS::CG, OpenMP over blocks) on the same system, host cores:
-    a bounded number of PCG iterations, scaled by the iterations the FP64 solve needs."""
+    """Oracle (C restatement of FFVILS::BiCGSTAB, OpenMP over blocks) on the same system, host cores:
+    a bounded number of iterations, scaled by the iterations the FP64 solve needs."""
     import ctypes
     from oracle import oracle as O
     grid, _tab = oracle_grid(tree)
@@ -103,15 +104,15 @@ def cpu_baseline(tree, A, b, bct, iters_per_solve, budget_s=15.0):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     threads = min(threads, 16)
     L.orc_set_threads(threads)
-    scratch = [np.zeros_like(b) for _ in range(6)]
+    scratch = [np.zeros_like(b) for _ in range(10)]
     cnt, res = ctypes.c_int(), ctypes.c_double()
 
     def run(k):
-        x = scratch[5]
+        x = scratch[9]
         x[:] = 0.0
-        L.orc_cg(ctypes.byref(grid), O.ptr(x), O.parr(A), O.ptr(b), *[O.ptr(q) for q in scratch[:5]],
-                 O.cint6(bct), O.cdbl6([0.0] * 6), ctypes.c_double(1.0), 1, k, ctypes.c_double(-1.0),
-                 ctypes.byref(cnt), ctypes.byref(res))
+        L.orc_bicgstab(ctypes.byref(grid), O.ptr(x), O.parr(A), O.ptr(b), O.parr(scratch[:9]),
+                       O.cint6(bct), O.cdbl6([0.0] * 6), ctypes.c_double(1.0), 1, k, ctypes.c_double(-1.0),
+                       ctypes.byref(cnt), ctypes.byref(res))
 
     run(2)
     k, t0 = 4, time.perf_counter()
@@ -126,15 +127,21 @@ def cpu_baseline(tree, A, b, bct, iters_per_solve, budget_s=15.0):
     per_it = dt / k
     return {"value": cells / (per_it * iters_per_solve) / 1e6, "unit": "Mcell-steps/s", "cores": threads,
             "kind": "port",
-            "sample": "%d Jacobi-PCG iterations of the same %d-block system in %.2f s on %d threads, scaled by "
+            "sample": "%d Jacobi-BiCGSTAB iterations of the same %d-block system in %.2f s on %d threads, scaled by "
                       "the %d iterations the FP64 solve to 1e-10 needs" % (k, tree.nblk, dt, threads,
                                                                            iters_per_solve)}
 
 
-# algorithmic bytes per cell per launch of the fused kernels (interior arrays; halo: 6x256 ghosts
-# per 4096 cells read or written once = 1.5 words)
+# algorithmic bytes per cell per launch of the fused kernels: interior arrays read/written once,
+# plus the ghost layer (6 x 256 per 4096 cells = 0.375 words) read or pushed once
 def kernel_bytes(w):
-    return {"cg_p": (4 + 1.5) * w, "cg_q": (9 + 1.5) * w, "cg_xr": 7 * w, "rbgs": (10 + 3.0) * w}
+    h = 0.375
+    return {"bi_p": (6 + h) * w, "bi_q": (10 + h) * w, "bi_s": (5 + h) * w, "bi_t": (10 + h) * w, "bi_xr": 8 * w,
+            "rbgs": (10 + 2 * h) * w}
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def main():
@@ -161,28 +168,31 @@ def main():
     ctx, cid, f64, f32 = setup(tree, A, b, bct)
     P = ffvhc.BC_P
 
+    log("%d blocks, %d cells resident; timing kernels" % (tree.nblk, cells))
     # per-kernel device times (HIP events on the library's stream)
     nit = 20 if args.profile else 100
     kt = {}
     for tag, f, w in (("64", f64, 8), ("32", f32, 4)):
         ctx.fill(f["x"], 0.0)
-        ctx.cg_timing(f["x"], f["A"], f["b"], 1.0, 5)
-        ms = ctx.cg_timing(f["x"], f["A"], f["b"], 1.0, nit) / nit
+        ctx.bicgstab_timing(f["x"], f["A"], f["b"], 1.0, 5)
+        ms = ctx.bicgstab_timing(f["x"], f["A"], f["b"], 1.0, nit) / nit
         ctx.fill(f["x"], 0.0)
         ctx.rbgs_fixed(f["x"], f["A"], f["b"], P, 1.2, 5)
         mr = ctx.rbgs_fixed(f["x"], f["A"], f["b"], P, 1.2, nit) / nit
         by = kernel_bytes(w)
+        names = ("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr")
         kt[tag] = {name: {"us": round(t * 1e3, 2), "GBs": round(by[name] * cells / (t * 1e-3) / 1e9, 1)}
-                   for name, t in (("cg_p", ms[0]), ("cg_q", ms[1]), ("cg_xr", ms[2]), ("rbgs", mr))}
+                   for name, t in list(zip(names, ms)) + [("rbgs", mr)]}
+        log("fp%s kernels: %s" % (tag, json.dumps(kt[tag])))
 
     def solve64():
         ctx.fill(f64["x"], 0.0)
-        cnt, res, rc = ctx.cg(f64["x"], f64["A"], f64["b"], P, 1.0, 1, args.count_max, args.eps, ncases=1)
+        cnt, res, rc = ctx.bicgstab(f64["x"], f64["A"], f64["b"], P, 1.0, 1, args.count_max, args.eps, ncases=1)
         return int(cnt[0]), float(res[0]), 1
 
     def solve_mixed():
         ctx.fill(f64["x"], 0.0)
-        cnt, res, outer, rc = ctx.solve_mixed(1, f64["x"], f64["A"], f64["b"], P, 1.0, args.count_max, args.eps,
+        cnt, res, outer, rc = ctx.solve_mixed(2, f64["x"], f64["A"], f64["b"], P, 1.0, args.count_max, args.eps,
                                               args.inner_eps, ncases=1)
         return int(cnt[0]), float(res[0]), int(outer[0])
 
@@ -194,7 +204,10 @@ def main():
             fn()
         ctx.sync()
         t0 = time.perf_counter()
-        st = [fn() for _ in range(steps)]
+        st = []
+        for _ in range(steps):
+            st.append(fn())
+            log("%s solve: %d iterations, residual %.3e, %d corrections" % (tag, st[-1][0], st[-1][1], st[-1][2]))
         ctx.sync()
         dt = time.perf_counter() - t0
         timings[tag] = {"seconds": dt, "iters": st[-1][0], "residual": st[-1][1], "outer": st[-1][2],
@@ -204,8 +217,9 @@ def main():
         ctx.close()
         return
     mix, d64 = timings["mixed"], timings["fp64"]
-    # dominant kernel of the headline (mixed) path: the FP32 matvec of the inner PCG
-    dom = kt["32"]["cg_q"]
+    # dominant kernel of the headline (mixed) path: the slowest FP32 kernel of the inner BiCGSTAB
+    dname = max(("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"), key=lambda k: kt["32"][k]["us"])
+    dom = kt["32"][dname]
     out = {
         "metric": "Mcell-steps/s mixed vs FP64 at equal residual; p50/p99 served-case latency",
         "value": round(mix["mcell_steps_s"], 3),
@@ -221,7 +235,7 @@ def main():
         "data": "synthetic (seeded uniform[-1,1] RHS, mean removed)",
         "config": {"workload": "standalone Poisson, 2-level octree: roots 8^3 of 16^3-cell blocks, level 1 in a "
                                "ball of radius L/4 (1-cell margin)",
-                   "blocks": tree.nblk, "cells": cells, "solver": "Jacobi-PCG (PreCount 1, omega 1)",
+                   "blocks": tree.nblk, "cells": cells, "solver": "Jacobi-BiCGSTAB (PreCount 1, omega 1)",
                    "eps": args.eps, "inner_eps": args.inner_eps},
         "fp64": {"value": round(d64["mcell_steps_s"], 3), "ms_per_step": round(d64["ms_per_step"], 3),
                  "iterations": d64["iters"], "residual": d64["residual"]},
@@ -229,11 +243,12 @@ def main():
         "speedup_mixed_vs_fp64": round(mix["mcell_steps_s"] / d64["mcell_steps_s"], 3),
         "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "cg_q_kernel<float>", "kernel_us": dom["us"],
-                     "bytes_per_cell": kernel_bytes(4)["cg_q"]},
+                     "kernel": dname + " (FP32)", "kernel_us": dom["us"],
+                     "bytes_per_cell": kernel_bytes(4)[dname]},
         "kernels": kt,
     }
     if not args.no_cpu:
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(tree, A, b, bct, d64["iters"], args.cpu_seconds)
     ctx.close()
     print(json.dumps(out))

@@ -55,7 +55,7 @@ struct SolveState {
   double bb;        // b.b
   double rr;        // last residual norm squared
   double rr0, rr1;  // CG
-  double alpha, beta;
+  double alpha, beta, gamma;
   double residual;  // FFVILS IsConverged residual
   double omega, eps;
   int countMax;
@@ -72,6 +72,13 @@ struct SolveState {
 template <class T>
 struct Coef7 {
   const T* p[7];
+};
+
+// BiCGSTAB work vectors r r0 p p_ q_ s s_ t_ and the ghost layers of p_ and s_
+template <class T>
+struct BiVecs {
+  T *r, *r0, *p, *pp, *q, *s, *ss, *t;
+  T *hpp, *hss;
 };
 
 __host__ __device__ inline int halo_index(int face, int a, int b) { return face * NF + a + NB * b; }

@@ -37,6 +37,14 @@ void launch_cg_q(hipStream_t sm, const int* active, int nact, const DevBlock* bl
 template <class T>
 void launch_cg_xr(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs, SolveState* st,
                   Coef7<T> A, T* x, T* r, const T* p, const T* q, double* part, size_t nslots, int count);
+template <class T>
+void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                          SolveState* st, Coef7<T> A, const T* b, const T* x, const T* hx, BiVecs<T> V, double* part,
+                          size_t nslots);
+template <class T>
+void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                          SolveState* st, Coef7<T> A, T* x, BiVecs<T> V, double* part, size_t nslots, int count,
+                          int mask = 31);
 void launch_defect(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                    SolveState* st, Coef7<double> A, const double* b, const double* x, const double* hx, float* r32,
                    double* part, size_t nslots, int outer);

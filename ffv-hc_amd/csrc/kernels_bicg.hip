@@ -1,0 +1,277 @@
+// Jacobi right-preconditioned BiCGSTAB (FFVILS::BiCGSTAB, FFVILS.h:793-974) as five fused kernels
+// per iteration; every scalar (alpha, gamma, beta, the convergence test) is computed on the device
+// by the last workgroup of each case.  The preconditioner is one damped-Jacobi sweep from zero
+// (countPreConditioner = 1), value-identical to omega * (v / Ap) (see kernels_cg.hip).
+//
+//   init : r = b - A x, r0 = r; b.b, r.r0                               [FFVILS.h:810-826]
+//   P    : p = r | r + beta (p - gamma q_); p_ = M p, push p_ ghosts   [828-858]
+//   Q    : q_ = A p_; q_.r0 -> alpha                                   [859-872]
+//   S    : s = r - alpha q_; s_ = M s, push s_ ghosts                  [873-893]
+//   T    : t_ = A s_; t_.s, t_.t_ -> gamma                             [894-915]
+//   XR   : x += alpha p_ + gamma s_; r = s - gamma t_; r.r -> IsConverged; r.r0 -> beta [916-965]
+#include "kcommon.h"
+#include "kernels.h"
+
+namespace ffvhc {
+
+constexpr int BI_NT = 256;
+
+template <class T>
+__device__ inline void stage(T* s, const T* __restrict__ x, const T* __restrict__ h, int slot) {
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  for (int k = 0; k < NB; k++) s[lds(tx, ty, k)] = x[base + tx + NB * (ty + NB * k)];
+  for (int q = t; q < NH; q += BI_NT) {
+    int f = q >> 8, p = q & 255, i, j, kk;
+    face_cell(f, p & 15, p >> 4, -1, i, j, kk);
+    s[lds(i, j, kk)] = h[(size_t)slot * NH + q];
+  }
+}
+
+template <class T>
+__device__ inline T matvec(const Coef7<T>& A, size_t g, const T* s, int li) {
+  // calc_ax order (bils.f90:195-202)
+  return A.p[1][g] * s[li - 1] + A.p[2][g] * s[li + 1] + A.p[3][g] * s[li - LP] + A.p[4][g] * s[li + LP] +
+         A.p[5][g] * s[li - LP * LP] + A.p[6][g] * s[li + LP * LP] + A.p[0][g] * s[li];
+}
+
+template <class T>
+__global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                        const DevCase* __restrict__ cs, SolveState* __restrict__ st,
+                                                        Coef7<T> A, const T* __restrict__ bv, const T* __restrict__ x,
+                                                        const T* __restrict__ hx, BiVecs<T> V,
+                                                        double* __restrict__ part, size_t nslots) {
+  __shared__ T s[LP * LP * LP];
+  __shared__ double red[BI_NT / 64];
+  __shared__ int flag;
+  const int slot = active[blockIdx.x];
+  const int c = blk[slot].case_id;
+  if (st[c].done) return;
+  stage<T>(s, x, hx, slot);
+  __syncthreads();
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  double bb = 0.0, rr1 = 0.0;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = base + tx + NB * (ty + NB * k);
+    const int li = lds(tx, ty, k);
+    const T b0 = bv[g];
+    const T r = b0 - A.p[1][g] * s[li - 1] - A.p[2][g] * s[li + 1] - A.p[3][g] * s[li - LP] -
+                A.p[4][g] * s[li + LP] - A.p[5][g] * s[li - LP * LP] - A.p[6][g] * s[li + LP * LP] -
+                A.p[0][g] * s[li];
+    V.r[g] = r;
+    V.r0[g] = r;
+    bb += (double)b0 * (double)b0;
+    rr1 += (double)r * (double)r;
+  }
+  double sb = block_sum<BI_NT>(bb, red);
+  double sr = block_sum<BI_NT>(rr1, red);
+  if (t == 0) {
+    part[slot] = sb;
+    part[nslots + slot] = sr;
+  }
+  const DevCase& C = cs[c];
+  if (!last_of_case(st, c, C.nblk, &flag)) return;
+  double tb = case_sum<BI_NT>(part, C.slot_begin, C.nblk, red);
+  double tr = case_sum<BI_NT>(part + nslots, C.slot_begin, C.nblk, red);
+  if (t == 0) {
+    SolveState& S = st[c];
+    S.ticket = 0;
+    S.bb = tb;
+    S.rr0 = 1.0;
+    S.alpha = 0.0;
+    S.gamma = 1.0;
+    if (fabs(tb) < FLT_MIN) {
+      S.residual = 0.0;
+      S.count = 0;
+      S.done = 1;
+    } else if (fabs(tr) < FLT_MIN) {
+      S.residual = tr;
+      S.count = 0;
+      S.done = 1;
+    } else {
+      S.rr1 = tr;
+    }
+  }
+}
+
+// P: p update and p_ = M p with its ghost layer (mode 0); S: s = r - alpha q_, s_ = M s (mode 1)
+template <class T, int MODE>
+__global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict__ active,
+                                                           const DevBlock* __restrict__ blk,
+                                                           const DevCase* __restrict__ cs,
+                                                           const SolveState* __restrict__ st, Coef7<T> A, BiVecs<T> V,
+                                                           int count) {
+  __shared__ T s[LP * LP * LP];
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const int c = B.case_id;
+  if (st[c].done) return;
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  const T omega = (T)st[c].omega;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = base + tx + NB * (ty + NB * k);
+    T v;
+    if (MODE == 0) {
+      if (count == 1) {
+        v = V.r[g];
+      } else {
+        const T ng = (T)(-st[c].gamma);
+        const T beta = (T)st[c].beta;
+        const T tmp = V.p[g] + ng * V.q[g];  // AXPY(p, q_, -gamma)
+        v = V.r[g] + beta * tmp;              // XPAY(p, r, beta)
+      }
+      V.p[g] = v;
+    } else {
+      const T na = (T)(-st[c].alpha);
+      v = na * V.q[g] + V.r[g];  // AXPYZ(s, q_, r, -alpha)
+      V.s[g] = v;
+    }
+    const T m = omega * (v / A.p[0][g]);
+    if (MODE == 0) V.pp[g] = m;
+    else V.ss[g] = m;
+    s[lds(tx, ty, k)] = m;
+  }
+  __syncthreads();
+  T* h = MODE == 0 ? V.hpp : V.hss;
+  push_halo<T, BI_NT>(B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return s[lds(i, j, k)]; }, h, h);
+}
+
+// Q: q_ = A p_, q_.r0 -> alpha (mode 0);  T: t_ = A s_, t_.s and t_.t_ -> gamma (mode 1)
+template <class T, int MODE>
+__global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict__ active,
+                                                          const DevBlock* __restrict__ blk,
+                                                          const DevCase* __restrict__ cs, SolveState* __restrict__ st,
+                                                          Coef7<T> A, BiVecs<T> V, double* __restrict__ part,
+                                                          size_t nslots) {
+  __shared__ T s[LP * LP * LP];
+  __shared__ double red[BI_NT / 64];
+  __shared__ int flag;
+  const int slot = active[blockIdx.x];
+  const int c = blk[slot].case_id;
+  if (st[c].done) return;
+  stage<T>(s, MODE == 0 ? V.pp : V.ss, MODE == 0 ? V.hpp : V.hss, slot);
+  __syncthreads();
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  double d0 = 0.0, d1 = 0.0;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = base + tx + NB * (ty + NB * k);
+    const T y = matvec<T>(A, g, s, lds(tx, ty, k));
+    if (MODE == 0) {
+      V.q[g] = y;
+      d0 += (double)y * (double)V.r0[g];
+    } else {
+      V.t[g] = y;
+      d0 += (double)y * (double)V.s[g];
+      d1 += (double)y * (double)y;
+    }
+  }
+  double s0 = block_sum<BI_NT>(d0, red);
+  double s1 = MODE == 1 ? block_sum<BI_NT>(d1, red) : 0.0;
+  if (t == 0) {
+    part[slot] = s0;
+    part[nslots + slot] = s1;
+  }
+  const DevCase& C = cs[c];
+  if (!last_of_case(st, c, C.nblk, &flag)) return;
+  double t0 = case_sum<BI_NT>(part, C.slot_begin, C.nblk, red);
+  double t1 = MODE == 1 ? case_sum<BI_NT>(part + nslots, C.slot_begin, C.nblk, red) : 0.0;
+  if (t == 0) {
+    SolveState& S = st[c];
+    S.ticket = 0;
+    if (MODE == 0) S.alpha = S.rr1 / t0;
+    else S.gamma = t0 / t1;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                      const DevCase* __restrict__ cs, SolveState* __restrict__ st,
+                                                      T* __restrict__ x, BiVecs<T> V, double* __restrict__ part,
+                                                      size_t nslots, int count) {
+  __shared__ double red[BI_NT / 64];
+  __shared__ int flag;
+  const int slot = active[blockIdx.x];
+  const int c = blk[slot].case_id;
+  if (st[c].done) return;
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  const T alpha = (T)st[c].alpha, gamma = (T)st[c].gamma, ng = (T)(-st[c].gamma);
+  double rr = 0.0, rr1 = 0.0;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = base + tx + NB * (ty + NB * k);
+    x[g] = alpha * V.pp[g] + gamma * V.ss[g] + x[g];  // AXPBYPZ(x, p_, s_, alpha, gamma)
+    const T r = ng * V.t[g] + V.s[g];                 // AXPYZ(r, t_, s, -gamma)
+    V.r[g] = r;
+    rr += (double)r * (double)r;
+    rr1 += (double)r * (double)V.r0[g];
+  }
+  double s0 = block_sum<BI_NT>(rr, red);
+  double s1 = block_sum<BI_NT>(rr1, red);
+  if (t == 0) {
+    part[slot] = s0;
+    part[nslots + slot] = s1;
+  }
+  const DevCase& C = cs[c];
+  if (!last_of_case(st, c, C.nblk, &flag)) return;
+  double tr = case_sum<BI_NT>(part, C.slot_begin, C.nblk, red);
+  double t1 = case_sum<BI_NT>(part + nslots, C.slot_begin, C.nblk, red);
+  if (t == 0) {
+    SolveState& S = st[c];
+    S.ticket = 0;
+    S.iter = count;
+    converged(S, tr, count);
+    if (!S.done) {
+      S.rr0 = S.rr1;
+      if (fabs(t1) < FLT_MIN) {  // next iteration's r.r0 test (FFVILS.h:829-834)
+        S.residual = t1;
+        S.count = 0;
+        S.done = 1;
+      } else {
+        S.rr1 = t1;
+        S.beta = t1 / S.rr0 * S.alpha / S.gamma;
+      }
+    }
+  }
+}
+
+template <class T>
+void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                          SolveState* st, Coef7<T> A, const T* b, const T* x, const T* hx, BiVecs<T> V, double* part,
+                          size_t nslots) {
+  if (nact) hipLaunchKernelGGL(bi_init_kernel<T>, dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, b, x, hx, V,
+                               part, nslots);
+}
+
+template <class T>
+void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                          SolveState* st, Coef7<T> A, T* x, BiVecs<T> V, double* part, size_t nslots, int count,
+                          int mask) {
+  if (!nact) return;
+  if (mask & 1)
+    hipLaunchKernelGGL((bi_precond_kernel<T, 0>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, count);
+  if (mask & 2)
+    hipLaunchKernelGGL((bi_matvec_kernel<T, 0>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, part,
+                       nslots);
+  if (mask & 4)
+    hipLaunchKernelGGL((bi_precond_kernel<T, 1>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, count);
+  if (mask & 8)
+    hipLaunchKernelGGL((bi_matvec_kernel<T, 1>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, part,
+                       nslots);
+  if (mask & 16)
+    hipLaunchKernelGGL(bi_xr_kernel<T>, dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part, nslots,
+                       count);
+}
+
+#define INST(T)                                                                                                     \
+  template void launch_bicgstab_init<T>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,  \
+                                        Coef7<T>, const T*, const T*, const T*, BiVecs<T>, double*, size_t);         \
+  template void launch_bicgstab_iter<T>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,  \
+                                        Coef7<T>, T*, BiVecs<T>, double*, size_t, int, int);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace ffvhc

@@ -16,7 +16,7 @@ constexpr int kBatchMin = 8;
 constexpr int kBatchMax = 256;
 
 // scratch roles
-enum { R_R = 1, R_P = 2, R_Q = 3, R_E = 4, R_RHS = 5, R_A = 10 };
+enum { R_R = 1, R_P = 2, R_Q = 3, R_E = 4, R_RHS = 5, R_A = 10, R_BI = 20 };
 
 template <class T>
 Coef7<T> coef(ffvhc_ctx* c, const int* A) {
@@ -105,6 +105,52 @@ int cg_core(ffvhc_ctx* c, FieldRec& X, Coef7<T> K, const T* bv, int bcg, double 
   }
   if (launched) *launched = n - 1;
   // FFVILS.h:789 plsx->ImposeBoundaryCondition after the loop
+  launch_ghost_pull<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, (const T*)X.d[0], (T*)X.h[0], bcg, 1,
+                       c->multilevel);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// BiCGSTAB (PreCount = 1) on x, matrix K, rhs bv; eight scratch vectors of the same precision.
+template <class T>
+int bicg_vectors(ffvhc_ctx* c, BiVecs<T>* V) {
+  const int prec = (int)sizeof(T);
+  int f[8];
+  for (int q = 0; q < 8; q++) {
+    int rc = scratch_field(c, prec, R_BI + q, &f[q]);
+    if (rc) return rc;
+  }
+  T** vp[8] = {&V->r, &V->r0, &V->p, &V->pp, &V->q, &V->s, &V->ss, &V->t};
+  for (int q = 0; q < 8; q++) *vp[q] = (T*)c->fields[f[q]].d[0];
+  V->hpp = (T*)c->fields[f[3]].h[0];
+  V->hss = (T*)c->fields[f[6]].h[0];
+  return 0;
+}
+
+template <class T>
+int bicgstab_core(ffvhc_ctx* c, FieldRec& X, Coef7<T> K, const T* bv, int bcg, double omega, int countMax,
+                  double eps, SolveState* st, std::vector<SolveState>& hst, const SolveState* outer, int* launched) {
+  const std::vector<int> ids = selected_cases(c);
+  BiVecs<T> V;
+  int rc0 = bicg_vectors<T>(c, &V);
+  if (rc0) return rc0;
+  launch_solve_init(c->sm, st, (int)c->cases.size(), omega, eps, countMax, outer);
+  launch_bicgstab_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, (const T*)X.d[0],
+                          (const T*)X.h[0], V, c->dpart, (size_t)c->cap);
+  int n = 1, batch = kBatchMin;
+  bool done = false;
+  while (!done) {
+    int last = std::min(n + batch - 1, countMax);
+    for (; n <= last; n++)
+      launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, (T*)X.d[0], V, c->dpart,
+                              (size_t)c->cap, n);
+    HIPCHK(hipGetLastError());
+    int rc = read_state(c, st, hst, ids, &done);
+    if (rc) return rc;
+    if (n > countMax) done = true;
+    batch = std::min(batch * 2, kBatchMax);
+  }
+  if (launched) *launched = n - 1;
   launch_ghost_pull<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, (const T*)X.d[0], (T*)X.h[0], bcg, 1,
                        c->multilevel);
   HIPCHK(hipGetLastError());
@@ -200,6 +246,42 @@ int cg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int ni
     for (int k = 0; k < 3; k++) {
       float t = 0.f;
       HIPCHK(hipEventElapsedTime(&t, ev[4 * n + k], ev[4 * n + k + 1]));
+      ms[k] += t;
+    }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return 0;
+}
+
+// niter BiCGSTAB iterations with events between its five kernels; ms[k] = summed device time.
+template <class T>
+int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int niter, double* ms) {
+  BiVecs<T> V;
+  int rc = bicg_vectors<T>(c, &V);
+  if (rc) return rc;
+  FieldRec& X = c->fields[x];
+  Coef7<T> K = coef<T>(c, A);
+  launch_solve_init(c->sm, c->dst, (int)c->cases.size(), omega, -1.0, niter);
+  launch_bicgstab_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K,
+                          (const T*)c->fields[b].d[0], (const T*)X.d[0], (const T*)X.h[0], V, c->dpart,
+                          (size_t)c->cap);
+  std::vector<hipEvent_t> ev(6 * niter);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  for (int n = 1; n <= niter; n++) {
+    hipEvent_t* e = &ev[6 * (n - 1)];
+    HIPCHK(hipEventRecord(e[0], c->sm));
+    for (int k = 0; k < 5; k++) {
+      launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, (T*)X.d[0], V, c->dpart,
+                              (size_t)c->cap, n, 1 << k);
+      HIPCHK(hipEventRecord(e[k + 1], c->sm));
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->sm));
+  for (int k = 0; k < 5; k++) ms[k] = 0.0;
+  for (int n = 0; n < niter; n++)
+    for (int k = 0; k < 5; k++) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, ev[6 * n + k], ev[6 * n + k + 1]));
       ms[k] += t;
     }
   for (auto& e : ev) (void)hipEventDestroy(e);
@@ -313,6 +395,40 @@ int ffvhc_cg_timing(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
                    : cg_timing_run<float>(c, x, A, b, omega, niter, ms3);
 }
 
+int ffvhc_bicgstab_timing(ffvhc_ctx* c, int x, const int* A, int b, double omega, int niter, double* ms5) {
+  int prec = 8;
+  int rc = prologue(c, x, A, b, BC_NULL, &prec);
+  if (rc) return rc;
+  if (niter < 1 || !ms5) return fail(-1, "niter >= 1 and ms[5] required");
+  return prec == 8 ? bicg_timing_run<double>(c, x, A, b, omega, niter, ms5)
+                   : bicg_timing_run<float>(c, x, A, b, omega, niter, ms5);
+}
+
+int ffvhc_bicgstab(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countPre, int countMax,
+                   double eps, int* count, double* residual) {
+  int prec = 8;
+  int rc = prologue(c, x, A, b, bcg, &prec);
+  if (rc) return rc;
+  if (countPre != 1)
+    return fail(-1, "the device BiCGSTAB implements countPreConditioner = 1 (M v = omega v / Ap); see DESIGN.md");
+  if (countMax < 1) {
+    const std::vector<int> ids = selected_cases(c);
+    for (size_t i = 0; i < ids.size(); i++)
+      if (count) count[i] = 1;
+    return 0;
+  }
+  if (prec == 8)
+    rc = bicgstab_core<double>(c, c->fields[x], coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega,
+                               countMax, eps, c->dst, c->hst, nullptr, nullptr);
+  else
+    rc = bicgstab_core<float>(c, c->fields[x], coef<float>(c, A), (const float*)c->fields[b].d[0], bcg, omega,
+                              countMax, eps, c->dst, c->hst, nullptr, nullptr);
+  if (rc) return rc;
+  rc = sync_state(c, c->dst, c->hst);
+  if (rc) return rc;
+  return report(c, c->hst, count, residual);
+}
+
 // FP32 inner solve in FP64 defect correction.  count = inner iterations summed over the
 // corrections, residual = final FP64 relative residual, outer = corrections applied.
 int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int bcg, double omega, int countMax,
@@ -321,7 +437,7 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
-  if (method != 0 && method != 1) return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG");
+  if (method < 0 || method > 2) return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB");
   const std::vector<int> ids = selected_cases(c);
   int fa[7], fr, fe;
   for (int q = 0; q < 7; q++) {
@@ -359,9 +475,12 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
     if (method == 0)
       rc = rbgs_core<float>(c, E, K32, (const float*)Rf.d[0], BC_HOM, omega, countMax, inner_eps, c->dst, c->hst,
                             c->dst2, nullptr);
-    else
+    else if (method == 1)
       rc = cg_core<float>(c, E, K32, (const float*)Rf.d[0], BC_HOM, omega, countMax, inner_eps, c->dst, c->hst,
                           c->dst2, nullptr);
+    else
+      rc = bicgstab_core<float>(c, E, K32, (const float*)Rf.d[0], BC_HOM, omega, countMax, inner_eps, c->dst,
+                                c->hst, c->dst2, nullptr);
     if (rc) return rc;
     rc = sync_state(c, c->dst, c->hst);
     if (rc) return rc;

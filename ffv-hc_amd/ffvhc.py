@@ -75,6 +75,8 @@ _sig = {
     "ffvhc_rbgs_fixed": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, _P(c_double)]),
     "ffvhc_rbgs_fixed_average": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
     "ffvhc_cg": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_int, c_double, _vp, _vp]),
+    "ffvhc_bicgstab": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_int, c_double, _vp, _vp]),
+    "ffvhc_bicgstab_timing": (c_int, [_vp, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_cg_timing": (c_int, [_vp, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_jacobi": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
     "ffvhc_solve_mixed": (c_int, [_vp, c_int, c_int, _vp, c_int, c_int, c_double, c_int, c_double, c_double, _vp,
@@ -315,6 +317,23 @@ class Context:
         if rc < 0:
             check(rc, "ffvhc_cg")
         return cnt, res, rc
+
+    def bicgstab(self, x, A, b, group, omega, count_pre, count_max, eps, ncases=None):
+        nc = ncases or self._nsel()
+        cnt = np.zeros(nc, np.int32)
+        res = np.zeros(nc)
+        a = np.asarray(A, np.int32)
+        rc = _L.ffvhc_bicgstab(self.h, x, _ptr(a), b, group, omega, count_pre, count_max, eps, _ptr(cnt), _ptr(res))
+        if rc < 0:
+            check(rc, "ffvhc_bicgstab")
+        return cnt, res, rc
+
+    def bicgstab_timing(self, x, A, b, omega, niter):
+        """Device ms of the five BiCGSTAB kernels summed over niter iterations."""
+        ms = np.zeros(5)
+        a = np.asarray(A, np.int32)
+        check(_L.ffvhc_bicgstab_timing(self.h, x, _ptr(a), b, omega, niter, _ptr(ms)), "ffvhc_bicgstab_timing")
+        return ms
 
     def cg_timing(self, x, A, b, omega, niter):
         """Device ms of the three PCG kernels summed over niter iterations."""

@@ -150,6 +150,14 @@ int ffvhc_rbgs_fixed_average(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_
 /* FFVILS::CG (FFVILS.h:661-791): Jacobi-preconditioned CG, countPre damped-Jacobi sweeps */
 int ffvhc_cg(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countPre, int countMax,
              double eps, int* count, double* residual);
+/* FFVILS::BiCGSTAB (FFVILS.h:793-974): Jacobi right-preconditioned BiCGSTAB, countPre = 1.  The
+ * coarse/fine ghost interpolation makes the octree Poisson operator non-symmetric; this is the
+ * solver that converges on it (and the reference's default for P and implicit U). */
+int ffvhc_bicgstab(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countPre,
+                   int countMax, double eps, int* count, double* residual);
+/* niter BiCGSTAB iterations with HIP events between the five fused kernels; ms[0..4] = total
+ * device time of P (p update + preconditioner), Q (q_ = A p_), S, T (t_ = A s_), XR (timing entry). */
+int ffvhc_bicgstab_timing(ffvhc_ctx* ctx, int x, const int* A, int b, double omega, int niter, double* ms);
 /* niter PCG iterations (no convergence test) with HIP events between the three fused kernels;
  * ms[0..2] = total device time of the p-update, matvec and x/r-update kernels (timing entry). */
 int ffvhc_cg_timing(ffvhc_ctx* ctx, int x, const int* A, int b, double omega, int niter, double* ms);
@@ -158,7 +166,8 @@ int ffvhc_jacobi(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, doubl
                  int* count, double* residual);
 
 /* Mixed precision: inner sweeps / matvecs in FP32 wrapped in FP64 defect correction to the same
- * relative residual eps (FP64 residual, dots and state).  method 0 RBGS, 1 Jacobi-PCG.
+ * relative residual eps (FP64 residual, dots and state).  method 0 RBGS, 1 Jacobi-PCG,
+ * 2 Jacobi-BiCGSTAB.
  * inner_eps: relative reduction per correction.  outer[c] = corrections taken. */
 int ffvhc_solve_mixed(ffvhc_ctx* ctx, int method, int x, const int* A, int b, int bc_group, double omega,
                       int countMax, double eps, double inner_eps, int* count, double* residual, int* outer);

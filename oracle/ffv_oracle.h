@@ -156,6 +156,10 @@ int orc_cg(const orc_grid* gr, real* x, real* const* A, const real* b, real* r, 
            real* x0, const int* bctype, const real* bcval, real omega, int countPre, int countMax,
            real eps, int* count, real* residual);
 
+/* FFVILS::BiCGSTAB (FFVILS.h:793-974); sc = {r, r0, p, p_, q_, s, s_, t_, x0} */
+int orc_bicgstab(const orc_grid* gr, real* x, real* const* A, const real* b, real* const* sc, const int* bctype,
+                 const real* bcval, real omega, int countPre, int countMax, real eps, int* count, real* residual);
+
 /* Fractional step (Solver::Update, Solver.cpp:3344-3416) with explicit U (UpdateUXe 3535-3744),
  * UpdateP (5294-5688) and UpdateU (5690-5771).  Temperature/fan stages are out of scope. */
 typedef struct {

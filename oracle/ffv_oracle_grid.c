@@ -332,6 +332,87 @@ int orc_cg(const orc_grid* gr, real* x, real* const* A, const real* b, real* r, 
   return ret;
 }
 
+/* FFVILS::BiCGSTAB (FFVILS.h:793-974): Jacobi right-preconditioned BiCGSTAB; scratch r, r0, p, p_,
+ * q_, s, s_, t_ (Solver.cpp:2026-2042) and x0 (the preconditioner's swap array). */
+int orc_bicgstab(const orc_grid* gr, real* x, real* const* A, const real* b, real* const* sc, const int* bctype,
+                 const real* bcval, real omega, int countPre, int countMax, real eps, int* count, real* residual) {
+  const int sz[3] = {gr->n, gr->n, gr->n};
+  const int nb = gr->nblk;
+  const int bt_null[6] = {1, 1, 1, 1, 1, 1};
+  const real bv_null[6] = {0, 0, 0, 0, 0, 0};
+  real *r = sc[0], *r0 = sc[1], *p = sc[2], *q_ = sc[4], *s = sc[5], *t_ = sc[7];
+  real *p_ = sc[3], *s_ = sc[6], *x0 = sc[8];
+  real bb = orc_grid_dot(gr, b, b);
+  if (fabs(bb) < FLT_MIN) { *residual = 0.0; *count = 0; return 0; }
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nb; k++)
+    orc_calc_r(BLK(r, gr, k), BLK(A[0], gr, k), BLK(A[1], gr, k), BLK(A[2], gr, k), BLK(A[3], gr, k),
+               BLK(A[4], gr, k), BLK(A[5], gr, k), BLK(A[6], gr, k), BLK(x, gr, k), BLK(b, gr, k), sz, gr->g);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nb; k++) orc_copy(BLK(r0, gr, k), BLK(r, gr, k), sz, gr->g);
+  real rr0 = 1.0, alpha = 0.0, gamma = 1.0;
+  int c, ret = 0;
+  for (c = 1; c <= countMax; ++c) {
+    real rr1 = orc_grid_dot(gr, r, r0);
+    if (fabs(rr1) < FLT_MIN) { *residual = rr1; c = 0; break; }
+    if (c == 1) {
+#pragma omp parallel for schedule(static)
+      for (int k = 0; k < nb; k++) orc_copy(BLK(p, gr, k), BLK(r, gr, k), sz, gr->g);
+    } else {
+      real beta = rr1 / rr0 * alpha / gamma;
+#pragma omp parallel for schedule(static)
+      for (int k = 0; k < nb; k++) {
+        orc_axpy(BLK(p, gr, k), BLK(q_, gr, k), -gamma, sz, gr->g);
+        orc_xpay(BLK(p, gr, k), BLK(r, gr, k), beta, sz, gr->g);
+      }
+    }
+    orc_ghost_fill(gr, p, 1, bt_null, bv_null);
+    real* pp = p_;
+    real* xx0 = x0;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nb; k++) orc_fill(BLK(pp, gr, k), 0.0, sz, gr->g);
+    for (int it = 0; it < countPre; it++) jacobi_sweep(gr, &pp, A, p, &xx0, omega, bt_null, bv_null);
+    if (pp != p_) { /* keep p_ naming stable for the caller-visible scratch */
+      memcpy(p_, pp, sizeof(real) * NPAD(gr) * (size_t)nb);
+    }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nb; k++)
+      orc_calc_ax(BLK(q_, gr, k), BLK(A[0], gr, k), BLK(A[1], gr, k), BLK(A[2], gr, k), BLK(A[3], gr, k),
+                  BLK(A[4], gr, k), BLK(A[5], gr, k), BLK(A[6], gr, k), BLK(p_, gr, k), sz, gr->g);
+    real q_r0 = orc_grid_dot(gr, q_, r0);
+    alpha = rr1 / q_r0;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nb; k++) orc_axpyz(BLK(s, gr, k), BLK(q_, gr, k), BLK(r, gr, k), -alpha, sz, gr->g);
+    orc_ghost_fill(gr, s, 1, bt_null, bv_null);
+    real* ss = s_;
+    xx0 = x0;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nb; k++) orc_fill(BLK(ss, gr, k), 0.0, sz, gr->g);
+    for (int it = 0; it < countPre; it++) jacobi_sweep(gr, &ss, A, s, &xx0, omega, bt_null, bv_null);
+    if (ss != s_) memcpy(s_, ss, sizeof(real) * NPAD(gr) * (size_t)nb);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nb; k++)
+      orc_calc_ax(BLK(t_, gr, k), BLK(A[0], gr, k), BLK(A[1], gr, k), BLK(A[2], gr, k), BLK(A[3], gr, k),
+                  BLK(A[4], gr, k), BLK(A[5], gr, k), BLK(A[6], gr, k), BLK(s_, gr, k), sz, gr->g);
+    real t_s = orc_grid_dot(gr, t_, s);
+    real t_t_ = orc_grid_dot(gr, t_, t_);
+    gamma = t_s / t_t_;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nb; k++) {
+      orc_axpbypz(BLK(x, gr, k), BLK(p_, gr, k), BLK(s_, gr, k), alpha, gamma, sz, gr->g);
+      orc_axpyz(BLK(r, gr, k), BLK(t_, gr, k), BLK(s, gr, k), -gamma, sz, gr->g);
+    }
+    real rr = orc_grid_dot(gr, r, r);
+    int st = is_converged(residual, rr, bb, eps);
+    if (st < 0) { ret = 1; break; }
+    if (st > 0) break;
+    rr0 = rr1;
+  }
+  orc_ghost_fill(gr, x, 1, bctype, bcval);
+  *count = c;
+  return ret;
+}
+
 /* ------------------------------------------------------------ time step */
 
 #define GEOM(F, gr, k)                                                                               \

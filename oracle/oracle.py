@@ -106,7 +106,7 @@ def _declare(L):
     L.orc_bstl_detect_zerocut.restype = i
     L.orc_bstl_fill_holes_v1.restype = i
     L.orc_bstl_fill_holes_v2.restype = i
-    for name in ["orc_rbgs", "orc_rbgs_fixed_average", "orc_jacobi", "orc_cg"]:
+    for name in ["orc_rbgs", "orc_rbgs_fixed_average", "orc_jacobi", "orc_cg", "orc_bicgstab"]:
         getattr(L, name).restype = i
 
 

@@ -289,3 +289,66 @@ def test_mixed_precision_reaches_fp64_residual(ctx, method):
             ctx.free(f)
     finally:
         ctx.remove_case(cid)
+
+
+def run_oracle_bicgstab(og, x, A, b, bct, bcv, omega, cmax, eps):
+    cnt, res = ctypes.c_int(), ctypes.c_double()
+    sc = [og.zeros() for _ in range(9)]
+    og.L.orc_bicgstab(og.ref(), O.ptr(x), O.parr(A), O.ptr(b), O.parr(sc), O.cint6(bct), O.cdbl6(bcv),
+                      ctypes.c_double(omega), 1, cmax, ctypes.c_double(eps), ctypes.byref(cnt), ctypes.byref(res))
+    return cnt.value, res.value
+
+
+@pytest.mark.parametrize("name", ["flat", "three_level"])
+def test_bicgstab_fixed_count_matches_oracle(ctx, name):
+    tree = trees()[name]
+    params = ffvhc.case_params(bc=BC)
+    cid = ctx.add_case(tree, params)
+    try:
+        og = OracleGrid(tree)
+        rng = np.random.default_rng(12)
+        bct = [params.bct[ffvhc.BC_P][q] for q in range(6)]
+        bcv = [params.bcv[ffvhc.BC_P][q] for q in range(6)]
+        A, b = make_system(og, rng, bct, bcv)
+        x = og.zeros()
+        fa, fb, fx = upload_system(ctx, cid, A, b, x)
+        cnt, res, rc = ctx.bicgstab(fx, fa, fb, ffvhc.BC_P, 1.0, 1, 8, 0.0, ncases=1)
+        got = ctx.download(fx, cid, out=x.copy())
+        c2, r2 = run_oracle_bicgstab(og, x, A, b, bct, bcv, 1.0, 8, 0.0)
+        assert rc == 0 and cnt[0] == c2 == 9
+        assert abs(res[0] - r2) <= 1e-9 * abs(r2)
+        scale = np.max(np.abs(x[:, LIVE]))
+        assert np.max(np.abs(got[:, LIVE] - x[:, LIVE])) <= 1e-11 * scale
+        for f in fa + [fb, fx]:
+            ctx.free(f)
+    finally:
+        ctx.remove_case(cid)
+
+
+def test_bicgstab_converges_and_mixed_matches(ctx):
+    tree = trees()["three_level"]
+    params = ffvhc.case_params(bc=BC)
+    cid = ctx.add_case(tree, params)
+    try:
+        og = OracleGrid(tree)
+        rng = np.random.default_rng(13)
+        bct = [params.bct[ffvhc.BC_P][q] for q in range(6)]
+        bcv = [params.bcv[ffvhc.BC_P][q] for q in range(6)]
+        A, b = make_system(og, rng, bct, bcv)
+        x = og.zeros()
+        fa, fb, fx = upload_system(ctx, cid, A, b, x)
+        cnt, res, rc = ctx.bicgstab(fx, fa, fb, ffvhc.BC_P, 1.0, 1, 5000, 1e-10, ncases=1)
+        assert rc == 0 and res[0] <= 1e-10 and 0 < cnt[0] < 5000
+        ref = ctx.download(fx, cid, out=x.copy())
+        assert true_residual(og, A, ref, b) <= 1.001e-10
+        ctx.upload(fx, cid, og.zeros())
+        cm, rm, outer, rc = ctx.solve_mixed(2, fx, fa, fb, ffvhc.BC_P, 1.0, 5000, 1e-10, 1e-5, ncases=1)
+        assert rc == 0 and rm[0] <= 1e-10 and outer[0] >= 2
+        got = ctx.download(fx, cid, out=x.copy())
+        assert true_residual(og, A, got, b) <= 1.001e-10
+        scale = np.max(np.abs(ref[:, LIVE]))
+        assert np.max(np.abs(got[:, LIVE] - ref[:, LIVE])) <= 1e-7 * scale
+        for f in fa + [fb, fx]:
+            ctx.free(f)
+    finally:
+        ctx.remove_case(cid)
