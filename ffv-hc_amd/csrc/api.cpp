@@ -62,8 +62,9 @@ int alloc_field(ffvhc_ctx* ctx, int prec, int* field) {
       break;
     }
   if (id < 0) {
+    if ((int)ctx->fields.size() >= MAX_FIELDS) return fail(-3, "too many fields");
     id = (int)ctx->fields.size();
-    ctx->fields.emplace_back();
+    ctx->fields.emplace_back();  // capacity reserved at creation: existing records never move
   }
   FieldRec& f = ctx->fields[id];
   f = FieldRec();
@@ -193,6 +194,7 @@ int ffvhc_ctx_create(int device, long long max_blocks, ffvhc_ctx** out) {
   c->hcase.assign(MAX_CASES, DevCase{});
   c->hst.assign(MAX_CASES, SolveState{});
   c->hst2.assign(MAX_CASES, SolveState{});
+  c->fields.reserve(MAX_FIELDS);
   HIPCHK(hipMalloc(&c->dblk, sizeof(DevBlock) * max_blocks));
   HIPCHK(hipMalloc(&c->dcase, sizeof(DevCase) * MAX_CASES));
   HIPCHK(hipMalloc(&c->dactive, sizeof(int) * max_blocks));

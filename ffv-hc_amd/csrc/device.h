@@ -20,6 +20,7 @@ constexpr int NC = NB * NB * NB;     // interior cells per block
 constexpr int NF = NB * NB;          // cells per face
 constexpr int NH = 6 * NF;           // halo cells per block
 constexpr int MAX_CASES = 1024;
+constexpr int MAX_FIELDS = 512;
 
 // Outer-BC groups carried by each case: the velocity components, pressure and the
 // boundaryTypeNULL fields (Neumann 0: A, b, face velocities, scratch, Solver.cpp:2024-2034).

@@ -45,9 +45,12 @@ int check_args(ffvhc_ctx* c, int x, const int* A, int b, int prec) {
 
 // RBGS on field x (ping-pong), matrix K, rhs bv; state in st. Result left in buffer 0.
 template <class T>
-int rbgs_core(ffvhc_ctx* c, FieldRec& X, Coef7<T> K, const T* bv, int bcg, double omega, int countMax, double eps,
+int rbgs_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double omega, int countMax, double eps,
               SolveState* st, std::vector<SolveState>& hst, const SolveState* outer, int* launched) {
   const std::vector<int> ids = selected_cases(c);
+  int rc0 = ensure_second_buffer(c, xf);
+  if (rc0) return rc0;
+  const FieldRec X = c->fields[xf];  // copy: no reference into the field table is held
   launch_solve_init(c->sm, st, (int)c->cases.size(), omega, eps, countMax, outer);
   T* xs[2] = {(T*)X.d[0], (T*)X.d[1]};
   T* hs[2] = {(T*)X.h[0], (T*)X.h[1]};
@@ -72,7 +75,7 @@ int rbgs_core(ffvhc_ctx* c, FieldRec& X, Coef7<T> K, const T* bv, int bcg, doubl
 
 // Jacobi-PCG (PreCount = 1) on x, matrix K, rhs bv; scratch r,p,q of the same precision.
 template <class T>
-int cg_core(ffvhc_ctx* c, FieldRec& X, Coef7<T> K, const T* bv, int bcg, double omega, int countMax, double eps,
+int cg_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double omega, int countMax, double eps,
             SolveState* st, std::vector<SolveState>& hst, const SolveState* outer, int* launched) {
   const std::vector<int> ids = selected_cases(c);
   const int prec = (int)sizeof(T);
@@ -81,7 +84,8 @@ int cg_core(ffvhc_ctx* c, FieldRec& X, Coef7<T> K, const T* bv, int bcg, double 
   if (!rc) rc = scratch_field(c, prec, R_P, &fp);
   if (!rc) rc = scratch_field(c, prec, R_Q, &fq);
   if (rc) return rc;
-  FieldRec &Rf = c->fields[fr], &Pf = c->fields[fp], &Qf = c->fields[fq];
+  // copies taken after every allocation
+  const FieldRec X = c->fields[xf], Rf = c->fields[fr], Pf = c->fields[fp], Qf = c->fields[fq];
   launch_solve_init(c->sm, st, (int)c->cases.size(), omega, eps, countMax, outer);
   launch_cg_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, (const T*)X.d[0],
                     (const T*)X.h[0], (T*)Rf.d[0], c->dpart, (size_t)c->cap);
@@ -128,12 +132,13 @@ int bicg_vectors(ffvhc_ctx* c, BiVecs<T>* V) {
 }
 
 template <class T>
-int bicgstab_core(ffvhc_ctx* c, FieldRec& X, Coef7<T> K, const T* bv, int bcg, double omega, int countMax,
+int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double omega, int countMax,
                   double eps, SolveState* st, std::vector<SolveState>& hst, const SolveState* outer, int* launched) {
   const std::vector<int> ids = selected_cases(c);
   BiVecs<T> V;
   int rc0 = bicg_vectors<T>(c, &V);
   if (rc0) return rc0;
+  const FieldRec X = c->fields[xf];  // copy taken after the scratch allocations
   launch_solve_init(c->sm, st, (int)c->cases.size(), omega, eps, countMax, outer);
   launch_bicgstab_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, (const T*)X.d[0],
                           (const T*)X.h[0], V, c->dpart, (size_t)c->cap);
@@ -180,7 +185,7 @@ template <class T>
 int rbgs_fixed_run(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int nsweep, double* ms) {
   int rc = ensure_second_buffer(c, x);
   if (rc) return rc;
-  FieldRec& X = c->fields[x];
+  const FieldRec X = c->fields[x];
   launch_solve_init(c->sm, c->dst, (int)c->cases.size(), omega, -1.0, nsweep);
   Coef7<T> K = coef<T>(c, A);
   const T* bv = (const T*)c->fields[b].d[0];
@@ -219,7 +224,7 @@ int cg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int ni
   if (!rc) rc = scratch_field(c, prec, R_P, &fp);
   if (!rc) rc = scratch_field(c, prec, R_Q, &fq);
   if (rc) return rc;
-  FieldRec &X = c->fields[x], &Rf = c->fields[fr], &Pf = c->fields[fp], &Qf = c->fields[fq];
+  const FieldRec X = c->fields[x], Rf = c->fields[fr], Pf = c->fields[fp], Qf = c->fields[fq];
   Coef7<T> K = coef<T>(c, A);
   launch_solve_init(c->sm, c->dst, (int)c->cases.size(), omega, -1.0, niter);
   launch_cg_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, (const T*)c->fields[b].d[0],
@@ -258,7 +263,7 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
   BiVecs<T> V;
   int rc = bicg_vectors<T>(c, &V);
   if (rc) return rc;
-  FieldRec& X = c->fields[x];
+  const FieldRec X = c->fields[x];
   Coef7<T> K = coef<T>(c, A);
   launch_solve_init(c->sm, c->dst, (int)c->cases.size(), omega, -1.0, niter);
   launch_bicgstab_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K,
@@ -341,10 +346,10 @@ int ffvhc_rbgs(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, 
   rc = ensure_second_buffer(c, x);
   if (rc) return rc;
   if (prec == 8)
-    rc = rbgs_core<double>(c, c->fields[x], coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega, countMax,
+    rc = rbgs_core<double>(c, x, coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega, countMax,
                            eps, c->dst, c->hst, nullptr, nullptr);
   else
-    rc = rbgs_core<float>(c, c->fields[x], coef<float>(c, A), (const float*)c->fields[b].d[0], bcg, omega, countMax,
+    rc = rbgs_core<float>(c, x, coef<float>(c, A), (const float*)c->fields[b].d[0], bcg, omega, countMax,
                           eps, c->dst, c->hst, nullptr, nullptr);
   if (rc) return rc;
   rc = sync_state(c, c->dst, c->hst);
@@ -375,10 +380,10 @@ int ffvhc_cg(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, in
     return 0;
   }
   if (prec == 8)
-    rc = cg_core<double>(c, c->fields[x], coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega, countMax,
+    rc = cg_core<double>(c, x, coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega, countMax,
                          eps, c->dst, c->hst, nullptr, nullptr);
   else
-    rc = cg_core<float>(c, c->fields[x], coef<float>(c, A), (const float*)c->fields[b].d[0], bcg, omega, countMax, eps,
+    rc = cg_core<float>(c, x, coef<float>(c, A), (const float*)c->fields[b].d[0], bcg, omega, countMax, eps,
                         c->dst, c->hst, nullptr, nullptr);
   if (rc) return rc;
   rc = sync_state(c, c->dst, c->hst);
@@ -418,10 +423,10 @@ int ffvhc_bicgstab(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double ome
     return 0;
   }
   if (prec == 8)
-    rc = bicgstab_core<double>(c, c->fields[x], coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega,
+    rc = bicgstab_core<double>(c, x, coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega,
                                countMax, eps, c->dst, c->hst, nullptr, nullptr);
   else
-    rc = bicgstab_core<float>(c, c->fields[x], coef<float>(c, A), (const float*)c->fields[b].d[0], bcg, omega,
+    rc = bicgstab_core<float>(c, x, coef<float>(c, A), (const float*)c->fields[b].d[0], bcg, omega,
                               countMax, eps, c->dst, c->hst, nullptr, nullptr);
   if (rc) return rc;
   rc = sync_state(c, c->dst, c->hst);
@@ -457,7 +462,7 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
                    (const double*)c->fields[A[q]].d[0]);
   Coef7<double> K64 = coef<double>(c, A);
   Coef7<float> K32 = coef<float>(c, fa);
-  FieldRec &X = c->fields[x], &E = c->fields[fe], &Rf = c->fields[fr];
+  const FieldRec X = c->fields[x], E = c->fields[fe], Rf = c->fields[fr];
   std::vector<int> inner_total(c->cases.size(), 0), outers(c->cases.size(), -1);
   const int max_outer = 64;
   launch_solve_init(c->sm, c->dst2, (int)c->cases.size(), omega, eps, max_outer);
@@ -473,13 +478,13 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
     if (done || o >= max_outer) break;
     launch_fill<float>(c->sm, c->dactive, c->nactive, (float*)E.d[0], (float*)E.h[0], 0.0f);
     if (method == 0)
-      rc = rbgs_core<float>(c, E, K32, (const float*)Rf.d[0], BC_HOM, omega, countMax, inner_eps, c->dst, c->hst,
+      rc = rbgs_core<float>(c, fe, K32, (const float*)Rf.d[0], BC_HOM, omega, countMax, inner_eps, c->dst, c->hst,
                             c->dst2, nullptr);
     else if (method == 1)
-      rc = cg_core<float>(c, E, K32, (const float*)Rf.d[0], BC_HOM, omega, countMax, inner_eps, c->dst, c->hst,
+      rc = cg_core<float>(c, fe, K32, (const float*)Rf.d[0], BC_HOM, omega, countMax, inner_eps, c->dst, c->hst,
                           c->dst2, nullptr);
     else
-      rc = bicgstab_core<float>(c, E, K32, (const float*)Rf.d[0], BC_HOM, omega, countMax, inner_eps, c->dst,
+      rc = bicgstab_core<float>(c, fe, K32, (const float*)Rf.d[0], BC_HOM, omega, countMax, inner_eps, c->dst,
                                 c->hst, c->dst2, nullptr);
     if (rc) return rc;
     rc = sync_state(c, c->dst, c->hst);
