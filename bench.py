@@ -213,7 +213,15 @@ def main():
         timings[tag] = {"seconds": dt, "iters": st[-1][0], "residual": st[-1][1], "outer": st[-1][2],
                         "mcell_steps_s": cells * steps / dt / 1e6, "ms_per_step": dt / steps * 1e3}
     if args.profile:
-        print(json.dumps({"profile": True, "kernels": kt, "timings": timings}))
+        # calibration for HBM counters: plain field copies of known size (interior + one ghost layer)
+        for w in (8, 4):
+            tmp, src = ctx.field(w), (f64 if w == 8 else f32)["b"]
+            for _ in range(10):
+                ctx.copy(tmp, src)
+            ctx.free(tmp)
+        ctx.sync()
+        copy_bytes = {w: 2 * w * tree.nblk * (N ** 3 + 6 * N * N) for w in (8, 4)}
+        print(json.dumps({"profile": True, "kernels": kt, "timings": timings, "copy_bytes": copy_bytes}))
         ctx.close()
         return
     mix, d64 = timings["mixed"], timings["fp64"]

@@ -48,8 +48,8 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const Dev
 void launch_defect(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                    SolveState* st, Coef7<double> A, const double* b, const double* x, const double* hx, float* r32,
                    double* part, size_t nslots, int outer);
-void launch_add_correction(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const SolveState* st,
-                           double* x, const float* e);
+void launch_add_correction(hipStream_t sm, const int* active, int nact, const DevBlock* blk, SolveState* st,
+                           const SolveState* inner, double* x, const float* e);
 void launch_convert(hipStream_t sm, const int* active, int nact, float* y, const double* x);
 
 void launch_solve_init(hipStream_t sm, SolveState* st, int ncase, double omega, double eps, int countMax,

@@ -320,10 +320,20 @@ __global__ __launch_bounds__(CG_NT) void defect_kernel(const int* __restrict__ a
 
 __global__ __launch_bounds__(256) void add_correction_kernel(const int* __restrict__ active,
                                                              const DevBlock* __restrict__ blk,
-                                                             const SolveState* __restrict__ st,
+                                                             SolveState* __restrict__ st,
+                                                             const SolveState* __restrict__ inner,
                                                              double* __restrict__ x, const float* __restrict__ e) {
   const int slot = active[blockIdx.x];
-  if (st[blk[slot].case_id].done) return;
+  const int c = blk[slot].case_id;
+  if (st[c].done) return;
+  if (inner[c].status) {  // the inner solve broke down (NaN/Inf): the case ends with that status
+    if (threadIdx.x == 0) {
+      st[c].status = 1;
+      st[c].residual = inner[c].residual;
+      st[c].done = 1;
+    }
+    return;
+  }
   const size_t base = (size_t)slot * NC;
   for (int q = threadIdx.x; q < NC; q += 256) x[base + q] = x[base + q] + (double)e[base + q];
 }
@@ -341,9 +351,9 @@ void launch_defect(hipStream_t sm, const int* active, int nact, const DevBlock* 
   if (nact) hipLaunchKernelGGL(defect_kernel, dim3(nact), dim3(CG_NT), 0, sm, active, blk, cs, st, A, b, x, hx, r32,
                                part, nslots, outer);
 }
-void launch_add_correction(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const SolveState* st,
-                           double* x, const float* e) {
-  if (nact) hipLaunchKernelGGL(add_correction_kernel, dim3(nact), dim3(256), 0, sm, active, blk, st, x, e);
+void launch_add_correction(hipStream_t sm, const int* active, int nact, const DevBlock* blk, SolveState* st,
+                           const SolveState* inner, double* x, const float* e) {
+  if (nact) hipLaunchKernelGGL(add_correction_kernel, dim3(nact), dim3(256), 0, sm, active, blk, st, inner, x, e);
 }
 void launch_convert(hipStream_t sm, const int* active, int nact, float* y, const double* x) {
   if (nact) hipLaunchKernelGGL(convert_kernel, dim3(nact), dim3(256), 0, sm, active, y, x);

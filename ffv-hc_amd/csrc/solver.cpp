@@ -491,7 +491,8 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
     if (rc) return rc;
     for (int id : ids)
       if (!c->hst2[id].done) inner_total[id] += std::max(c->hst[id].count, 0);
-    launch_add_correction(c->sm, c->dactive, c->nactive, c->dblk, c->dst2, (double*)X.d[0], (const float*)E.d[0]);
+    launch_add_correction(c->sm, c->dactive, c->nactive, c->dblk, c->dst2, c->dst, (double*)X.d[0],
+                          (const float*)E.d[0]);
     launch_ghost_pull<double>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, (const double*)X.d[0],
                               (double*)X.h[0], bcg, 1, c->multilevel);
     HIPCHK(hipGetLastError());
@@ -508,6 +509,27 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   }
   if (status) set_error("residual is NaN or Inf");
   return status;
+}
+
+int ffvhc_residual(ffvhc_ctx* c, int x, const int* A, int b, double* residual) {
+  int prec = 8;
+  int rc = prologue(c, x, A, b, BC_P, &prec);
+  if (rc) return rc;
+  if (prec != 8) return fail(-1, "residual takes FP64 x, A, b");
+  int fr;
+  rc = scratch_field(c, 4, R_RHS, &fr);
+  if (rc) return rc;
+  const std::vector<int> ids = selected_cases(c);
+  launch_solve_init(c->sm, c->dst2, (int)c->cases.size(), 1.0, -1.0, 1);
+  launch_defect(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst2, coef<double>(c, A),
+                (const double*)c->fields[b].d[0], (const double*)c->fields[x].d[0],
+                (const double*)c->fields[x].h[0], (float*)c->fields[fr].d[0], c->dpart, (size_t)c->cap, 0);
+  HIPCHK(hipGetLastError());
+  rc = sync_state(c, c->dst2, c->hst2);
+  if (rc) return rc;
+  for (size_t i = 0; i < ids.size(); i++)
+    if (residual) residual[i] = c->hst2[ids[i]].residual;
+  return 0;
 }
 
 }  // extern "C"

@@ -79,6 +79,7 @@ _sig = {
     "ffvhc_bicgstab_timing": (c_int, [_vp, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_cg_timing": (c_int, [_vp, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_jacobi": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
+    "ffvhc_residual": (c_int, [_vp, c_int, _vp, c_int, _vp]),
     "ffvhc_solve_mixed": (c_int, [_vp, c_int, c_int, _vp, c_int, c_int, c_double, c_int, c_double, c_double, _vp,
                                   _vp, _vp]),
 }
@@ -341,6 +342,13 @@ class Context:
         a = np.asarray(A, np.int32)
         check(_L.ffvhc_cg_timing(self.h, x, _ptr(a), b, omega, niter, _ptr(ms)), "ffvhc_cg_timing")
         return ms
+
+    def residual(self, x, A, b, ncases=None):
+        """FP64 ||b - Ax|| / ||b|| per selected case (x's ghost layer must be current)."""
+        res = np.zeros(ncases or self._nsel())
+        a = np.asarray(A, np.int32)
+        check(_L.ffvhc_residual(self.h, x, _ptr(a), b, _ptr(res)), "ffvhc_residual")
+        return res
 
     def solve_mixed(self, method, x, A, b, group, omega, count_max, eps, inner_eps, ncases=None):
         nc = ncases or self._nsel()

@@ -165,10 +165,16 @@ int ffvhc_cg_timing(ffvhc_ctx* ctx, int x, const int* A, int b, double omega, in
 int ffvhc_jacobi(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countMax, double eps,
                  int* count, double* residual);
 
+/* True FP64 relative residual ||b - Ax||_2 / ||b||_2 of every selected case (FFVILS::CalcR2 and
+ * IsConverged, FFVILS.h:247-296 and 24-47); x's ghost layer must be current. */
+int ffvhc_residual(ffvhc_ctx* ctx, int x, const int* A, int b, double* residual);
+
 /* Mixed precision: inner sweeps / matvecs in FP32 wrapped in FP64 defect correction to the same
  * relative residual eps (FP64 residual, dots and state).  method 0 RBGS, 1 Jacobi-PCG,
  * 2 Jacobi-BiCGSTAB.
- * inner_eps: relative reduction per correction.  outer[c] = corrections taken. */
+ * inner_eps: relative reduction per correction; countMax caps each inner solve (a capped inner
+ * solve still yields a correction).  count[c] = inner iterations in total, outer[c] = corrections
+ * taken, residual[c] = FP64 ||b - Ax|| / ||b|| after the last correction. */
 int ffvhc_solve_mixed(ffvhc_ctx* ctx, int method, int x, const int* A, int b, int bc_group, double omega,
                       int countMax, double eps, double inner_eps, int* count, double* residual, int* outer);
 
