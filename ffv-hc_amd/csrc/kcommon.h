@@ -47,6 +47,41 @@ __device__ inline double case_sum(const double* part, int begin, int count, doub
   return r;
 }
 
+// Planes a thread loads per batch: every global load of a batch is issued before any store of
+// the batch, so the memory latency is paid once per batch instead of once per plane (a store
+// through a pointer held in a struct may alias later loads as far as the compiler can tell).
+template <class T>
+struct Batch {
+  static constexpr int K = sizeof(T) == 4 ? 8 : 4;
+};
+
+constexpr int LP = NB + 2;  // LDS pitch
+__device__ inline int lds(int i, int j, int k) { return (i + 1) + LP * ((j + 1) + LP * (k + 1)); }
+
+// Stage a block's interior and its one-cell ghost layer into LDS (pitch LP); NT threads with
+// thread t owning column (t & 15, t >> 4) of every plane (NT = 256).
+template <class T, int NT>
+__device__ inline void stage_block(T* s, const T* __restrict__ x, const T* __restrict__ h, int slot) {
+  static_assert(NT == 256, "one thread per (i, j) column");
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  T v[NB];
+#pragma unroll
+  for (int k = 0; k < NB; k++) v[k] = x[base + tx + NB * (ty + NB * k)];
+  T hv[NH / NT];
+#pragma unroll
+  for (int q = 0; q < NH / NT; q++) hv[q] = h[(size_t)slot * NH + t + q * NT];
+#pragma unroll
+  for (int k = 0; k < NB; k++) s[lds(tx, ty, k)] = v[k];
+#pragma unroll
+  for (int q = 0; q < NH / NT; q++) {
+    const int w = t + q * NT;
+    int f = w >> 8, p = w & 255, i, j, kk;
+    face_cell(f, p & 15, p >> 4, -1, i, j, kk);
+    s[lds(i, j, kk)] = hv[q];
+  }
+}
+
 // returns true in every thread of the last workgroup of case c
 __device__ inline bool last_of_case(SolveState* st, int c, int nblk, int* flag) {
   __syncthreads();
@@ -79,8 +114,6 @@ __device__ inline void converged(SolveState& s, double rr, int count) {
   }
 }
 
-constexpr int LP = NB + 2;  // LDS pitch
-__device__ inline int lds(int i, int j, int k) { return (i + 1) + LP * ((j + 1) + LP * (k + 1)); }
 
 // Push this block's ghost-layer contributions into the next halo buffer: its own outer faces
 // (bcx.f90), the facing layer of same-level neighbours, F2C means into a coarser neighbour's

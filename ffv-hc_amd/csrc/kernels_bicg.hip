@@ -17,25 +17,6 @@ namespace ffvhc {
 constexpr int BI_NT = 256;
 
 template <class T>
-__device__ inline void stage(T* s, const T* __restrict__ x, const T* __restrict__ h, int slot) {
-  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
-  const size_t base = (size_t)slot * NC;
-  for (int k = 0; k < NB; k++) s[lds(tx, ty, k)] = x[base + tx + NB * (ty + NB * k)];
-  for (int q = t; q < NH; q += BI_NT) {
-    int f = q >> 8, p = q & 255, i, j, kk;
-    face_cell(f, p & 15, p >> 4, -1, i, j, kk);
-    s[lds(i, j, kk)] = h[(size_t)slot * NH + q];
-  }
-}
-
-template <class T>
-__device__ inline T matvec(const Coef7<T>& A, size_t g, const T* s, int li) {
-  // calc_ax order (bils.f90:195-202)
-  return A.p[1][g] * s[li - 1] + A.p[2][g] * s[li + 1] + A.p[3][g] * s[li - LP] + A.p[4][g] * s[li + LP] +
-         A.p[5][g] * s[li - LP * LP] + A.p[6][g] * s[li + LP * LP] + A.p[0][g] * s[li];
-}
-
-template <class T>
 __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
                                                         const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                         Coef7<T> A, const T* __restrict__ bv, const T* __restrict__ x,
@@ -47,22 +28,36 @@ __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ 
   const int slot = active[blockIdx.x];
   const int c = blk[slot].case_id;
   if (st[c].done) return;
-  stage<T>(s, x, hx, slot);
-  __syncthreads();
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
+  constexpr int KB = Batch<T>::K;
   double bb = 0.0, rr1 = 0.0;
-  for (int k = 0; k < NB; k++) {
-    const size_t g = base + tx + NB * (ty + NB * k);
-    const int li = lds(tx, ty, k);
-    const T b0 = bv[g];
-    const T r = b0 - A.p[1][g] * s[li - 1] - A.p[2][g] * s[li + 1] - A.p[3][g] * s[li - LP] -
-                A.p[4][g] * s[li + LP] - A.p[5][g] * s[li - LP * LP] - A.p[6][g] * s[li + LP * LP] -
-                A.p[0][g] * s[li];
-    V.r[g] = r;
-    V.r0[g] = r;
-    bb += (double)b0 * (double)b0;
-    rr1 += (double)r * (double)r;
+#pragma unroll
+  for (int k0 = 0; k0 < NB; k0 += KB) {
+    T a[KB][7], b0[KB];
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+#pragma unroll
+      for (int q = 0; q < 7; q++) a[kk][q] = A.p[q][g];
+      b0[kk] = bv[g];
+    }
+    if (k0 == 0) {
+      stage_block<T, BI_NT>(s, x, hx, slot);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      const int li = lds(tx, ty, k0 + kk);
+      const T r = b0[kk] - a[kk][1] * s[li - 1] - a[kk][2] * s[li + 1] - a[kk][3] * s[li - LP] -
+                  a[kk][4] * s[li + LP] - a[kk][5] * s[li - LP * LP] - a[kk][6] * s[li + LP * LP] -
+                  a[kk][0] * s[li];
+      V.r[g] = r;
+      V.r0[g] = r;
+      bb += (double)b0[kk] * (double)b0[kk];
+      rr1 += (double)r * (double)r;
+    }
   }
   double sb = block_sum<BI_NT>(bb, red);
   double sr = block_sum<BI_NT>(rr1, red);
@@ -110,28 +105,40 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
   const T omega = (T)st[c].omega;
-  for (int k = 0; k < NB; k++) {
-    const size_t g = base + tx + NB * (ty + NB * k);
-    T v;
-    if (MODE == 0) {
-      if (count == 1) {
-        v = V.r[g];
-      } else {
-        const T ng = (T)(-st[c].gamma);
-        const T beta = (T)st[c].beta;
-        const T tmp = V.p[g] + ng * V.q[g];  // AXPY(p, q_, -gamma)
-        v = V.r[g] + beta * tmp;              // XPAY(p, r, beta)
-      }
-      V.p[g] = v;
-    } else {
-      const T na = (T)(-st[c].alpha);
-      v = na * V.q[g] + V.r[g];  // AXPYZ(s, q_, r, -alpha)
-      V.s[g] = v;
+  const T ng = (T)(-st[c].gamma), beta = (T)st[c].beta, na = (T)(-st[c].alpha);
+  constexpr int KB = Batch<T>::K;
+#pragma unroll
+  for (int k0 = 0; k0 < NB; k0 += KB) {
+    T rv[KB], pv[KB], qv[KB], ap[KB];
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      rv[kk] = V.r[g];
+      ap[kk] = A.p[0][g];
+      if (MODE == 1 || count != 1) qv[kk] = V.q[g];
+      if (MODE == 0 && count != 1) pv[kk] = V.p[g];
     }
-    const T m = omega * (v / A.p[0][g]);
-    if (MODE == 0) V.pp[g] = m;
-    else V.ss[g] = m;
-    s[lds(tx, ty, k)] = m;
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      T v;
+      if (MODE == 0) {
+        if (count == 1) {
+          v = rv[kk];
+        } else {
+          const T tmp = pv[kk] + ng * qv[kk];  // AXPY(p, q_, -gamma)
+          v = rv[kk] + beta * tmp;             // XPAY(p, r, beta)
+        }
+        V.p[g] = v;
+      } else {
+        v = na * qv[kk] + rv[kk];  // AXPYZ(s, q_, r, -alpha)
+        V.s[g] = v;
+      }
+      const T m = omega * (v / ap[kk]);
+      if (MODE == 0) V.pp[g] = m;
+      else V.ss[g] = m;
+      s[lds(tx, ty, k0 + kk)] = m;
+    }
   }
   __syncthreads();
   T* h = MODE == 0 ? V.hpp : V.hss;
@@ -151,21 +158,36 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
   const int slot = active[blockIdx.x];
   const int c = blk[slot].case_id;
   if (st[c].done) return;
-  stage<T>(s, MODE == 0 ? V.pp : V.ss, MODE == 0 ? V.hpp : V.hss, slot);
-  __syncthreads();
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
+  const T* __restrict__ w = MODE == 0 ? V.r0 : V.s;
+  T* __restrict__ y = MODE == 0 ? V.q : V.t;
+  constexpr int KB = Batch<T>::K;
   double d0 = 0.0, d1 = 0.0;
-  for (int k = 0; k < NB; k++) {
-    const size_t g = base + tx + NB * (ty + NB * k);
-    const T y = matvec<T>(A, g, s, lds(tx, ty, k));
-    if (MODE == 0) {
-      V.q[g] = y;
-      d0 += (double)y * (double)V.r0[g];
-    } else {
-      V.t[g] = y;
-      d0 += (double)y * (double)V.s[g];
-      d1 += (double)y * (double)y;
+#pragma unroll
+  for (int k0 = 0; k0 < NB; k0 += KB) {
+    T a[KB][7], wv[KB];
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+#pragma unroll
+      for (int q = 0; q < 7; q++) a[kk][q] = A.p[q][g];
+      wv[kk] = w[g];
+    }
+    if (k0 == 0) {
+      stage_block<T, BI_NT>(s, MODE == 0 ? V.pp : V.ss, MODE == 0 ? V.hpp : V.hss, slot);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      const int li = lds(tx, ty, k0 + kk);
+      // calc_ax order (bils.f90:195-202)
+      const T v = a[kk][1] * s[li - 1] + a[kk][2] * s[li + 1] + a[kk][3] * s[li - LP] + a[kk][4] * s[li + LP] +
+                  a[kk][5] * s[li - LP * LP] + a[kk][6] * s[li + LP * LP] + a[kk][0] * s[li];
+      y[g] = v;
+      d0 += (double)v * (double)wv[kk];
+      if (MODE == 1) d1 += (double)v * (double)v;
     }
   }
   double s0 = block_sum<BI_NT>(d0, red);
@@ -199,14 +221,30 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
   const T alpha = (T)st[c].alpha, gamma = (T)st[c].gamma, ng = (T)(-st[c].gamma);
+  constexpr int KB = Batch<T>::K;
   double rr = 0.0, rr1 = 0.0;
-  for (int k = 0; k < NB; k++) {
-    const size_t g = base + tx + NB * (ty + NB * k);
-    x[g] = alpha * V.pp[g] + gamma * V.ss[g] + x[g];  // AXPBYPZ(x, p_, s_, alpha, gamma)
-    const T r = ng * V.t[g] + V.s[g];                 // AXPYZ(r, t_, s, -gamma)
-    V.r[g] = r;
-    rr += (double)r * (double)r;
-    rr1 += (double)r * (double)V.r0[g];
+#pragma unroll
+  for (int k0 = 0; k0 < NB; k0 += KB) {
+    T pv[KB], sv_[KB], xv[KB], tv[KB], sv[KB], r0v[KB];
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      pv[kk] = V.pp[g];
+      sv_[kk] = V.ss[g];
+      xv[kk] = x[g];
+      tv[kk] = V.t[g];
+      sv[kk] = V.s[g];
+      r0v[kk] = V.r0[g];
+    }
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      x[g] = alpha * pv[kk] + gamma * sv_[kk] + xv[kk];  // AXPBYPZ(x, p_, s_, alpha, gamma)
+      const T r = ng * tv[kk] + sv[kk];                  // AXPYZ(r, t_, s, -gamma)
+      V.r[g] = r;
+      rr += (double)r * (double)r;
+      rr1 += (double)r * (double)r0v[kk];
+    }
   }
   double s0 = block_sum<BI_NT>(rr, red);
   double s1 = block_sum<BI_NT>(rr1, red);

@@ -36,28 +36,35 @@ __global__ __launch_bounds__(CG_NT) void cg_init_kernel(const int* __restrict__ 
   if (st[c].done) return;
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
-  for (int k = 0; k < NB; k++) s[lds(tx, ty, k)] = x[base + tx + NB * (ty + NB * k)];
-  for (int q = t; q < NH; q += CG_NT) {
-    int f = q >> 8, p = q & 255, i, j, kk;
-    face_cell(f, p & 15, p >> 4, -1, i, j, kk);
-    s[lds(i, j, kk)] = hx[(size_t)slot * NH + q];
-  }
-  __syncthreads();
   const T omega = (T)st[c].omega;
+  constexpr int KB = Batch<T>::K;
   double bb = 0.0, rz = 0.0;
-  for (int k = 0; k < NB; k++) {
-    const size_t g = base + tx + NB * (ty + NB * k);
-    const int li = lds(tx, ty, k);
-    T a[7];
 #pragma unroll
-    for (int q = 0; q < 7; q++) a[q] = A.p[q][g];
-    const T b0 = bv[g];
-    const T rr = b0 - a[1] * s[li - 1] - a[2] * s[li + 1] - a[3] * s[li - LP] - a[4] * s[li + LP] -
-                 a[5] * s[li - LP * LP] - a[6] * s[li + LP * LP] - a[0] * s[li];
-    r[g] = rr;
-    const T z = jacobi_z<T>(rr, a[0], omega);
-    bb += (double)b0 * (double)b0;
-    rz += (double)rr * (double)z;
+  for (int k0 = 0; k0 < NB; k0 += KB) {
+    T a[KB][7], b0[KB];
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+#pragma unroll
+      for (int q = 0; q < 7; q++) a[kk][q] = A.p[q][g];
+      b0[kk] = bv[g];
+    }
+    if (k0 == 0) {
+      stage_block<T, CG_NT>(s, x, hx, slot);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      const int li = lds(tx, ty, k0 + kk);
+      const T rr = b0[kk] - a[kk][1] * s[li - 1] - a[kk][2] * s[li + 1] - a[kk][3] * s[li - LP] -
+                   a[kk][4] * s[li + LP] - a[kk][5] * s[li - LP * LP] - a[kk][6] * s[li + LP * LP] -
+                   a[kk][0] * s[li];
+      r[g] = rr;
+      const T z = jacobi_z<T>(rr, a[kk][0], omega);
+      bb += (double)b0[kk] * (double)b0[kk];
+      rz += (double)rr * (double)z;
+    }
   }
   double sb = block_sum<CG_NT>(bb, red);
   double sz = block_sum<CG_NT>(rz, red);
@@ -103,12 +110,25 @@ __global__ __launch_bounds__(CG_NT) void cg_p_kernel(const int* __restrict__ act
   const size_t base = (size_t)slot * NC;
   const T omega = (T)st[c].omega;
   const T beta = (T)st[c].beta;
-  for (int k = 0; k < NB; k++) {
-    const size_t g = base + tx + NB * (ty + NB * k);
-    const T z = jacobi_z<T>(r[g], A.p[0][g], omega);
-    const T pn = (count == 1) ? z : z + beta * p[g];
-    p[g] = pn;
-    s[lds(tx, ty, k)] = pn;
+  constexpr int KB = Batch<T>::K;
+#pragma unroll
+  for (int k0 = 0; k0 < NB; k0 += KB) {
+    T rv[KB], ap[KB], pv[KB];
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      rv[kk] = r[g];
+      ap[kk] = A.p[0][g];
+      if (count != 1) pv[kk] = p[g];
+    }
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      const T z = jacobi_z<T>(rv[kk], ap[kk], omega);
+      const T pn = (count == 1) ? z : z + beta * pv[kk];
+      p[g] = pn;
+      s[lds(tx, ty, k0 + kk)] = pn;
+    }
   }
   __syncthreads();
   push_halo<T, CG_NT>(B, cs[c], bcg, slot, [&](int i, int j, int k) { return s[lds(i, j, k)]; }, hp, hp);
@@ -128,25 +148,31 @@ __global__ __launch_bounds__(CG_NT) void cg_q_kernel(const int* __restrict__ act
   if (st[c].done) return;
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
-  for (int k = 0; k < NB; k++) s[lds(tx, ty, k)] = p[base + tx + NB * (ty + NB * k)];
-  for (int w = t; w < NH; w += CG_NT) {
-    int f = w >> 8, pp = w & 255, i, j, kk;
-    face_cell(f, pp & 15, pp >> 4, -1, i, j, kk);
-    s[lds(i, j, kk)] = hp[(size_t)slot * NH + w];
-  }
-  __syncthreads();
+  constexpr int KB = Batch<T>::K;
   double qp = 0.0;
-  for (int k = 0; k < NB; k++) {
-    const size_t g = base + tx + NB * (ty + NB * k);
-    const int li = lds(tx, ty, k);
-    T a[7];
 #pragma unroll
-    for (int w = 0; w < 7; w++) a[w] = A.p[w][g];
-    const T pv = s[li];
-    const T qv = a[1] * s[li - 1] + a[2] * s[li + 1] + a[3] * s[li - LP] + a[4] * s[li + LP] +
-                 a[5] * s[li - LP * LP] + a[6] * s[li + LP * LP] + a[0] * pv;
-    q[g] = qv;
-    qp += (double)qv * (double)pv;
+  for (int k0 = 0; k0 < NB; k0 += KB) {
+    T a[KB][7];
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+#pragma unroll
+      for (int w = 0; w < 7; w++) a[kk][w] = A.p[w][g];
+    }
+    if (k0 == 0) {
+      stage_block<T, CG_NT>(s, p, hp, slot);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      const int li = lds(tx, ty, k0 + kk);
+      const T pv = s[li];
+      const T qv = a[kk][1] * s[li - 1] + a[kk][2] * s[li + 1] + a[kk][3] * s[li - LP] + a[kk][4] * s[li + LP] +
+                   a[kk][5] * s[li - LP * LP] + a[kk][6] * s[li + LP * LP] + a[kk][0] * pv;
+      q[g] = qv;
+      qp += (double)qv * (double)pv;
+    }
   }
   double sq = block_sum<CG_NT>(qp, red);
   if (t == 0) part[slot] = sq;
@@ -177,15 +203,30 @@ __global__ __launch_bounds__(CG_NT) void cg_xr_kernel(const int* __restrict__ ac
   const T alpha = (T)st[c].alpha;
   const T nalpha = -alpha;
   const T omega = (T)st[c].omega;
+  constexpr int KB = Batch<T>::K;
   double rr = 0.0, rz = 0.0;
-  for (int k = 0; k < NB; k++) {
-    const size_t g = base + tx + NB * (ty + NB * k);
-    x[g] = x[g] + alpha * p[g];
-    const T rn = r[g] + nalpha * q[g];
-    r[g] = rn;
-    const T z = jacobi_z<T>(rn, A.p[0][g], omega);
-    rr += (double)rn * (double)rn;
-    rz += (double)rn * (double)z;
+#pragma unroll
+  for (int k0 = 0; k0 < NB; k0 += KB) {
+    T xv[KB], pv[KB], rv[KB], qv[KB], ap[KB];
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      xv[kk] = x[g];
+      pv[kk] = p[g];
+      rv[kk] = r[g];
+      qv[kk] = q[g];
+      ap[kk] = A.p[0][g];
+    }
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      x[g] = xv[kk] + alpha * pv[kk];
+      const T rn = rv[kk] + nalpha * qv[kk];
+      r[g] = rn;
+      const T z = jacobi_z<T>(rn, ap[kk], omega);
+      rr += (double)rn * (double)rn;
+      rz += (double)rn * (double)z;
+    }
   }
   double sr = block_sum<CG_NT>(rr, red);
   double sz = block_sum<CG_NT>(rz, red);
@@ -278,26 +319,33 @@ __global__ __launch_bounds__(CG_NT) void defect_kernel(const int* __restrict__ a
   if (st[c].done) return;
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
-  for (int k = 0; k < NB; k++) s[lds(tx, ty, k)] = x[base + tx + NB * (ty + NB * k)];
-  for (int q = t; q < NH; q += CG_NT) {
-    int f = q >> 8, p = q & 255, i, j, kk;
-    face_cell(f, p & 15, p >> 4, -1, i, j, kk);
-    s[lds(i, j, kk)] = hx[(size_t)slot * NH + q];
-  }
-  __syncthreads();
+  constexpr int KB = Batch<double>::K;
   double rr = 0.0, bb = 0.0;
-  for (int k = 0; k < NB; k++) {
-    const size_t g = base + tx + NB * (ty + NB * k);
-    const int li = lds(tx, ty, k);
-    double a[7];
 #pragma unroll
-    for (int q = 0; q < 7; q++) a[q] = A.p[q][g];
-    const double b0 = bv[g];
-    const double r = b0 - a[1] * s[li - 1] - a[2] * s[li + 1] - a[3] * s[li - LP] - a[4] * s[li + LP] -
-                     a[5] * s[li - LP * LP] - a[6] * s[li + LP * LP] - a[0] * s[li];
-    r32[g] = (float)r;
-    rr += r * r;
-    if (outer == 0) bb += b0 * b0;
+  for (int k0 = 0; k0 < NB; k0 += KB) {
+    double a[KB][7], b0[KB];
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+#pragma unroll
+      for (int q = 0; q < 7; q++) a[kk][q] = A.p[q][g];
+      b0[kk] = bv[g];
+    }
+    if (k0 == 0) {
+      stage_block<double, CG_NT>(s, x, hx, slot);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      const int li = lds(tx, ty, k0 + kk);
+      const double r = b0[kk] - a[kk][1] * s[li - 1] - a[kk][2] * s[li + 1] - a[kk][3] * s[li - LP] -
+                       a[kk][4] * s[li + LP] - a[kk][5] * s[li - LP * LP] - a[kk][6] * s[li + LP * LP] -
+                       a[kk][0] * s[li];
+      r32[g] = (float)r;
+      rr += r * r;
+      if (outer == 0) bb += b0[kk] * b0[kk];
+    }
   }
   double sr = block_sum<CG_NT>(rr, red);
   double sb = outer == 0 ? block_sum<CG_NT>(bb, red) : 0.0;
