@@ -175,7 +175,7 @@ def main():
     for tag, f, w in (("64", f64, 8), ("32", f32, 4)):
         ctx.fill(f["x"], 0.0)
         ctx.bicgstab_timing(f["x"], f["A"], f["b"], 1.0, 5)
-        ms = ctx.bicgstab_timing(f["x"], f["A"], f["b"], 1.0, nit) / nit
+        ms = ctx.bicgstab_timing(f["x"], f["A"], f["b"], 1.0, nit)[:5] / nit
         ctx.fill(f["x"], 0.0)
         ctx.rbgs_fixed(f["x"], f["A"], f["b"], P, 1.2, 5)
         mr = ctx.rbgs_fixed(f["x"], f["A"], f["b"], P, 1.2, nit) / nit

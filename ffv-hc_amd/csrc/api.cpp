@@ -38,15 +38,21 @@ std::vector<int> selected_cases(ffvhc_ctx* ctx) {
 int refresh_active(ffvhc_ctx* ctx) {
   if (!ctx->dirty) return 0;
   ctx->hactive.clear();
+  ctx->hcaseact.clear();
   ctx->multilevel = false;
   for (int c : selected_cases(ctx)) {
     const CaseRec& r = ctx->cases[c];
     for (int s = 0; s < r.nblk; s++) ctx->hactive.push_back(r.slot_begin + s);
+    if (r.nblk > 0) ctx->hcaseact.push_back(c);
     if (r.max_level > 0) ctx->multilevel = true;
   }
   ctx->nactive = (int)ctx->hactive.size();
+  ctx->ncaseact = (int)ctx->hcaseact.size();
   if (ctx->nactive)
     HIPCHK(hipMemcpyAsync(ctx->dactive, ctx->hactive.data(), sizeof(int) * ctx->nactive, hipMemcpyHostToDevice,
+                          ctx->sm));
+  if (ctx->ncaseact)
+    HIPCHK(hipMemcpyAsync(ctx->dcaseact, ctx->hcaseact.data(), sizeof(int) * ctx->ncaseact, hipMemcpyHostToDevice,
                           ctx->sm));
   HIPCHK(hipStreamSynchronize(ctx->sm));
   ctx->dirty = false;
@@ -198,6 +204,7 @@ int ffvhc_ctx_create(int device, long long max_blocks, ffvhc_ctx** out) {
   HIPCHK(hipMalloc(&c->dblk, sizeof(DevBlock) * max_blocks));
   HIPCHK(hipMalloc(&c->dcase, sizeof(DevCase) * MAX_CASES));
   HIPCHK(hipMalloc(&c->dactive, sizeof(int) * max_blocks));
+  HIPCHK(hipMalloc(&c->dcaseact, sizeof(int) * MAX_CASES));
   HIPCHK(hipMalloc(&c->dst, sizeof(SolveState) * MAX_CASES));
   HIPCHK(hipMalloc(&c->dst2, sizeof(SolveState) * MAX_CASES));
   HIPCHK(hipMalloc(&c->dpart, sizeof(double) * 4 * max_blocks));
@@ -224,6 +231,7 @@ void ffvhc_ctx_destroy(ffvhc_ctx* c) {
   (void)hipFree(c->dblk);
   (void)hipFree(c->dcase);
   (void)hipFree(c->dactive);
+  (void)hipFree(c->dcaseact);
   (void)hipFree(c->dst);
   (void)hipFree(c->dst2);
   (void)hipFree(c->dpart);

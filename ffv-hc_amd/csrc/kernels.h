@@ -7,6 +7,21 @@
 
 namespace ffvhc {
 
+// scalar steps applied per case by launch_finalize after a reducing kernel
+enum FinalOp {
+  FIN_BI_INIT = 0,
+  FIN_BI_ALPHA,
+  FIN_BI_GAMMA,
+  FIN_BI_XR,
+  FIN_CG_INIT,
+  FIN_CG_ALPHA,
+  FIN_CG_XR,
+  FIN_RBGS,
+  FIN_DEFECT
+};
+void launch_finalize(hipStream_t sm, const int* caseact, int ncase, const DevCase* cs, SolveState* st,
+                     const double* part, size_t nslots, int op, int arg);
+
 template <class T>
 void launch_ghost_pull(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                        const T* x, T* h, int bcg, int updater, bool multilevel);
@@ -42,9 +57,9 @@ void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const Dev
                           SolveState* st, Coef7<T> A, const T* b, const T* x, const T* hx, BiVecs<T> V, double* part,
                           size_t nslots);
 template <class T>
-void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
-                          SolveState* st, Coef7<T> A, T* x, BiVecs<T> V, double* part, size_t nslots, int count,
-                          int mask = 31);
+void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase,
+                          const DevBlock* blk, const DevCase* cs, SolveState* st, Coef7<T> A, T* x, BiVecs<T> V,
+                          double* part, size_t nslots, int count, int mask = 31);
 void launch_defect(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                    SolveState* st, Coef7<double> A, const double* b, const double* x, const double* hx, float* r32,
                    double* part, size_t nslots, int outer);

@@ -24,7 +24,6 @@ __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ 
                                                         double* __restrict__ part, size_t nslots) {
   __shared__ T s[LP * LP * LP];
   __shared__ double red[BI_NT / 64];
-  __shared__ int flag;
   const int slot = active[blockIdx.x];
   const int c = blk[slot].case_id;
   if (st[c].done) return;
@@ -64,29 +63,6 @@ __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ 
   if (t == 0) {
     part[slot] = sb;
     part[nslots + slot] = sr;
-  }
-  const DevCase& C = cs[c];
-  if (!last_of_case(st, c, C.nblk, &flag)) return;
-  double tb = case_sum<BI_NT>(part, C.slot_begin, C.nblk, red);
-  double tr = case_sum<BI_NT>(part + nslots, C.slot_begin, C.nblk, red);
-  if (t == 0) {
-    SolveState& S = st[c];
-    S.ticket = 0;
-    S.bb = tb;
-    S.rr0 = 1.0;
-    S.alpha = 0.0;
-    S.gamma = 1.0;
-    if (fabs(tb) < FLT_MIN) {
-      S.residual = 0.0;
-      S.count = 0;
-      S.done = 1;
-    } else if (fabs(tr) < FLT_MIN) {
-      S.residual = tr;
-      S.count = 0;
-      S.done = 1;
-    } else {
-      S.rr1 = tr;
-    }
   }
 }
 
@@ -154,7 +130,6 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
                                                           size_t nslots) {
   __shared__ T s[LP * LP * LP];
   __shared__ double red[BI_NT / 64];
-  __shared__ int flag;
   const int slot = active[blockIdx.x];
   const int c = blk[slot].case_id;
   if (st[c].done) return;
@@ -196,16 +171,6 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
     part[slot] = s0;
     part[nslots + slot] = s1;
   }
-  const DevCase& C = cs[c];
-  if (!last_of_case(st, c, C.nblk, &flag)) return;
-  double t0 = case_sum<BI_NT>(part, C.slot_begin, C.nblk, red);
-  double t1 = MODE == 1 ? case_sum<BI_NT>(part + nslots, C.slot_begin, C.nblk, red) : 0.0;
-  if (t == 0) {
-    SolveState& S = st[c];
-    S.ticket = 0;
-    if (MODE == 0) S.alpha = S.rr1 / t0;
-    else S.gamma = t0 / t1;
-  }
 }
 
 template <class T>
@@ -214,7 +179,6 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
                                                       T* __restrict__ x, BiVecs<T> V, double* __restrict__ part,
                                                       size_t nslots, int count) {
   __shared__ double red[BI_NT / 64];
-  __shared__ int flag;
   const int slot = active[blockIdx.x];
   const int c = blk[slot].case_id;
   if (st[c].done) return;
@@ -252,27 +216,6 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
     part[slot] = s0;
     part[nslots + slot] = s1;
   }
-  const DevCase& C = cs[c];
-  if (!last_of_case(st, c, C.nblk, &flag)) return;
-  double tr = case_sum<BI_NT>(part, C.slot_begin, C.nblk, red);
-  double t1 = case_sum<BI_NT>(part + nslots, C.slot_begin, C.nblk, red);
-  if (t == 0) {
-    SolveState& S = st[c];
-    S.ticket = 0;
-    S.iter = count;
-    converged(S, tr, count);
-    if (!S.done) {
-      S.rr0 = S.rr1;
-      if (fabs(t1) < FLT_MIN) {  // next iteration's r.r0 test (FFVILS.h:829-834)
-        S.residual = t1;
-        S.count = 0;
-        S.done = 1;
-      } else {
-        S.rr1 = t1;
-        S.beta = t1 / S.rr0 * S.alpha / S.gamma;
-      }
-    }
-  }
 }
 
 template <class T>
@@ -284,30 +227,38 @@ void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const Dev
 }
 
 template <class T>
-void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
-                          SolveState* st, Coef7<T> A, T* x, BiVecs<T> V, double* part, size_t nslots, int count,
-                          int mask) {
+void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase,
+                          const DevBlock* blk, const DevCase* cs, SolveState* st, Coef7<T> A, T* x, BiVecs<T> V,
+                          double* part, size_t nslots, int count, int mask) {
+  // mask bits 0-4 select P, Q, S, T, XR; bit 5 leaves out the per-case finalisations
   if (!nact) return;
   if (mask & 1)
     hipLaunchKernelGGL((bi_precond_kernel<T, 0>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, count);
-  if (mask & 2)
+  if (mask & 2) {
     hipLaunchKernelGGL((bi_matvec_kernel<T, 0>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, part,
                        nslots);
+    if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_ALPHA, count);
+  }
   if (mask & 4)
     hipLaunchKernelGGL((bi_precond_kernel<T, 1>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, count);
-  if (mask & 8)
+  if (mask & 8) {
     hipLaunchKernelGGL((bi_matvec_kernel<T, 1>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, part,
                        nslots);
-  if (mask & 16)
+    if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_GAMMA, count);
+  }
+  if (mask & 16) {
     hipLaunchKernelGGL(bi_xr_kernel<T>, dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part, nslots,
                        count);
+    if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_XR, count);
+  }
 }
 
 #define INST(T)                                                                                                     \
   template void launch_bicgstab_init<T>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,  \
                                         Coef7<T>, const T*, const T*, const T*, BiVecs<T>, double*, size_t);         \
-  template void launch_bicgstab_iter<T>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,  \
-                                        Coef7<T>, T*, BiVecs<T>, double*, size_t, int, int);
+  template void launch_bicgstab_iter<T>(hipStream_t, const int*, int, const int*, int, const DevBlock*,            \
+                                        const DevCase*, SolveState*, Coef7<T>, T*, BiVecs<T>, double*, size_t, int,  \
+                                        int);
 INST(double)
 INST(float)
 #undef INST

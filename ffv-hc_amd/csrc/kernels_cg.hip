@@ -29,7 +29,6 @@ __global__ __launch_bounds__(CG_NT) void cg_init_kernel(const int* __restrict__ 
                                                         double* __restrict__ part, size_t nslots) {
   __shared__ T s[LP * LP * LP];
   __shared__ double red[CG_NT / 64];
-  __shared__ int flag;
   const int slot = active[blockIdx.x];
   const DevBlock& B = blk[slot];
   const int c = B.case_id;
@@ -71,28 +70,6 @@ __global__ __launch_bounds__(CG_NT) void cg_init_kernel(const int* __restrict__ 
   if (t == 0) {
     part[slot] = sb;
     part[nslots + slot] = sz;
-  }
-  const DevCase& C = cs[c];
-  if (!last_of_case(st, c, C.nblk, &flag)) return;
-  double tb = case_sum<CG_NT>(part, C.slot_begin, C.nblk, red);
-  double tz = case_sum<CG_NT>(part + nslots, C.slot_begin, C.nblk, red);
-  if (t == 0) {
-    SolveState& S = st[c];
-    S.ticket = 0;
-    S.bb = tb;
-    S.rr0 = 1.0;
-    if (fabs(tb) < FLT_MIN) {  // FFVILS.h:683-687
-      S.residual = 0.0;
-      S.count = 0;
-      S.done = 1;
-    } else if (fabs(tz) < FLT_MIN) {  // FFVILS.h:718-722
-      S.residual = tz;
-      S.count = 0;
-      S.done = 1;
-    } else {
-      S.rr1 = tz;
-      S.beta = tz / S.rr0;
-    }
   }
 }
 
@@ -141,7 +118,6 @@ __global__ __launch_bounds__(CG_NT) void cg_q_kernel(const int* __restrict__ act
                                                      T* __restrict__ q, double* __restrict__ part) {
   __shared__ T s[LP * LP * LP];
   __shared__ double red[CG_NT / 64];
-  __shared__ int flag;
   const int slot = active[blockIdx.x];
   const DevBlock& B = blk[slot];
   const int c = B.case_id;
@@ -176,14 +152,6 @@ __global__ __launch_bounds__(CG_NT) void cg_q_kernel(const int* __restrict__ act
   }
   double sq = block_sum<CG_NT>(qp, red);
   if (t == 0) part[slot] = sq;
-  const DevCase& C = cs[c];
-  if (!last_of_case(st, c, C.nblk, &flag)) return;
-  double tq = case_sum<CG_NT>(part, C.slot_begin, C.nblk, red);
-  if (t == 0) {
-    SolveState& S = st[c];
-    S.ticket = 0;
-    S.alpha = S.rr1 / tq;
-  }
 }
 
 template <class T>
@@ -193,7 +161,6 @@ __global__ __launch_bounds__(CG_NT) void cg_xr_kernel(const int* __restrict__ ac
                                                       const T* __restrict__ p, const T* __restrict__ q,
                                                       double* __restrict__ part, size_t nslots, int count) {
   __shared__ double red[CG_NT / 64];
-  __shared__ int flag;
   const int slot = active[blockIdx.x];
   const DevBlock& B = blk[slot];
   const int c = B.case_id;
@@ -233,27 +200,6 @@ __global__ __launch_bounds__(CG_NT) void cg_xr_kernel(const int* __restrict__ ac
   if (t == 0) {
     part[slot] = sr;
     part[nslots + slot] = sz;
-  }
-  const DevCase& C = cs[c];
-  if (!last_of_case(st, c, C.nblk, &flag)) return;
-  double tr = case_sum<CG_NT>(part, C.slot_begin, C.nblk, red);
-  double tz = case_sum<CG_NT>(part + nslots, C.slot_begin, C.nblk, red);
-  if (t == 0) {
-    SolveState& S = st[c];
-    S.ticket = 0;
-    S.iter = count;
-    converged(S, tr, count);
-    if (!S.done) {
-      S.rr0 = S.rr1;
-      if (fabs(tz) < FLT_MIN) {
-        S.residual = tz;
-        S.count = 0;
-        S.done = 1;
-      } else {
-        S.rr1 = tz;
-        S.beta = tz / S.rr0;
-      }
-    }
   }
 }
 
@@ -312,7 +258,6 @@ __global__ __launch_bounds__(CG_NT) void defect_kernel(const int* __restrict__ a
                                                        size_t nslots, int outer) {
   __shared__ double s[LP * LP * LP];
   __shared__ double red[CG_NT / 64];
-  __shared__ int flag;
   const int slot = active[blockIdx.x];
   const DevBlock& B = blk[slot];
   const int c = B.case_id;
@@ -352,17 +297,6 @@ __global__ __launch_bounds__(CG_NT) void defect_kernel(const int* __restrict__ a
   if (t == 0) {
     part[slot] = sr;
     part[nslots + slot] = sb;
-  }
-  const DevCase& C = cs[c];
-  if (!last_of_case(st, c, C.nblk, &flag)) return;
-  double tr = case_sum<CG_NT>(part, C.slot_begin, C.nblk, red);
-  double tb = outer == 0 ? case_sum<CG_NT>(part + nslots, C.slot_begin, C.nblk, red) : 0.0;
-  if (t == 0) {
-    SolveState& S = st[c];
-    S.ticket = 0;
-    if (outer == 0) S.bb = tb;
-    S.iter = outer;
-    converged(S, tr, outer);
   }
 }
 

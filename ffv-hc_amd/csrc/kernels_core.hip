@@ -94,7 +94,6 @@ __global__ __launch_bounds__(RB_NT) void rbgs_iter_kernel(const int* __restrict_
                                                           size_t nslots, int n, int bcg) {
   __shared__ T s[LP * LP * LP];
   __shared__ double red[RB_NT / 64];
-  __shared__ int flag;
   const int slot = active[blockIdx.x];
   const DevBlock& B = blk[slot];
   const int c = B.case_id;
@@ -174,22 +173,6 @@ __global__ __launch_bounds__(RB_NT) void rbgs_iter_kernel(const int* __restrict_
   if (t == 0) {
     part[slot] = rsum;
     part[nslots + slot] = bsum;
-  }
-  const DevCase& C = cs[c];
-  if (!last_of_case(st, c, C.nblk, &flag)) return;
-  double tr = case_sum<RB_NT>(part, C.slot_begin, C.nblk, red);
-  double tb = (n == 1) ? case_sum<RB_NT>(part + nslots, C.slot_begin, C.nblk, red) : 0.0;
-  if (t == 0) {
-    SolveState& S = st[c];
-    S.ticket = 0;
-    S.iter = n;
-    if (n == 1) {
-      S.bb = tb;
-      if (S.countMax < 1) S.done = 1;
-    } else {
-      converged(S, tr, n - 1);
-      S.parity = (n - 1) & 1;
-    }
   }
 }
 

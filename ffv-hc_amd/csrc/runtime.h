@@ -65,6 +65,9 @@ struct ffvhc_ctx {
   std::vector<int> hactive;
   int* dactive = nullptr;
   int nactive = 0;
+  std::vector<int> hcaseact;  // selected case ids with at least one block
+  int* dcaseact = nullptr;
+  int ncaseact = 0;
   bool multilevel = false;
   bool dirty = true;
   // solver state

@@ -43,6 +43,11 @@ int check_args(ffvhc_ctx* c, int x, const int* A, int b, int prec) {
   return 0;
 }
 
+// per-case scalar step after a reducing kernel (sums the per-block partials in slot order)
+inline void fin(ffvhc_ctx* c, SolveState* st, int op, int arg) {
+  launch_finalize(c->sm, c->dcaseact, c->ncaseact, c->dcase, st, c->dpart, (size_t)c->cap, op, arg);
+}
+
 // RBGS on field x (ping-pong), matrix K, rhs bv; state in st. Result left in buffer 0.
 template <class T>
 int rbgs_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double omega, int countMax, double eps,
@@ -59,8 +64,11 @@ int rbgs_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double ome
   while (!done) {
     int last = std::min(n + batch - 1, countMax + 1);
     for (; n <= last; n++)
+    {
       launch_rbgs_iter<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, xs[(n - 1) & 1], xs[n & 1],
                           hs[(n - 1) & 1], hs[n & 1], c->dpart, (size_t)c->cap, n, bcg);
+      fin(c, st, FIN_RBGS, n);
+    }
     HIPCHK(hipGetLastError());
     int rc = read_state(c, st, hst, ids, &done);
     if (rc) return rc;
@@ -89,6 +97,7 @@ int cg_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double omega
   launch_solve_init(c->sm, st, (int)c->cases.size(), omega, eps, countMax, outer);
   launch_cg_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, (const T*)X.d[0],
                     (const T*)X.h[0], (T*)Rf.d[0], c->dpart, (size_t)c->cap);
+  fin(c, st, FIN_CG_INIT, 0);
   int n = 1, batch = kBatchMin;
   bool done = false;
   while (!done) {
@@ -98,8 +107,10 @@ int cg_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double omega
                      (T*)Pf.h[0], n, BC_NULL);
       launch_cg_q<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, (const T*)Pf.d[0], (const T*)Pf.h[0],
                      (T*)Qf.d[0], c->dpart);
+      fin(c, st, FIN_CG_ALPHA, n);
       launch_cg_xr<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, (T*)X.d[0], (T*)Rf.d[0],
                       (const T*)Pf.d[0], (const T*)Qf.d[0], c->dpart, (size_t)c->cap, n);
+      fin(c, st, FIN_CG_XR, n);
     }
     HIPCHK(hipGetLastError());
     rc = read_state(c, st, hst, ids, &done);
@@ -142,13 +153,14 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
   launch_solve_init(c->sm, st, (int)c->cases.size(), omega, eps, countMax, outer);
   launch_bicgstab_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, (const T*)X.d[0],
                           (const T*)X.h[0], V, c->dpart, (size_t)c->cap);
+  fin(c, st, FIN_BI_INIT, 0);
   int n = 1, batch = kBatchMin;
   bool done = false;
   while (!done) {
     int last = std::min(n + batch - 1, countMax);
     for (; n <= last; n++)
-      launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, (T*)X.d[0], V, c->dpart,
-                              (size_t)c->cap, n);
+      launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase, st, K,
+                              (T*)X.d[0], V, c->dpart, (size_t)c->cap, n);
     HIPCHK(hipGetLastError());
     int rc = read_state(c, st, hst, ids, &done);
     if (rc) return rc;
@@ -192,20 +204,31 @@ int rbgs_fixed_run(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double ome
   T* xs[2] = {(T*)X.d[0], (T*)X.d[1]};
   T* hs[2] = {(T*)X.h[0], (T*)X.h[1]};
   hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, c->sm));
-  for (int n = 1; n <= nsweep; n++)
+  std::vector<hipEvent_t> ev(2 * nsweep + 1);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventRecord(ev[0], c->sm));
+  for (int n = 1; n <= nsweep; n++) {
     launch_rbgs_iter<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, bv, xs[(n - 1) & 1], xs[n & 1],
                         hs[(n - 1) & 1], hs[n & 1], c->dpart, (size_t)c->cap, n, bcg);
-  HIPCHK(hipEventRecord(e1, c->sm));
+    HIPCHK(hipEventRecord(ev[2 * n - 1], c->sm));
+    fin(c, c->dst, FIN_RBGS, n);
+    HIPCHK(hipEventRecord(ev[2 * n], c->sm));
+  }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventSynchronize(e1));
-  float t = 0.f;
-  HIPCHK(hipEventElapsedTime(&t, e0, e1));
-  if (ms) *ms = t;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  HIPCHK(hipEventSynchronize(ev[2 * nsweep]));
+  double tk = 0.0, tf = 0.0;
+  for (int n = 1; n <= nsweep; n++) {
+    float a = 0.f, b2 = 0.f;
+    HIPCHK(hipEventElapsedTime(&a, ev[2 * n - 2], ev[2 * n - 1]));
+    HIPCHK(hipEventElapsedTime(&b2, ev[2 * n - 1], ev[2 * n]));
+    tk += a;
+    tf += b2;
+  }
+  if (ms) {
+    ms[0] = tk;
+    ms[1] = tf;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
   if (nsweep & 1) {  // the result of sweep nsweep is in buffer (nsweep & 1)
     launch_copy<T>(c->sm, c->dactive, c->nactive, xs[0], hs[0], xs[1], hs[1]);
     HIPCHK(hipGetLastError());
@@ -229,10 +252,11 @@ int cg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int ni
   launch_solve_init(c->sm, c->dst, (int)c->cases.size(), omega, -1.0, niter);
   launch_cg_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, (const T*)c->fields[b].d[0],
                     (const T*)X.d[0], (const T*)X.h[0], (T*)Rf.d[0], c->dpart, (size_t)c->cap);
-  std::vector<hipEvent_t> ev(4 * niter);
+  fin(c, c->dst, FIN_CG_INIT, 0);
+  std::vector<hipEvent_t> ev(6 * niter);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   for (int n = 1; n <= niter; n++) {
-    hipEvent_t* e = &ev[4 * (n - 1)];
+    hipEvent_t* e = &ev[6 * (n - 1)];
     HIPCHK(hipEventRecord(e[0], c->sm));
     launch_cg_p<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, (const T*)Rf.d[0], (T*)Pf.d[0],
                    (T*)Pf.h[0], n, BC_NULL);
@@ -240,18 +264,23 @@ int cg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int ni
     launch_cg_q<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, (const T*)Pf.d[0], (const T*)Pf.h[0],
                    (T*)Qf.d[0], c->dpart);
     HIPCHK(hipEventRecord(e[2], c->sm));
+    fin(c, c->dst, FIN_CG_ALPHA, n);
+    HIPCHK(hipEventRecord(e[3], c->sm));
     launch_cg_xr<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, (T*)X.d[0], (T*)Rf.d[0],
                     (const T*)Pf.d[0], (const T*)Qf.d[0], c->dpart, (size_t)c->cap, n);
-    HIPCHK(hipEventRecord(e[3], c->sm));
+    HIPCHK(hipEventRecord(e[4], c->sm));
+    fin(c, c->dst, FIN_CG_XR, n);
+    HIPCHK(hipEventRecord(e[5], c->sm));
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->sm));
-  ms[0] = ms[1] = ms[2] = 0.0;
+  for (int k = 0; k < 4; k++) ms[k] = 0.0;
+  static const int slot_of[5] = {0, 1, 3, 2, 3};  // p, q, fin, xr, fin
   for (int n = 0; n < niter; n++)
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 5; k++) {
       float t = 0.f;
-      HIPCHK(hipEventElapsedTime(&t, ev[4 * n + k], ev[4 * n + k + 1]));
-      ms[k] += t;
+      HIPCHK(hipEventElapsedTime(&t, ev[6 * n + k], ev[6 * n + k + 1]));
+      ms[slot_of[k]] += t;
     }
   for (auto& e : ev) (void)hipEventDestroy(e);
   return 0;
@@ -269,25 +298,33 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
   launch_bicgstab_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K,
                           (const T*)c->fields[b].d[0], (const T*)X.d[0], (const T*)X.h[0], V, c->dpart,
                           (size_t)c->cap);
-  std::vector<hipEvent_t> ev(6 * niter);
+  fin(c, c->dst, FIN_BI_INIT, 0);
+  // stages in launch order: P, Q, fin(alpha), S, T, fin(gamma), XR, fin(convergence, beta)
+  static const int bit_of[8] = {1, 2, 0, 4, 8, 0, 16, 0};
+  static const int fin_of[8] = {-1, -1, FIN_BI_ALPHA, -1, -1, FIN_BI_GAMMA, -1, FIN_BI_XR};
+  static const int slot_of[8] = {0, 1, 5, 2, 3, 5, 4, 5};
+  std::vector<hipEvent_t> ev(9 * niter);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   for (int n = 1; n <= niter; n++) {
-    hipEvent_t* e = &ev[6 * (n - 1)];
+    hipEvent_t* e = &ev[9 * (n - 1)];
     HIPCHK(hipEventRecord(e[0], c->sm));
-    for (int k = 0; k < 5; k++) {
-      launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K, (T*)X.d[0], V, c->dpart,
-                              (size_t)c->cap, n, 1 << k);
+    for (int k = 0; k < 8; k++) {
+      if (bit_of[k])
+        launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase, c->dst,
+                                K, (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, bit_of[k] | 32);
+      else
+        fin(c, c->dst, fin_of[k], n);
       HIPCHK(hipEventRecord(e[k + 1], c->sm));
     }
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->sm));
-  for (int k = 0; k < 5; k++) ms[k] = 0.0;
+  for (int k = 0; k < 6; k++) ms[k] = 0.0;
   for (int n = 0; n < niter; n++)
-    for (int k = 0; k < 5; k++) {
+    for (int k = 0; k < 8; k++) {
       float t = 0.f;
-      HIPCHK(hipEventElapsedTime(&t, ev[6 * n + k], ev[6 * n + k + 1]));
-      ms[k] += t;
+      HIPCHK(hipEventElapsedTime(&t, ev[9 * n + k], ev[9 * n + k + 1]));
+      ms[slot_of[k]] += t;
     }
   for (auto& e : ev) (void)hipEventDestroy(e);
   return 0;
@@ -469,6 +506,7 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   for (int o = 0;; o++) {
     launch_defect(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst2, K64, (const double*)c->fields[b].d[0],
                   (const double*)X.d[0], (const double*)X.h[0], (float*)Rf.d[0], c->dpart, (size_t)c->cap, o);
+    fin(c, c->dst2, FIN_DEFECT, o);
     HIPCHK(hipGetLastError());
     bool done = false;
     rc = read_state(c, c->dst2, c->hst2, ids, &done);
@@ -524,6 +562,7 @@ int ffvhc_residual(ffvhc_ctx* c, int x, const int* A, int b, double* residual) {
   launch_defect(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst2, coef<double>(c, A),
                 (const double*)c->fields[b].d[0], (const double*)c->fields[x].d[0],
                 (const double*)c->fields[x].h[0], (float*)c->fields[fr].d[0], c->dpart, (size_t)c->cap, 0);
+  fin(c, c->dst2, FIN_DEFECT, 0);
   HIPCHK(hipGetLastError());
   rc = sync_state(c, c->dst2, c->hst2);
   if (rc) return rc;

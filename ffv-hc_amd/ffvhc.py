@@ -72,7 +72,7 @@ _sig = {
     "ffvhc_field_copy": (c_int, [_vp, c_int, c_int]),
     "ffvhc_ghost_fill": (c_int, [_vp, c_int, c_int, c_int]),
     "ffvhc_rbgs": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
-    "ffvhc_rbgs_fixed": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, _P(c_double)]),
+    "ffvhc_rbgs_fixed": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, _vp]),
     "ffvhc_rbgs_fixed_average": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
     "ffvhc_cg": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_int, c_double, _vp, _vp]),
     "ffvhc_bicgstab": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_int, c_double, _vp, _vp]),
@@ -292,12 +292,13 @@ class Context:
             check(rc, "ffvhc_rbgs")
         return cnt, res, rc
 
-    def rbgs_fixed(self, x, A, b, group, omega, nsweep):
-        """nsweep fused RBGS iterations; returns the device time in ms."""
-        ms = c_double()
+    def rbgs_fixed(self, x, A, b, group, omega, nsweep, with_finalize=False):
+        """nsweep fused RBGS iterations; device ms of the iteration kernels (and of the per-case
+        residual finalisations when with_finalize)."""
+        ms = np.zeros(2)
         a = np.asarray(A, np.int32)
-        check(_L.ffvhc_rbgs_fixed(self.h, x, _ptr(a), b, group, omega, nsweep, ctypes.byref(ms)), "ffvhc_rbgs_fixed")
-        return ms.value
+        check(_L.ffvhc_rbgs_fixed(self.h, x, _ptr(a), b, group, omega, nsweep, _ptr(ms)), "ffvhc_rbgs_fixed")
+        return (ms[0], ms[1]) if with_finalize else ms[0]
 
     def rbgs_fixed_average(self, x, A, b, group, omega, count_max, eps, ncases=None):
         nc = ncases or self._nsel()
@@ -330,15 +331,15 @@ class Context:
         return cnt, res, rc
 
     def bicgstab_timing(self, x, A, b, omega, niter):
-        """Device ms of the five BiCGSTAB kernels summed over niter iterations."""
-        ms = np.zeros(5)
+        """Device ms of the five BiCGSTAB kernels and of the finalisations, summed over niter."""
+        ms = np.zeros(6)
         a = np.asarray(A, np.int32)
         check(_L.ffvhc_bicgstab_timing(self.h, x, _ptr(a), b, omega, niter, _ptr(ms)), "ffvhc_bicgstab_timing")
         return ms
 
     def cg_timing(self, x, A, b, omega, niter):
-        """Device ms of the three PCG kernels summed over niter iterations."""
-        ms = np.zeros(3)
+        """Device ms of the three PCG kernels and of the finalisations, summed over niter."""
+        ms = np.zeros(4)
         a = np.asarray(A, np.int32)
         check(_L.ffvhc_cg_timing(self.h, x, _ptr(a), b, omega, niter, _ptr(ms)), "ffvhc_cg_timing")
         return ms

@@ -142,7 +142,8 @@ int ffvhc_ghost_fill(ffvhc_ctx* ctx, int field, int bc_group, int updater);
 int ffvhc_rbgs(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countMax, double eps,
                int* count, double* residual);
 /* Exactly nsweep fused RBGS iterations (FFVILS::RBGS with eps < 0, countMax = nsweep) on the
- * context stream, bracketed by HIP events; *ms = device time of the sweeps (timing entry). */
+ * context stream with HIP events around every launch; ms[0] = device time of the iteration
+ * kernels, ms[1] = device time of the per-case residual finalisations (timing entry). */
 int ffvhc_rbgs_fixed(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int nsweep, double* ms);
 /* FFVILS::RBGS_WithFixedAverage (FFVILS.h:596-659): mean over Ncells removed every iteration */
 int ffvhc_rbgs_fixed_average(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countMax,
@@ -155,11 +156,13 @@ int ffvhc_cg(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double om
  * solver that converges on it (and the reference's default for P and implicit U). */
 int ffvhc_bicgstab(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countPre,
                    int countMax, double eps, int* count, double* residual);
-/* niter BiCGSTAB iterations with HIP events between the five fused kernels; ms[0..4] = total
- * device time of P (p update + preconditioner), Q (q_ = A p_), S, T (t_ = A s_), XR (timing entry). */
+/* niter BiCGSTAB iterations with HIP events between the launches; ms[0..4] = total device time of
+ * P (p update + preconditioner), Q (q_ = A p_), S, T (t_ = A s_), XR; ms[5] = the per-case
+ * scalar finalisations (timing entry). */
 int ffvhc_bicgstab_timing(ffvhc_ctx* ctx, int x, const int* A, int b, double omega, int niter, double* ms);
-/* niter PCG iterations (no convergence test) with HIP events between the three fused kernels;
- * ms[0..2] = total device time of the p-update, matvec and x/r-update kernels (timing entry). */
+/* niter PCG iterations (no convergence test) with HIP events between the launches; ms[0..2] =
+ * total device time of the p-update, matvec and x/r-update kernels, ms[3] = the per-case scalar
+ * finalisations (timing entry). */
 int ffvhc_cg_timing(ffvhc_ctx* ctx, int x, const int* A, int b, double omega, int niter, double* ms);
 /* FFVILS::Jacobi (FFVILS.h:492-545) */
 int ffvhc_jacobi(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countMax, double eps,
