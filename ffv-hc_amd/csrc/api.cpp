@@ -111,6 +111,37 @@ int ensure_second_buffer(ffvhc_ctx* ctx, int field) {
   return 0;
 }
 
+int upload_case_meta(ffvhc_ctx* c, int id) {
+  const CaseRec& r = c->cases[id];
+  DevCase& d = c->hcase[id];
+  d.slot_begin = r.slot_begin;
+  d.nblk = r.nblk;
+  d.rho = r.params.rho;
+  d.mu = r.params.mu;
+  d.dt = r.params.dt;
+  d.gx = r.params.gx;
+  d.gy = r.params.gy;
+  d.gz = r.params.gz;
+  d.betag = r.params.betag;
+  d.tr = r.params.tr;
+  for (int g = 0; g < NUM_USER_BC; g++)
+    for (int f = 0; f < 6; f++) {
+      d.bct[g][f] = r.params.bct[g][f];
+      d.bcv[g][f] = r.params.bcv[g][f];
+    }
+  for (int f = 0; f < 6; f++) {
+    d.bct[BC_HOM][f] = r.params.bct[BC_P][f];
+    d.bcv[BC_HOM][f] = 0.0;
+  }
+  d.step = r.step;
+  // Solver.cpp:3347-3355: the X_M inflow value of UX ramps up over the acceleration steps
+  if (r.accel_steps > 0 && r.step <= r.accel_steps)
+    d.bcv[BC_UX][0] = r.params.bcv[BC_UX][0] * (double)r.step / (double)r.accel_steps;
+  HIPCHK(hipMemcpyAsync(c->dcase + id, &d, sizeof(DevCase), hipMemcpyHostToDevice, c->sm));
+  HIPCHK(hipStreamSynchronize(c->sm));
+  return 0;
+}
+
 }  // namespace ffvhc
 
 extern "C" {
@@ -263,32 +294,6 @@ int ffvhc_ctx_select(ffvhc_ctx* c, const int* ids, int n) {
 
 // ------------------------------------------------------------------------------------ cases
 
-static int upload_case_meta(ffvhc_ctx* c, int id) {
-  const CaseRec& r = c->cases[id];
-  DevCase& d = c->hcase[id];
-  d.slot_begin = r.slot_begin;
-  d.nblk = r.nblk;
-  d.rho = r.params.rho;
-  d.mu = r.params.mu;
-  d.dt = r.params.dt;
-  d.gx = r.params.gx;
-  d.gy = r.params.gy;
-  d.gz = r.params.gz;
-  d.betag = r.params.betag;
-  d.tr = r.params.tr;
-  for (int g = 0; g < NUM_USER_BC; g++)
-    for (int f = 0; f < 6; f++) {
-      d.bct[g][f] = r.params.bct[g][f];
-      d.bcv[g][f] = r.params.bcv[g][f];
-    }
-  for (int f = 0; f < 6; f++) {
-    d.bct[BC_HOM][f] = r.params.bct[BC_P][f];
-    d.bcv[BC_HOM][f] = 0.0;
-  }
-  HIPCHK(hipMemcpyAsync(c->dcase + id, &d, sizeof(DevCase), hipMemcpyHostToDevice, c->sm));
-  HIPCHK(hipStreamSynchronize(c->sm));
-  return 0;
-}
 
 int ffvhc_case_add(ffvhc_ctx* c, const ffvhc_tree* t, const ffvhc_case_params* p, int g, const double* cut,
                    const int* cid, const int* pid, int* case_id) {

@@ -48,6 +48,28 @@ struct DevCase {
   double gx, gy, gz, betag, tr;
   int bct[NUM_BC][6];
   double bcv[NUM_BC][6];
+  int step;  // time step counter of the case (0 primes the Adams-Bashforth history)
+  int pad_;
+};
+
+// Device pointers of the time-step fields (FP64, interior + one ghost layer where a stencil
+// reads neighbours).  uc/ud receive this step's convection/diffusion terms, ucp/udp hold the
+// previous step's (the caller swaps the roles after the step instead of copying).
+struct StepFields {
+  double* u[3];
+  double* hu[3];
+  double* p;
+  double* hp;
+  double* uc[3];
+  double* ud[3];
+  const double* ucp[3];
+  const double* udp[3];
+  double* v[6];
+  double* A[7];
+  double* b;
+  double* lapp;
+  const double* t;     // temperature (buoyancy) or null = 0
+  const double* f[3];  // body force or null = 0
 };
 
 // Per-case state of an iterative solve, updated on the device by the last workgroup of each

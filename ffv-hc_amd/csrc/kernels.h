@@ -67,6 +67,19 @@ void launch_add_correction(hipStream_t sm, const int* active, int nact, const De
                            const SolveState* inner, double* x, const float* e);
 void launch_convert(hipStream_t sm, const int* active, int nact, float* y, const double* x);
 
+void launch_predict(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                    const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc);
+void launch_p1(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+               const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc);
+void launch_ab_p(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                 const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc);
+void launch_refval(hipStream_t sm, const int* slots, int n, const DevBlock* blk, const StepFields& F, double xr,
+                   double yr, double zr, double pr);
+void launch_matrix_bc(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                      const StepFields& F, int bcg);
+void launch_corr(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                 const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc);
+
 void launch_solve_init(hipStream_t sm, SolveState* st, int ncase, double omega, double eps, int countMax,
                        const SolveState* outer = nullptr);
 

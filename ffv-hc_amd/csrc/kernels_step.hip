@@ -1,0 +1,405 @@
+// Stage kernels of the fractional step (Solver::Update, Solver.cpp:3344-3416) with explicit U:
+//
+//   predict  : per velocity component d, bcut_calc_c_f_c2 (convection, bcut.f90:1583-1764),
+//              bcut_calc_d_u (diffusion, 3397-3552) and bcut_update_u (Adams-Bashforth 2,
+//              4299-4523) fused; step 0 primes ucp/udp with this step's terms (Solver.cpp:3693-3702)
+//   p1       : bcut_add_g (3793-3833), bcut_add_f (3835-3877), bcut_remove_p (3879-4061) fused
+//   ab_p     : bcut_calc_ab_p (608-832): Poisson matrix, b = div(v)/dt, face velocities
+//   refval   : bcut_set_reference_value (5626-5710), including its Ae(i+1) index at 5682
+//   matrix BC: bc_a*_{d,n} (bca.f90), face by face in face order (BlockScalar3D.h:183-190)
+//   corr     : bcut_corr_u (4063-4297): projection, face-velocity correction, lapp
+//
+// One 256-thread workgroup per 16^3 block, thread = (i, j) column over the 16 k planes. Every
+// cell evaluates the reference's expressions in the reference's order (FP64), so results are
+// bit-identical to the per-block Fortran.  Geometry per cell: six float32 cut distances, a
+// flag word (bit f: face f is cut, bit 6: fluid), the cut ids only for forces.
+#include "kcommon.h"
+#include "kernels.h"
+
+namespace ffvhc {
+
+constexpr int ST_NT = 256;
+
+// value of a field at (i, j, k) with at most one coordinate one cell outside the block
+__device__ inline double at(const double* __restrict__ x, const double* __restrict__ h, int slot, int i, int j,
+                            int k) {
+  const size_t hb = (size_t)slot * NH;
+  if (i < 0) return h[hb + halo_index(0, j, k)];
+  if (i >= NB) return h[hb + halo_index(1, j, k)];
+  if (j < 0) return h[hb + halo_index(2, i, k)];
+  if (j >= NB) return h[hb + halo_index(3, i, k)];
+  if (k < 0) return h[hb + halo_index(4, i, j)];
+  if (k >= NB) return h[hb + halo_index(5, i, j)];
+  return x[(size_t)slot * NC + i + NB * (j + NB * k)];
+}
+
+struct Geo {
+  double d[6];
+  uint32_t fl;
+  __device__ bool cut(int f) const { return (fl >> f) & 1u; }
+  __device__ bool fluid() const { return (fl >> 6) & 1u; }
+};
+
+__device__ inline Geo load_geo(const float* __restrict__ cut, const uint32_t* __restrict__ flag, size_t capnc,
+                               size_t g) {
+  Geo G;
+#pragma unroll
+  for (int f = 0; f < 6; f++) G.d[f] = (double)cut[(size_t)f * capnc + g];
+  G.fl = flag[g];
+  return G;
+}
+
+struct Face6 {
+  double x0, x1, y2, y3, z4, z5;
+};
+
+// pressure-gradient face terms shared by update_u / remove_p / corr_u (bcut.f90), with the
+// "- g*0.0" terms of update_u when with_g
+__device__ inline Face6 grad_faces(const double* __restrict__ p, const double* __restrict__ hp, int slot, int i,
+                                   int j, int k, double r0, double dx, bool with_g, double gx, double gy, double gz,
+                                   const Geo& G) {
+  const double pc = at(p, hp, slot, i, j, k);
+  const double dpx0 = (pc - at(p, hp, slot, i - 1, j, k)) / dx, dpx1 = (at(p, hp, slot, i + 1, j, k) - pc) / dx;
+  const double dpy2 = (pc - at(p, hp, slot, i, j - 1, k)) / dx, dpy3 = (at(p, hp, slot, i, j + 1, k) - pc) / dx;
+  const double dpz4 = (pc - at(p, hp, slot, i, j, k - 1)) / dx, dpz5 = (at(p, hp, slot, i, j, k + 1) - pc) / dx;
+  Face6 r;
+  if (with_g) {
+    r.x0 = r0 * dpx0 - gx * 0.0;
+    r.x1 = r0 * dpx1 - gx * 0.0;
+    r.y2 = r0 * dpy2 - gy * 0.0;
+    r.y3 = r0 * dpy3 - gy * 0.0;
+    r.z4 = r0 * dpz4 - gz * 0.0;
+    r.z5 = r0 * dpz5 - gz * 0.0;
+  } else {
+    r.x0 = r0 * dpx0;
+    r.x1 = r0 * dpx1;
+    r.y2 = r0 * dpy2;
+    r.y3 = r0 * dpy3;
+    r.z4 = r0 * dpz4;
+    r.z5 = r0 * dpz5;
+  }
+  const double* d = G.d;
+  if (G.cut(0)) r.x0 = r.x1 * (d[0] - 0.5) / (d[0] + 0.5);
+  if (G.cut(1)) r.x1 = r.x0 * (d[1] - 0.5) / (d[1] + 0.5);
+  if (G.cut(2)) r.y2 = r.y3 * (d[2] - 0.5) / (d[2] + 0.5);
+  if (G.cut(3)) r.y3 = r.y2 * (d[3] - 0.5) / (d[3] + 0.5);
+  if (G.cut(4)) r.z4 = r.z5 * (d[4] - 0.5) / (d[4] + 0.5);
+  if (G.cut(5)) r.z5 = r.z4 * (d[5] - 0.5) / (d[5] + 0.5);
+  if (G.cut(0) && G.cut(1)) { r.x0 = 0.0; r.x1 = 0.0; }
+  if (G.cut(2) && G.cut(3)) { r.y2 = 0.0; r.y3 = 0.0; }
+  if (G.cut(4) && G.cut(5)) { r.z4 = 0.0; r.z5 = 0.0; }
+  return r;
+}
+
+// ------------------------------------------------------------------ U predictor (explicit)
+__global__ __launch_bounds__(ST_NT) void predict_kernel(const int* __restrict__ active,
+                                                        const DevBlock* __restrict__ blk,
+                                                        const DevCase* __restrict__ cs, StepFields F,
+                                                        const float* __restrict__ cut,
+                                                        const uint32_t* __restrict__ flag, size_t capnc) {
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const DevCase& C = cs[B.case_id];
+  const int t = threadIdx.x, i = t & 15, j = t >> 4;
+  const double dx = B.dx, dt = C.dt, rhof = C.rho, mu = C.mu, Uc = 0.0;
+  const bool step0 = C.step == 0;
+  for (int dir = 0; dir < 3; dir++) {
+    const double* __restrict__ u = F.u[dir];
+    const double* __restrict__ hu = F.hu[dir];
+    double unew[NB];
+    for (int k = 0; k < NB; k++) {
+      const size_t g = (size_t)slot * NC + i + NB * (j + NB * k);
+      const Geo G = load_geo(cut, flag, capnc, g);
+      const double* d = G.d;
+      const bool fluid = G.fluid();
+      const double fp = u[g];
+      const double fw = at(u, hu, slot, i - 1, j, k), fe = at(u, hu, slot, i + 1, j, k);
+      const double fs = at(u, hu, slot, i, j - 1, k), fn = at(u, hu, slot, i, j + 1, k);
+      const double fb = at(u, hu, slot, i, j, k - 1), ft = at(u, hu, slot, i, j, k + 1);
+      // bcut_calc_c_f_c2
+      double fc;
+      {
+        const double vx0 = F.v[0][g], vx1 = F.v[1][g], vy2 = F.v[2][g], vy3 = F.v[3][g], vz4 = F.v[4][g],
+                     vz5 = F.v[5][g];
+        const double f0 = 0.5 * (fp + fw), f1 = 0.5 * (fp + fe), f2 = 0.5 * (fp + fs);
+        const double f3 = 0.5 * (fp + fn), f4 = 0.5 * (fp + fb), f5 = 0.5 * (fp + ft);
+        double q0 = vx0 * f0, q1 = vx1 * f1, q2 = vy2 * f2, q3 = vy3 * f3, q4 = vz4 * f4, q5 = vz5 * f5;
+        if (G.cut(0)) q0 = (d[0] - 0.5) / (d[0] + 0.5) * q1;
+        if (G.cut(1)) q1 = (d[1] - 0.5) / (d[1] + 0.5) * q0;
+        if (G.cut(2)) q2 = (d[2] - 0.5) / (d[2] + 0.5) * q3;
+        if (G.cut(3)) q3 = (d[3] - 0.5) / (d[3] + 0.5) * q2;
+        if (G.cut(4)) q4 = (d[4] - 0.5) / (d[4] + 0.5) * q5;
+        if (G.cut(5)) q5 = (d[5] - 0.5) / (d[5] + 0.5) * q4;
+        if (G.cut(0) && G.cut(1)) { q0 = 0.0; q1 = 0.0; }
+        if (G.cut(2) && G.cut(3)) { q2 = 0.0; q3 = 0.0; }
+        if (G.cut(4) && G.cut(5)) { q4 = 0.0; q5 = 0.0; }
+        fc = (q1 - q0) / dx + (q3 - q2) / dx + (q5 - q4) / dx;
+        if (!fluid) fc = 0.0;
+      }
+      // bcut_calc_d_u
+      double ud;
+      {
+        double mu0 = mu, mu1 = mu, mu2 = mu, mu3 = mu, mu4 = mu, mu5 = mu;
+        double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0, m4 = 0.0, m5 = 0.0;
+        if (G.cut(0)) { mu0 = mu / d[0] * 2.0 / (d[0] + d[1]); mu1 = mu / d[1] * 2.0 / (d[0] + d[1]); m0 = 1.0; }
+        if (G.cut(1)) { mu0 = mu / d[0] * 2.0 / (d[0] + d[1]); mu1 = mu / d[1] * 2.0 / (d[0] + d[1]); m1 = 1.0; }
+        if (G.cut(2)) { mu2 = mu / d[2] * 2.0 / (d[2] + d[3]); mu3 = mu / d[3] * 2.0 / (d[2] + d[3]); m2 = 1.0; }
+        if (G.cut(3)) { mu2 = mu / d[2] * 2.0 / (d[2] + d[3]); mu3 = mu / d[3] * 2.0 / (d[2] + d[3]); m3 = 1.0; }
+        if (G.cut(4)) { mu4 = mu / d[4] * 2.0 / (d[4] + d[5]); mu5 = mu / d[5] * 2.0 / (d[4] + d[5]); m4 = 1.0; }
+        if (G.cut(5)) { mu4 = mu / d[4] * 2.0 / (d[4] + d[5]); mu5 = mu / d[5] * 2.0 / (d[4] + d[5]); m5 = 1.0; }
+        const double l0 = mu0 / rhof / (dx * dx), l1 = mu1 / rhof / (dx * dx), l2 = mu2 / rhof / (dx * dx);
+        const double l3 = mu3 / rhof / (dx * dx), l4 = mu4 / rhof / (dx * dx), l5 = mu5 / rhof / (dx * dx);
+        const double uw = fw * (1.0 - m0) + Uc * m0;
+        const double ue = fe * (1.0 - m1) + Uc * m1;
+        const double us = fs * (1.0 - m2) + Uc * m2;
+        const double un = fn * (1.0 - m3) + Uc * m3;
+        const double ub = fb * (1.0 - m4) + Uc * m4;
+        const double ut = ft * (1.0 - m5) + Uc * m5;
+        ud = l1 * (ue - fp) - l0 * (fp - uw) + l3 * (un - fp) - l2 * (fp - us) + l5 * (ut - fp) - l4 * (fp - ub);
+        if (!fluid) ud = 0.0;
+      }
+      // bcut_update_u
+      const double ucp = step0 ? fc : F.ucp[dir][g];
+      const double udp = step0 ? ud : F.udp[dir][g];
+      const Face6 r = grad_faces(F.p, F.hp, slot, i, j, k, 1.0 / rhof, dx, true, C.gx, C.gy, C.gz, G);
+      const double rdpx = 0.5 * (r.x0 + r.x1), rdpy = 0.5 * (r.y2 + r.y3), rdpz = 0.5 * (r.z4 + r.z5);
+      const double rdp = dir == 0 ? rdpx : (dir == 1 ? rdpy : rdpz);
+      double un = fp - (1.5 * fc - 0.5 * ucp) * dt + (1.5 * ud - 0.5 * udp) * dt - rdp * dt;
+      if (!fluid) un = Uc;
+      unew[k] = un;
+      F.uc[dir][g] = fc;
+      F.ud[dir][g] = ud;
+    }
+    __syncthreads();  // every neighbour read of u[dir] in this block is done
+    for (int k = 0; k < NB; k++) F.u[dir][(size_t)slot * NC + i + NB * (j + NB * k)] = unew[k];
+  }
+}
+
+// ------------------------------------------------------------------ add_g, add_f, remove_p
+__global__ __launch_bounds__(ST_NT) void p1_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                   const DevCase* __restrict__ cs, StepFields F,
+                                                   const float* __restrict__ cut, const uint32_t* __restrict__ flag,
+                                                   size_t capnc) {
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const DevCase& C = cs[B.case_id];
+  const int t = threadIdx.x, i = t & 15, j = t >> 4;
+  const double dx = B.dx, dt = C.dt, rhof = C.rho;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = (size_t)slot * NC + i + NB * (j + NB * k);
+    const Geo G = load_geo(cut, flag, capnc, g);
+    double ux = F.u[0][g], uy = F.u[1][g], uz = F.u[2][g];
+    const double tv = F.t ? F.t[g] : 0.0;
+    // bcut_add_g
+    ux = ux + C.gx * dt;
+    uy = uy + C.gy * dt;
+    uz = uz + C.gz * dt + C.betag * (tv - C.tr) * dt;
+    // bcut_add_f (fluid cells)
+    if (G.fluid()) {
+      const double fx = F.f[0] ? F.f[0][g] : 0.0, fy = F.f[1] ? F.f[1][g] : 0.0, fz = F.f[2] ? F.f[2][g] : 0.0;
+      ux = ux + fx / rhof * dt;
+      uy = uy + fy / rhof * dt;
+      uz = uz + fz / rhof * dt;
+    }
+    // bcut_remove_p
+    const Face6 r = grad_faces(F.p, F.hp, slot, i, j, k, 1.0 / rhof, dx, false, 0.0, 0.0, 0.0, G);
+    const double rdpx = 0.5 * (r.x0 + r.x1), rdpy = 0.5 * (r.y2 + r.y3), rdpz = 0.5 * (r.z4 + r.z5);
+    ux = ux + rdpx * dt;
+    uy = uy + rdpy * dt;
+    uz = uz + rdpz * dt;
+    if (!G.fluid()) { ux = 0.0; uy = 0.0; uz = 0.0; }
+    F.u[0][g] = ux;
+    F.u[1][g] = uy;
+    F.u[2][g] = uz;
+  }
+}
+
+// ------------------------------------------------------------------ Poisson matrix and RHS
+__global__ __launch_bounds__(ST_NT) void ab_p_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                     const DevCase* __restrict__ cs, StepFields F,
+                                                     const float* __restrict__ cut, const uint32_t* __restrict__ flag,
+                                                     size_t capnc) {
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const DevCase& C = cs[B.case_id];
+  const int t = threadIdx.x, i = t & 15, j = t >> 4;
+  const double dx = B.dx, dt = C.dt, rhof = C.rho;
+  const double *ux = F.u[0], *uy = F.u[1], *uz = F.u[2];
+  for (int k = 0; k < NB; k++) {
+    const size_t g = (size_t)slot * NC + i + NB * (j + NB * k);
+    const Geo G = load_geo(cut, flag, capnc, g);
+    const double* d = G.d;
+    double r0 = 1.0 / rhof, r1 = 1.0 / rhof, r2 = 1.0 / rhof, r3 = 1.0 / rhof, r4 = 1.0 / rhof, r5 = 1.0 / rhof;
+    double vx0 = 0.5 * (ux[g] + at(ux, F.hu[0], slot, i - 1, j, k));
+    double vx1 = 0.5 * (ux[g] + at(ux, F.hu[0], slot, i + 1, j, k));
+    double vy2 = 0.5 * (uy[g] + at(uy, F.hu[1], slot, i, j - 1, k));
+    double vy3 = 0.5 * (uy[g] + at(uy, F.hu[1], slot, i, j + 1, k));
+    double vz4 = 0.5 * (uz[g] + at(uz, F.hu[2], slot, i, j, k - 1));
+    double vz5 = 0.5 * (uz[g] + at(uz, F.hu[2], slot, i, j, k + 1));
+    if (G.cut(0)) { r0 = 0.0; vx0 = vx1 * (d[0] - 0.5) / (d[0] + 0.5); r1 = (1.0 / rhof) / (d[0] + 0.5); }
+    if (G.cut(1)) { r1 = 0.0; vx1 = vx0 * (d[1] - 0.5) / (d[1] + 0.5); r0 = (1.0 / rhof) / (d[1] + 0.5); }
+    if (G.cut(2)) { r2 = 0.0; vy2 = vy3 * (d[2] - 0.5) / (d[2] + 0.5); r3 = (1.0 / rhof) / (d[2] + 0.5); }
+    if (G.cut(3)) { r3 = 0.0; vy3 = vy2 * (d[3] - 0.5) / (d[3] + 0.5); r2 = (1.0 / rhof) / (d[3] + 0.5); }
+    if (G.cut(4)) { r4 = 0.0; vz4 = vz5 * (d[4] - 0.5) / (d[4] + 0.5); r5 = (1.0 / rhof) / (d[4] + 0.5); }
+    if (G.cut(5)) { r5 = 0.0; vz5 = vz4 * (d[5] - 0.5) / (d[5] + 0.5); r4 = (1.0 / rhof) / (d[5] + 0.5); }
+    if (G.cut(0) && G.cut(1)) { r0 = 0.0; r1 = 0.0; vx0 = 0.0; vx1 = 0.0; }
+    if (G.cut(2) && G.cut(3)) { r2 = 0.0; r3 = 0.0; vy2 = 0.0; vy3 = 0.0; }
+    if (G.cut(4) && G.cut(5)) { r4 = 0.0; r5 = 0.0; vz4 = 0.0; vz5 = 0.0; }
+    const double l0 = r0 / (dx * dx), l1 = r1 / (dx * dx), l2 = r2 / (dx * dx);
+    const double l3 = r3 / (dx * dx), l4 = r4 / (dx * dx), l5 = r5 / (dx * dx);
+    const double divv = (vx1 - vx0 + vy3 - vy2 + vz5 - vz4) / dx;
+    double a[7] = {-(l0 + l1) - (l2 + l3) - (l4 + l5), l0, l1, l2, l3, l4, l5};
+    double bv = divv / dt;
+    double v[6] = {vx0, vx1, vy2, vy3, vz4, vz5};
+    if (!G.fluid()) {
+      a[0] = -6.0;
+      for (int q = 1; q < 7; q++) a[q] = 1.0;
+      bv = 0.0;
+      for (int q = 0; q < 6; q++) v[q] = 0.0;
+    }
+    for (int q = 0; q < 7; q++) F.A[q][g] = a[q];
+    F.b[g] = bv;
+    for (int q = 0; q < 6; q++) F.v[q][g] = v[q];
+  }
+}
+
+// ------------------------------------------------------------------ reference pressure
+// One thread per candidate block walks the block's cells in the reference loop order, so a
+// point on a cell boundary is applied to each matching cell in turn exactly as the reference
+// does; writes that land in the ghost layer are dropped (no interior loop reads them).
+__global__ void refval_kernel(const int* __restrict__ slots, const DevBlock* __restrict__ blk, StepFields F,
+                              double xr, double yr, double zr, double pr) {
+  if (threadIdx.x != 0) return;
+  const int slot = slots[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const double dx = B.dx;
+  double* A[7];
+  for (int q = 0; q < 7; q++) A[q] = F.A[q] + (size_t)slot * NC;
+  double* b = F.b + (size_t)slot * NC;
+  auto in = [](int i, int j, int k) { return i >= 0 && i < NB && j >= 0 && j < NB && k >= 0 && k < NB; };
+  auto ix = [](int i, int j, int k) { return i + NB * (j + NB * k); };
+  for (int k = 0; k < NB; k++)
+    for (int j = 0; j < NB; j++)
+      for (int i = 0; i < NB; i++) {
+        // Fortran cell (i+1, j+1, k+1): [org + (i) dx, org + (i+1) dx]
+        const double x0 = B.org[0] + (double)i * dx, y0 = B.org[1] + (double)j * dx, z0 = B.org[2] + (double)k * dx;
+        const double x1 = B.org[0] + (double)(i + 1) * dx, y1 = B.org[1] + (double)(j + 1) * dx,
+                     z1 = B.org[2] + (double)(k + 1) * dx;
+        if (!((x0 <= xr && xr <= x1) && (y0 <= yr && yr <= y1) && (z0 <= zr && zr <= z1))) continue;
+        const int m = ix(i, j, k);
+        A[0][m] = 1.0;
+        for (int q = 1; q < 7; q++) A[q][m] = 0.0;
+        b[m] = pr;
+        // neighbours e, w, n, s, t, b; the w line reads Ae of the e cell (bcut.f90:5682)
+        const int ne[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+        const int band[6] = {1, 2, 3, 4, 5, 6};  // Aw of e, Ae of w, As of n, An of s, Ab of t, At of b
+        for (int q = 0; q < 6; q++) {
+          const int a0 = ne[q][0], a1 = ne[q][1], a2 = ne[q][2];
+          if (!in(a0, a1, a2)) continue;
+          const int n = ix(a0, a1, a2);
+          // the w line subtracts Ae of the e cell; when e lies in the ghost layer that matrix
+          // entry is never assigned (calc_ab_p writes interior cells only) and stays zero
+          const double ap_sub = q == 1 ? (in(i + 1, j, k) ? A[2][ix(i + 1, j, k)] : 0.0) : A[band[q]][n];
+          A[0][n] = A[0][n] - ap_sub;
+          b[n] = b[n] - A[band[q]][n] * pr * 2.0;
+          A[band[q]][n] = 0.0;
+        }
+      }
+}
+
+// ------------------------------------------------------------------ matrix outer BC
+__global__ __launch_bounds__(ST_NT) void matrix_bc_kernel(const int* __restrict__ active,
+                                                          const DevBlock* __restrict__ blk,
+                                                          const DevCase* __restrict__ cs, StepFields F, int bcg) {
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const DevCase& C = cs[B.case_id];
+  const int t = threadIdx.x;
+  for (int f = 0; f < 6; f++) {
+    if (B.ftype[f] != FACE_OUTER) continue;
+    const int type = C.bct[bcg][f];
+    if (type != BCT_DIRICHLET && type != BCT_NEUMANN) continue;
+    const double xc = type == BCT_NEUMANN ? C.bcv[bcg][f] * B.dx : C.bcv[bcg][f];
+    int i, j, k;
+    face_cell(f, t & 15, t >> 4, 0, i, j, k);
+    const size_t g = (size_t)slot * NC + i + NB * (j + NB * k);
+    double* Ap = F.A[0];
+    double* An = F.A[1 + f];
+    if (type == BCT_DIRICHLET) {
+      Ap[g] = Ap[g] - An[g];
+      F.b[g] = F.b[g] - An[g] * xc * 2.0;
+    } else {
+      Ap[g] = Ap[g] + An[g];
+      F.b[g] = (f & 1) ? F.b[g] - An[g] * xc : F.b[g] + An[g] * xc;
+    }
+    An[g] = 0.0;
+    __syncthreads();  // faces in order: a corner cell folds face f before face f+1
+  }
+}
+
+// ------------------------------------------------------------------ velocity correction
+__global__ __launch_bounds__(ST_NT) void corr_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                     const DevCase* __restrict__ cs, StepFields F,
+                                                     const float* __restrict__ cut, const uint32_t* __restrict__ flag,
+                                                     size_t capnc) {
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const DevCase& C = cs[B.case_id];
+  const int t = threadIdx.x, i = t & 15, j = t >> 4;
+  const double dx = B.dx, dt = C.dt, rhof = C.rho;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = (size_t)slot * NC + i + NB * (j + NB * k);
+    const Geo G = load_geo(cut, flag, capnc, g);
+    const Face6 r = grad_faces(F.p, F.hp, slot, i, j, k, 1.0 / rhof, dx, false, 0.0, 0.0, 0.0, G);
+    const double rdpx = 0.5 * (r.x0 + r.x1), rdpy = 0.5 * (r.y2 + r.y3), rdpz = 0.5 * (r.z4 + r.z5);
+    double ux = F.u[0][g] - rdpx * dt, uy = F.u[1][g] - rdpy * dt, uz = F.u[2][g] - rdpz * dt;
+    double v[6] = {F.v[0][g] - r.x0 * dt, F.v[1][g] - r.x1 * dt, F.v[2][g] - r.y2 * dt,
+                   F.v[3][g] - r.y3 * dt, F.v[4][g] - r.z4 * dt, F.v[5][g] - r.z5 * dt};
+    const double pc = F.p[g];
+    double lapp = (at(F.p, F.hp, slot, i + 1, j, k) + at(F.p, F.hp, slot, i - 1, j, k) +
+                   at(F.p, F.hp, slot, i, j + 1, k) + at(F.p, F.hp, slot, i, j - 1, k) +
+                   at(F.p, F.hp, slot, i, j, k + 1) + at(F.p, F.hp, slot, i, j, k - 1) - 6.0 * pc) /
+                  (dx * dx);
+    if (!G.fluid()) {
+      ux = 0.0;
+      uy = 0.0;
+      uz = 0.0;
+      for (int q = 0; q < 6; q++) v[q] = 0.0;
+      lapp = 0.0;
+    }
+    if (G.fl & 63u) lapp = 0.0;
+    F.u[0][g] = ux;
+    F.u[1][g] = uy;
+    F.u[2][g] = uz;
+    for (int q = 0; q < 6; q++) F.v[q][g] = v[q];
+    F.lapp[g] = lapp;
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+void launch_predict(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                    const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc) {
+  if (nact) hipLaunchKernelGGL(predict_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, blk, cs, F, cut, flag, capnc);
+}
+void launch_p1(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+               const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc) {
+  if (nact) hipLaunchKernelGGL(p1_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, blk, cs, F, cut, flag, capnc);
+}
+void launch_ab_p(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                 const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc) {
+  if (nact) hipLaunchKernelGGL(ab_p_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, blk, cs, F, cut, flag, capnc);
+}
+void launch_refval(hipStream_t sm, const int* slots, int n, const DevBlock* blk, const StepFields& F, double xr,
+                   double yr, double zr, double pr) {
+  if (n) hipLaunchKernelGGL(refval_kernel, dim3(n), dim3(64), 0, sm, slots, blk, F, xr, yr, zr, pr);
+}
+void launch_matrix_bc(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                      const StepFields& F, int bcg) {
+  if (nact) hipLaunchKernelGGL(matrix_bc_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, blk, cs, F, bcg);
+}
+void launch_corr(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                 const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc) {
+  if (nact) hipLaunchKernelGGL(corr_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, blk, cs, F, cut, flag, capnc);
+}
+
+}  // namespace ffvhc

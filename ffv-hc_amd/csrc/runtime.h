@@ -40,6 +40,8 @@ struct CaseRec {
   int nblk = 0;
   int max_level = 0;
   int64_t ncells = 0;
+  int step = 0;         // Solver::Update step counter
+  int accel_steps = 0;  // TimeControl/Acceleration of the X_M inflow (0 = off)
   ffvhc_case_params params{};
 };
 
@@ -91,4 +93,5 @@ int ensure_second_buffer(ffvhc_ctx* ctx, int field);
 int alloc_field(ffvhc_ctx* ctx, int prec, int* field);
 int scratch_field(ffvhc_ctx* ctx, int prec, int role, int* field);
 bool field_ok(ffvhc_ctx* ctx, int f, int prec);
+int upload_case_meta(ffvhc_ctx* c, int id);
 }  // namespace ffvhc

@@ -1,6 +1,7 @@
 // Linear solvers behind the C ABI (FFVILS.h): fused device-resident RBGS and Jacobi-PCG, and the
 // FP32-inner / FP64-defect-correction mixed-precision drivers.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "kernels.h"
@@ -203,7 +204,6 @@ int rbgs_fixed_run(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double ome
   const T* bv = (const T*)c->fields[b].d[0];
   T* xs[2] = {(T*)X.d[0], (T*)X.d[1]};
   T* hs[2] = {(T*)X.h[0], (T*)X.h[1]};
-  hipEvent_t e0, e1;
   std::vector<hipEvent_t> ev(2 * nsweep + 1);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   HIPCHK(hipEventRecord(ev[0], c->sm));
@@ -569,6 +569,159 @@ int ffvhc_residual(ffvhc_ctx* c, int x, const int* A, int b, double* residual) {
   for (size_t i = 0; i < ids.size(); i++)
     if (residual) residual[i] = c->hst2[ids[i]].residual;
   return 0;
+}
+
+int ffvhc_case_set_step(ffvhc_ctx* c, int id, int step, int accel_steps) {
+  if (!c || id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
+  if (step < 0 || accel_steps < 0) return fail(-1, "step and accel_steps must be >= 0");
+  c->cases[id].step = step;
+  c->cases[id].accel_steps = accel_steps;
+  return upload_case_meta(c, id);
+}
+
+int ffvhc_case_step(const ffvhc_ctx* c, int id) {
+  if (!c || id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
+  return c->cases[id].step;
+}
+
+int ffvhc_step(ffvhc_ctx* c, ffvhc_step_fields* F, const ffvhc_step_params* P, ffvhc_step_info* info,
+               double* stage_ms) {
+  if (!c || !F || !P) return fail(-1, "null argument");
+  const int req[] = {F->ux, F->uy, F->uz, F->p, F->uc[0], F->uc[1], F->uc[2], F->ucp[0], F->ucp[1], F->ucp[2],
+                     F->ud[0], F->ud[1], F->ud[2], F->udp[0], F->udp[1], F->udp[2], F->v[0], F->v[1], F->v[2],
+                     F->v[3], F->v[4], F->v[5], F->A[0], F->A[1], F->A[2], F->A[3], F->A[4], F->A[5], F->A[6],
+                     F->b, F->lapp};
+  for (int f : req)
+    if (!field_ok(c, f, 8)) return fail(-1, "time-step fields must be allocated FP64 fields");
+  const int opt[] = {F->t, F->f[0], F->f[1], F->f[2]};
+  for (int f : opt)
+    if (f != -1 && !field_ok(c, f, 8)) return fail(-1, "t / f must be -1 or FP64 fields");
+  if (P->solver != 0 && P->solver != 2 && P->solver != 3)
+    return fail(-1, "solver: 0 RBGS, 2 Jacobi-PCG, 3 Jacobi-BiCGSTAB");
+  if (P->solver != 0 && P->countPre != 1) return fail(-1, "countPre must be 1");
+  int rc = refresh_active(c);
+  if (rc) return rc;
+  const std::vector<int> ids = selected_cases(c);
+  for (int id : ids) {
+    rc = upload_case_meta(c, id);
+    if (rc) return rc;
+  }
+  auto D = [&](int f) { return (double*)c->fields[f].d[0]; };
+  auto H = [&](int f) { return (double*)c->fields[f].h[0]; };
+  StepFields S;
+  const int u[3] = {F->ux, F->uy, F->uz};
+  for (int d = 0; d < 3; d++) {
+    S.u[d] = D(u[d]);
+    S.hu[d] = H(u[d]);
+    S.uc[d] = D(F->uc[d]);
+    S.ud[d] = D(F->ud[d]);
+    S.ucp[d] = D(F->ucp[d]);
+    S.udp[d] = D(F->udp[d]);
+    S.f[d] = F->f[d] < 0 ? nullptr : D(F->f[d]);
+  }
+  S.p = D(F->p);
+  S.hp = H(F->p);
+  for (int q = 0; q < 6; q++) S.v[q] = D(F->v[q]);
+  for (int q = 0; q < 7; q++) S.A[q] = D(F->A[q]);
+  S.b = D(F->b);
+  S.lapp = D(F->lapp);
+  S.t = F->t < 0 ? nullptr : D(F->t);
+  const size_t capnc = (size_t)c->cap * NC;
+  auto fill_u = [&]() {
+    for (int d = 0; d < 3; d++)
+      launch_ghost_pull<double>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S.u[d], S.hu[d], BC_UX + d, 5,
+                                c->multilevel);
+  };
+  hipEvent_t ev[4];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventRecord(ev[0], c->sm));
+  // UpdateUXe / UYe / UZe: the three components are independent (face velocities are fixed)
+  launch_predict(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S, c->dcut, c->dflag, capnc);
+  fill_u();
+  HIPCHK(hipEventRecord(ev[1], c->sm));
+  // UpdateP
+  launch_p1(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S, c->dcut, c->dflag, capnc);
+  fill_u();
+  launch_ab_p(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S, c->dcut, c->dflag, capnc);
+  if (P->ref_active) {
+    std::vector<int> cand;
+    for (int id : ids) {
+      const CaseRec& r = c->cases[id];
+      for (int q = 0; q < r.nblk; q++) {
+        const DevBlock& B = c->hblk[r.slot_begin + q];
+        const double L = B.dx * NB;
+        bool inside = true;
+        for (int a = 0; a < 3; a++)
+          if (!(B.org[a] <= P->ref[a] && P->ref[a] <= B.org[a] + L)) inside = false;
+        if (inside) cand.push_back(r.slot_begin + q);
+      }
+    }
+    if (!cand.empty()) {
+      int* dc = nullptr;
+      HIPCHK(hipMallocAsync((void**)&dc, sizeof(int) * cand.size(), c->sm));
+      HIPCHK(hipMemcpyAsync(dc, cand.data(), sizeof(int) * cand.size(), hipMemcpyHostToDevice, c->sm));
+      launch_refval(c->sm, dc, (int)cand.size(), c->dblk, S, P->ref[0], P->ref[1], P->ref[2], P->ref_p);
+      HIPCHK(hipFreeAsync(dc, c->sm));
+    }
+  }
+  launch_matrix_bc(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S, BC_P);
+  HIPCHK(hipGetLastError());
+  const size_t nc = ids.size();
+  std::vector<int> cnt(nc, 0), outer(nc, 0);
+  std::vector<double> res(nc, 0.0);
+  int st = 0;
+  if (P->mixed) {
+    const int method = P->solver == 0 ? 0 : (P->solver == 2 ? 1 : 2);
+    st = ffvhc_solve_mixed(c, method, F->p, F->A, F->b, BC_P, P->omega, P->countMax, P->eps, P->inner_eps,
+                           cnt.data(), res.data(), outer.data());
+  } else if (P->solver == 0) {
+    st = ffvhc_rbgs(c, F->p, F->A, F->b, BC_P, P->omega, P->countMax, P->eps, cnt.data(), res.data());
+  } else if (P->solver == 2) {
+    st = ffvhc_cg(c, F->p, F->A, F->b, BC_P, P->omega, P->countPre, P->countMax, P->eps, cnt.data(), res.data());
+  } else {
+    st = ffvhc_bicgstab(c, F->p, F->A, F->b, BC_P, P->omega, P->countPre, P->countMax, P->eps, cnt.data(),
+                        res.data());
+  }
+  if (st < 0) return st;
+  launch_ghost_pull<double>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S.p, S.hp, BC_P, 1, c->multilevel);
+  HIPCHK(hipEventRecord(ev[2], c->sm));
+  // UpdateU
+  launch_corr(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S, c->dcut, c->dflag, capnc);
+  fill_u();
+  launch_ghost_pull<double>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S.p, S.hp, BC_P, 1, c->multilevel);
+  launch_ghost_pull<double>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S.lapp, H(F->lapp), BC_NULL, 1,
+                            c->multilevel);
+  HIPCHK(hipEventRecord(ev[3], c->sm));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventSynchronize(ev[3]));
+  float t01 = 0.f, t12 = 0.f, t23 = 0.f;
+  HIPCHK(hipEventElapsedTime(&t01, ev[0], ev[1]));
+  HIPCHK(hipEventElapsedTime(&t12, ev[1], ev[2]));
+  HIPCHK(hipEventElapsedTime(&t23, ev[2], ev[3]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (stage_ms) {
+    stage_ms[0] = t01 + t12 + t23;
+    stage_ms[1] = 0.0;
+    stage_ms[2] = t01;
+    stage_ms[3] = 0.0;
+    stage_ms[4] = 0.0;
+    stage_ms[5] = t12;
+    stage_ms[6] = t23;
+  }
+  for (int id : ids) c->cases[id].step++;
+  for (int d = 0; d < 3; d++) {
+    std::swap(F->uc[d], F->ucp[d]);
+    std::swap(F->ud[d], F->udp[d]);
+  }
+  for (size_t i = 0; i < nc; i++)
+    if (info) {
+      info[i].countP = cnt[i];
+      info[i].residualP = res[i];
+      info[i].outer = outer[i];
+      info[i].status = (std::isnan(res[i]) || std::isinf(res[i])) ? 1 : 0;
+    }
+  if (st) set_error("Poisson residual is NaN or Inf");
+  return st;
 }
 
 }  // extern "C"

@@ -181,6 +181,55 @@ int ffvhc_residual(ffvhc_ctx* ctx, int x, const int* A, int b, double* residual)
 int ffvhc_solve_mixed(ffvhc_ctx* ctx, int method, int x, const int* A, int b, int bc_group, double omega,
                       int countMax, double eps, double inner_eps, int* count, double* residual, int* outer);
 
+/* ------------------------------------------------------------------ fractional time step */
+
+/* Field handles of Solver::Update (Solver.cpp:3344-3416), all FP64.  uc/ud receive this step's
+ * convection/diffusion terms and ucp/udp hold the previous step's: on return the library swaps
+ * uc<->ucp and ud<->udp in the struct, which leaves ucp/udp holding this step's terms exactly as
+ * the reference's copies do (Solver.cpp:3725-3732).  t and f may be -1 (zero temperature
+ * deviation / zero body force; the additions are still evaluated, as the reference does). */
+typedef struct ffvhc_step_fields {
+  int ux, uy, uz, p;
+  int uc[3], ucp[3], ud[3], udp[3];
+  int v[6];    /* face velocities vw ve vs vn vb vt */
+  int A[7], b; /* Poisson matrix Ap..At and right-hand side */
+  int lapp;    /* Laplacian of p (bcut_corr_u) */
+  int t;
+  int f[3];
+} ffvhc_step_fields;
+
+/* SolvingMethod/P and its parameters (FFVConfig.cpp:164-165, 434) */
+typedef struct ffvhc_step_params {
+  int solver;        /* 0 RBGS, 2 Jacobi-PCG, 3 Jacobi-BiCGSTAB */
+  int mixed;         /* 1: FP32 inner solve + FP64 defect correction to eps */
+  double omega;
+  int countMax;
+  double eps;
+  int countPre;      /* must be 1 for PCG / BiCGSTAB */
+  double inner_eps;  /* mixed only */
+  int ref_active;    /* bcut_set_reference_value (off by default) */
+  double ref[3];
+  double ref_p;
+} ffvhc_step_params;
+
+typedef struct ffvhc_step_info {
+  int countP;        /* Poisson iterations (inner iterations in total when mixed) */
+  double residualP;  /* relative residual at exit */
+  int outer;         /* defect corrections (mixed) */
+  int status;        /* 0 ok, 1 NaN/Inf residual */
+} ffvhc_step_info;
+
+/* One Solver::Update step with explicit U (UpdateUXe/UYe/UZe 3535-3744, UpdateP 5294-5688,
+ * UpdateU 5690-5771) for every selected case; each case advances its own step counter
+ * (step 0 primes the Adams-Bashforth history).  info[c] per selected case; stage_ms (7 doubles
+ * or NULL) = device ms of {total, -, UX, UY+UZ (0: fused into UX), -, P, U} as Solver::times.
+ * Temperature (stage T) is out of scope. */
+int ffvhc_step(ffvhc_ctx* ctx, ffvhc_step_fields* fields, const ffvhc_step_params* params, ffvhc_step_info* info,
+               double* stage_ms);
+/* Step counter and inflow ramp (TimeControl/Acceleration, Solver.cpp:3347-3355) of a case. */
+int ffvhc_case_set_step(ffvhc_ctx* ctx, int case_id, int step, int accel_steps);
+int ffvhc_case_step(const ffvhc_ctx* ctx, int case_id);
+
 #ifdef __cplusplus
 }
 #endif
