@@ -17,7 +17,8 @@ enum FinalOp {
   FIN_CG_ALPHA,
   FIN_CG_XR,
   FIN_RBGS,
-  FIN_DEFECT
+  FIN_DEFECT,
+  FIN_FIXAVG
 };
 void launch_finalize(hipStream_t sm, const int* caseact, int ncase, const DevCase* cs, SolveState* st,
                      const double* part, size_t nslots, int op, int arg);
@@ -29,7 +30,10 @@ void launch_ghost_pull(hipStream_t sm, const int* active, int nact, const DevBlo
 template <class T>
 void launch_rbgs_iter(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                       SolveState* st, Coef7<T> A, const T* b, const T* xin, T* xout, const T* hin, T* hout,
-                      double* part, size_t nslots, int n, int bcg);
+                      double* part, size_t nslots, int n, int bcg, int mode = 0);
+template <class T>
+void launch_shift_push(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                       const SolveState* st, T* x, const T* hin, T* hout, int bcg);
 
 template <class T>
 void launch_select(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const SolveState* st, T* x0,

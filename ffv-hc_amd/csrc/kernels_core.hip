@@ -84,7 +84,11 @@ __global__ __launch_bounds__(256) void ghost_pull_kernel(const int* __restrict__
 // ------------------------------------------------------------------------------------------
 constexpr int RB_NT = 512;
 
-template <class T>
+// MODE 0: FFVILS::RBGS iteration.  MODE 1: FFVILS::Jacobi iteration (jacobi_smoother,
+// bils.f90:5-55: every cell from the old values, then the ghost fill).  MODE 2: the sweep of
+// FFVILS::RBGS_WithFixedAverage (FFVILS.h:596-659): x^n unshifted, no ghost push, and the block
+// sum of x^n (sf3d_calc_sum) into part[2][slot]; shift_push_kernel then removes the mean.
+template <class T, int MODE>
 __global__ __launch_bounds__(RB_NT) void rbgs_iter_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
                                                           const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                           Coef7<T> A,
@@ -117,7 +121,7 @@ __global__ __launch_bounds__(RB_NT) void rbgs_iter_kernel(const int* __restrict_
   const T omega = (T)st[c].omega;
   const int par = (tx + ty) & 1;
   double rr = 0.0, bb = 0.0;
-  T xr[4], sA[4][7], sb[4];
+  T xr[4], sA[4][7], sb[4], xj[8];
 #pragma unroll
   for (int m = 0; m < 4; m++) {
 #pragma unroll
@@ -136,7 +140,9 @@ __global__ __launch_bounds__(RB_NT) void rbgs_iter_kernel(const int* __restrict_
       const T r = rs - a[0] * xp;
       rr += (double)r * (double)r;
       if (n == 1) bb += (double)b0 * (double)b0;
-      if (isred) {
+      if (MODE == 1) {
+        xj[2 * m + h] = xp - omega * (xp - rs / a[0]);
+      } else if (isred) {
         xr[m] = xp - omega * (xp - rs / a[0]);
       } else {
 #pragma unroll
@@ -145,7 +151,16 @@ __global__ __launch_bounds__(RB_NT) void rbgs_iter_kernel(const int* __restrict_
       }
     }
   }
-  if (sweep) {
+  double xsum = 0.0;
+  if (sweep && MODE == 1) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; q++) s[lds(tx, ty, tz * 8 + q)] = xj[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; q++) xout[base + tx + NB * (ty + NB * (tz * 8 + q))] = xj[q];
+    push_halo<T, RB_NT>(B, cs[c], bcg, slot, [&](int i, int j, int k) { return s[lds(i, j, k)]; }, hin, hout);
+  } else if (sweep) {
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 4; m++) s[lds(tx, ty, tz * 8 + 2 * m + par)] = xr[m];
@@ -163,17 +178,48 @@ __global__ __launch_bounds__(RB_NT) void rbgs_iter_kernel(const int* __restrict_
 #pragma unroll
     for (int kk = 0; kk < 8; kk++) {
       const int k = tz * 8 + kk;
-      xout[base + tx + NB * (ty + NB * k)] = s[lds(tx, ty, k)];
+      const T v = s[lds(tx, ty, k)];
+      xout[base + tx + NB * (ty + NB * k)] = v;
+      if (MODE == 2) xsum += (double)v;
     }
-    push_halo<T, RB_NT>(B, cs[c], bcg, slot, [&](int i, int j, int k) { return s[lds(i, j, k)]; }, hin, hout);
+    if (MODE == 0)
+      push_halo<T, RB_NT>(B, cs[c], bcg, slot, [&](int i, int j, int k) { return s[lds(i, j, k)]; }, hin, hout);
   }
   // residual of x^{n-1} (and b.b on the first launch)
   double rsum = block_sum<RB_NT>(rr, red);
   double bsum = (n == 1) ? block_sum<RB_NT>(bb, red) : 0.0;
+  double ssum = MODE == 2 ? block_sum<RB_NT>(xsum, red) : 0.0;
   if (t == 0) {
     part[slot] = rsum;
     part[nslots + slot] = bsum;
+    if (MODE == 2) part[2 * nslots + slot] = ssum;
   }
+}
+
+// RBGS_WithFixedAverage, second half of an iteration: x^n += a (a = -sum/Ncells from the
+// finalisation, ShiftByA / adda, blas.f90:162-193), then the ghost fill of the shifted x^n.
+template <class T>
+__global__ __launch_bounds__(256) void shift_push_kernel(const int* __restrict__ active,
+                                                         const DevBlock* __restrict__ blk,
+                                                         const DevCase* __restrict__ cs,
+                                                         const SolveState* __restrict__ st, T* __restrict__ x,
+                                                         const T* __restrict__ hin, T* __restrict__ hout, int bcg) {
+  __shared__ T s[LP * LP * LP];
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const int c = B.case_id;
+  if (st[c].done) return;
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  const T a = (T)st[c].alpha;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = base + tx + NB * (ty + NB * k);
+    const T v = x[g] + a;
+    x[g] = v;
+    s[lds(tx, ty, k)] = v;
+  }
+  __syncthreads();
+  push_halo<T, 256>(B, cs[c], bcg, slot, [&](int i, int j, int k) { return s[lds(i, j, k)]; }, hin, hout);
 }
 
 // Copy buffer 1 into buffer 0 for every case whose result landed in buffer 1.
@@ -218,10 +264,24 @@ void launch_ghost_pull(hipStream_t sm, const int* active, int nact, const DevBlo
 template <class T>
 void launch_rbgs_iter(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                       SolveState* st, Coef7<T> A, const T* b, const T* xin, T* xout, const T* hin,
-                      T* hout, double* part, size_t nslots, int n, int bcg) {
+                      T* hout, double* part, size_t nslots, int n, int bcg, int mode) {
   if (nact == 0) return;
-  hipLaunchKernelGGL(rbgs_iter_kernel<T>, dim3(nact), dim3(RB_NT), 0, sm, active, blk, cs, st, A, b, xin,
-                     xout, hin, hout, part, nslots, n, bcg);
+  if (mode == 0)
+    hipLaunchKernelGGL((rbgs_iter_kernel<T, 0>), dim3(nact), dim3(RB_NT), 0, sm, active, blk, cs, st, A, b, xin,
+                       xout, hin, hout, part, nslots, n, bcg);
+  else if (mode == 1)
+    hipLaunchKernelGGL((rbgs_iter_kernel<T, 1>), dim3(nact), dim3(RB_NT), 0, sm, active, blk, cs, st, A, b, xin,
+                       xout, hin, hout, part, nslots, n, bcg);
+  else
+    hipLaunchKernelGGL((rbgs_iter_kernel<T, 2>), dim3(nact), dim3(RB_NT), 0, sm, active, blk, cs, st, A, b, xin,
+                       xout, hin, hout, part, nslots, n, bcg);
+}
+
+template <class T>
+void launch_shift_push(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                       const SolveState* st, T* x, const T* hin, T* hout, int bcg) {
+  if (nact == 0) return;
+  hipLaunchKernelGGL(shift_push_kernel<T>, dim3(nact), dim3(256), 0, sm, active, blk, cs, st, x, hin, hout, bcg);
 }
 
 template <class T>
@@ -244,10 +304,14 @@ template void launch_ghost_pull<float>(hipStream_t, const int*, int, const DevBl
                                        float*, int, int, bool);
 template void launch_rbgs_iter<double>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,
                                        Coef7<double>, const double*, const double*, double*, const double*, double*,
-                                       double*, size_t, int, int);
+                                       double*, size_t, int, int, int);
 template void launch_rbgs_iter<float>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,
                                       Coef7<float>, const float*, const float*, float*, const float*, float*, double*,
-                                      size_t, int, int);
+                                      size_t, int, int, int);
+template void launch_shift_push<double>(hipStream_t, const int*, int, const DevBlock*, const DevCase*,
+                                        const SolveState*, double*, const double*, double*, int);
+template void launch_shift_push<float>(hipStream_t, const int*, int, const DevBlock*, const DevCase*,
+                                       const SolveState*, float*, const float*, float*, int);
 template void launch_select<double>(hipStream_t, const int*, int, const DevBlock*, const SolveState*, double*,
                                     const double*, double*, const double*);
 template void launch_select<float>(hipStream_t, const int*, int, const DevBlock*, const SolveState*, float*,

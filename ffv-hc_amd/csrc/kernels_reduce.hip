@@ -20,9 +20,11 @@ __global__ __launch_bounds__(FIN_NT) void finalize_kernel(const int* __restrict_
   if (st[c].done) return;
   const DevCase& C = cs[c];
   const bool two = op == FIN_BI_INIT || op == FIN_BI_GAMMA || op == FIN_BI_XR || op == FIN_CG_INIT ||
-                   op == FIN_CG_XR || (op == FIN_RBGS && arg == 1) || (op == FIN_DEFECT && arg == 0);
+                   op == FIN_CG_XR || ((op == FIN_RBGS || op == FIN_FIXAVG) && arg == 1) ||
+                   (op == FIN_DEFECT && arg == 0);
   const double t0 = case_sum<FIN_NT>(part, C.slot_begin, C.nblk, red);
   const double t1 = two ? case_sum<FIN_NT>(part + nslots, C.slot_begin, C.nblk, red) : 0.0;
+  const double t2 = op == FIN_FIXAVG ? case_sum<FIN_NT>(part + 2 * nslots, C.slot_begin, C.nblk, red) : 0.0;
   if (threadIdx.x != 0) return;
   SolveState& S = st[c];
   switch (op) {
@@ -107,6 +109,17 @@ __global__ __launch_bounds__(FIN_NT) void finalize_kernel(const int* __restrict_
         converged(S, t0, arg - 1);
         S.parity = (arg - 1) & 1;
       }
+      break;
+    case FIN_FIXAVG:  // as FIN_RBGS, plus the shift a = -sum(x^n) / Ncells for x^n
+      S.iter = arg;
+      if (arg == 1) {
+        S.bb = t1;
+        if (S.countMax < 1) S.done = 1;
+      } else {
+        converged(S, t0, arg - 1);
+        S.parity = (arg - 1) & 1;
+      }
+      if (!S.done) S.alpha = -t2 / (double)((long long)C.nblk * NC);
       break;
     case FIN_DEFECT:  // outer pass arg: FP64 ||b - A x||^2 (and b.b on the first pass)
       if (arg == 0) S.bb = t1;

@@ -49,10 +49,11 @@ inline void fin(ffvhc_ctx* c, SolveState* st, int op, int arg) {
   launch_finalize(c->sm, c->dcaseact, c->ncaseact, c->dcase, st, c->dpart, (size_t)c->cap, op, arg);
 }
 
-// RBGS on field x (ping-pong), matrix K, rhs bv; state in st. Result left in buffer 0.
+// RBGS (mode 0), Jacobi (mode 1) or RBGS with fixed average (mode 2) on field x (ping-pong),
+// matrix K, rhs bv; state in st.  Result left in buffer 0.
 template <class T>
 int rbgs_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double omega, int countMax, double eps,
-              SolveState* st, std::vector<SolveState>& hst, const SolveState* outer, int* launched) {
+              SolveState* st, std::vector<SolveState>& hst, const SolveState* outer, int* launched, int mode = 0) {
   const std::vector<int> ids = selected_cases(c);
   int rc0 = ensure_second_buffer(c, xf);
   if (rc0) return rc0;
@@ -67,8 +68,11 @@ int rbgs_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double ome
     for (; n <= last; n++)
     {
       launch_rbgs_iter<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, xs[(n - 1) & 1], xs[n & 1],
-                          hs[(n - 1) & 1], hs[n & 1], c->dpart, (size_t)c->cap, n, bcg);
-      fin(c, st, FIN_RBGS, n);
+                          hs[(n - 1) & 1], hs[n & 1], c->dpart, (size_t)c->cap, n, bcg, mode);
+      fin(c, st, mode == 2 ? FIN_FIXAVG : FIN_RBGS, n);
+      if (mode == 2)
+        launch_shift_push<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, xs[n & 1], hs[(n - 1) & 1],
+                             hs[n & 1], bcg);
     }
     HIPCHK(hipGetLastError());
     int rc = read_state(c, st, hst, ids, &done);
@@ -369,8 +373,12 @@ int ffvhc_field_copy(ffvhc_ctx* c, int dst, int src) {
   return 0;
 }
 
-int ffvhc_rbgs(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countMax, double eps, int* count,
-               double* residual) {
+}  // extern "C"
+
+namespace {
+
+int stationary(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countMax, double eps,
+               int* count, double* residual, int mode) {
   int prec = 8;
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
@@ -384,14 +392,33 @@ int ffvhc_rbgs(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, 
   if (rc) return rc;
   if (prec == 8)
     rc = rbgs_core<double>(c, x, coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega, countMax,
-                           eps, c->dst, c->hst, nullptr, nullptr);
+                           eps, c->dst, c->hst, nullptr, nullptr, mode);
   else
     rc = rbgs_core<float>(c, x, coef<float>(c, A), (const float*)c->fields[b].d[0], bcg, omega, countMax,
-                          eps, c->dst, c->hst, nullptr, nullptr);
+                          eps, c->dst, c->hst, nullptr, nullptr, mode);
   if (rc) return rc;
   rc = sync_state(c, c->dst, c->hst);
   if (rc) return rc;
   return report(c, c->hst, count, residual);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffvhc_rbgs(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countMax, double eps, int* count,
+               double* residual) {
+  return stationary(c, x, A, b, bcg, omega, countMax, eps, count, residual, 0);
+}
+
+int ffvhc_jacobi(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countMax, double eps,
+                 int* count, double* residual) {
+  return stationary(c, x, A, b, bcg, omega, countMax, eps, count, residual, 1);
+}
+
+int ffvhc_rbgs_fixed_average(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countMax,
+                             double eps, int* count, double* residual) {
+  return stationary(c, x, A, b, bcg, omega, countMax, eps, count, residual, 2);
 }
 
 int ffvhc_rbgs_fixed(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int nsweep, double* ms) {
@@ -596,9 +623,10 @@ int ffvhc_step(ffvhc_ctx* c, ffvhc_step_fields* F, const ffvhc_step_params* P, f
   const int opt[] = {F->t, F->f[0], F->f[1], F->f[2]};
   for (int f : opt)
     if (f != -1 && !field_ok(c, f, 8)) return fail(-1, "t / f must be -1 or FP64 fields");
-  if (P->solver != 0 && P->solver != 2 && P->solver != 3)
-    return fail(-1, "solver: 0 RBGS, 2 Jacobi-PCG, 3 Jacobi-BiCGSTAB");
-  if (P->solver != 0 && P->countPre != 1) return fail(-1, "countPre must be 1");
+  if (P->solver < 0 || P->solver > 3)
+    return fail(-1, "solver: 0 RBGS, 1 RBGS_WithFixedAverage, 2 Jacobi-PCG, 3 Jacobi-BiCGSTAB");
+  if (P->solver >= 2 && P->countPre != 1) return fail(-1, "countPre must be 1");
+  if (P->mixed && P->solver == 1) return fail(-1, "mixed precision takes RBGS, PCG or BiCGSTAB");
   int rc = refresh_active(c);
   if (rc) return rc;
   const std::vector<int> ids = selected_cases(c);
@@ -676,6 +704,9 @@ int ffvhc_step(ffvhc_ctx* c, ffvhc_step_fields* F, const ffvhc_step_params* P, f
                            cnt.data(), res.data(), outer.data());
   } else if (P->solver == 0) {
     st = ffvhc_rbgs(c, F->p, F->A, F->b, BC_P, P->omega, P->countMax, P->eps, cnt.data(), res.data());
+  } else if (P->solver == 1) {
+    st = ffvhc_rbgs_fixed_average(c, F->p, F->A, F->b, BC_P, P->omega, P->countMax, P->eps, cnt.data(),
+                                  res.data());
   } else if (P->solver == 2) {
     st = ffvhc_cg(c, F->p, F->A, F->b, BC_P, P->omega, P->countPre, P->countMax, P->eps, cnt.data(), res.data());
   } else {

@@ -46,6 +46,23 @@ class CaseParams(ctypes.Structure):
                 ("bcv", (c_double * 6) * 6)]
 
 
+class StepFields(ctypes.Structure):
+    """Field handles of one time step (ffvhc_step_fields); uc/ucp and ud/udp swap on return."""
+    _fields_ = [("ux", c_int), ("uy", c_int), ("uz", c_int), ("p", c_int), ("uc", c_int * 3), ("ucp", c_int * 3),
+                ("ud", c_int * 3), ("udp", c_int * 3), ("v", c_int * 6), ("A", c_int * 7), ("b", c_int),
+                ("lapp", c_int), ("t", c_int), ("f", c_int * 3)]
+
+
+class StepParams(ctypes.Structure):
+    _fields_ = [("solver", c_int), ("mixed", c_int), ("omega", c_double), ("countMax", c_int), ("eps", c_double),
+                ("countPre", c_int), ("inner_eps", c_double), ("ref_active", c_int), ("ref", c_double * 3),
+                ("ref_p", c_double)]
+
+
+class StepInfo(ctypes.Structure):
+    _fields_ = [("countP", c_int), ("residualP", c_double), ("outer", c_int), ("status", c_int)]
+
+
 _vp = ctypes.c_void_p
 _sig = {
     "ffvhc_last_error": (ctypes.c_char_p, []),
@@ -79,6 +96,9 @@ _sig = {
     "ffvhc_bicgstab_timing": (c_int, [_vp, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_cg_timing": (c_int, [_vp, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_jacobi": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
+    "ffvhc_step": (c_int, [_vp, _P(StepFields), _P(StepParams), _vp, _vp]),
+    "ffvhc_case_set_step": (c_int, [_vp, c_int, c_int, c_int]),
+    "ffvhc_case_step": (c_int, [_vp, c_int]),
     "ffvhc_residual": (c_int, [_vp, c_int, _vp, c_int, _vp]),
     "ffvhc_solve_mixed": (c_int, [_vp, c_int, c_int, _vp, c_int, c_int, c_double, c_int, c_double, c_double, _vp,
                                   _vp, _vp]),
@@ -300,6 +320,16 @@ class Context:
         check(_L.ffvhc_rbgs_fixed(self.h, x, _ptr(a), b, group, omega, nsweep, _ptr(ms)), "ffvhc_rbgs_fixed")
         return (ms[0], ms[1]) if with_finalize else ms[0]
 
+    def jacobi(self, x, A, b, group, omega, count_max, eps, ncases=None):
+        nc = ncases or self._nsel()
+        cnt = np.zeros(nc, np.int32)
+        res = np.zeros(nc)
+        a = np.asarray(A, np.int32)
+        rc = _L.ffvhc_jacobi(self.h, x, _ptr(a), b, group, omega, count_max, eps, _ptr(cnt), _ptr(res))
+        if rc < 0:
+            check(rc, "ffvhc_jacobi")
+        return cnt, res, rc
+
     def rbgs_fixed_average(self, x, A, b, group, omega, count_max, eps, ncases=None):
         nc = ncases or self._nsel()
         cnt = np.zeros(nc, np.int32)
@@ -343,6 +373,44 @@ class Context:
         a = np.asarray(A, np.int32)
         check(_L.ffvhc_cg_timing(self.h, x, _ptr(a), b, omega, niter, _ptr(ms)), "ffvhc_cg_timing")
         return ms
+
+    def step_fields(self, with_t=False, with_f=False):
+        """Allocate every FP64 field of a time step; returns a StepFields struct."""
+        F = StepFields()
+        F.ux, F.uy, F.uz, F.p = (self.field(8) for _ in range(4))
+        for d in range(3):
+            F.uc[d], F.ucp[d], F.ud[d], F.udp[d] = (self.field(8) for _ in range(4))
+        for q in range(6):
+            F.v[q] = self.field(8)
+        for q in range(7):
+            F.A[q] = self.field(8)
+        F.b, F.lapp = self.field(8), self.field(8)
+        F.t = self.field(8) if with_t else -1
+        for d in range(3):
+            F.f[d] = self.field(8) if with_f else -1
+        return F
+
+    def step(self, F, solver=0, omega=1.2, count_max=100, eps=1e-8, count_pre=1, mixed=False, inner_eps=1e-5,
+             ref=None, ncases=None):
+        """One Solver::Update step of the selected cases; F's uc/ucp, ud/udp swap in place.
+        Returns (per-case StepInfo list, stage ms [7], status)."""
+        P = StepParams()
+        P.solver, P.mixed, P.omega, P.countMax, P.eps = solver, int(mixed), omega, count_max, eps
+        P.countPre, P.inner_eps = count_pre, inner_eps
+        if ref is not None:
+            P.ref_active = 1
+            P.ref[0], P.ref[1], P.ref[2] = ref[:3]
+            P.ref_p = ref[3]
+        nc = ncases or self._nsel()
+        info = (StepInfo * nc)()
+        ms = np.zeros(7)
+        rc = _L.ffvhc_step(self.h, ctypes.byref(F), ctypes.byref(P), info, _ptr(ms))
+        if rc < 0:
+            check(rc, "ffvhc_step")
+        return list(info), ms, rc
+
+    def set_step(self, case_id, step, accel_steps=0):
+        check(_L.ffvhc_case_set_step(self.h, case_id, step, accel_steps), "ffvhc_case_set_step")
 
     def residual(self, x, A, b, ncases=None):
         """FP64 ||b - Ax|| / ||b|| per selected case (x's ghost layer must be current)."""

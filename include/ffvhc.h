@@ -145,7 +145,8 @@ int ffvhc_rbgs(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double 
  * context stream with HIP events around every launch; ms[0] = device time of the iteration
  * kernels, ms[1] = device time of the per-case residual finalisations (timing entry). */
 int ffvhc_rbgs_fixed(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int nsweep, double* ms);
-/* FFVILS::RBGS_WithFixedAverage (FFVILS.h:596-659): mean over Ncells removed every iteration */
+/* FFVILS::RBGS_WithFixedAverage (FFVILS.h:596-659): after every sweep the mean over the case's
+ * cells (Ncells = blocks x 16^3) is removed, then the ghost layer is filled. */
 int ffvhc_rbgs_fixed_average(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, double omega, int countMax,
                              double eps, int* count, double* residual);
 /* FFVILS::CG (FFVILS.h:661-791): Jacobi-preconditioned CG, countPre damped-Jacobi sweeps */
@@ -200,7 +201,7 @@ typedef struct ffvhc_step_fields {
 
 /* SolvingMethod/P and its parameters (FFVConfig.cpp:164-165, 434) */
 typedef struct ffvhc_step_params {
-  int solver;        /* 0 RBGS, 2 Jacobi-PCG, 3 Jacobi-BiCGSTAB */
+  int solver;        /* 0 RBGS, 1 RBGS_WithFixedAverage, 2 Jacobi-PCG, 3 Jacobi-BiCGSTAB */
   int mixed;         /* 1: FP32 inner solve + FP64 defect correction to eps */
   double omega;
   int countMax;

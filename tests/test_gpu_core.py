@@ -121,6 +121,66 @@ def test_rbgs_fixed_count_bit_exact(ctx, name):
         ctx.remove_case(cid)
 
 
+@pytest.mark.parametrize("name", ["flat", "three_level"])
+def test_jacobi_fixed_count_bit_exact(ctx, name):
+    """FFVILS::Jacobi: every update is local, so a fixed sweep count is bit-exact."""
+    tree = trees()[name]
+    params = ffvhc.case_params(bc=BC)
+    cid = ctx.add_case(tree, params)
+    try:
+        og = OracleGrid(tree)
+        rng = np.random.default_rng(11)
+        bct = [params.bct[ffvhc.BC_P][q] for q in range(6)]
+        bcv = [params.bcv[ffvhc.BC_P][q] for q in range(6)]
+        A, b = make_system(og, rng, bct, bcv)
+        x = rng.uniform(-1, 1, og.shape)
+        og.ghost_fill(x, 1, bct, bcv)
+        fa, fb, fx = upload_system(ctx, cid, A, b, x)
+        cnt, res, rc = ctx.jacobi(fx, fa, fb, ffvhc.BC_P, 0.8, 7, 0.0, ncases=1)
+        got = ctx.download(fx, cid, out=x.copy())
+        c2, r2 = ctypes.c_int(), ctypes.c_double()
+        og.L.orc_jacobi(og.ref(), O.ptr(x), O.parr(A), O.ptr(b), O.ptr(og.zeros()), O.cint6(bct), O.cdbl6(bcv),
+                        ctypes.c_double(0.8), 7, ctypes.c_double(0.0), ctypes.byref(c2), ctypes.byref(r2))
+        assert rc == 0 and cnt[0] == c2.value == 8
+        assert abs(res[0] - r2.value) <= 1e-12 * abs(r2.value)
+        assert_live_equal(got, x, "jacobi %s" % name)
+        for f in fa + [fb, fx]:
+            ctx.free(f)
+    finally:
+        ctx.remove_case(cid)
+
+
+def test_rbgs_fixed_average_matches(ctx):
+    """FFVILS::RBGS_WithFixedAverage on an all-Neumann system: the mean is removed every
+    iteration; the mean is a global sum, so agreement is to 1e-12 (sum order differs)."""
+    tree = trees()["two_level"]
+    bc = {ffvhc.BC_P: [(1, 0.0)] * 6}
+    params = ffvhc.case_params(bc=bc)
+    cid = ctx.add_case(tree, params)
+    try:
+        og = OracleGrid(tree)
+        rng = np.random.default_rng(12)
+        bct, bcv = [1] * 6, [0.0] * 6
+        A, b = make_system(og, rng, bct, bcv)
+        x = og.zeros()
+        fa, fb, fx = upload_system(ctx, cid, A, b, x)
+        cnt, res, rc = ctx.rbgs_fixed_average(fx, fa, fb, ffvhc.BC_P, 1.2, 12, 0.0, ncases=1)
+        got = ctx.download(fx, cid, out=x.copy())
+        c2, r2 = ctypes.c_int(), ctypes.c_double()
+        og.L.orc_rbgs_fixed_average(og.ref(), O.ptr(x), O.parr(A), O.ptr(b), O.cint6(bct), O.cdbl6(bcv),
+                                    og.nb * N ** 3, ctypes.c_double(1.2), 12, ctypes.c_double(0.0),
+                                    ctypes.byref(c2), ctypes.byref(r2))
+        assert rc == 0 and cnt[0] == c2.value == 13
+        assert abs(res[0] - r2.value) <= 1e-10 * abs(r2.value)
+        scale = np.max(np.abs(x[:, LIVE]))
+        assert np.max(np.abs(got - x)[:, LIVE]) <= 1e-12 * scale
+        assert abs(np.mean(interior(got))) <= 1e-12 * scale
+        for f in fa + [fb, fx]:
+            ctx.free(f)
+    finally:
+        ctx.remove_case(cid)
+
+
 def test_rbgs_converges_like_oracle(ctx):
     tree = trees()["two_level"]
     params = ffvhc.case_params(bc=BC)
