@@ -134,6 +134,39 @@ def cpu_baseline(tree, A, b, bct, iters_per_solve, budget_s=15.0):
 
 # algorithmic bytes per cell per launch of the fused kernels: interior arrays read/written once,
 # plus the ghost layer (6 x 256 per 4096 cells = 0.375 words) read or pushed once
+def serve_leg(ncases, seed, mixed, budget_s):
+    """Served stream: flow cases of uneven block counts arrive as a Poisson process and run a
+    number of time steps each under continuous batching; p50/p99 time-to-solution."""
+    import ffvhc
+    from serve import Request, Server, uniform_flow_init
+    rng = np.random.default_rng(seed)
+    bc = {ffvhc.BC_UX: [(0, 1.0)] + [(1, 0.0)] * 5,
+          ffvhc.BC_P: [(1, 0.0), (0, 0.0)] + [(1, 0.0)] * 4}
+    reqs, t = [], 0.0
+    for i in range(ncases):
+        nx, ny = int(rng.integers(2, 7)), int(rng.integers(2, 5))
+        regions = []
+        if rng.random() < 0.5:  # refined ball: a 2-level case
+            regions = [dict(kind="ball", a=(nx / 2, ny / 2, ny / 2), r=float(rng.uniform(0.3, 0.8)), level=1)]
+        tree = ffvhc.Tree(root=(nx, ny, ny), root_length=1.0, regions=regions)
+        t += float(rng.exponential(0.15))
+        reqs.append(Request(name="case%d" % i, arrival=t, tree=tree,
+                            params=ffvhc.case_params(mu=float(1.0 / rng.uniform(50, 300)), dt=0.002, bc=bc),
+                            steps=int(rng.integers(5, 21)), init=uniform_flow_init(i)))
+    cap = 4 * max(r.tree.nblk for r in reqs)
+    ctx = ffvhc.Context(max_blocks=cap)
+    F = ctx.step_fields()
+    kw = dict(solver=3, omega=1.0, count_max=2000, eps=1e-8, mixed=mixed, inner_eps=1e-2)
+    srv = Server(ctx, F, kw)
+    srv.run(reqs, t_limit=budget_s)
+    rep = srv.report()
+    rep.update(solver="Jacobi-BiCGSTAB to 1e-8" + (" (FP32 inner + FP64 correction)" if mixed else " (FP64)"),
+               blocks_capacity=cap, blocks_per_case=[int(r.tree.nblk) for r in reqs],
+               arrival_rate_per_s=round(ncases / t, 2))
+    ctx.close()
+    return rep
+
+
 def kernel_bytes(w):
     h = 0.375
     return {"bi_p": (6 + h) * w, "bi_q": (10 + h) * w, "bi_s": (5 + h) * w, "bi_t": (10 + h) * w, "bi_xr": 8 * w,
@@ -154,6 +187,7 @@ def main():
     ap.add_argument("--count-max", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-cases", type=int, default=16)
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: timing loops and one solve per precision")
     args = ap.parse_args()
@@ -255,6 +289,10 @@ def main():
                      "bytes_per_cell": kernel_bytes(4)[dname]},
         "kernels": kt,
     }
+    if args.serve_cases > 0:
+        log("served stream of %d cases" % args.serve_cases)
+        out["serving"] = serve_leg(args.serve_cases, 7, False, 60.0)
+        log("serving: " + json.dumps({k: v for k, v in out["serving"].items() if k != "blocks_per_case"}))
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(tree, A, b, bct, d64["iters"], args.cpu_seconds)

@@ -1,0 +1,68 @@
+"""Served stream on the GPU: a case's fields after its steps are bit-identical whether it runs
+alone or is admitted into a running mix of other cases of different sizes (continuous batching
+with per-case reductions)."""
+import numpy as np
+import pytest
+
+import ffvhc
+from serve import Request, Server, uniform_flow_init
+
+pytestmark = pytest.mark.gpu
+
+BC = {ffvhc.BC_UX: [(0, 1.0), (1, 0.0), (1, 0.0), (1, 0.0), (1, 0.0), (1, 0.0)],
+      ffvhc.BC_P: [(1, 0.0), (0, 0.0), (1, 0.0), (1, 0.0), (1, 0.0), (1, 0.0)]}
+
+
+class StepClock:
+    """Deterministic clock: advances a fixed amount per call."""
+
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 0.25
+        return self.t
+
+
+def tree(kind):
+    if kind == "a":
+        return ffvhc.Tree(root=(2, 2, 2), root_length=1.0, regions=[dict(kind="ball", a=(1.0, 1.0, 1.0), r=0.3,
+                                                                          level=1)])
+    if kind == "b":
+        return ffvhc.Tree(root=(4, 2, 2), root_length=1.0)
+    return ffvhc.Tree(root=(3, 3, 2), root_length=1.0, regions=[dict(kind="ball", a=(1.5, 1.5, 1.0), r=0.4,
+                                                                      level=1)])
+
+
+def serve(plan, step_kwargs, cap=2048):
+    ctx = ffvhc.Context(max_blocks=cap)
+    F = ctx.step_fields()
+    out = {}
+
+    def grab(name):
+        def result(ctx_, cid, F_):
+            out[name] = [ctx_.download(f, cid, out=np.zeros((ctx_.cases[cid].nblk, 20, 20, 20)))
+                         for f in (F_.ux, F_.uy, F_.uz, F_.p, F_.lapp)]
+        return result
+
+    reqs = [Request(name=n, arrival=a, tree=tree(k), params=ffvhc.case_params(mu=0.01, dt=0.005, bc=BC),
+                    steps=s, init=uniform_flow_init(seed), result=grab(n))
+            for n, k, a, s, seed in plan]
+    srv = Server(ctx, F, step_kwargs, clock=StepClock())
+    srv.run(reqs)
+    rep = srv.report()
+    ctx.close()
+    return out, rep, reqs
+
+
+@pytest.mark.parametrize("solver", [0, 3])
+def test_case_alone_equals_case_in_mix(solver):
+    kw = dict(solver=solver, omega=1.2 if solver == 0 else 1.0, count_max=300, eps=1e-6)
+    alone, _, _ = serve([("A", "a", 0.0, 4, 1)], kw)
+    mix, rep, reqs = serve([("B", "b", 0.0, 6, 2), ("C", "c", 0.0, 3, 3), ("A", "a", 1.0, 4, 1)], kw)
+    # A was admitted into a running mix
+    a = [r for r in reqs if r.name == "A"][0]
+    assert a.admitted > 0.0
+    assert rep["cases"] == 3
+    for x, y in zip(alone["A"], mix["A"]):
+        assert np.array_equal(x, y)
