@@ -73,6 +73,8 @@ void launch_convert(hipStream_t sm, const int* active, int nact, float* y, const
 
 void launch_predict(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                     const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc);
+void launch_abd_u(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                  const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc, int dir, double alpha);
 void launch_p1(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc);
 void launch_ab_p(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,

@@ -91,6 +91,78 @@ __device__ inline Face6 grad_faces(const double* __restrict__ p, const double* _
   return r;
 }
 
+// neighbour values of u around a cell: w e s n b t
+struct Nb6 {
+  double p, w, e, s, n, b, t;
+};
+
+__device__ inline Nb6 load_nb(const double* __restrict__ u, const double* __restrict__ hu, int slot, int i, int j,
+                              int k) {
+  Nb6 v;
+  v.p = u[(size_t)slot * NC + i + NB * (j + NB * k)];
+  v.w = at(u, hu, slot, i - 1, j, k);
+  v.e = at(u, hu, slot, i + 1, j, k);
+  v.s = at(u, hu, slot, i, j - 1, k);
+  v.n = at(u, hu, slot, i, j + 1, k);
+  v.b = at(u, hu, slot, i, j, k - 1);
+  v.t = at(u, hu, slot, i, j, k + 1);
+  return v;
+}
+
+// bcut_calc_c_f_c2 (bcut.f90:1583-1764) at one cell
+__device__ inline double conv_c2(const Nb6& f, const double* const* V, size_t g, const Geo& G, double dx) {
+  const double* d = G.d;
+  const double vx0 = V[0][g], vx1 = V[1][g], vy2 = V[2][g], vy3 = V[3][g], vz4 = V[4][g], vz5 = V[5][g];
+  const double f0 = 0.5 * (f.p + f.w), f1 = 0.5 * (f.p + f.e), f2 = 0.5 * (f.p + f.s);
+  const double f3 = 0.5 * (f.p + f.n), f4 = 0.5 * (f.p + f.b), f5 = 0.5 * (f.p + f.t);
+  double q0 = vx0 * f0, q1 = vx1 * f1, q2 = vy2 * f2, q3 = vy3 * f3, q4 = vz4 * f4, q5 = vz5 * f5;
+  if (G.cut(0)) q0 = (d[0] - 0.5) / (d[0] + 0.5) * q1;
+  if (G.cut(1)) q1 = (d[1] - 0.5) / (d[1] + 0.5) * q0;
+  if (G.cut(2)) q2 = (d[2] - 0.5) / (d[2] + 0.5) * q3;
+  if (G.cut(3)) q3 = (d[3] - 0.5) / (d[3] + 0.5) * q2;
+  if (G.cut(4)) q4 = (d[4] - 0.5) / (d[4] + 0.5) * q5;
+  if (G.cut(5)) q5 = (d[5] - 0.5) / (d[5] + 0.5) * q4;
+  if (G.cut(0) && G.cut(1)) { q0 = 0.0; q1 = 0.0; }
+  if (G.cut(2) && G.cut(3)) { q2 = 0.0; q3 = 0.0; }
+  if (G.cut(4) && G.cut(5)) { q4 = 0.0; q5 = 0.0; }
+  double fc = (q1 - q0) / dx + (q3 - q2) / dx + (q5 - q4) / dx;
+  if (!G.fluid()) fc = 0.0;
+  return fc;
+}
+
+// cut-face viscosities and wall masks shared by calc_d_u and calc_abd_u (bcut.f90)
+struct Visc {
+  double mu[6], m[6];
+};
+
+__device__ inline Visc cut_visc(double mu, const Geo& G) {
+  const double* d = G.d;
+  Visc v;
+  for (int q = 0; q < 6; q++) {
+    v.mu[q] = mu;
+    v.m[q] = 0.0;
+  }
+  if (G.cut(0)) { v.mu[0] = mu / d[0] * 2.0 / (d[0] + d[1]); v.mu[1] = mu / d[1] * 2.0 / (d[0] + d[1]); v.m[0] = 1.0; }
+  if (G.cut(1)) { v.mu[0] = mu / d[0] * 2.0 / (d[0] + d[1]); v.mu[1] = mu / d[1] * 2.0 / (d[0] + d[1]); v.m[1] = 1.0; }
+  if (G.cut(2)) { v.mu[2] = mu / d[2] * 2.0 / (d[2] + d[3]); v.mu[3] = mu / d[3] * 2.0 / (d[2] + d[3]); v.m[2] = 1.0; }
+  if (G.cut(3)) { v.mu[2] = mu / d[2] * 2.0 / (d[2] + d[3]); v.mu[3] = mu / d[3] * 2.0 / (d[2] + d[3]); v.m[3] = 1.0; }
+  if (G.cut(4)) { v.mu[4] = mu / d[4] * 2.0 / (d[4] + d[5]); v.mu[5] = mu / d[5] * 2.0 / (d[4] + d[5]); v.m[4] = 1.0; }
+  if (G.cut(5)) { v.mu[4] = mu / d[4] * 2.0 / (d[4] + d[5]); v.mu[5] = mu / d[5] * 2.0 / (d[4] + d[5]); v.m[5] = 1.0; }
+  return v;
+}
+
+// diffusion sum l1 (ue - up) - l0 (up - uw) + ... with wall value Uc behind masked faces
+__device__ inline double diff_sum(const Nb6& f, const double l[6], const double m[6], double Uc) {
+  const double uw = f.w * (1.0 - m[0]) + Uc * m[0];
+  const double ue = f.e * (1.0 - m[1]) + Uc * m[1];
+  const double us = f.s * (1.0 - m[2]) + Uc * m[2];
+  const double un = f.n * (1.0 - m[3]) + Uc * m[3];
+  const double ub = f.b * (1.0 - m[4]) + Uc * m[4];
+  const double ut = f.t * (1.0 - m[5]) + Uc * m[5];
+  return l[1] * (ue - f.p) - l[0] * (f.p - uw) + l[3] * (un - f.p) - l[2] * (f.p - us) + l[5] * (ut - f.p) -
+         l[4] * (f.p - ub);
+}
+
 // ------------------------------------------------------------------ U predictor (explicit)
 __global__ __launch_bounds__(ST_NT) void predict_kernel(const int* __restrict__ active,
                                                         const DevBlock* __restrict__ blk,
@@ -110,52 +182,16 @@ __global__ __launch_bounds__(ST_NT) void predict_kernel(const int* __restrict__ 
     for (int k = 0; k < NB; k++) {
       const size_t g = (size_t)slot * NC + i + NB * (j + NB * k);
       const Geo G = load_geo(cut, flag, capnc, g);
-      const double* d = G.d;
       const bool fluid = G.fluid();
-      const double fp = u[g];
-      const double fw = at(u, hu, slot, i - 1, j, k), fe = at(u, hu, slot, i + 1, j, k);
-      const double fs = at(u, hu, slot, i, j - 1, k), fn = at(u, hu, slot, i, j + 1, k);
-      const double fb = at(u, hu, slot, i, j, k - 1), ft = at(u, hu, slot, i, j, k + 1);
-      // bcut_calc_c_f_c2
-      double fc;
+      const Nb6 f = load_nb(u, hu, slot, i, j, k);
+      const double fp = f.p;
+      const double fc = conv_c2(f, F.v, g, G, dx);  // bcut_calc_c_f_c2
+      double ud;                                     // bcut_calc_d_u
       {
-        const double vx0 = F.v[0][g], vx1 = F.v[1][g], vy2 = F.v[2][g], vy3 = F.v[3][g], vz4 = F.v[4][g],
-                     vz5 = F.v[5][g];
-        const double f0 = 0.5 * (fp + fw), f1 = 0.5 * (fp + fe), f2 = 0.5 * (fp + fs);
-        const double f3 = 0.5 * (fp + fn), f4 = 0.5 * (fp + fb), f5 = 0.5 * (fp + ft);
-        double q0 = vx0 * f0, q1 = vx1 * f1, q2 = vy2 * f2, q3 = vy3 * f3, q4 = vz4 * f4, q5 = vz5 * f5;
-        if (G.cut(0)) q0 = (d[0] - 0.5) / (d[0] + 0.5) * q1;
-        if (G.cut(1)) q1 = (d[1] - 0.5) / (d[1] + 0.5) * q0;
-        if (G.cut(2)) q2 = (d[2] - 0.5) / (d[2] + 0.5) * q3;
-        if (G.cut(3)) q3 = (d[3] - 0.5) / (d[3] + 0.5) * q2;
-        if (G.cut(4)) q4 = (d[4] - 0.5) / (d[4] + 0.5) * q5;
-        if (G.cut(5)) q5 = (d[5] - 0.5) / (d[5] + 0.5) * q4;
-        if (G.cut(0) && G.cut(1)) { q0 = 0.0; q1 = 0.0; }
-        if (G.cut(2) && G.cut(3)) { q2 = 0.0; q3 = 0.0; }
-        if (G.cut(4) && G.cut(5)) { q4 = 0.0; q5 = 0.0; }
-        fc = (q1 - q0) / dx + (q3 - q2) / dx + (q5 - q4) / dx;
-        if (!fluid) fc = 0.0;
-      }
-      // bcut_calc_d_u
-      double ud;
-      {
-        double mu0 = mu, mu1 = mu, mu2 = mu, mu3 = mu, mu4 = mu, mu5 = mu;
-        double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0, m4 = 0.0, m5 = 0.0;
-        if (G.cut(0)) { mu0 = mu / d[0] * 2.0 / (d[0] + d[1]); mu1 = mu / d[1] * 2.0 / (d[0] + d[1]); m0 = 1.0; }
-        if (G.cut(1)) { mu0 = mu / d[0] * 2.0 / (d[0] + d[1]); mu1 = mu / d[1] * 2.0 / (d[0] + d[1]); m1 = 1.0; }
-        if (G.cut(2)) { mu2 = mu / d[2] * 2.0 / (d[2] + d[3]); mu3 = mu / d[3] * 2.0 / (d[2] + d[3]); m2 = 1.0; }
-        if (G.cut(3)) { mu2 = mu / d[2] * 2.0 / (d[2] + d[3]); mu3 = mu / d[3] * 2.0 / (d[2] + d[3]); m3 = 1.0; }
-        if (G.cut(4)) { mu4 = mu / d[4] * 2.0 / (d[4] + d[5]); mu5 = mu / d[5] * 2.0 / (d[4] + d[5]); m4 = 1.0; }
-        if (G.cut(5)) { mu4 = mu / d[4] * 2.0 / (d[4] + d[5]); mu5 = mu / d[5] * 2.0 / (d[4] + d[5]); m5 = 1.0; }
-        const double l0 = mu0 / rhof / (dx * dx), l1 = mu1 / rhof / (dx * dx), l2 = mu2 / rhof / (dx * dx);
-        const double l3 = mu3 / rhof / (dx * dx), l4 = mu4 / rhof / (dx * dx), l5 = mu5 / rhof / (dx * dx);
-        const double uw = fw * (1.0 - m0) + Uc * m0;
-        const double ue = fe * (1.0 - m1) + Uc * m1;
-        const double us = fs * (1.0 - m2) + Uc * m2;
-        const double un = fn * (1.0 - m3) + Uc * m3;
-        const double ub = fb * (1.0 - m4) + Uc * m4;
-        const double ut = ft * (1.0 - m5) + Uc * m5;
-        ud = l1 * (ue - fp) - l0 * (fp - uw) + l3 * (un - fp) - l2 * (fp - us) + l5 * (ut - fp) - l4 * (fp - ub);
+        const Visc vs = cut_visc(mu, G);
+        double l[6];
+        for (int q = 0; q < 6; q++) l[q] = vs.mu[q] / rhof / (dx * dx);
+        ud = diff_sum(f, l, vs.m, Uc);
         if (!fluid) ud = 0.0;
       }
       // bcut_update_u
@@ -173,6 +209,75 @@ __global__ __launch_bounds__(ST_NT) void predict_kernel(const int* __restrict__ 
     __syncthreads();  // every neighbour read of u[dir] in this block is done
     for (int k = 0; k < NB; k++) F.u[dir][(size_t)slot * NC + i + NB * (j + NB * k)] = unew[k];
   }
+}
+
+// ------------------------------------------------------------------ U implicit: matrix and RHS
+// UpdateUX/UY/UZ (Solver.cpp:4391-4690) for component dir: bcut_calc_c_f_c2 into uc,
+// bcut_calc_d_u, then bcut_calc_abd_u (bcut.f90:43-322: A = 1 + alpha sum(l), b with the
+// Adams-Bashforth convection, (1 - alpha) diffusion and the pressure gradient; ud overwritten;
+// solid cells get the identity row, b = Uc and u = 0).  ucp is this step's previous convection
+// (zero before the first step, as the initial Fill leaves it).
+__global__ __launch_bounds__(ST_NT) void abd_u_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                      const DevCase* __restrict__ cs, StepFields F,
+                                                      const float* __restrict__ cut, const uint32_t* __restrict__ flag,
+                                                      size_t capnc, int dir, double alpha) {
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const DevCase& C = cs[B.case_id];
+  const int t = threadIdx.x, i = t & 15, j = t >> 4;
+  const double dx = B.dx, dt = C.dt, rhof = C.rho, mu = C.mu, Uc = 0.0;
+  const bool step0 = C.step == 0;
+  const double* __restrict__ u = F.u[dir];
+  const double* __restrict__ hu = F.hu[dir];
+  double unew[NB];
+  for (int k = 0; k < NB; k++) {
+    const size_t g = (size_t)slot * NC + i + NB * (j + NB * k);
+    const Geo G = load_geo(cut, flag, capnc, g);
+    const bool fluid = G.fluid();
+    const Nb6 f = load_nb(u, hu, slot, i, j, k);
+    const double fc = conv_c2(f, F.v, g, G, dx);
+    // (bcut_calc_d_u's result is overwritten by calc_abd_u below)
+    Visc vs = cut_visc(mu, G);
+    const Face6 r = grad_faces(F.p, F.hp, slot, i, j, k, 1.0 / rhof, dx, true, C.gx, C.gy, C.gz, G);
+    if (G.cut(0) && G.cut(1)) { vs.m[0] = 1.0; vs.m[1] = 1.0; }
+    if (G.cut(2) && G.cut(3)) { vs.m[2] = 1.0; vs.m[3] = 1.0; }
+    if (G.cut(4) && G.cut(5)) { vs.m[4] = 1.0; vs.m[5] = 1.0; }
+    const double rdpx = 0.5 * (r.x0 + r.x1), rdpy = 0.5 * (r.y2 + r.y3), rdpz = 0.5 * (r.z4 + r.z5);
+    const double rdp = dir == 0 ? rdpx : (dir == 1 ? rdpy : rdpz);
+    double l[6];
+    for (int q = 0; q < 6; q++) l[q] = vs.mu[q] / rhof * dt / (dx * dx);
+    // the reference loop zeroes u of solid cells as it goes (u(i,j,k) = 0 at the end of each
+    // cell), so the w, s and b neighbours inside the block, already visited, read 0 if solid
+    Nb6 fz = f;
+    const size_t g0 = (size_t)slot * NC;
+    if (i > 0 && !((flag[g0 + (i - 1) + NB * (j + NB * k)] >> 6) & 1u)) fz.w = 0.0;
+    if (j > 0 && !((flag[g0 + i + NB * ((j - 1) + NB * k)] >> 6) & 1u)) fz.s = 0.0;
+    if (k > 0 && !((flag[g0 + i + NB * (j + NB * (k - 1))] >> 6) & 1u)) fz.b = 0.0;
+    double ud = diff_sum(fz, l, vs.m, Uc);
+    if (!fluid) ud = 0.0;
+    const double* m = vs.m;
+    double a[7];
+    a[0] = 1.0 + alpha * (l[0] + l[1]) + alpha * (l[2] + l[3]) + alpha * (l[4] + l[5]);
+    for (int q = 0; q < 6; q++) a[1 + q] = -(alpha * l[q] * (1.0 - m[q]));
+    const double ucp = step0 ? 0.0 : F.ucp[dir][g];
+    double bv = f.p - (1.5 * fc - 0.5 * ucp) * dt + (1.0 - alpha) * ud * dt - rdp * dt + alpha * l[0] * Uc * m[0] +
+                alpha * l[1] * Uc * m[1] + alpha * l[2] * Uc * m[2] + alpha * l[3] * Uc * m[3] +
+                alpha * l[4] * Uc * m[4] + alpha * l[5] * Uc * m[5];
+    double un = f.p;
+    if (!fluid) {
+      a[0] = 1.0;
+      for (int q = 1; q < 7; q++) a[q] = 0.0;
+      bv = Uc;
+      un = 0.0;
+    }
+    for (int q = 0; q < 7; q++) F.A[q][g] = a[q];
+    F.b[g] = bv;
+    F.uc[dir][g] = fc;
+    F.ud[dir][g] = ud;
+    unew[k] = un;
+  }
+  __syncthreads();  // every neighbour read of u[dir] in this block is done
+  for (int k = 0; k < NB; k++) F.u[dir][(size_t)slot * NC + i + NB * (j + NB * k)] = unew[k];
 }
 
 // ------------------------------------------------------------------ add_g, add_f, remove_p
@@ -380,6 +485,11 @@ __global__ __launch_bounds__(ST_NT) void corr_kernel(const int* __restrict__ act
 void launch_predict(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                     const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc) {
   if (nact) hipLaunchKernelGGL(predict_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, blk, cs, F, cut, flag, capnc);
+}
+void launch_abd_u(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                  const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc, int dir, double alpha) {
+  if (nact)
+    hipLaunchKernelGGL(abd_u_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, blk, cs, F, cut, flag, capnc, dir, alpha);
 }
 void launch_p1(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc) {

@@ -627,6 +627,7 @@ int ffvhc_step(ffvhc_ctx* c, ffvhc_step_fields* F, const ffvhc_step_params* P, f
     return fail(-1, "solver: 0 RBGS, 1 RBGS_WithFixedAverage, 2 Jacobi-PCG, 3 Jacobi-BiCGSTAB");
   if (P->solver >= 2 && P->countPre != 1) return fail(-1, "countPre must be 1");
   if (P->mixed && P->solver == 1) return fail(-1, "mixed precision takes RBGS, PCG or BiCGSTAB");
+  if (P->u_implicit && P->countMax_u < 0) return fail(-1, "countMax_u must be >= 0");
   int rc = refresh_active(c);
   if (rc) return rc;
   const std::vector<int> ids = selected_cases(c);
@@ -663,9 +664,26 @@ int ffvhc_step(ffvhc_ctx* c, ffvhc_step_fields* F, const ffvhc_step_params* P, f
   hipEvent_t ev[4];
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   HIPCHK(hipEventRecord(ev[0], c->sm));
-  // UpdateUXe / UYe / UZe: the three components are independent (face velocities are fixed)
-  launch_predict(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S, c->dcut, c->dflag, capnc);
-  fill_u();
+  const size_t nc = ids.size();
+  std::vector<int> cntu(3 * nc, 0);
+  std::vector<double> resu(3 * nc, 0.0);
+  if (P->u_implicit) {
+    // UpdateUX / UY / UZ: matrix and RHS, the component's matrix BC, BiCGSTAB, ghost fill
+    for (int d = 0; d < 3; d++) {
+      launch_abd_u(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S, c->dcut, c->dflag, capnc, d, P->alpha_u);
+      launch_matrix_bc(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S, BC_UX + d);
+      HIPCHK(hipGetLastError());
+      int stu = ffvhc_bicgstab(c, u[d], F->A, F->b, BC_UX + d, P->omega_u, 1, P->countMax_u, P->eps_u,
+                               cntu.data() + d * nc, resu.data() + d * nc);
+      if (stu < 0) return stu;
+      launch_ghost_pull<double>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S.u[d], S.hu[d], BC_UX + d, 5,
+                                c->multilevel);
+    }
+  } else {
+    // UpdateUXe / UYe / UZe: the three components are independent (face velocities are fixed)
+    launch_predict(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S, c->dcut, c->dflag, capnc);
+    fill_u();
+  }
   HIPCHK(hipEventRecord(ev[1], c->sm));
   // UpdateP
   launch_p1(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S, c->dcut, c->dflag, capnc);
@@ -694,7 +712,6 @@ int ffvhc_step(ffvhc_ctx* c, ffvhc_step_fields* F, const ffvhc_step_params* P, f
   }
   launch_matrix_bc(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, S, BC_P);
   HIPCHK(hipGetLastError());
-  const size_t nc = ids.size();
   std::vector<int> cnt(nc, 0), outer(nc, 0);
   std::vector<double> res(nc, 0.0);
   int st = 0;
@@ -750,6 +767,11 @@ int ffvhc_step(ffvhc_ctx* c, ffvhc_step_fields* F, const ffvhc_step_params* P, f
       info[i].residualP = res[i];
       info[i].outer = outer[i];
       info[i].status = (std::isnan(res[i]) || std::isinf(res[i])) ? 1 : 0;
+      for (int d = 0; d < 3; d++) {
+        info[i].countU[d] = cntu[d * nc + i];
+        info[i].residualU[d] = resu[d * nc + i];
+        if (std::isnan(resu[d * nc + i]) || std::isinf(resu[d * nc + i])) info[i].status = 1;
+      }
     }
   if (st) set_error("Poisson residual is NaN or Inf");
   return st;

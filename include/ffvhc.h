@@ -211,6 +211,13 @@ typedef struct ffvhc_step_params {
   int ref_active;    /* bcut_set_reference_value (off by default) */
   double ref[3];
   double ref_p;
+  /* SolvingMethod/U: 0 explicit (UpdateUXe, Adams-Bashforth 2), 1 implicit (UpdateUX:
+   * bcut_calc_abd_u + Jacobi-BiCGSTAB per component, Solver.cpp:4391-4690) */
+  int u_implicit;
+  double alpha_u;    /* TimeIntegrationMethodForU_Alpha */
+  double omega_u;
+  int countMax_u;
+  double eps_u;
 } ffvhc_step_params;
 
 typedef struct ffvhc_step_info {
@@ -218,6 +225,8 @@ typedef struct ffvhc_step_info {
   double residualP;  /* relative residual at exit */
   int outer;         /* defect corrections (mixed) */
   int status;        /* 0 ok, 1 NaN/Inf residual */
+  int countU[3];     /* implicit U: BiCGSTAB iterations per component */
+  double residualU[3];
 } ffvhc_step_info;
 
 /* One Solver::Update step with explicit U (UpdateUXe/UYe/UZe 3535-3744, UpdateP 5294-5688,

@@ -170,6 +170,7 @@ typedef struct {
   real *lapp;
   real *t0, *f[3];  /* temperature (buoyancy) and body force, executed as in UpdateP */
   real *cgs[5];     /* CG scratch r p q z x0 */
+  real *bis[9];     /* BiCGSTAB scratch r r0 p p_ q_ s s_ t_ x0 (implicit U) */
   real *c[6];
   int *cid[6];
   int *pid;
@@ -189,6 +190,10 @@ typedef struct {
   int ncells; /* numTotalCells (fixed-average mean) */
   int accel_steps; /* TimeControl/Acceleration (0 = off): ramps ux X_M inflow */
   int ref_active; double ref_x, ref_y, ref_z, ref_p;
+  /* implicit U (UpdateUX/UY/UZ, Solver.cpp:4391-4690): calc_abd_u + BiCGSTAB per component */
+  int u_implicit;
+  double alpha_u, omega_u, eps_u;
+  int countMax_u;
 } orc_params;
 
 typedef struct {
@@ -196,6 +201,8 @@ typedef struct {
   double residualP;
   int status;
   double times[7];
+  int countU[3];
+  double residualU[3];
 } orc_step_info;
 
 void orc_update(const orc_grid* gr, orc_fields* F, const orc_params* P, int step, orc_step_info* info);

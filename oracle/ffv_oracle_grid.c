@@ -441,6 +441,34 @@ void orc_update(const orc_grid* gr, orc_fields* F, const orc_params* P, int step
   tt[0] = t0;
   tt[1] = now_s(); /* UpdateT: out of scope */
 
+  if (P->u_implicit) {
+    /* UpdateUX / UY / UZ (Solver.cpp:4391-4690): calc_c_f_c2, calc_d_u, calc_abd_u, copy uc to
+     * ucp per block; matrix BC of the component; BiCGSTAB; ghost fill (Updater5). */
+    for (int d = 0; d < 3; d++) {
+#pragma omp parallel for schedule(static)
+      for (int k = 0; k < nb; k++) {
+        GEOM(F, gr, k)
+        real dx = gr->dx[k];
+        real* uk = BLK(u[d], gr, k);
+        real* Ab[7];
+        for (int q = 0; q < 7; q++) Ab[q] = BLK(F->A[q], gr, k);
+        orc_bcut_calc_c_f_c2(BLK(F->uc[d], gr, k), uk, BLK(F->v[0], gr, k), BLK(F->v[1], gr, k),
+                             BLK(F->v[2], gr, k), BLK(F->v[3], gr, k), BLK(F->v[4], gr, k),
+                             BLK(F->v[5], gr, k), cb, ib, pb, dx, dt, Uc, sz, g);
+        orc_bcut_calc_d_u(BLK(F->ud[d], gr, k), uk, cb, ib, pb, rhof, mu, dx, dt, Uc, sz, g);
+        orc_bcut_calc_abd_u(Ab, BLK(F->b, gr, k), uk, BLK(F->uc[d], gr, k), BLK(F->ucp[d], gr, k),
+                            BLK(F->ud[d], gr, k), BLK(F->p, gr, k), cb, ib, pb, d, P->alpha_u, rhof, mu, dx, dt,
+                            Uc, P->gx, P->gy, P->gz, sz, g);
+        orc_copy(BLK(F->ucp[d], gr, k), BLK(F->uc[d], gr, k), sz, g);
+      }
+      orc_matrix_bc(gr, F->A, F->b, P->bct_u[d], bcv_u[d]);
+      int st = orc_bicgstab(gr, u[d], F->A, F->b, F->bis, P->bct_u[d], bcv_u[d], P->omega_u, 1, P->countMax_u,
+                            P->eps_u, &info->countU[d], &info->residualU[d]);
+      (void)st;
+      orc_ghost_fill(gr, u[d], 5, P->bct_u[d], bcv_u[d]);
+      tt[2 + d] = now_s();
+    }
+  } else {
   /* UpdateUXe / UYe / UZe */
   for (int d = 0; d < 3; d++) {
 #pragma omp parallel for schedule(static)
@@ -464,6 +492,8 @@ void orc_update(const orc_grid* gr, orc_fields* F, const orc_params* P, int step
     }
     orc_ghost_fill(gr, u[d], 5, P->bct_u[d], bcv_u[d]);
     tt[2 + d] = now_s();
+  }
+
   }
 
   /* UpdateP */

@@ -73,7 +73,7 @@ class Fields(ctypes.Structure):
     _fields_ = [("ux", dp), ("uy", dp), ("uz", dp), ("p", dp),
                 ("uc", dp * 3), ("ucp", dp * 3), ("ud", dp * 3), ("udp", dp * 3),
                 ("v", dp * 6), ("A", dp * 7), ("b", dp), ("lapp", dp),
-                ("t0", dp), ("f", dp * 3), ("cgs", dp * 5),
+                ("t0", dp), ("f", dp * 3), ("cgs", dp * 5), ("bis", dp * 9),
                 ("c", dp * 6), ("cid", ip * 6), ("pid", ip)]
 
 
@@ -87,12 +87,14 @@ class Params(ctypes.Structure):
                 ("eps", ctypes.c_double), ("countPre", ctypes.c_int), ("ncells", ctypes.c_int),
                 ("accel_steps", ctypes.c_int),
                 ("ref_active", ctypes.c_int), ("ref_x", ctypes.c_double), ("ref_y", ctypes.c_double),
-                ("ref_z", ctypes.c_double), ("ref_p", ctypes.c_double)]
+                ("ref_z", ctypes.c_double), ("ref_p", ctypes.c_double),
+                ("u_implicit", ctypes.c_int), ("alpha_u", ctypes.c_double), ("omega_u", ctypes.c_double),
+                ("eps_u", ctypes.c_double), ("countMax_u", ctypes.c_int)]
 
 
 class StepInfo(ctypes.Structure):
     _fields_ = [("countP", ctypes.c_int), ("residualP", ctypes.c_double), ("status", ctypes.c_int),
-                ("times", ctypes.c_double * 7)]
+                ("times", ctypes.c_double * 7), ("countU", ctypes.c_int * 3), ("residualU", ctypes.c_double * 3)]
 
 
 def _declare(L):

@@ -49,7 +49,7 @@ def geometry(og, rng):
 class OracleStep:
     """orc_update on host arrays mirroring the device fields."""
 
-    def __init__(self, og, params, geo, arrays, solver, omega, count_max, eps):
+    def __init__(self, og, params, geo, arrays, solver, omega, count_max, eps, implicit=None):
         self.og = og
         self.a = arrays
         self.geo = geo
@@ -68,6 +68,8 @@ class OracleStep:
             F.A[q] = O.ptr(arrays["A"][q])
         for q in range(5):
             F.cgs[q] = O.ptr(arrays["cgs"][q])
+        for q in range(9):
+            F.bis[q] = O.ptr(arrays["bis"][q])
         F.t0 = O.ptr(arrays["t"])
         F.pid = O.ptr(geo[2])
         self.F = F
@@ -84,6 +86,9 @@ class OracleStep:
         P.solver = {0: 0, 2: 2}[solver]
         P.omega, P.countMax, P.eps, P.countPre = omega, count_max, eps, 1
         P.ncells = og.nb * N ** 3
+        if implicit is not None:
+            P.u_implicit = 1
+            P.alpha_u, P.omega_u, P.eps_u, P.countMax_u = implicit
         self.P = P
 
     def step(self, n):
@@ -100,7 +105,7 @@ def initial_arrays(og, rng):
          "uc": [z() for _ in range(3)], "ucp": [z() for _ in range(3)],
          "ud": [z() for _ in range(3)], "udp": [z() for _ in range(3)],
          "f": [z() for _ in range(3)], "v": [z() for _ in range(6)], "A": [z() for _ in range(7)],
-         "cgs": [z() for _ in range(5)]}
+         "cgs": [z() for _ in range(5)], "bis": [z() for _ in range(9)]}
     # face velocities consistent with the initial field (what calc_ab_p of step -1 would leave)
     a["v"][0][:] = a["ux"]
     a["v"][1][:] = a["ux"]
@@ -147,7 +152,7 @@ def ctx():
     c.close()
 
 
-def run_case(ctx, name, solver, omega, count_max, eps, tol, seed=5, nsteps=2):
+def run_case(ctx, name, solver, omega, count_max, eps, tol, seed=5, nsteps=2, implicit=None):
     tree = trees()[name]
     og = OracleGrid(tree)
     rng = np.random.default_rng(seed)
@@ -163,13 +168,19 @@ def run_case(ctx, name, solver, omega, count_max, eps, tol, seed=5, nsteps=2):
             og.ghost_fill(a[key], 5, *bl(d))
         og.ghost_fill(a["p"], 1, *bl(ffvhc.BC_P))
         upload_all(ctx, cid, F, a)
-        orc = OracleStep(og, params, geo, a, solver, omega, count_max, eps)
+        orc = OracleStep(og, params, geo, a, solver, omega, count_max, eps, implicit)
         ctx.select([cid])
+        kw = {}
+        if implicit is not None:
+            kw = dict(implicit_u=True, alpha_u=implicit[0], omega_u=implicit[1], count_max_u=implicit[3],
+                      eps_u=implicit[2])
         for n in range(nsteps):
             info_o = orc.step(n)
-            info_d, ms, rc = ctx.step(F, solver=solver, omega=omega, count_max=count_max, eps=eps, ncases=1)
+            info_d, ms, rc = ctx.step(F, solver=solver, omega=omega, count_max=count_max, eps=eps, ncases=1, **kw)
             assert rc == 0 and info_o.status == 0
             assert info_d[0].countP == info_o.countP, (info_d[0].countP, info_o.countP)
+            if implicit is not None:
+                assert list(info_d[0].countU) == list(info_o.countU)
             compare(ctx, cid, F, a, tol, "%s step %d" % (name, n))
     finally:
         ctx.select(None)
@@ -187,3 +198,10 @@ def test_step_rbgs_bit_exact(ctx, name):
 @pytest.mark.parametrize("name", ["two_level"])
 def test_step_pcg_matches(ctx, name):
     run_case(ctx, name, solver=2, omega=1.0, count_max=15, eps=1e-30, tol=1e-10)
+
+
+@pytest.mark.parametrize("name", ["flat", "two_level"])
+def test_step_implicit_u_matches(ctx, name):
+    """UpdateUX/UY/UZ: calc_abd_u (bit-exact per cell) + BiCGSTAB per component (reduction order
+    differs) then the RBGS pressure solve."""
+    run_case(ctx, name, solver=0, omega=1.2, count_max=20, eps=1e-30, tol=1e-9, implicit=(0.5, 1.0, 1e-30, 8))
