@@ -481,6 +481,62 @@ __global__ __launch_bounds__(ST_NT) void corr_kernel(const int* __restrict__ act
   }
 }
 
+// ------------------------------------------------------------------ pressure force
+// bcut_calc_f_p (bcut.f90:4666-4838) summed as Solver::PrintForceCID does (Solver.cpp:2994-3160):
+// on every face of a fluid cell whose boundary id is cid_target the face pressure is extrapolated
+// from the cell and its opposite neighbour, force += (m1 p1 - m0 p0, ...) dx^2.  One partial
+// triple per block, then fixed-order per-case sums in force_sum_kernel.
+__global__ __launch_bounds__(ST_NT) void force_p_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                        const double* __restrict__ p, const double* __restrict__ hp,
+                                                        const uint8_t* __restrict__ cid8,
+                                                        const uint32_t* __restrict__ flag, size_t capnc, int target,
+                                                        double* __restrict__ part, size_t nslots) {
+  __shared__ double red[ST_NT / 64];
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const int t = threadIdx.x, i = t & 15, j = t >> 4;
+  const double dx = B.dx;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = (size_t)slot * NC + i + NB * (j + NB * k);
+    const double pp = p[g];
+    const double pw = at(p, hp, slot, i - 1, j, k), pe = at(p, hp, slot, i + 1, j, k);
+    const double ps = at(p, hp, slot, i, j - 1, k), pn = at(p, hp, slot, i, j + 1, k);
+    const double pb = at(p, hp, slot, i, j, k - 1), pt = at(p, hp, slot, i, j, k + 1);
+    double p0 = 0.5 * (pp + pw), p1 = 0.5 * (pp + pe), p2 = 0.5 * (pp + ps);
+    double p3 = 0.5 * (pp + pn), p4 = 0.5 * (pp + pb), p5 = 0.5 * (pp + pt);
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0, m4 = 0.0, m5 = 0.0;
+    if (cid8[0 * capnc + g] == target) { p0 = pp + (pp - pe) * 0.5; m0 = 1.0; }
+    if (cid8[1 * capnc + g] == target) { p1 = pp + (pp - pw) * 0.5; m1 = 1.0; }
+    if (cid8[2 * capnc + g] == target) { p2 = pp + (pp - pn) * 0.5; m2 = 1.0; }
+    if (cid8[3 * capnc + g] == target) { p3 = pp + (pp - ps) * 0.5; m3 = 1.0; }
+    if (cid8[4 * capnc + g] == target) { p4 = pp + (pp - pt) * 0.5; m4 = 1.0; }
+    if (cid8[5 * capnc + g] == target) { p5 = pp + (pp - pb) * 0.5; m5 = 1.0; }
+    if (!((flag[g] >> 6) & 1u)) { m0 = m1 = m2 = m3 = m4 = m5 = 0.0; }
+    fx += (m1 * p1 - m0 * p0) * dx * dx;
+    fy += (m3 * p3 - m2 * p2) * dx * dx;
+    fz += (m5 * p5 - m4 * p4) * dx * dx;
+  }
+  const double sx = block_sum<ST_NT>(fx, red), sy = block_sum<ST_NT>(fy, red), sz = block_sum<ST_NT>(fz, red);
+  if (t == 0) {
+    part[slot] = sx;
+    part[nslots + slot] = sy;
+    part[2 * nslots + slot] = sz;
+  }
+}
+
+__global__ __launch_bounds__(ST_NT) void force_sum_kernel(const int* __restrict__ caseact,
+                                                          const DevCase* __restrict__ cs,
+                                                          const double* __restrict__ part, size_t nslots,
+                                                          double* __restrict__ out) {
+  __shared__ double red[ST_NT / 64];
+  const DevCase& C = cs[caseact[blockIdx.x]];
+  for (int q = 0; q < 3; q++) {
+    const double v = case_sum<ST_NT>(part + q * nslots, C.slot_begin, C.nblk, red);
+    if (threadIdx.x == 0) out[3 * blockIdx.x + q] = v;
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 void launch_predict(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                     const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc) {
@@ -506,6 +562,14 @@ void launch_refval(hipStream_t sm, const int* slots, int n, const DevBlock* blk,
 void launch_matrix_bc(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                       const StepFields& F, int bcg) {
   if (nact) hipLaunchKernelGGL(matrix_bc_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, blk, cs, F, bcg);
+}
+void launch_force_p(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase, const DevBlock* blk,
+                    const DevCase* cs, const double* p, const double* hp, const uint8_t* cid8, const uint32_t* flag,
+                    size_t capnc, int target, double* part, size_t nslots, double* out) {
+  if (!nact || !ncase) return;
+  hipLaunchKernelGGL(force_p_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, blk, p, hp, cid8, flag, capnc, target,
+                     part, nslots);
+  hipLaunchKernelGGL(force_sum_kernel, dim3(ncase), dim3(ST_NT), 0, sm, caseact, cs, part, nslots, out);
 }
 void launch_corr(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                  const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc) {

@@ -598,6 +598,36 @@ int ffvhc_residual(ffvhc_ctx* c, int x, const int* A, int b, double* residual) {
   return 0;
 }
 
+int ffvhc_force_p(ffvhc_ctx* c, int p, int cut_id, double* f) {
+  if (!c || !f) return fail(-1, "null argument");
+  if (!field_ok(c, p, 8)) return fail(-1, "p must be an FP64 field");
+  if (cut_id < 1 || cut_id > 255) return fail(-1, "cut_id must be in 1..255");
+  int rc = refresh_active(c);
+  if (rc) return rc;
+  double* dout = nullptr;
+  HIPCHK(hipMallocAsync((void**)&dout, sizeof(double) * 3 * std::max(c->ncaseact, 1), c->sm));
+  launch_force_p(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase,
+                 (const double*)c->fields[p].d[0], (const double*)c->fields[p].h[0], c->dcid, c->dflag,
+                 (size_t)c->cap * NC, cut_id, c->dpart, (size_t)c->cap, dout);
+  HIPCHK(hipGetLastError());
+  std::vector<double> h(3 * std::max(c->ncaseact, 1), 0.0);
+  HIPCHK(hipMemcpyAsync(h.data(), dout, sizeof(double) * 3 * c->ncaseact, hipMemcpyDeviceToHost, c->sm));
+  HIPCHK(hipFreeAsync(dout, c->sm));
+  HIPCHK(hipStreamSynchronize(c->sm));
+  // caseact lists the selected cases with blocks in selection order
+  const std::vector<int> ids = selected_cases(c);
+  size_t q = 0;
+  for (size_t i = 0; i < ids.size(); i++) {
+    if (c->cases[ids[i]].nblk > 0) {
+      for (int d = 0; d < 3; d++) f[3 * i + d] = h[3 * q + d];
+      q++;
+    } else {
+      for (int d = 0; d < 3; d++) f[3 * i + d] = 0.0;
+    }
+  }
+  return 0;
+}
+
 int ffvhc_case_set_step(ffvhc_ctx* c, int id, int step, int accel_steps) {
   if (!c || id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
   if (step < 0 || accel_steps < 0) return fail(-1, "step and accel_steps must be >= 0");

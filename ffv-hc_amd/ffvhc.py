@@ -100,6 +100,7 @@ _sig = {
     "ffvhc_jacobi": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
     "ffvhc_step": (c_int, [_vp, _P(StepFields), _P(StepParams), _vp, _vp]),
     "ffvhc_case_set_step": (c_int, [_vp, c_int, c_int, c_int]),
+    "ffvhc_force_p": (c_int, [_vp, c_int, c_int, _vp]),
     "ffvhc_case_step": (c_int, [_vp, c_int]),
     "ffvhc_residual": (c_int, [_vp, c_int, _vp, c_int, _vp]),
     "ffvhc_solve_mixed": (c_int, [_vp, c_int, c_int, _vp, c_int, c_int, c_double, c_int, c_double, c_double, _vp,
@@ -411,6 +412,12 @@ class Context:
         if rc < 0:
             check(rc, "ffvhc_step")
         return list(info), ms, rc
+
+    def force_p(self, p, cut_id, ncases=None):
+        """Pressure force (fx, fy, fz) on the faces with boundary id cut_id, per selected case."""
+        f = np.zeros((ncases or self._nsel(), 3))
+        check(_L.ffvhc_force_p(self.h, p, cut_id, _ptr(f)), "ffvhc_force_p")
+        return f
 
     def set_step(self, case_id, step, accel_steps=0):
         check(_L.ffvhc_case_set_step(self.h, case_id, step, accel_steps), "ffvhc_case_set_step")

@@ -236,6 +236,11 @@ typedef struct ffvhc_step_info {
  * Temperature (stage T) is out of scope. */
 int ffvhc_step(ffvhc_ctx* ctx, ffvhc_step_fields* fields, const ffvhc_step_params* params, ffvhc_step_info* info,
                double* stage_ms);
+/* Pressure force on the faces with boundary id cut_id (bcut_calc_f_p, bcut.f90:4666-4838, as summed
+ * by Solver::PrintForceCID, Solver.cpp:2994-3160): f[3*i .. 3*i+2] for the i-th selected case.
+ * p's ghost layer must be current. */
+int ffvhc_force_p(ffvhc_ctx* ctx, int p, int cut_id, double* f);
+
 /* Step counter and inflow ramp (TimeControl/Acceleration, Solver.cpp:3347-3355) of a case. */
 int ffvhc_case_set_step(ffvhc_ctx* ctx, int case_id, int step, int accel_steps);
 int ffvhc_case_step(const ffvhc_ctx* ctx, int case_id);

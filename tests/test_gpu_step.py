@@ -205,3 +205,41 @@ def test_step_implicit_u_matches(ctx, name):
     """UpdateUX/UY/UZ: calc_abd_u (bit-exact per cell) + BiCGSTAB per component (reduction order
     differs) then the RBGS pressure solve."""
     run_case(ctx, name, solver=0, omega=1.2, count_max=20, eps=1e-30, tol=1e-9, implicit=(0.5, 1.0, 1e-30, 8))
+
+
+@pytest.mark.parametrize("name", ["flat", "two_level"])
+def test_force_p_matches_oracle(ctx, name):
+    """bcut_calc_f_p summed over the blocks (Solver::PrintForceCID) for each boundary id."""
+    tree = trees()[name]
+    og = OracleGrid(tree)
+    rng = np.random.default_rng(21)
+    geo = geometry(og, rng)
+    params = ffvhc.case_params(bc=BC)
+    cid = ctx.add_case(tree, params, geometry=(geo[0], geo[1], geo[2]))
+    pf = ctx.field(8)
+    try:
+        p = rng.standard_normal(og.shape)
+        og.ghost_fill(p, 1, [params.bct[ffvhc.BC_P][f] for f in range(6)],
+                      [params.bcv[ffvhc.BC_P][f] for f in range(6)])
+        ctx.upload(pf, cid, p)
+        ctx.select([cid])
+        sz = (ctypes.c_int * 3)(N, N, N)
+        for target in (1, 2):
+            got = ctx.force_p(pf, target, ncases=1)[0]
+            want = np.zeros(3)
+            npad = (N + 2 * G) ** 3
+            for b in range(og.nb):
+                fs = [np.zeros(npad) for _ in range(3)]
+                out = np.zeros(3)
+                cb = [np.ascontiguousarray(geo[0][f][b]) for f in range(6)]
+                ib = [np.ascontiguousarray(geo[1][f][b]) for f in range(6)]
+                og.L.orc_bcut_calc_f_p(O.ptr(fs[0]), O.ptr(fs[1]), O.ptr(fs[2]), O.ptr(out), target,
+                                       O.ptr(np.ascontiguousarray(p[b])), O.parr(cb), O.parr(ib, ctypes.c_int),
+                                       O.ptr(np.ascontiguousarray(geo[2][b])), ctypes.c_double(og.tb["dx"][b]),
+                                       ctypes.c_double(0.01), sz, G)
+                want += out
+            assert np.all(np.abs(got - want) <= 1e-12 * np.max(np.abs(want)) + 1e-300), (got, want)
+    finally:
+        ctx.select(None)
+        ctx.free(pf)
+        ctx.remove_case(cid)
