@@ -72,8 +72,8 @@ struct StepFields {
   const double* f[3];  // body force or null = 0
 };
 
-// Per-case state of an iterative solve, updated on the device by the last workgroup of each
-// reduction (deterministic: partials are summed in slot order).
+// Per-case state of an iterative solve, updated on the device by the per-case finalisation of
+// each reduction (deterministic: partials are summed in slot order).
 struct SolveState {
   double bb;        // b.b
   double rr;        // last residual norm squared
@@ -87,7 +87,7 @@ struct SolveState {
   int status;       // 0 ok, 1 NaN/Inf residual (reference Exit(0))
   int parity;       // buffer that holds the result (RBGS ping-pong)
   int iter;         // iterations launched so far
-  unsigned int ticket;  // last-block counter
+  int restart;      // Krylov restart pending: the next P step takes p = r (reliable update)
   int pad;
 };
 

@@ -11,7 +11,7 @@ namespace ffvhc {
 
 // ------------------------------------------------------------------------------------------
 // Per-case deterministic reductions: every workgroup writes its block's partial sum(s); the
-// last workgroup of the case (ticket) adds the partials in slot order with a fixed tree.
+// per-case finalisation adds the partials in slot order with a fixed tree.
 // ------------------------------------------------------------------------------------------
 
 template <int NT>
@@ -82,19 +82,6 @@ __device__ inline void stage_block(T* s, const T* __restrict__ x, const T* __res
   }
 }
 
-// returns true in every thread of the last workgroup of case c
-__device__ inline bool last_of_case(SolveState* st, int c, int nblk, int* flag) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned int prev = atomicAdd(&st[c].ticket, 1u);
-    *flag = (prev == (unsigned int)(nblk - 1));
-  }
-  __syncthreads();
-  bool last = *flag;
-  if (last) __threadfence();
-  return last;
-}
 
 __device__ inline void converged(SolveState& s, double rr, int count) {
   // FFVILS.h:24-47 IsConverged

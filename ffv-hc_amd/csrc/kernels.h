@@ -18,10 +18,12 @@ enum FinalOp {
   FIN_CG_XR,
   FIN_RBGS,
   FIN_DEFECT,
-  FIN_FIXAVG
+  FIN_FIXAVG,
+  FIN_BI_REANCHOR
 };
+// gate: for FIN_BI_REANCHOR, the outer state; cases whose outer solve is done are skipped
 void launch_finalize(hipStream_t sm, const int* caseact, int ncase, const DevCase* cs, SolveState* st,
-                     const double* part, size_t nslots, int op, int arg);
+                     const double* part, size_t nslots, int op, int arg, const SolveState* gate = nullptr);
 
 template <class T>
 void launch_ghost_pull(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
@@ -64,6 +66,12 @@ template <class T>
 void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase,
                           const DevBlock* blk, const DevCase* cs, SolveState* st, Coef7<T> A, T* x, BiVecs<T> V,
                           double* part, size_t nslots, int count, int mask = 31);
+template <class T>
+void launch_bicgstab_reanchor(hipStream_t sm, const int* active, int nact, const DevBlock* blk,
+                              const SolveState* gate, BiVecs<T> V, double* part, size_t nslots);
+template <class T>
+void launch_bicgstab_restart(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const SolveState* st,
+                             BiVecs<T> V);
 void launch_defect(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                    SolveState* st, Coef7<double> A, const double* b, const double* x, const double* hx, float* r32,
                    double* part, size_t nslots, int outer);

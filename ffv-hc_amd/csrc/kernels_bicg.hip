@@ -1,6 +1,6 @@
 // Jacobi right-preconditioned BiCGSTAB (FFVILS::BiCGSTAB, FFVILS.h:793-974) as five fused kernels
 // per iteration; every scalar (alpha, gamma, beta, the convergence test) is computed on the device
-// by the last workgroup of each case.  The preconditioner is one damped-Jacobi sweep from zero
+// by the per-case finalisation (kernels_reduce.hip).  The preconditioner is one damped-Jacobi sweep from zero
 // (countPreConditioner = 1), value-identical to omega * (v / Ap) (see kernels_cg.hip).
 //
 //   init : r = b - A x, r0 = r; b.b, r.r0                               [FFVILS.h:810-826]
@@ -82,6 +82,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
   const size_t base = (size_t)slot * NC;
   const T omega = (T)st[c].omega;
   const T ng = (T)(-st[c].gamma), beta = (T)st[c].beta, na = (T)(-st[c].alpha);
+  const bool first = count == 1 || st[c].restart;
   constexpr int KB = Batch<T>::K;
 #pragma unroll
   for (int k0 = 0; k0 < NB; k0 += KB) {
@@ -91,15 +92,15 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
       rv[kk] = V.r[g];
       ap[kk] = A.p[0][g];
-      if (MODE == 1 || count != 1) qv[kk] = V.q[g];
-      if (MODE == 0 && count != 1) pv[kk] = V.p[g];
+      if (MODE == 1 || !first) qv[kk] = V.q[g];
+      if (MODE == 0 && !first) pv[kk] = V.p[g];
     }
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
       T v;
       if (MODE == 0) {
-        if (count == 1) {
+        if (first) {
           v = rv[kk];
         } else {
           const T tmp = pv[kk] + ng * qv[kk];  // AXPY(p, q_, -gamma)
@@ -218,6 +219,53 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
   }
 }
 
+// Reliable update (mixed precision): r has just been replaced by the true residual of the FP64
+// solution; r.r becomes the new reference norm and r.r0 the new rho (part[0], part[1]).
+template <class T>
+__global__ __launch_bounds__(BI_NT) void bi_reanchor_kernel(const int* __restrict__ active,
+                                                            const DevBlock* __restrict__ blk,
+                                                            const SolveState* __restrict__ gate, BiVecs<T> V,
+                                                            double* __restrict__ part, size_t nslots) {
+  __shared__ double red[BI_NT / 64];
+  const int slot = active[blockIdx.x];
+  if (gate[blk[slot].case_id].done) return;
+  const size_t base = (size_t)slot * NC;
+  double rr = 0.0, rr0 = 0.0;
+  for (int q = threadIdx.x; q < NC; q += BI_NT) {
+    const double r = (double)V.r[base + q];
+    rr += r * r;
+    rr0 += r * (double)V.r0[base + q];
+  }
+  const double s0 = block_sum<BI_NT>(rr, red), s1 = block_sum<BI_NT>(rr0, red);
+  if (threadIdx.x == 0) {
+    part[slot] = s0;
+    part[nslots + slot] = s1;
+  }
+}
+
+// Restart after a breakdown at a reliable update: r0 := r for the cases that asked for it.
+template <class T>
+__global__ __launch_bounds__(BI_NT) void bi_restart_kernel(const int* __restrict__ active,
+                                                           const DevBlock* __restrict__ blk,
+                                                           const SolveState* __restrict__ st, BiVecs<T> V) {
+  const int slot = active[blockIdx.x];
+  if (!st[blk[slot].case_id].restart) return;
+  const size_t base = (size_t)slot * NC;
+  for (int q = threadIdx.x; q < NC; q += BI_NT) V.r0[base + q] = V.r[base + q];
+}
+
+template <class T>
+void launch_bicgstab_reanchor(hipStream_t sm, const int* active, int nact, const DevBlock* blk,
+                              const SolveState* gate, BiVecs<T> V, double* part, size_t nslots) {
+  if (nact) hipLaunchKernelGGL(bi_reanchor_kernel<T>, dim3(nact), dim3(BI_NT), 0, sm, active, blk, gate, V, part, nslots);
+}
+
+template <class T>
+void launch_bicgstab_restart(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const SolveState* st,
+                             BiVecs<T> V) {
+  if (nact) hipLaunchKernelGGL(bi_restart_kernel<T>, dim3(nact), dim3(BI_NT), 0, sm, active, blk, st, V);
+}
+
 template <class T>
 void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                           SolveState* st, Coef7<T> A, const T* b, const T* x, const T* hx, BiVecs<T> V, double* part,
@@ -258,7 +306,10 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
                                         Coef7<T>, const T*, const T*, const T*, BiVecs<T>, double*, size_t);         \
   template void launch_bicgstab_iter<T>(hipStream_t, const int*, int, const int*, int, const DevBlock*,            \
                                         const DevCase*, SolveState*, Coef7<T>, T*, BiVecs<T>, double*, size_t, int,  \
-                                        int);
+                                        int);                                                                        \
+  template void launch_bicgstab_reanchor<T>(hipStream_t, const int*, int, const DevBlock*, const SolveState*,         \
+                                            BiVecs<T>, double*, size_t);                                             \
+  template void launch_bicgstab_restart<T>(hipStream_t, const int*, int, const DevBlock*, const SolveState*, BiVecs<T>);
 INST(double)
 INST(float)
 #undef INST

@@ -1,6 +1,6 @@
 // Jacobi-preconditioned CG (FFVILS::CG, FFVILS.h:661-791) as three fused kernels per iteration,
-// with every scalar (alpha, beta, the convergence test) computed on the device by the last
-// workgroup of each case.  The preconditioner is the reference's Jacobi_PreConditioner with
+// with every scalar (alpha, beta, the convergence test) computed on the device by the per-case
+// finalisation (kernels_reduce.hip).  The preconditioner is the reference's Jacobi_PreConditioner with
 // countPreConditioner = 1 from z = 0 (FFVILS.h:468-490, bils.f90:5-55):
 // z = 0 - omega*(0 - (r - A_nb*0 - ...)/Ap), which equals omega*(r/Ap) exactly in IEEE arithmetic
 // (every A_nb*0 is a signed zero, x - (+-0) = x, 0 - (0 - y) = y), so only Ap is read.

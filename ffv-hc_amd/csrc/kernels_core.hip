@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void ghost_pull_kernel(const int* __restrict__
 //   * red then black SOR on block-local parity with the ghost layer frozen,
 //   * x^n to the other ping-pong buffer, its ghost layer pushed into the neighbours' next halo
 //     (same-level copy, F2C mean, C2F trilinear: Scalar3DUpdater1 semantics) and the outer BC.
-// The last workgroup of each case finishes the residual of x^{n-1} and sets `done`; later
+// The per-case finalisation completes the residual of x^{n-1} and sets `done`; later
 // launches of a finished case return at once, so the device never waits for the host.
 // ------------------------------------------------------------------------------------------
 constexpr int RB_NT = 512;

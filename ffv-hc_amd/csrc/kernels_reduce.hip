@@ -14,14 +14,18 @@ constexpr int FIN_NT = 256;
 __global__ __launch_bounds__(FIN_NT) void finalize_kernel(const int* __restrict__ caseact,
                                                           const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                           const double* __restrict__ part, size_t nslots, int op,
-                                                          int arg) {
+                                                          int arg, const SolveState* __restrict__ gate) {
   __shared__ double red[FIN_NT / 64];
   const int c = caseact[blockIdx.x];
-  if (st[c].done) return;
+  if (op == FIN_BI_REANCHOR) {
+    if (gate && gate[c].done) return;
+  } else if (st[c].done) {
+    return;
+  }
   const DevCase& C = cs[c];
   const bool two = op == FIN_BI_INIT || op == FIN_BI_GAMMA || op == FIN_BI_XR || op == FIN_CG_INIT ||
                    op == FIN_CG_XR || ((op == FIN_RBGS || op == FIN_FIXAVG) && arg == 1) ||
-                   (op == FIN_DEFECT && arg == 0);
+                   (op == FIN_DEFECT && arg == 0) || op == FIN_BI_REANCHOR;
   const double t0 = case_sum<FIN_NT>(part, C.slot_begin, C.nblk, red);
   const double t1 = two ? case_sum<FIN_NT>(part + nslots, C.slot_begin, C.nblk, red) : 0.0;
   const double t2 = op == FIN_FIXAVG ? case_sum<FIN_NT>(part + 2 * nslots, C.slot_begin, C.nblk, red) : 0.0;
@@ -47,6 +51,7 @@ __global__ __launch_bounds__(FIN_NT) void finalize_kernel(const int* __restrict_
       break;
     case FIN_BI_ALPHA:  // FFVILS.h:868-872: alpha = rr1 / (q_.r0)
       S.alpha = S.rr1 / t0;
+      S.restart = 0;
       break;
     case FIN_BI_GAMMA:  // FFVILS.h:905-915: gamma = (t_.s) / (t_.t_)
       S.gamma = t0 / t1;
@@ -54,8 +59,8 @@ __global__ __launch_bounds__(FIN_NT) void finalize_kernel(const int* __restrict_
     case FIN_BI_XR:  // FFVILS.h:925-965: t0 = r.r (IsConverged), t1 = next r.r0
       S.iter = arg;
       converged(S, t0, arg);
+      S.rr0 = S.rr1;  // this iteration's rho becomes the previous one (also kept for a reliable update)
       if (!S.done) {
-        S.rr0 = S.rr1;
         if (fabs(t1) < FLT_MIN) {  // next iteration's r.r0 test (FFVILS.h:829-834)
           S.residual = t1;
           S.count = 0;
@@ -121,6 +126,18 @@ __global__ __launch_bounds__(FIN_NT) void finalize_kernel(const int* __restrict_
       }
       if (!S.done) S.alpha = -t2 / (double)((long long)C.nblk * NC);
       break;
+    case FIN_BI_REANCHOR:  // reliable update: t0 = r.r (new reference), t1 = r.r0 (new rho)
+      S.bb = t0;
+      S.done = 0;
+      S.status = 0;
+      if (fabs(t1) < FLT_MIN || !(fabs(t1) <= DBL_MAX)) {
+        S.restart = 1;  // r0 := r is done by the host; p := r on the next P step
+        S.rr1 = t0;
+      } else {
+        S.rr1 = t1;
+        S.beta = t1 / S.rr0 * S.alpha / S.gamma;
+      }
+      break;
     case FIN_DEFECT:  // outer pass arg: FP64 ||b - A x||^2 (and b.b on the first pass)
       if (arg == 0) S.bb = t1;
       S.iter = arg;
@@ -132,9 +149,10 @@ __global__ __launch_bounds__(FIN_NT) void finalize_kernel(const int* __restrict_
 }
 
 void launch_finalize(hipStream_t sm, const int* caseact, int ncase, const DevCase* cs, SolveState* st,
-                     const double* part, size_t nslots, int op, int arg) {
+                     const double* part, size_t nslots, int op, int arg, const SolveState* gate) {
   if (ncase > 0)
-    hipLaunchKernelGGL(finalize_kernel, dim3(ncase), dim3(FIN_NT), 0, sm, caseact, cs, st, part, nslots, op, arg);
+    hipLaunchKernelGGL(finalize_kernel, dim3(ncase), dim3(FIN_NT), 0, sm, caseact, cs, st, part, nslots, op, arg,
+                       gate);
 }
 
 }  // namespace ffvhc

@@ -45,8 +45,8 @@ int check_args(ffvhc_ctx* c, int x, const int* A, int b, int prec) {
 }
 
 // per-case scalar step after a reducing kernel (sums the per-block partials in slot order)
-inline void fin(ffvhc_ctx* c, SolveState* st, int op, int arg) {
-  launch_finalize(c->sm, c->dcaseact, c->ncaseact, c->dcase, st, c->dpart, (size_t)c->cap, op, arg);
+inline void fin(ffvhc_ctx* c, SolveState* st, int op, int arg, const SolveState* gate = nullptr) {
+  launch_finalize(c->sm, c->dcaseact, c->ncaseact, c->dcase, st, c->dpart, (size_t)c->cap, op, arg, gate);
 }
 
 // RBGS (mode 0), Jacobi (mode 1) or RBGS with fixed average (mode 2) on field x (ping-pong),
@@ -404,6 +404,104 @@ int stationary(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, 
 
 }  // namespace
 
+namespace {
+
+// Mixed-precision BiCGSTAB with reliable updates: the Krylov recurrence runs in FP32 on the
+// correction e (x = x64 + e); whenever the recursive residual has dropped by delta since the
+// last anchor, e is folded into the FP64 solution, the true residual b - A x64 is computed in
+// FP64 (the convergence test), and it replaces the FP32 residual without restarting the
+// recurrence (rho recomputed; a restart with r0 := r only after a breakdown).
+int bicgstab_reliable(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countMax, double eps,
+                      double delta, int* count, double* residual, int* outer) {
+  const std::vector<int> ids = selected_cases(c);
+  int fa[7], fe;
+  for (int q = 0; q < 7; q++) {
+    int rc = scratch_field(c, 4, R_A + q, &fa[q]);
+    if (rc) return rc;
+  }
+  int rc = scratch_field(c, 4, R_E, &fe);
+  if (rc) return rc;
+  BiVecs<float> V;
+  rc = bicg_vectors<float>(c, &V);
+  if (rc) return rc;
+  for (int q = 0; q < 7; q++)
+    launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
+                   (const double*)c->fields[A[q]].d[0]);
+  Coef7<double> K64 = coef<double>(c, A);
+  Coef7<float> K32 = coef<float>(c, fa);
+  const FieldRec X = c->fields[x], E = c->fields[fe];
+  const double* b64 = (const double*)c->fields[b].d[0];
+  const int max_outer = 100000;
+  std::vector<int> outers(c->cases.size(), -1);
+  launch_solve_init(c->sm, c->dst2, (int)c->cases.size(), omega, eps, max_outer);
+  int o = 0;
+  auto defect = [&]() {
+    launch_defect(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst2, K64, b64, (const double*)X.d[0],
+                  (const double*)X.h[0], V.r, c->dpart, (size_t)c->cap, o);
+    fin(c, c->dst2, FIN_DEFECT, o);
+  };
+  defect();
+  bool done = false;
+  rc = read_state(c, c->dst2, c->hst2, ids, &done);
+  if (rc) return rc;
+  for (int id : ids)
+    if (c->hst2[id].done) outers[id] = 0;
+  int n = 1;
+  if (!done) {
+    launch_fill<float>(c->sm, c->dactive, c->nactive, (float*)E.d[0], (float*)E.h[0], 0.0f);
+    launch_solve_init(c->sm, c->dst, (int)c->cases.size(), omega, delta, countMax, c->dst2);
+    launch_bicgstab_init<float>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K32, V.r,
+                                (const float*)E.d[0], (const float*)E.h[0], V, c->dpart, (size_t)c->cap);
+    fin(c, c->dst, FIN_BI_INIT, 0);
+    while (true) {
+      // one segment: FP32 iterations until the recursive residual dropped by delta
+      int batch = kBatchMin;
+      bool seg = false;
+      while (!seg && n <= countMax) {
+        const int last = std::min(n + batch - 1, countMax);
+        for (; n <= last; n++)
+          launch_bicgstab_iter<float>(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase,
+                                      c->dst, K32, (float*)E.d[0], V, c->dpart, (size_t)c->cap, n);
+        HIPCHK(hipGetLastError());
+        rc = read_state(c, c->dst, c->hst, ids, &seg);
+        if (rc) return rc;
+        batch = std::min(batch * 2, 64);
+      }
+      // reliable update: x64 += e, e = 0, FP64 residual
+      launch_add_correction(c->sm, c->dactive, c->nactive, c->dblk, c->dst2, c->dst, (double*)X.d[0],
+                            (const float*)E.d[0]);
+      launch_fill<float>(c->sm, c->dactive, c->nactive, (float*)E.d[0], (float*)E.h[0], 0.0f);
+      launch_ghost_pull<double>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, (const double*)X.d[0],
+                                (double*)X.h[0], bcg, 1, c->multilevel);
+      o++;
+      defect();
+      rc = read_state(c, c->dst2, c->hst2, ids, &done);
+      if (rc) return rc;
+      for (int id : ids)
+        if (c->hst2[id].done && outers[id] < 0) outers[id] = o;
+      if (done || n > countMax) break;
+      launch_bicgstab_reanchor<float>(c->sm, c->dactive, c->nactive, c->dblk, c->dst2, V, c->dpart, (size_t)c->cap);
+      fin(c, c->dst, FIN_BI_REANCHOR, n, c->dst2);
+      launch_bicgstab_restart<float>(c->sm, c->dactive, c->nactive, c->dblk, c->dst, V);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  rc = sync_state(c, c->dst2, c->hst2);
+  if (rc) return rc;
+  int status = 0;
+  for (size_t i = 0; i < ids.size(); i++) {
+    const SolveState& s = c->hst2[ids[i]];
+    if (count) count[i] = n - 1;
+    if (residual) residual[i] = s.residual;
+    if (outer) outer[i] = outers[ids[i]] < 0 ? o : outers[ids[i]];
+    if (s.status) status = 1;
+  }
+  if (status) set_error("residual is NaN or Inf");
+  return status;
+}
+
+}  // namespace
+
 extern "C" {
 
 int ffvhc_rbgs(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countMax, double eps, int* count,
@@ -506,7 +604,12 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
-  if (method < 0 || method > 2) return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB");
+  if (method < 0 || method > 3)
+    return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB, 3 Jacobi-BiCGSTAB with reliable updates");
+  if (method == 3) {
+    if (!(inner_eps > 0.0 && inner_eps < 1.0)) return fail(-1, "reliable-update factor must be in (0, 1)");
+    return bicgstab_reliable(c, x, A, b, bcg, omega, countMax, eps, inner_eps, count, residual, outer);
+  }
   const std::vector<int> ids = selected_cases(c);
   int fa[7], fr, fe;
   for (int q = 0; q < 7; q++) {
