@@ -167,6 +167,24 @@ def serve_leg(ncases, seed, mixed, budget_s):
     return rep
 
 
+# kernel names of the timed stages as rocprofv3 reports them
+PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0>", "bi_q": "bi_matvec_kernel<%s, 0>", "bi_s": "bi_precond_kernel<%s, 1>",
+              "bi_t": "bi_matvec_kernel<%s, 1>", "bi_xr": "bi_xr_kernel<%s>", "rbgs": "rbgs_iter_kernel<%s, 0>"}
+
+
+def pmc_traffic(stage, w):
+    """HBM bytes per launch of a stage's kernel from the committed rocprofv3 PMC summary of this
+    workload (FETCH_SIZE + WRITE_SIZE, calibrated with a plain copy); None if absent."""
+    path = os.path.join(ROOT, "profiles", "c2_poisson_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    name = PROF_NAMES[stage] % ("float" if w == 4 else "double")
+    v = d.get("bytes_per_launch", {}).get(name)
+    return None if v is None else int(round(v))
+
+
 def kernel_bytes(w):
     h = 0.375
     return {"bi_p": (6 + h) * w, "bi_q": (10 + h) * w, "bi_s": (5 + h) * w, "bi_t": (10 + h) * w, "bi_xr": 8 * w,
@@ -183,7 +201,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--eps", type=float, default=1e-10)
-    ap.add_argument("--inner-eps", type=float, default=1e-5)
+    ap.add_argument("--inner-eps", type=float, default=0.0,
+                    help="reliable-update factor of the mixed solve (0: chosen by the autotuner)")
     ap.add_argument("--count-max", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -224,10 +243,20 @@ def main():
         cnt, res, rc = ctx.bicgstab(f64["x"], f64["A"], f64["b"], P, 1.0, 1, args.count_max, args.eps, ncases=1)
         return int(cnt[0]), float(res[0]), 1
 
+    # schedule of the mixed-precision solve: picked by the autotuner on this system (untimed)
+    import autotune
+    if args.inner_eps > 0:
+        sched = (3, args.inner_eps, args.count_max)
+        tuned = []
+    else:
+        log("autotuning the mixed-precision schedule")
+        sched, tuned = autotune.tune_mixed(ctx, f64["x"], f64["A"], f64["b"], P, args.eps, omega=1.0)
+        log("schedule %s; candidates %s" % (sched, [(c, round(t, 4), it, o) for c, t, it, o, r in tuned]))
+
     def solve_mixed():
         ctx.fill(f64["x"], 0.0)
-        cnt, res, outer, rc = ctx.solve_mixed(2, f64["x"], f64["A"], f64["b"], P, 1.0, args.count_max, args.eps,
-                                              args.inner_eps, ncases=1)
+        cnt, res, outer, rc = ctx.solve_mixed(sched[0], f64["x"], f64["A"], f64["b"], P, 1.0, sched[2], args.eps,
+                                              sched[1], ncases=1)
         return int(cnt[0]), float(res[0]), int(outer[0])
 
     timings = {}
@@ -278,13 +307,20 @@ def main():
         "config": {"workload": "standalone Poisson, 2-level octree: roots 8^3 of 16^3-cell blocks, level 1 in a "
                                "ball of radius L/4 (1-cell margin)",
                    "blocks": tree.nblk, "cells": cells, "solver": "Jacobi-BiCGSTAB (PreCount 1, omega 1)",
-                   "eps": args.eps, "inner_eps": args.inner_eps},
+                   "eps": args.eps},
         "fp64": {"value": round(d64["mcell_steps_s"], 3), "ms_per_step": round(d64["ms_per_step"], 3),
                  "iterations": d64["iters"], "residual": d64["residual"]},
-        "mixed": {"iterations_inner": mix["iters"], "corrections": mix["outer"], "residual": mix["residual"]},
+        "mixed": {"iterations_inner": mix["iters"], "corrections": mix["outer"], "residual": mix["residual"],
+                  "ms_per_step": round(mix["ms_per_step"], 3),
+                  "schedule": {"method": ["RBGS", "Jacobi-PCG", "Jacobi-BiCGSTAB restarted",
+                                          "Jacobi-BiCGSTAB with reliable updates"][sched[0]],
+                               "inner_eps_or_update_factor": sched[1], "inner_cap": sched[2]},
+                  "autotune": [{"method": c[0], "factor": c[1], "cap": c[2],
+                                "seconds": (None if t == float("inf") else round(t, 4)), "inner": it,
+                                "corrections": o} for c, t, it, o, r in tuned]},
         "speedup_mixed_vs_fp64": round(mix["mcell_steps_s"] / d64["mcell_steps_s"], 3),
         "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dname, 4),
                      "kernel": dname + " (FP32)", "kernel_us": dom["us"],
                      "bytes_per_cell": kernel_bytes(4)[dname]},
         "kernels": kt,

@@ -1,0 +1,50 @@
+"""Autotuner for the solver schedule of a resident case set.
+
+The parameters that decide the throughput of a Poisson solve on the device are schedule
+parameters, not numerics of the result: the mixed-precision method and its reliable-update factor
+(how far the FP32 recurrence runs before the FP64 residual re-anchors it), the inner tolerance
+and iteration cap of a defect-correction solve.  Every candidate solves the same system from the
+same start to the same FP64 relative residual, so the candidates are interchangeable; the tuner
+times each on the device and keeps the fastest that reached the tolerance.
+"""
+import time
+
+import numpy as np
+
+
+DEFAULT_CANDIDATES = (
+    # (method, inner_eps or reliable-update factor, inner iteration cap)
+    (3, 0.1, 100000),
+    (3, 0.03, 100000),
+    (3, 0.01, 100000),
+    (3, 1e-3, 100000),
+    (1, 1e-2, 500),
+)
+
+
+def tune_mixed(ctx, x, A, b, group, eps, omega=1.0, candidates=DEFAULT_CANDIDATES, repeats=1, ncases=1,
+               x0=None):
+    """Time every candidate mixed-precision schedule on the resident system; returns
+    (best candidate, list of (candidate, seconds, inner iterations, corrections, residual))."""
+    results = []
+    for cand in candidates:
+        method, ieps, cap = cand
+        best_t = None
+        for _ in range(repeats):
+            if x0 is None:
+                ctx.fill(x, 0.0)
+            else:
+                x0()
+            ctx.sync()
+            t0 = time.perf_counter()
+            cnt, res, outer, rc = ctx.solve_mixed(method, x, A, b, group, omega, cap, eps, ieps, ncases=ncases)
+            ctx.sync()
+            dt = time.perf_counter() - t0
+            best_t = dt if best_t is None else min(best_t, dt)
+        ok = rc == 0 and bool(np.all(res <= eps))
+        results.append((cand, best_t if ok else float("inf"), int(np.max(cnt)), int(np.max(outer)),
+                        float(np.max(res))))
+    best = min(results, key=lambda r: r[1])
+    if not np.isfinite(best[1]):
+        raise RuntimeError("no mixed-precision schedule reached the tolerance")
+    return best[0], results

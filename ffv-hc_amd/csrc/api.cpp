@@ -1,5 +1,6 @@
 // C ABI implementation: octree, context, cases, fields, ghost fill.
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -525,6 +526,62 @@ int ffvhc_field_download(ffvhc_ctx* c, int f, int id, int g, void* padded) {
   if (F.prec == 8) unpack((const double*)in.data(), (const double*)halo.data(), g, (double*)padded, nb);
   else unpack((const float*)in.data(), (const float*)halo.data(), g, (float*)padded, nb);
   return 0;
+}
+
+/* LocalScalar3D<real>::Dump2 (LocalScalar3D.cpp:215-264) file layout: six ints {nx, ny, nz, vc,
+ * nE = 1, nB} then, per block in block order, the padded (n+2vc)^3 FP64 array. */
+int ffvhc_field_dump(ffvhc_ctx* c, int f, int id, int g, const char* path) {
+  if (!c || !path || g < 1) return fail(-1, "bad arguments");
+  if (!field_ok(c, f, 8)) return fail(-1, "Dump2 writes FP64 fields");
+  if (id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
+  const int nb = c->cases[id].nblk;
+  const size_t npad = (size_t)(NB + 2 * g) * (NB + 2 * g) * (NB + 2 * g);
+  std::vector<double> data((size_t)nb * npad, 0.0);
+  int rc = ffvhc_field_download(c, f, id, g, data.data());
+  if (rc) return rc;
+  FILE* fp = fopen(path, "wb");
+  if (!fp) return fail(-1, std::string("cannot open ") + path);
+  const int head[6] = {NB, NB, NB, g, 1, nb};
+  bool ok = fwrite(head, sizeof(int), 6, fp) == 6 && fwrite(data.data(), sizeof(double), data.size(), fp) == data.size();
+  ok = (fclose(fp) == 0) && ok;
+  return ok ? 0 : fail(-1, std::string("write failed: ") + path);
+}
+
+/* Header of a Dump2 file: {nx, ny, nz, vc, nE, nB}. */
+int ffvhc_dump_header(const char* path, int* head6) {
+  if (!path || !head6) return fail(-1, "bad arguments");
+  FILE* fp = fopen(path, "rb");
+  if (!fp) return fail(-1, std::string("cannot open ") + path);
+  const bool ok = fread(head6, sizeof(int), 6, fp) == 6;
+  fclose(fp);
+  return ok ? 0 : fail(-1, std::string("short header: ") + path);
+}
+
+/* LocalScalar3D<real>::Load2 (LocalScalar3D.cpp:266-342), same-resolution branch: the file must
+ * hold nE = 1 and the case's block count of 16^3 blocks; the interior and the first ghost layer
+ * are uploaded (the caller fills the ghosts afterwards, as Load2 does). */
+int ffvhc_field_load(ffvhc_ctx* c, int f, int id, const char* path) {
+  if (!c || !path) return fail(-1, "bad arguments");
+  if (!field_ok(c, f, 8)) return fail(-1, "Load2 reads FP64 fields");
+  if (id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
+  FILE* fp = fopen(path, "rb");
+  if (!fp) return fail(-1, std::string("cannot open ") + path);
+  int head[6];
+  if (fread(head, sizeof(int), 6, fp) != 6) {
+    fclose(fp);
+    return fail(-1, "short header");
+  }
+  const int nb = c->cases[id].nblk, g = head[3];
+  if (head[0] != NB || head[1] != NB || head[2] != NB || head[4] != 1 || head[5] != nb || g < 1 || g > 8) {
+    fclose(fp);
+    return fail(-1, "file does not match the case (block size, nE, block count)");
+  }
+  const size_t npad = (size_t)(NB + 2 * g) * (NB + 2 * g) * (NB + 2 * g);
+  std::vector<double> data((size_t)nb * npad);
+  const bool ok = fread(data.data(), sizeof(double), data.size(), fp) == data.size();
+  fclose(fp);
+  if (!ok) return fail(-1, "short file");
+  return ffvhc_field_upload(c, f, id, g, data.data());
 }
 
 int ffvhc_ghost_fill(ffvhc_ctx* c, int f, int bcg, int updater) {

@@ -101,6 +101,9 @@ _sig = {
     "ffvhc_step": (c_int, [_vp, _P(StepFields), _P(StepParams), _vp, _vp]),
     "ffvhc_case_set_step": (c_int, [_vp, c_int, c_int, c_int]),
     "ffvhc_force_p": (c_int, [_vp, c_int, c_int, _vp]),
+    "ffvhc_field_dump": (c_int, [_vp, c_int, c_int, c_int, ctypes.c_char_p]),
+    "ffvhc_field_load": (c_int, [_vp, c_int, c_int, ctypes.c_char_p]),
+    "ffvhc_dump_header": (c_int, [ctypes.c_char_p, _vp]),
     "ffvhc_case_step": (c_int, [_vp, c_int]),
     "ffvhc_residual": (c_int, [_vp, c_int, _vp, c_int, _vp]),
     "ffvhc_solve_mixed": (c_int, [_vp, c_int, c_int, _vp, c_int, c_int, c_double, c_int, c_double, c_double, _vp,
@@ -124,6 +127,13 @@ def check(rc, where):
     if rc != 0:
         raise FFVError(rc, where)
     return rc
+
+
+def dump_header(path):
+    """{nx, ny, nz, vc, nE, nB} of a Dump2 file."""
+    h = np.zeros(6, np.int32)
+    check(_L.ffvhc_dump_header(str(path).encode(), _ptr(h)), "ffvhc_dump_header")
+    return h
 
 
 def lib():
@@ -418,6 +428,14 @@ class Context:
         f = np.zeros((ncases or self._nsel(), 3))
         check(_L.ffvhc_force_p(self.h, p, cut_id, _ptr(f)), "ffvhc_force_p")
         return f
+
+    def dump(self, f, case_id, path, g=2):
+        """Write a field of a case in the reference's Dump2 layout."""
+        check(_L.ffvhc_field_dump(self.h, f, case_id, g, str(path).encode()), "ffvhc_field_dump")
+
+    def load(self, f, case_id, path):
+        """Read a Dump2 file into a field of a case (ghosts: call ghost_fill afterwards)."""
+        check(_L.ffvhc_field_load(self.h, f, case_id, str(path).encode()), "ffvhc_field_load")
 
     def set_step(self, case_id, step, accel_steps=0):
         check(_L.ffvhc_case_set_step(self.h, case_id, step, accel_steps), "ffvhc_case_set_step")

@@ -124,6 +124,14 @@ int ffvhc_field_alloc(ffvhc_ctx* ctx, int precision, int* field);
 int ffvhc_field_free(ffvhc_ctx* ctx, int field);
 int ffvhc_field_upload(ffvhc_ctx* ctx, int field, int case_id, int g, const void* padded);
 int ffvhc_field_download(ffvhc_ctx* ctx, int field, int case_id, int g, void* padded);
+/* Checkpoint in the reference's Dump2 layout (LocalScalar3D.cpp:215-264): six ints {nx, ny, nz,
+ * vc, nE = 1, nB} then each block's padded (n+2vc)^3 FP64 array; g = vc.  The device keeps one
+ * ghost layer: layers beyond the first are written as 0 (Load2 refills every ghost anyway).
+ * The file name is the caller's (the reference: <path>/<prefix><label>-<rank:5>-<step:10>.bin). */
+int ffvhc_field_dump(ffvhc_ctx* ctx, int field, int case_id, int g, const char* path);
+/* Load2 (LocalScalar3D.cpp:266-342), same-resolution branch; then call ffvhc_ghost_fill. */
+int ffvhc_field_load(ffvhc_ctx* ctx, int field, int case_id, const char* path);
+int ffvhc_dump_header(const char* path, int* head6);
 /* FFVILS::Fill (FFVILS.h:425-445): every cell and ghost = value */
 int ffvhc_field_fill(ffvhc_ctx* ctx, int field, double value);
 /* FFVILS::Copy (FFVILS.h:447-466) */

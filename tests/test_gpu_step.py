@@ -243,3 +243,28 @@ def test_force_p_matches_oracle(ctx, name):
         ctx.select(None)
         ctx.free(pf)
         ctx.remove_case(cid)
+
+
+def test_checkpoint_round_trip(ctx, tmp_path):
+    """Dump2 of a field, then Load2 into another: interior and first ghost layer identical; the
+    file has the reference's layout (header + padded blocks)."""
+    tree = trees()["two_level"]
+    og = OracleGrid(tree)
+    cid = ctx.add_case(tree, ffvhc.case_params(bc=BC))
+    f1, f2 = ctx.field(8), ctx.field(8)
+    try:
+        x = np.random.default_rng(3).standard_normal(og.shape)
+        ctx.upload(f1, cid, x)
+        path = tmp_path / "pP-00000-0000000007.bin"
+        ctx.dump(f1, cid, path)
+        head = ffvhc.dump_header(path)
+        assert list(head) == [N, N, N, G, 1, og.nb]
+        raw = np.fromfile(path, dtype=np.float64, offset=24).reshape(og.shape)
+        assert np.array_equal(raw[:, LIVE], x[:, LIVE])
+        ctx.load(f2, cid, path)
+        got = ctx.download(f2, cid, out=np.zeros(og.shape))
+        assert np.array_equal(got[:, LIVE], x[:, LIVE])
+    finally:
+        ctx.free(f1)
+        ctx.free(f2)
+        ctx.remove_case(cid)
