@@ -239,7 +239,7 @@ int ffvhc_ctx_create(int device, long long max_blocks, ffvhc_ctx** out) {
   HIPCHK(hipMalloc(&c->dcaseact, sizeof(int) * MAX_CASES));
   HIPCHK(hipMalloc(&c->dst, sizeof(SolveState) * MAX_CASES));
   HIPCHK(hipMalloc(&c->dst2, sizeof(SolveState) * MAX_CASES));
-  HIPCHK(hipMalloc(&c->dpart, sizeof(double) * 4 * max_blocks));
+  HIPCHK(hipMalloc(&c->dpart, sizeof(double) * 6 * max_blocks));
   HIPCHK(hipMalloc(&c->dcut, sizeof(float) * 6 * NC * max_blocks));
   HIPCHK(hipMalloc(&c->dflag, sizeof(uint32_t) * NC * max_blocks));
   HIPCHK(hipMalloc(&c->dcid, sizeof(uint8_t) * 6 * NC * max_blocks));
@@ -331,6 +331,7 @@ int ffvhc_case_add(ffvhc_ctx* c, const ffvhc_tree* t, const ffvhc_case_params* p
   r.slot_begin = (int)begin;
   r.nblk = (int)nb;
   r.max_level = t->tree->max_level();
+  r.root_length = t->tree->spec().root_length;
   r.ncells = nb * NC;
   r.params = *p;
   // block table with global slot numbers

@@ -537,6 +537,74 @@ __global__ __launch_bounds__(ST_NT) void force_sum_kernel(const int* __restrict_
   }
 }
 
+// ------------------------------------------------------------------ field statistics
+// LocalScalar3D::CalcStats (LocalScalar3D.cpp:9-80, sf3d_calc_stats): sum, max, min, |max|,
+// |min| over the interior cells of each case.  Block partials, then per-case combination in slot
+// order (the sum with a fixed tree; max/min are order-free).
+__device__ inline double block_op(double v, double* red, int op) {  // op 0 max, 1 min
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off, 64);
+    v = op == 0 ? fmax(v, o) : fmin(v, o);
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = red[0];
+  for (int q = 1; q < ST_NT / 64; q++) r = op == 0 ? fmax(r, red[q]) : fmin(r, red[q]);
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(ST_NT) void stats_kernel(const int* __restrict__ active, const double* __restrict__ x,
+                                                      double* __restrict__ part, size_t nslots) {
+  __shared__ double red[ST_NT / 64];
+  const int slot = active[blockIdx.x];
+  const size_t base = (size_t)slot * NC;
+  double s = 0.0, mx = -DBL_MAX, mn = DBL_MAX, amx = 0.0, amn = DBL_MAX;
+  for (int q = threadIdx.x; q < NC; q += ST_NT) {
+    const double v = x[base + q];
+    s += v;
+    mx = fmax(mx, v);
+    mn = fmin(mn, v);
+    amx = fmax(amx, fabs(v));
+    amn = fmin(amn, fabs(v));
+  }
+  const double ss = block_sum<ST_NT>(s, red);
+  const double m0 = block_op(mx, red, 0), m1 = block_op(mn, red, 1), m2 = block_op(amx, red, 0),
+               m3 = block_op(amn, red, 1);
+  if (threadIdx.x == 0) {
+    part[slot] = ss;
+    part[nslots + slot] = m0;
+    part[2 * nslots + slot] = m1;
+    part[3 * nslots + slot] = m2;
+    part[4 * nslots + slot] = m3;
+  }
+}
+
+__global__ __launch_bounds__(ST_NT) void stats_sum_kernel(const int* __restrict__ caseact,
+                                                          const DevCase* __restrict__ cs,
+                                                          const double* __restrict__ part, size_t nslots,
+                                                          double* __restrict__ out) {
+  __shared__ double red[ST_NT / 64];
+  const DevCase& C = cs[caseact[blockIdx.x]];
+  const double sum = case_sum<ST_NT>(part, C.slot_begin, C.nblk, red);
+  if (threadIdx.x != 0) return;
+  double mx = -DBL_MAX, mn = DBL_MAX, amx = 0.0, amn = DBL_MAX;
+  for (int b = 0; b < C.nblk; b++) {
+    const size_t q = (size_t)C.slot_begin + b;
+    mx = fmax(mx, part[nslots + q]);
+    mn = fmin(mn, part[2 * nslots + q]);
+    amx = fmax(amx, part[3 * nslots + q]);
+    amn = fmin(amn, part[4 * nslots + q]);
+  }
+  double* o = out + 5 * blockIdx.x;
+  o[0] = sum;
+  o[1] = mx;
+  o[2] = mn;
+  o[3] = amx;
+  o[4] = amn;
+}
+
 // ------------------------------------------------------------------ launchers
 void launch_predict(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                     const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc) {
@@ -570,6 +638,12 @@ void launch_force_p(hipStream_t sm, const int* active, int nact, const int* case
   hipLaunchKernelGGL(force_p_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, blk, p, hp, cid8, flag, capnc, target,
                      part, nslots);
   hipLaunchKernelGGL(force_sum_kernel, dim3(ncase), dim3(ST_NT), 0, sm, caseact, cs, part, nslots, out);
+}
+void launch_stats(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase, const DevCase* cs,
+                  const double* x, double* part, size_t nslots, double* out) {
+  if (!nact || !ncase) return;
+  hipLaunchKernelGGL(stats_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, x, part, nslots);
+  hipLaunchKernelGGL(stats_sum_kernel, dim3(ncase), dim3(ST_NT), 0, sm, caseact, cs, part, nslots, out);
 }
 void launch_corr(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                  const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc) {

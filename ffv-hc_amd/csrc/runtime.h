@@ -39,6 +39,7 @@ struct CaseRec {
   int slot_begin = 0;
   int nblk = 0;
   int max_level = 0;
+  double root_length = 1.0;
   int64_t ncells = 0;
   int step = 0;         // Solver::Update step counter
   int accel_steps = 0;  // TimeControl/Acceleration of the X_M inflow (0 = off)
@@ -79,7 +80,7 @@ struct ffvhc_ctx {
   std::vector<ffvhc::SolveState> hst2;
   // internally allocated scratch fields, keyed by precision * 100 + role
   std::vector<std::pair<int, int>> scratch_ids;
-  double* dpart = nullptr;  // 4 * cap partial sums
+  double* dpart = nullptr;  // 6 * cap per-block partials
   std::vector<ffvhc::FieldRec> fields;
   long long bytes = 0;
   // free slot intervals [begin, end)

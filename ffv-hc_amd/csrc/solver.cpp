@@ -731,6 +731,50 @@ int ffvhc_force_p(ffvhc_ctx* c, int p, int cut_id, double* f) {
   return 0;
 }
 
+int ffvhc_field_stats(ffvhc_ctx* c, int f, double* out) {
+  if (!c || !out) return fail(-1, "null argument");
+  if (!field_ok(c, f, 8)) return fail(-1, "stats take an FP64 field");
+  int rc = refresh_active(c);
+  if (rc) return rc;
+  const std::vector<int> ids = selected_cases(c);
+  const int n = std::max(c->ncaseact, 1);
+  double* dout = nullptr;
+  HIPCHK(hipMallocAsync((void**)&dout, sizeof(double) * 5 * n, c->sm));
+  launch_stats(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dcase, (const double*)c->fields[f].d[0],
+               c->dpart, (size_t)c->cap, dout);
+  HIPCHK(hipGetLastError());
+  std::vector<double> h(5 * n, 0.0);
+  HIPCHK(hipMemcpyAsync(h.data(), dout, sizeof(double) * 5 * c->ncaseact, hipMemcpyDeviceToHost, c->sm));
+  HIPCHK(hipFreeAsync(dout, c->sm));
+  HIPCHK(hipStreamSynchronize(c->sm));
+  size_t q = 0;
+  for (size_t i = 0; i < ids.size(); i++) {
+    const bool has = c->cases[ids[i]].nblk > 0;
+    for (int d = 0; d < 5; d++) out[5 * i + d] = has ? h[5 * q + d] : 0.0;
+    if (has) q++;
+  }
+  return 0;
+}
+
+int ffvhc_cfl_check(ffvhc_ctx* c, int ux, double* cfl) {
+  std::vector<double> st;
+  const std::vector<int> ids = c ? selected_cases(c) : std::vector<int>();
+  st.resize(5 * std::max<size_t>(ids.size(), 1));
+  int rc = ffvhc_field_stats(c, ux, st.data());
+  if (rc) return rc;
+  int status = 0;
+  for (size_t i = 0; i < ids.size(); i++) {
+    const CaseRec& r = c->cases[ids[i]];
+    // Solver::PrintStats (Solver.cpp:2655-2676): dx = RootBlockLength / 2^maxLevel
+    const double dx = r.root_length / (double)(1 << r.max_level);
+    const double v = st[5 * i + 3] * r.params.dt / dx;
+    if (cfl) cfl[i] = v;
+    if (v > 0.9) status = 2;
+  }
+  if (status) set_error("CFL guard: |ux|max dt / dx > 0.9");
+  return status;
+}
+
 int ffvhc_case_set_step(ffvhc_ctx* c, int id, int step, int accel_steps) {
   if (!c || id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
   if (step < 0 || accel_steps < 0) return fail(-1, "step and accel_steps must be >= 0");

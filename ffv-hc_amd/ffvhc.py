@@ -101,6 +101,8 @@ _sig = {
     "ffvhc_step": (c_int, [_vp, _P(StepFields), _P(StepParams), _vp, _vp]),
     "ffvhc_case_set_step": (c_int, [_vp, c_int, c_int, c_int]),
     "ffvhc_force_p": (c_int, [_vp, c_int, c_int, _vp]),
+    "ffvhc_field_stats": (c_int, [_vp, c_int, _vp]),
+    "ffvhc_cfl_check": (c_int, [_vp, c_int, _vp]),
     "ffvhc_field_dump": (c_int, [_vp, c_int, c_int, c_int, ctypes.c_char_p]),
     "ffvhc_field_load": (c_int, [_vp, c_int, c_int, ctypes.c_char_p]),
     "ffvhc_dump_header": (c_int, [ctypes.c_char_p, _vp]),
@@ -428,6 +430,20 @@ class Context:
         f = np.zeros((ncases or self._nsel(), 3))
         check(_L.ffvhc_force_p(self.h, p, cut_id, _ptr(f)), "ffvhc_force_p")
         return f
+
+    def stats(self, f, ncases=None):
+        """(sum, max, min, |max|, |min|) over the interior of each selected case."""
+        out = np.zeros((ncases or self._nsel(), 5))
+        check(_L.ffvhc_field_stats(self.h, f, _ptr(out)), "ffvhc_field_stats")
+        return out
+
+    def cfl(self, ux, ncases=None):
+        """CFL numbers of the reference's guard per selected case and the status (2 = tripped)."""
+        out = np.zeros(ncases or self._nsel())
+        rc = _L.ffvhc_cfl_check(self.h, ux, _ptr(out))
+        if rc < 0:
+            check(rc, "ffvhc_cfl_check")
+        return out, rc
 
     def dump(self, f, case_id, path, g=2):
         """Write a field of a case in the reference's Dump2 layout."""

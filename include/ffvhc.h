@@ -249,6 +249,14 @@ int ffvhc_step(ffvhc_ctx* ctx, ffvhc_step_fields* fields, const ffvhc_step_param
  * p's ghost layer must be current. */
 int ffvhc_force_p(ffvhc_ctx* ctx, int p, int cut_id, double* f);
 
+/* LocalScalar3D::CalcStats (LocalScalar3D.cpp:9-80): out[5*i ..] = {sum, max, min, |max|, |min|}
+ * over the interior cells of the i-th selected case. */
+int ffvhc_field_stats(ffvhc_ctx* ctx, int field, double* out);
+/* The CFL guard of Solver::PrintStats (Solver.cpp:2655-2676): cfl[i] = |ux|max dt / dx with
+ * dx = RootBlockLength / 2^maxLevel; returns 2 when any selected case exceeds 0.9 (the reference
+ * exits there). */
+int ffvhc_cfl_check(ffvhc_ctx* ctx, int ux, double* cfl);
+
 /* Step counter and inflow ramp (TimeControl/Acceleration, Solver.cpp:3347-3355) of a case. */
 int ffvhc_case_set_step(ffvhc_ctx* ctx, int case_id, int step, int accel_steps);
 int ffvhc_case_step(const ffvhc_ctx* ctx, int case_id);

@@ -268,3 +268,30 @@ def test_checkpoint_round_trip(ctx, tmp_path):
         ctx.free(f1)
         ctx.free(f2)
         ctx.remove_case(cid)
+
+
+def test_field_stats_and_cfl_guard(ctx):
+    """CalcStats per case against numpy on the same interior; the CFL guard of PrintStats."""
+    tree = trees()["two_level"]
+    og = OracleGrid(tree)
+    cid = ctx.add_case(tree, ffvhc.case_params(dt=0.01, bc=BC))
+    f = ctx.field(8)
+    try:
+        x = np.random.default_rng(4).standard_normal(og.shape)
+        ctx.upload(f, cid, x)
+        ctx.select([cid])
+        got = ctx.stats(f, ncases=1)[0]
+        xi = x[:, G:G + N, G:G + N, G:G + N]
+        assert abs(got[0] - xi.sum()) <= 1e-12 * np.abs(xi).sum()
+        assert got[1] == xi.max() and got[2] == xi.min()
+        assert got[3] == np.abs(xi).max() and got[4] == np.abs(xi).min()
+        dxl = 1.0 / 2 ** tree.max_level
+        cfl, rc = ctx.cfl(f, ncases=1)
+        assert abs(cfl[0] - np.abs(xi).max() * 0.01 / dxl) <= 1e-15 * cfl[0] and rc == 0
+        ctx.upload(f, cid, x * 1e3)
+        cfl, rc = ctx.cfl(f, ncases=1)
+        assert rc == 2
+    finally:
+        ctx.select(None)
+        ctx.free(f)
+        ctx.remove_case(cid)
