@@ -70,6 +70,9 @@ template <class T>
 void launch_bicgstab_reanchor(hipStream_t sm, const int* active, int nact, const DevBlock* blk,
                               const SolveState* gate, BiVecs<T> V, double* part, size_t nslots);
 template <class T>
+void launch_bicgstab_shadow(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                            const SolveState* gate, BiVecs<T> V);
+template <class T>
 void launch_bicgstab_restart(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const SolveState* st,
                              BiVecs<T> V);
 void launch_defect(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,

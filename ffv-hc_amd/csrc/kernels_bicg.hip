@@ -243,6 +243,39 @@ __global__ __launch_bounds__(BI_NT) void bi_reanchor_kernel(const int* __restric
   }
 }
 
+// Seeded shadow residual for the mixed-precision solver: r0 = uniform(-1, 1) from a hash of the
+// block's index within its case and the cell (so a case draws the same r0 whatever slots it
+// holds).  With r0 = r (the reference's choice) rho = r.r0 decays far below FP32 resolution on
+// the coarse/fine octree operator; a random shadow keeps it near |r||r0|/sqrt(N).
+__device__ inline float shadow_value(uint32_t blk, uint32_t cell) {
+  uint32_t h = blk * 0x9E3779B1u ^ (cell + 0x7F4A7C15u) * 0x85EBCA77u;  // integer mix (murmur-style)
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return (float)((double)h * (2.0 / 4294967296.0) - 1.0);
+}
+
+template <class T>
+__global__ __launch_bounds__(BI_NT) void bi_shadow_kernel(const int* __restrict__ active,
+                                                          const DevBlock* __restrict__ blk,
+                                                          const DevCase* __restrict__ cs,
+                                                          const SolveState* __restrict__ gate, BiVecs<T> V) {
+  const int slot = active[blockIdx.x];
+  const int c = blk[slot].case_id;
+  if (gate[c].done) return;
+  const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
+  const size_t base = (size_t)slot * NC;
+  for (int q = threadIdx.x; q < NC; q += BI_NT) V.r0[base + q] = (T)shadow_value(local, (uint32_t)q);
+}
+
+template <class T>
+void launch_bicgstab_shadow(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                            const SolveState* gate, BiVecs<T> V) {
+  if (nact) hipLaunchKernelGGL(bi_shadow_kernel<T>, dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, gate, V);
+}
+
 // Restart after a breakdown at a reliable update: r0 := r for the cases that asked for it.
 template <class T>
 __global__ __launch_bounds__(BI_NT) void bi_restart_kernel(const int* __restrict__ active,
@@ -309,7 +342,9 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
                                         int);                                                                        \
   template void launch_bicgstab_reanchor<T>(hipStream_t, const int*, int, const DevBlock*, const SolveState*,         \
                                             BiVecs<T>, double*, size_t);                                             \
-  template void launch_bicgstab_restart<T>(hipStream_t, const int*, int, const DevBlock*, const SolveState*, BiVecs<T>);
+  template void launch_bicgstab_restart<T>(hipStream_t, const int*, int, const DevBlock*, const SolveState*, BiVecs<T>); \
+  template void launch_bicgstab_shadow<T>(hipStream_t, const int*, int, const DevBlock*, const DevCase*,              \
+                                          const SolveState*, BiVecs<T>);
 INST(double)
 INST(float)
 #undef INST

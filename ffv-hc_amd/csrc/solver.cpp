@@ -412,7 +412,7 @@ namespace {
 // FP64 (the convergence test), and it replaces the FP32 residual without restarting the
 // recurrence (rho recomputed; a restart with r0 := r only after a breakdown).
 int bicgstab_reliable(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countMax, double eps,
-                      double delta, int* count, double* residual, int* outer) {
+                      double delta, int* count, double* residual, int* outer, bool random_shadow) {
   const std::vector<int> ids = selected_cases(c);
   int fa[7], fe;
   for (int q = 0; q < 7; q++) {
@@ -453,6 +453,11 @@ int bicgstab_reliable(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double 
     launch_bicgstab_init<float>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K32, V.r,
                                 (const float*)E.d[0], (const float*)E.h[0], V, c->dpart, (size_t)c->cap);
     fin(c, c->dst, FIN_BI_INIT, 0);
+    if (random_shadow) {  // seeded shadow residual, then rho = r.r0
+      launch_bicgstab_shadow<float>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst2, V);
+      launch_bicgstab_reanchor<float>(c->sm, c->dactive, c->nactive, c->dblk, c->dst2, V, c->dpart, (size_t)c->cap);
+      fin(c, c->dst, FIN_BI_REANCHOR, 0, c->dst2);
+    }
     while (true) {
       // one segment: FP32 iterations until the recursive residual dropped by delta
       int batch = kBatchMin;
@@ -604,11 +609,12 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
-  if (method < 0 || method > 3)
-    return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB, 3 Jacobi-BiCGSTAB with reliable updates");
-  if (method == 3) {
+  if (method < 0 || method > 4)
+    return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB, 3 Jacobi-BiCGSTAB with reliable updates, "
+                    "4 the same with a seeded shadow residual");
+  if (method >= 3) {
     if (!(inner_eps > 0.0 && inner_eps < 1.0)) return fail(-1, "reliable-update factor must be in (0, 1)");
-    return bicgstab_reliable(c, x, A, b, bcg, omega, countMax, eps, inner_eps, count, residual, outer);
+    return bicgstab_reliable(c, x, A, b, bcg, omega, countMax, eps, inner_eps, count, residual, outer, method == 4);
   }
   const std::vector<int> ids = selected_cases(c);
   int fa[7], fr, fe;

@@ -183,7 +183,9 @@ int ffvhc_residual(ffvhc_ctx* ctx, int x, const int* A, int b, double* residual)
 
 /* Mixed precision: inner sweeps / matvecs in FP32 wrapped in FP64 defect correction to the same
  * relative residual eps (FP64 residual, dots and state).  method 0 RBGS, 1 Jacobi-PCG,
- * 2 Jacobi-BiCGSTAB.
+ * 2 Jacobi-BiCGSTAB (each restarted per correction); 3 Jacobi-BiCGSTAB with reliable updates (the
+ * FP32 recurrence continues across corrections; inner_eps = residual reduction between updates);
+ * 4 as 3 with a seeded random shadow residual r0 instead of r0 = r.
  * inner_eps: relative reduction per correction; countMax caps each inner solve (a capped inner
  * solve still yields a correction).  count[c] = inner iterations in total, outer[c] = corrections
  * taken, residual[c] = FP64 ||b - Ax|| / ||b|| after the last correction. */
