@@ -65,7 +65,8 @@ void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const Dev
 template <class T>
 void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase,
                           const DevBlock* blk, const DevCase* cs, SolveState* st, Coef7<T> A, T* x, BiVecs<T> V,
-                          double* part, size_t nslots, int count, int mask = 31);
+                          double* part, size_t nslots, int count, int mask = 31,
+                          const Coef7<float>* M = nullptr, const Coef7<double>* M64 = nullptr);
 template <class T>
 void launch_bicgstab_reanchor(hipStream_t sm, const int* active, int nact, const DevBlock* blk,
                               const SolveState* gate, BiVecs<T> V, double* part, size_t nslots);

@@ -122,6 +122,73 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
   push_halo<T, BI_NT>(B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return s[lds(i, j, k)]; }, h, h);
 }
 
+// P / S with a block red-black Gauss-Seidel preconditioner: v (p or s) as above in FP64, then
+// z = one SOR sweep of M z = v from z = 0 in the matrix precision TM (FP32 copy of the matrix for
+// the mixed-precision solve, FP64 for its all-FP64 counterpart), red cells first, the block's ghost values zero (a block-Jacobi / red-black GS
+// preconditioner, fixed and linear); p_ (s_) = z widened to FP64, ghost layer pushed.
+template <int MODE, class TM>
+__global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restrict__ active,
+                                                              const DevBlock* __restrict__ blk,
+                                                              const DevCase* __restrict__ cs,
+                                                              const SolveState* __restrict__ st, Coef7<TM> M,
+                                                              BiVecs<double> V, int count) {
+  __shared__ TM z[LP * LP * LP];
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const int c = B.case_id;
+  if (st[c].done) return;
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  const TM omega = (TM)st[c].omega;
+  const double ng = -st[c].gamma, beta = st[c].beta, na = -st[c].alpha;
+  const bool first = count == 1 || st[c].restart;
+  for (int q = t; q < LP * LP * LP; q += BI_NT) z[q] = (TM)0;
+  TM vf[NB];
+#pragma unroll
+  for (int k = 0; k < NB; k++) {
+    const size_t g = base + tx + NB * (ty + NB * k);
+    double v;
+    if (MODE == 0) {
+      if (first) {
+        v = V.r[g];
+      } else {
+        const double tmp = V.p[g] + ng * V.q[g];
+        v = V.r[g] + beta * tmp;
+      }
+      V.p[g] = v;
+    } else {
+      v = na * V.q[g] + V.r[g];
+      V.s[g] = v;
+    }
+    vf[k] = (TM)v;
+  }
+  __syncthreads();
+  const int par = (tx + ty) & 1;
+#pragma unroll
+  for (int k = 0; k < NB; k++)
+    if ((k & 1) == par) {  // red: neighbours are zero
+      const size_t g = base + tx + NB * (ty + NB * k);
+      z[lds(tx, ty, k)] = omega * (vf[k] / M.p[0][g]);
+    }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NB; k++)
+    if ((k & 1) != par) {  // black: red neighbours inside the block, zero ghosts
+      const size_t g = base + tx + NB * (ty + NB * k);
+      const int li = lds(tx, ty, k);
+      const TM rs = vf[k] - M.p[1][g] * z[li - 1] - M.p[2][g] * z[li + 1] - M.p[3][g] * z[li - LP] -
+                       M.p[4][g] * z[li + LP] - M.p[5][g] * z[li - LP * LP] - M.p[6][g] * z[li + LP * LP];
+      z[li] = omega * (rs / M.p[0][g]);
+    }
+  __syncthreads();
+  double* out = MODE == 0 ? V.pp : V.ss;
+#pragma unroll
+  for (int k = 0; k < NB; k++) out[base + tx + NB * (ty + NB * k)] = (double)z[lds(tx, ty, k)];
+  double* h = MODE == 0 ? V.hpp : V.hss;
+  push_halo<double, BI_NT>(B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return (double)z[lds(i, j, k)]; }, h,
+                           h);
+}
+
 // Q: q_ = A p_, q_.r0 -> alpha (mode 0);  T: t_ = A s_, t_.s and t_.t_ -> gamma (mode 1)
 template <class T, int MODE>
 __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict__ active,
@@ -310,17 +377,33 @@ void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const Dev
 template <class T>
 void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase,
                           const DevBlock* blk, const DevCase* cs, SolveState* st, Coef7<T> A, T* x, BiVecs<T> V,
-                          double* part, size_t nslots, int count, int mask) {
-  // mask bits 0-4 select P, Q, S, T, XR; bit 5 leaves out the per-case finalisations
+                          double* part, size_t nslots, int count, int mask, const Coef7<float>* M,
+                          const Coef7<double>* M64) {
+  // mask bits 0-4 select P, Q, S, T, XR; bit 5 leaves out the per-case finalisations.
+  // M (FP64 solves only): FP32 matrix of the block red-black GS preconditioner instead of Jacobi.
   if (!nact) return;
-  if (mask & 1)
+  const bool gs = M != nullptr && sizeof(T) == 8;
+  if (sizeof(T) != 8) M64 = nullptr;
+  if ((mask & 1) && gs)
+    hipLaunchKernelGGL((bi_precond_gs_kernel<0, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, *M,
+                       *reinterpret_cast<BiVecs<double>*>(&V), count);
+  else if ((mask & 1) && M64)
+    hipLaunchKernelGGL((bi_precond_gs_kernel<0, double>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
+                       *M64, *reinterpret_cast<BiVecs<double>*>(&V), count);
+  else if (mask & 1)
     hipLaunchKernelGGL((bi_precond_kernel<T, 0>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, count);
   if (mask & 2) {
     hipLaunchKernelGGL((bi_matvec_kernel<T, 0>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, part,
                        nslots);
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_ALPHA, count);
   }
-  if (mask & 4)
+  if ((mask & 4) && gs)
+    hipLaunchKernelGGL((bi_precond_gs_kernel<1, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, *M,
+                       *reinterpret_cast<BiVecs<double>*>(&V), count);
+  else if ((mask & 4) && M64)
+    hipLaunchKernelGGL((bi_precond_gs_kernel<1, double>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
+                       *M64, *reinterpret_cast<BiVecs<double>*>(&V), count);
+  else if (mask & 4)
     hipLaunchKernelGGL((bi_precond_kernel<T, 1>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, count);
   if (mask & 8) {
     hipLaunchKernelGGL((bi_matvec_kernel<T, 1>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, part,
@@ -339,7 +422,7 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
                                         Coef7<T>, const T*, const T*, const T*, BiVecs<T>, double*, size_t);         \
   template void launch_bicgstab_iter<T>(hipStream_t, const int*, int, const int*, int, const DevBlock*,            \
                                         const DevCase*, SolveState*, Coef7<T>, T*, BiVecs<T>, double*, size_t, int,  \
-                                        int);                                                                        \
+                                        int, const Coef7<float>*, const Coef7<double>*);                             \
   template void launch_bicgstab_reanchor<T>(hipStream_t, const int*, int, const DevBlock*, const SolveState*,         \
                                             BiVecs<T>, double*, size_t);                                             \
   template void launch_bicgstab_restart<T>(hipStream_t, const int*, int, const DevBlock*, const SolveState*, BiVecs<T>); \

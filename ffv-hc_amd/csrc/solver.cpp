@@ -149,7 +149,8 @@ int bicg_vectors(ffvhc_ctx* c, BiVecs<T>* V) {
 
 template <class T>
 int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double omega, int countMax,
-                  double eps, SolveState* st, std::vector<SolveState>& hst, const SolveState* outer, int* launched) {
+                  double eps, SolveState* st, std::vector<SolveState>& hst, const SolveState* outer, int* launched,
+                  const Coef7<float>* M = nullptr, const Coef7<double>* M64 = nullptr) {
   const std::vector<int> ids = selected_cases(c);
   BiVecs<T> V;
   int rc0 = bicg_vectors<T>(c, &V);
@@ -165,7 +166,7 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
     int last = std::min(n + batch - 1, countMax);
     for (; n <= last; n++)
       launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase, st, K,
-                              (T*)X.d[0], V, c->dpart, (size_t)c->cap, n);
+                              (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, 31, M, M64);
     HIPCHK(hipGetLastError());
     int rc = read_state(c, st, hst, ids, &done);
     if (rc) return rc;
@@ -609,9 +610,42 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
-  if (method < 0 || method > 4)
+  if (method < 0 || method > 6)
     return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB, 3 Jacobi-BiCGSTAB with reliable updates, "
-                    "4 the same with a seeded shadow residual");
+                    "4 the same with a seeded shadow residual, 5 FP64 BiCGSTAB with an FP32 red-black GS "
+                    "preconditioner, 6 the same all in FP64 (reference point)");
+  if (method == 6) {
+    const Coef7<double> M64 = coef<double>(c, A);
+    rc = bicgstab_core<double>(c, x, M64, (const double*)c->fields[b].d[0], bcg, omega, countMax, eps, c->dst,
+                               c->hst, nullptr, nullptr, nullptr, &M64);
+    if (rc) return rc;
+    rc = sync_state(c, c->dst, c->hst);
+    if (rc) return rc;
+    const std::vector<int> ids6 = selected_cases(c);
+    for (size_t i = 0; i < ids6.size(); i++)
+      if (outer) outer[i] = 0;
+    return report(c, c->hst, count, residual);
+  }
+  if (method == 5) {  // FP64 Krylov recurrence, dots and residual; FP32 preconditioner sweeps
+    int fa[7];
+    for (int q = 0; q < 7; q++) {
+      rc = scratch_field(c, 4, R_A + q, &fa[q]);
+      if (rc) return rc;
+    }
+    for (int q = 0; q < 7; q++)
+      launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
+                     (const double*)c->fields[A[q]].d[0]);
+    const Coef7<float> M = coef<float>(c, fa);
+    rc = bicgstab_core<double>(c, x, coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega, countMax,
+                               eps, c->dst, c->hst, nullptr, nullptr, &M);
+    if (rc) return rc;
+    rc = sync_state(c, c->dst, c->hst);
+    if (rc) return rc;
+    const std::vector<int> ids5 = selected_cases(c);
+    for (size_t i = 0; i < ids5.size(); i++)
+      if (outer) outer[i] = 0;
+    return report(c, c->hst, count, residual);
+  }
   if (method >= 3) {
     if (!(inner_eps > 0.0 && inner_eps < 1.0)) return fail(-1, "reliable-update factor must be in (0, 1)");
     return bicgstab_reliable(c, x, A, b, bcg, omega, countMax, eps, inner_eps, count, residual, outer, method == 4);
