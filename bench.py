@@ -167,6 +167,12 @@ def serve_leg(ncases, seed, mixed, budget_s):
     return rep
 
 
+MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64 defect correction",
+                 "FP32 Jacobi-BiCGSTAB in FP64 defect correction",
+                 "FP32 Jacobi-BiCGSTAB with FP64 reliable updates", "the same with a seeded shadow residual",
+                 "FP64 BiCGSTAB with an FP32 block red-black GS preconditioner",
+                 "FP64 BiCGSTAB with an FP64 block red-black GS preconditioner"]
+
 # kernel names of the timed stages as rocprofv3 reports them
 PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0>", "bi_q": "bi_matvec_kernel<%s, 0>", "bi_s": "bi_precond_kernel<%s, 1>",
               "bi_t": "bi_matvec_kernel<%s, 1>", "bi_xr": "bi_xr_kernel<%s>", "rbgs": "rbgs_iter_kernel<%s, 0>"}
@@ -245,18 +251,18 @@ def main():
 
     # schedule of the mixed-precision solve: picked by the autotuner on this system (untimed)
     import autotune
-    if args.inner_eps > 0:
-        sched = (3, args.inner_eps, args.count_max)
+    if args.inner_eps > 0 or args.profile:
+        sched = (1, args.inner_eps or 1e-2, 500, 1.0)
         tuned = []
     else:
         log("autotuning the mixed-precision schedule")
-        sched, tuned = autotune.tune_mixed(ctx, f64["x"], f64["A"], f64["b"], P, args.eps, omega=1.0)
+        sched, tuned = autotune.tune_mixed(ctx, f64["x"], f64["A"], f64["b"], P, args.eps)
         log("schedule %s; candidates %s" % (sched, [(c, round(t, 4), it, o) for c, t, it, o, r in tuned]))
 
     def solve_mixed():
         ctx.fill(f64["x"], 0.0)
-        cnt, res, outer, rc = ctx.solve_mixed(sched[0], f64["x"], f64["A"], f64["b"], P, 1.0, sched[2], args.eps,
-                                              sched[1], ncases=1)
+        cnt, res, outer, rc = ctx.solve_mixed(sched[0], f64["x"], f64["A"], f64["b"], P, sched[3], sched[2],
+                                              args.eps, sched[1], ncases=1)
         return int(cnt[0]), float(res[0]), int(outer[0])
 
     timings = {}
@@ -312,10 +318,9 @@ def main():
                  "iterations": d64["iters"], "residual": d64["residual"]},
         "mixed": {"iterations_inner": mix["iters"], "corrections": mix["outer"], "residual": mix["residual"],
                   "ms_per_step": round(mix["ms_per_step"], 3),
-                  "schedule": {"method": ["RBGS", "Jacobi-PCG", "Jacobi-BiCGSTAB restarted",
-                                          "Jacobi-BiCGSTAB with reliable updates"][sched[0]],
-                               "inner_eps_or_update_factor": sched[1], "inner_cap": sched[2]},
-                  "autotune": [{"method": c[0], "factor": c[1], "cap": c[2],
+                  "schedule": {"method": MIXED_METHODS[sched[0]], "inner_eps_or_update_factor": sched[1],
+                               "inner_cap": sched[2], "omega": sched[3]},
+                  "autotune": [{"method": MIXED_METHODS[c[0]], "factor": c[1], "cap": c[2], "omega": c[3],
                                 "seconds": (None if t == float("inf") else round(t, 4)), "inner": it,
                                 "corrections": o} for c, t, it, o, r in tuned]},
         "speedup_mixed_vs_fp64": round(mix["mcell_steps_s"] / d64["mcell_steps_s"], 3),

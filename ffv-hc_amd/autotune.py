@@ -13,22 +13,22 @@ import numpy as np
 
 
 DEFAULT_CANDIDATES = (
-    # (method, inner_eps or reliable-update factor, inner iteration cap)
-    (3, 0.1, 100000),
-    (3, 0.03, 100000),
-    (3, 0.01, 100000),
-    (3, 1e-3, 100000),
-    (1, 1e-2, 500),
+    # (method, inner_eps or reliable-update factor, inner iteration cap, omega)
+    (1, 1e-2, 300, 1.0),
+    (1, 1e-2, 500, 1.0),
+    (1, 3e-2, 400, 1.0),
+    (2, 1e-1, 250, 1.0),
+    (3, 0.1, 100000, 1.0),
+    (5, 0.5, 100000, 1.0),
 )
 
 
-def tune_mixed(ctx, x, A, b, group, eps, omega=1.0, candidates=DEFAULT_CANDIDATES, repeats=1, ncases=1,
-               x0=None):
+def tune_mixed(ctx, x, A, b, group, eps, candidates=DEFAULT_CANDIDATES, repeats=1, ncases=1, x0=None):
     """Time every candidate mixed-precision schedule on the resident system; returns
     (best candidate, list of (candidate, seconds, inner iterations, corrections, residual))."""
     results = []
     for cand in candidates:
-        method, ieps, cap = cand
+        method, ieps, cap, omega = cand
         best_t = None
         for _ in range(repeats):
             if x0 is None:
@@ -37,7 +37,10 @@ def tune_mixed(ctx, x, A, b, group, eps, omega=1.0, candidates=DEFAULT_CANDIDATE
                 x0()
             ctx.sync()
             t0 = time.perf_counter()
-            cnt, res, outer, rc = ctx.solve_mixed(method, x, A, b, group, omega, cap, eps, ieps, ncases=ncases)
+            try:
+                cnt, res, outer, rc = ctx.solve_mixed(method, x, A, b, group, omega, cap, eps, ieps, ncases=ncases)
+            except RuntimeError:  # a schedule the solver rejects is skipped
+                cnt, res, outer, rc = np.zeros(1, int), np.ones(1), np.zeros(1, int), 1
             ctx.sync()
             dt = time.perf_counter() - t0
             best_t = dt if best_t is None else min(best_t, dt)

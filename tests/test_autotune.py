@@ -31,9 +31,9 @@ def test_picks_fastest_converged(monkeypatch):
         return ctx.now
 
     monkeypatch.setattr(autotune.time, "perf_counter", fake_perf)
-    cands = ((3, 0.1, 10), (3, 0.01, 10), (1, 0.01, 10))
+    cands = ((3, 0.1, 10, 1.0), (3, 0.01, 10, 1.0), (1, 0.01, 10, 1.0))
     best, res = autotune.tune_mixed(ctx, 0, [0] * 7, 0, 3, 1e-10, candidates=cands)
-    assert best == (1, 0.01, 10)
+    assert best == (1, 0.01, 10, 1.0)
     assert res[1][1] == float("inf")
 
 
@@ -41,4 +41,4 @@ def test_no_candidate_converges(monkeypatch):
     ctx = FakeCtx({(3, 0.1): (0.5, False)})
     monkeypatch.setattr(autotune.time, "perf_counter", lambda: ctx.now)
     with pytest.raises(RuntimeError):
-        autotune.tune_mixed(ctx, 0, [0] * 7, 0, 3, 1e-10, candidates=((3, 0.1, 10),))
+        autotune.tune_mixed(ctx, 0, [0] * 7, 0, 3, 1e-10, candidates=((3, 0.1, 10, 1.0),))
