@@ -175,7 +175,8 @@ MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64
 
 # kernel names of the timed stages as rocprofv3 reports them
 PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0>", "bi_q": "bi_matvec_kernel<%s, 0>", "bi_s": "bi_precond_kernel<%s, 1>",
-              "bi_t": "bi_matvec_kernel<%s, 1>", "bi_xr": "bi_xr_kernel<%s>", "rbgs": "rbgs_iter_kernel<%s, 0>"}
+              "bi_t": "bi_matvec_kernel<%s, 1>", "bi_xr": "bi_xr_kernel<%s>", "rbgs": "rbgs_iter_kernel<%s, 0>",
+              "cg_p": "cg_p_kernel<%s>", "cg_q": "cg_q_kernel<%s>", "cg_xr": "cg_xr_kernel<%s>"}
 
 
 def pmc_traffic(stage, w):
@@ -194,7 +195,7 @@ def pmc_traffic(stage, w):
 def kernel_bytes(w):
     h = 0.375
     return {"bi_p": (6 + h) * w, "bi_q": (10 + h) * w, "bi_s": (5 + h) * w, "bi_t": (10 + h) * w, "bi_xr": 8 * w,
-            "rbgs": (10 + 2 * h) * w}
+            "rbgs": (10 + 2 * h) * w, "cg_p": (4 + h) * w, "cg_q": (9 + h) * w, "cg_xr": 7 * w}
 
 
 def log(msg):
@@ -238,16 +239,15 @@ def main():
         ctx.fill(f["x"], 0.0)
         ctx.rbgs_fixed(f["x"], f["A"], f["b"], P, 1.2, 5)
         mr = ctx.rbgs_fixed(f["x"], f["A"], f["b"], P, 1.2, nit) / nit
+        ctx.fill(f["x"], 0.0)
+        ctx.cg_timing(f["x"], f["A"], f["b"], 1.0, 5)
+        mc = ctx.cg_timing(f["x"], f["A"], f["b"], 1.0, nit)[:3] / nit
         by = kernel_bytes(w)
         names = ("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr")
+        stages = list(zip(names, ms)) + [("rbgs", mr)] + list(zip(("cg_p", "cg_q", "cg_xr"), mc))
         kt[tag] = {name: {"us": round(t * 1e3, 2), "GBs": round(by[name] * cells / (t * 1e-3) / 1e9, 1)}
-                   for name, t in list(zip(names, ms)) + [("rbgs", mr)]}
+                   for name, t in stages}
         log("fp%s kernels: %s" % (tag, json.dumps(kt[tag])))
-
-    def solve64():
-        ctx.fill(f64["x"], 0.0)
-        cnt, res, rc = ctx.bicgstab(f64["x"], f64["A"], f64["b"], P, 1.0, 1, args.count_max, args.eps, ncases=1)
-        return int(cnt[0]), float(res[0]), 1
 
     # schedule of the mixed-precision solve: picked by the autotuner on this system (untimed)
     import autotune
@@ -294,9 +294,14 @@ def main():
         ctx.close()
         return
     mix, d64 = timings["mixed"], timings["fp64"]
-    # dominant kernel of the headline (mixed) path: the slowest FP32 kernel of the inner BiCGSTAB
-    dname = max(("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"), key=lambda k: kt["32"][k]["us"])
-    dom = kt["32"][dname]
+    # dominant kernel of the headline (mixed) path: the slowest per-iteration kernel of the
+    # schedule's inner solver
+    stage_set = {0: ("32", ("rbgs",)), 1: ("32", ("cg_p", "cg_q", "cg_xr")), 2: ("32", ("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr")),
+                 3: ("32", ("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr")), 4: ("32", ("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"))}
+    dtag, dstages = stage_set.get(sched[0], ("64", ("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr")))
+    dname = max(dstages, key=lambda k: kt[dtag][k]["us"])
+    dom = kt[dtag][dname]
+    dw = 4 if dtag == "32" else 8
     out = {
         "metric": "Mcell-steps/s mixed vs FP64 at equal residual; p50/p99 served-case latency",
         "value": round(mix["mcell_steps_s"], 3),
@@ -325,9 +330,10 @@ def main():
                                 "corrections": o} for c, t, it, o, r in tuned]},
         "speedup_mixed_vs_fp64": round(mix["mcell_steps_s"] / d64["mcell_steps_s"], 3),
         "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dname, 4),
-                     "kernel": dname + " (FP32)", "kernel_us": dom["us"],
-                     "bytes_per_cell": kernel_bytes(4)[dname]},
+                     "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dname, dw),
+                     "algorithmic_bytes": int(kernel_bytes(dw)[dname] * cells),
+                     "kernel": "%s (FP%d)" % (dname, 8 * dw), "kernel_us": dom["us"],
+                     "bytes_per_cell": kernel_bytes(dw)[dname]},
         "kernels": kt,
     }
     if args.serve_cases > 0:

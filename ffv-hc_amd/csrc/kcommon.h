@@ -29,22 +29,40 @@ __device__ inline double block_sum(double v, double* red) {
 }
 
 template <int NT>
-__device__ inline double case_sum(const double* part, int begin, int count, double* red) {
+__device__ inline double case_sum(const double* __restrict__ part, int begin, int count, double* red) {
+  // strided per-thread sums, then the fixed wave pattern and the waves in order (deterministic)
   double v = 0.0;
-  const volatile double* p = part;
-  for (int q = threadIdx.x; q < count; q += NT) v += p[begin + q];
-  // tree in shared memory with fixed pairing
-  __shared__ double tree[NT];
-  tree[threadIdx.x] = v;
-  __syncthreads();
-  for (int h = NT / 2; h > 0; h >>= 1) {
-    if (threadIdx.x < h) tree[threadIdx.x] = tree[threadIdx.x] + tree[threadIdx.x + h];
-    __syncthreads();
+  for (int q = threadIdx.x; q < count; q += NT) v += part[begin + q];
+  return block_sum<NT>(v, red);  // valid in thread 0
+}
+
+// two sums over the same partial range in one pass (both valid in thread 0)
+template <int NT>
+__device__ inline void case_sum2(const double* __restrict__ p0, const double* __restrict__ p1, int begin, int count,
+                                 double* red, double& s0, double& s1) {
+  double v0 = 0.0, v1 = 0.0;
+  for (int q = threadIdx.x; q < count; q += NT) {
+    v0 += p0[begin + q];
+    v1 += p1[begin + q];
   }
-  double r = tree[0];
+  for (int off = 32; off > 0; off >>= 1) {
+    v0 += __shfl_xor(v0, off, 64);
+    v1 += __shfl_xor(v1, off, 64);
+  }
+  const int w = threadIdx.x >> 6;
   __syncthreads();
-  (void)red;
-  return r;
+  if ((threadIdx.x & 63) == 0) {
+    red[w] = v0;
+    red[NT / 64 + w] = v1;
+  }
+  __syncthreads();
+  s0 = 0.0;
+  s1 = 0.0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NT / 64; q++) {
+      s0 += red[q];
+      s1 += red[NT / 64 + q];
+    }
 }
 
 // Planes a thread loads per batch: every global load of a batch is issued before any store of

@@ -62,11 +62,21 @@ template <class T>
 void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                           SolveState* st, Coef7<T> A, const T* b, const T* x, const T* hx, BiVecs<T> V, double* part,
                           size_t nslots);
+// Precision options of an FP64 BiCGSTAB: A32 = FP32 copy of the matrix for the matvecs and the
+// Jacobi preconditioner (widened to FP64 on load); gs = block red-black GS preconditioner with
+// an FP32 matrix and FP32 arithmetic, an FP32 matrix and FP64 arithmetic, or all FP64.
+struct BiOpts {
+  enum Gs { NONE = 0, GS32, GS32_ARITH64, GS64 };
+  const Coef7<float>* A32 = nullptr;
+  int gs = NONE;
+  const Coef7<float>* M32 = nullptr;
+  const Coef7<double>* M64 = nullptr;
+};
+
 template <class T>
 void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase,
                           const DevBlock* blk, const DevCase* cs, SolveState* st, Coef7<T> A, T* x, BiVecs<T> V,
-                          double* part, size_t nslots, int count, int mask = 31,
-                          const Coef7<float>* M = nullptr, const Coef7<double>* M64 = nullptr);
+                          double* part, size_t nslots, int count, int mask = 31, const BiOpts& o = BiOpts());
 template <class T>
 void launch_bicgstab_reanchor(hipStream_t sm, const int* active, int nact, const DevBlock* blk,
                               const SolveState* gate, BiVecs<T> V, double* part, size_t nslots);

@@ -9,6 +9,8 @@
 //   S    : s = r - alpha q_; s_ = M s, push s_ ghosts                  [873-893]
 //   T    : t_ = A s_; t_.s, t_.t_ -> gamma                             [894-915]
 //   XR   : x += alpha p_ + gamma s_; r = s - gamma t_; r.r -> IsConverged; r.r0 -> beta [916-965]
+#include <type_traits>
+
 #include "kcommon.h"
 #include "kernels.h"
 
@@ -67,11 +69,12 @@ __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ 
 }
 
 // P: p update and p_ = M p with its ghost layer (mode 0); S: s = r - alpha q_, s_ = M s (mode 1)
-template <class T, int MODE>
+// TA: storage precision of the matrix (FP32 copies for an FP64 solve are widened on load)
+template <class T, int MODE, class TA = T>
 __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict__ active,
                                                            const DevBlock* __restrict__ blk,
                                                            const DevCase* __restrict__ cs,
-                                                           const SolveState* __restrict__ st, Coef7<T> A, BiVecs<T> V,
+                                                           const SolveState* __restrict__ st, Coef7<TA> A, BiVecs<T> V,
                                                            int count) {
   __shared__ T s[LP * LP * LP];
   const int slot = active[blockIdx.x];
@@ -91,7 +94,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
     for (int kk = 0; kk < KB; kk++) {
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
       rv[kk] = V.r[g];
-      ap[kk] = A.p[0][g];
+      ap[kk] = (T)A.p[0][g];
       if (MODE == 1 || !first) qv[kk] = V.q[g];
       if (MODE == 0 && !first) pv[kk] = V.p[g];
     }
@@ -126,24 +129,24 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
 // z = one SOR sweep of M z = v from z = 0 in the matrix precision TM (FP32 copy of the matrix for
 // the mixed-precision solve, FP64 for its all-FP64 counterpart), red cells first, the block's ghost values zero (a block-Jacobi / red-black GS
 // preconditioner, fixed and linear); p_ (s_) = z widened to FP64, ghost layer pushed.
-template <int MODE, class TM>
+template <int MODE, class TM, class TZ = TM>
 __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restrict__ active,
                                                               const DevBlock* __restrict__ blk,
                                                               const DevCase* __restrict__ cs,
                                                               const SolveState* __restrict__ st, Coef7<TM> M,
                                                               BiVecs<double> V, int count) {
-  __shared__ TM z[LP * LP * LP];
+  __shared__ TZ z[LP * LP * LP];
   const int slot = active[blockIdx.x];
   const DevBlock& B = blk[slot];
   const int c = B.case_id;
   if (st[c].done) return;
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
-  const TM omega = (TM)st[c].omega;
+  const TZ omega = (TZ)st[c].omega;
   const double ng = -st[c].gamma, beta = st[c].beta, na = -st[c].alpha;
   const bool first = count == 1 || st[c].restart;
-  for (int q = t; q < LP * LP * LP; q += BI_NT) z[q] = (TM)0;
-  TM vf[NB];
+  for (int q = t; q < LP * LP * LP; q += BI_NT) z[q] = (TZ)0;
+  TZ vf[NB];
 #pragma unroll
   for (int k = 0; k < NB; k++) {
     const size_t g = base + tx + NB * (ty + NB * k);
@@ -160,7 +163,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
       v = na * V.q[g] + V.r[g];
       V.s[g] = v;
     }
-    vf[k] = (TM)v;
+    vf[k] = (TZ)v;
   }
   __syncthreads();
   const int par = (tx + ty) & 1;
@@ -168,7 +171,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
   for (int k = 0; k < NB; k++)
     if ((k & 1) == par) {  // red: neighbours are zero
       const size_t g = base + tx + NB * (ty + NB * k);
-      z[lds(tx, ty, k)] = omega * (vf[k] / M.p[0][g]);
+      z[lds(tx, ty, k)] = omega * (vf[k] / (TZ)M.p[0][g]);
     }
   __syncthreads();
 #pragma unroll
@@ -176,9 +179,9 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
     if ((k & 1) != par) {  // black: red neighbours inside the block, zero ghosts
       const size_t g = base + tx + NB * (ty + NB * k);
       const int li = lds(tx, ty, k);
-      const TM rs = vf[k] - M.p[1][g] * z[li - 1] - M.p[2][g] * z[li + 1] - M.p[3][g] * z[li - LP] -
-                       M.p[4][g] * z[li + LP] - M.p[5][g] * z[li - LP * LP] - M.p[6][g] * z[li + LP * LP];
-      z[li] = omega * (rs / M.p[0][g]);
+      const TZ rs = vf[k] - (TZ)M.p[1][g] * z[li - 1] - (TZ)M.p[2][g] * z[li + 1] - (TZ)M.p[3][g] * z[li - LP] -
+                    (TZ)M.p[4][g] * z[li + LP] - (TZ)M.p[5][g] * z[li - LP * LP] - (TZ)M.p[6][g] * z[li + LP * LP];
+      z[li] = omega * (rs / (TZ)M.p[0][g]);
     }
   __syncthreads();
   double* out = MODE == 0 ? V.pp : V.ss;
@@ -190,11 +193,11 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
 }
 
 // Q: q_ = A p_, q_.r0 -> alpha (mode 0);  T: t_ = A s_, t_.s and t_.t_ -> gamma (mode 1)
-template <class T, int MODE>
+template <class T, int MODE, class TA = T>
 __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict__ active,
                                                           const DevBlock* __restrict__ blk,
                                                           const DevCase* __restrict__ cs, SolveState* __restrict__ st,
-                                                          Coef7<T> A, BiVecs<T> V, double* __restrict__ part,
+                                                          Coef7<TA> A, BiVecs<T> V, double* __restrict__ part,
                                                           size_t nslots) {
   __shared__ T s[LP * LP * LP];
   __shared__ double red[BI_NT / 64];
@@ -214,7 +217,7 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
     for (int kk = 0; kk < KB; kk++) {
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
 #pragma unroll
-      for (int q = 0; q < 7; q++) a[kk][q] = A.p[q][g];
+      for (int q = 0; q < 7; q++) a[kk][q] = (T)A.p[q][g];
       wv[kk] = w[g];
     }
     if (k0 == 0) {
@@ -377,37 +380,48 @@ void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const Dev
 template <class T>
 void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase,
                           const DevBlock* blk, const DevCase* cs, SolveState* st, Coef7<T> A, T* x, BiVecs<T> V,
-                          double* part, size_t nslots, int count, int mask, const Coef7<float>* M,
-                          const Coef7<double>* M64) {
+                          double* part, size_t nslots, int count, int mask, const BiOpts& o) {
   // mask bits 0-4 select P, Q, S, T, XR; bit 5 leaves out the per-case finalisations.
-  // M (FP64 solves only): FP32 matrix of the block red-black GS preconditioner instead of Jacobi.
   if (!nact) return;
-  const bool gs = M != nullptr && sizeof(T) == 8;
-  if (sizeof(T) != 8) M64 = nullptr;
-  if ((mask & 1) && gs)
-    hipLaunchKernelGGL((bi_precond_gs_kernel<0, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, *M,
-                       *reinterpret_cast<BiVecs<double>*>(&V), count);
-  else if ((mask & 1) && M64)
-    hipLaunchKernelGGL((bi_precond_gs_kernel<0, double>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
-                       *M64, *reinterpret_cast<BiVecs<double>*>(&V), count);
-  else if (mask & 1)
-    hipLaunchKernelGGL((bi_precond_kernel<T, 0>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, count);
+  const bool dbl = sizeof(T) == 8;
+  BiVecs<double>& Vd = *reinterpret_cast<BiVecs<double>*>(&V);
+  auto precond = [&](auto mode_tag) {
+    constexpr int MODE = decltype(mode_tag)::value;
+    if (dbl && o.gs == BiOpts::GS32)
+      hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
+                         *o.M32, Vd, count);
+    else if (dbl && o.gs == BiOpts::GS32_ARITH64)
+      hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float, double>), dim3(nact), dim3(BI_NT), 0, sm, active, blk,
+                         cs, st, *o.M32, Vd, count);
+    else if (dbl && o.gs == BiOpts::GS64)
+      hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, double>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
+                         *o.M64, Vd, count);
+    else if (dbl && o.A32)
+      hipLaunchKernelGGL((bi_precond_kernel<double, MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs,
+                         st, *o.A32, Vd, count);
+    else
+      hipLaunchKernelGGL((bi_precond_kernel<T, MODE>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V,
+                         count);
+  };
+  auto matvec = [&](auto mode_tag) {
+    constexpr int MODE = decltype(mode_tag)::value;
+    if (dbl && o.A32)
+      hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs,
+                         st, *o.A32, Vd, part, nslots);
+    else
+      hipLaunchKernelGGL((bi_matvec_kernel<T, MODE>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V,
+                         part, nslots);
+  };
+  using M0 = std::integral_constant<int, 0>;
+  using M1 = std::integral_constant<int, 1>;
+  if (mask & 1) precond(M0{});
   if (mask & 2) {
-    hipLaunchKernelGGL((bi_matvec_kernel<T, 0>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, part,
-                       nslots);
+    matvec(M0{});
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_ALPHA, count);
   }
-  if ((mask & 4) && gs)
-    hipLaunchKernelGGL((bi_precond_gs_kernel<1, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, *M,
-                       *reinterpret_cast<BiVecs<double>*>(&V), count);
-  else if ((mask & 4) && M64)
-    hipLaunchKernelGGL((bi_precond_gs_kernel<1, double>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
-                       *M64, *reinterpret_cast<BiVecs<double>*>(&V), count);
-  else if (mask & 4)
-    hipLaunchKernelGGL((bi_precond_kernel<T, 1>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, count);
+  if (mask & 4) precond(M1{});
   if (mask & 8) {
-    hipLaunchKernelGGL((bi_matvec_kernel<T, 1>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V, part,
-                       nslots);
+    matvec(M1{});
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_GAMMA, count);
   }
   if (mask & 16) {
@@ -422,7 +436,7 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
                                         Coef7<T>, const T*, const T*, const T*, BiVecs<T>, double*, size_t);         \
   template void launch_bicgstab_iter<T>(hipStream_t, const int*, int, const int*, int, const DevBlock*,            \
                                         const DevCase*, SolveState*, Coef7<T>, T*, BiVecs<T>, double*, size_t, int,  \
-                                        int, const Coef7<float>*, const Coef7<double>*);                             \
+                                        int, const BiOpts&);                                                         \
   template void launch_bicgstab_reanchor<T>(hipStream_t, const int*, int, const DevBlock*, const SolveState*,         \
                                             BiVecs<T>, double*, size_t);                                             \
   template void launch_bicgstab_restart<T>(hipStream_t, const int*, int, const DevBlock*, const SolveState*, BiVecs<T>); \

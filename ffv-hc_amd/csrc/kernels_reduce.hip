@@ -15,7 +15,7 @@ __global__ __launch_bounds__(FIN_NT) void finalize_kernel(const int* __restrict_
                                                           const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                           const double* __restrict__ part, size_t nslots, int op,
                                                           int arg, const SolveState* __restrict__ gate) {
-  __shared__ double red[FIN_NT / 64];
+  __shared__ double red[2 * (FIN_NT / 64)];
   const int c = caseact[blockIdx.x];
   if (op == FIN_BI_REANCHOR) {
     if (gate && gate[c].done) return;
@@ -26,8 +26,9 @@ __global__ __launch_bounds__(FIN_NT) void finalize_kernel(const int* __restrict_
   const bool two = op == FIN_BI_INIT || op == FIN_BI_GAMMA || op == FIN_BI_XR || op == FIN_CG_INIT ||
                    op == FIN_CG_XR || ((op == FIN_RBGS || op == FIN_FIXAVG) && arg == 1) ||
                    (op == FIN_DEFECT && arg == 0) || op == FIN_BI_REANCHOR;
-  const double t0 = case_sum<FIN_NT>(part, C.slot_begin, C.nblk, red);
-  const double t1 = two ? case_sum<FIN_NT>(part + nslots, C.slot_begin, C.nblk, red) : 0.0;
+  double t0 = 0.0, t1 = 0.0;
+  if (two) case_sum2<FIN_NT>(part, part + nslots, C.slot_begin, C.nblk, red, t0, t1);
+  else t0 = case_sum<FIN_NT>(part, C.slot_begin, C.nblk, red);
   const double t2 = op == FIN_FIXAVG ? case_sum<FIN_NT>(part + 2 * nslots, C.slot_begin, C.nblk, red) : 0.0;
   if (threadIdx.x != 0) return;
   SolveState& S = st[c];

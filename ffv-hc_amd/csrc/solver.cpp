@@ -150,7 +150,7 @@ int bicg_vectors(ffvhc_ctx* c, BiVecs<T>* V) {
 template <class T>
 int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double omega, int countMax,
                   double eps, SolveState* st, std::vector<SolveState>& hst, const SolveState* outer, int* launched,
-                  const Coef7<float>* M = nullptr, const Coef7<double>* M64 = nullptr) {
+                  const BiOpts& opts = BiOpts()) {
   const std::vector<int> ids = selected_cases(c);
   BiVecs<T> V;
   int rc0 = bicg_vectors<T>(c, &V);
@@ -166,7 +166,7 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
     int last = std::min(n + batch - 1, countMax);
     for (; n <= last; n++)
       launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase, st, K,
-                              (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, 31, M, M64);
+                              (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, 31, opts);
     HIPCHK(hipGetLastError());
     int rc = read_state(c, st, hst, ids, &done);
     if (rc) return rc;
@@ -610,14 +610,18 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
-  if (method < 0 || method > 6)
+  if (method < 0 || method > 8)
     return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB, 3 Jacobi-BiCGSTAB with reliable updates, "
                     "4 the same with a seeded shadow residual, 5 FP64 BiCGSTAB with an FP32 red-black GS "
-                    "preconditioner, 6 the same all in FP64 (reference point)");
+                    "preconditioner, 6 the same all in FP64 (reference point), 7 FP64 Jacobi-BiCGSTAB with an FP32 "
+                    "matrix, 8 as 7 with the block red-black GS preconditioner (FP64 arithmetic)");
   if (method == 6) {
     const Coef7<double> M64 = coef<double>(c, A);
+    BiOpts o;
+    o.gs = BiOpts::GS64;
+    o.M64 = &M64;
     rc = bicgstab_core<double>(c, x, M64, (const double*)c->fields[b].d[0], bcg, omega, countMax, eps, c->dst,
-                               c->hst, nullptr, nullptr, nullptr, &M64);
+                               c->hst, nullptr, nullptr, o);
     if (rc) return rc;
     rc = sync_state(c, c->dst, c->hst);
     if (rc) return rc;
@@ -626,7 +630,7 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
       if (outer) outer[i] = 0;
     return report(c, c->hst, count, residual);
   }
-  if (method == 5) {  // FP64 Krylov recurrence, dots and residual; FP32 preconditioner sweeps
+  if (method == 5 || method == 7 || method == 8) {  // FP64 recurrence, dots and residual; FP32 matrix
     int fa[7];
     for (int q = 0; q < 7; q++) {
       rc = scratch_field(c, 4, R_A + q, &fa[q]);
@@ -636,8 +640,19 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
       launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
                      (const double*)c->fields[A[q]].d[0]);
     const Coef7<float> M = coef<float>(c, fa);
+    BiOpts o;
+    if (method == 5) {
+      o.gs = BiOpts::GS32;
+      o.M32 = &M;
+    } else if (method == 7) {
+      o.A32 = &M;
+    } else {  // 8: FP32 matrix storage everywhere, GS preconditioner in FP64 arithmetic
+      o.A32 = &M;
+      o.gs = BiOpts::GS32_ARITH64;
+      o.M32 = &M;
+    }
     rc = bicgstab_core<double>(c, x, coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega, countMax,
-                               eps, c->dst, c->hst, nullptr, nullptr, &M);
+                               eps, c->dst, c->hst, nullptr, nullptr, o);
     if (rc) return rc;
     rc = sync_state(c, c->dst, c->hst);
     if (rc) return rc;
