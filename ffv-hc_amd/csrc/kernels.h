@@ -64,9 +64,10 @@ void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const Dev
                           size_t nslots);
 // Precision options of an FP64 BiCGSTAB: A32 = FP32 copy of the matrix for the matvecs and the
 // Jacobi preconditioner (widened to FP64 on load); gs = block red-black GS preconditioner with
-// an FP32 matrix and FP32 arithmetic, an FP32 matrix and FP64 arithmetic, or all FP64.
+// an FP32 matrix and FP32 arithmetic, an FP32 matrix and FP64 arithmetic, all FP64, or the
+// symmetric (red, black, red) sweep with an FP32 matrix and FP64 arithmetic.
 struct BiOpts {
-  enum Gs { NONE = 0, GS32, GS32_ARITH64, GS64 };
+  enum Gs { NONE = 0, GS32, GS32_ARITH64, GS64, SGS32_ARITH64 };
   const Coef7<float>* A32 = nullptr;
   int gs = NONE;
   const Coef7<float>* M32 = nullptr;

@@ -129,7 +129,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
 // z = one SOR sweep of M z = v from z = 0 in the matrix precision TM (FP32 copy of the matrix for
 // the mixed-precision solve, FP64 for its all-FP64 counterpart), red cells first, the block's ghost values zero (a block-Jacobi / red-black GS
 // preconditioner, fixed and linear); p_ (s_) = z widened to FP64, ghost layer pushed.
-template <int MODE, class TM, class TZ = TM>
+template <int MODE, class TM, class TZ = TM, bool SYM = false>
 __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restrict__ active,
                                                               const DevBlock* __restrict__ blk,
                                                               const DevCase* __restrict__ cs,
@@ -184,6 +184,19 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
       z[li] = omega * (rs / (TZ)M.p[0][g]);
     }
   __syncthreads();
+  if (SYM) {  // backward half: red cells again from the updated black neighbours (symmetric GS)
+#pragma unroll
+    for (int k = 0; k < NB; k++)
+      if ((k & 1) == par) {
+        const size_t g = base + tx + NB * (ty + NB * k);
+        const int li = lds(tx, ty, k);
+        const TZ zr = z[li];
+        const TZ rs = vf[k] - (TZ)M.p[1][g] * z[li - 1] - (TZ)M.p[2][g] * z[li + 1] - (TZ)M.p[3][g] * z[li - LP] -
+                      (TZ)M.p[4][g] * z[li + LP] - (TZ)M.p[5][g] * z[li - LP * LP] - (TZ)M.p[6][g] * z[li + LP * LP];
+        z[li] = zr + omega * (rs / (TZ)M.p[0][g] - zr);
+      }
+    __syncthreads();
+  }
   double* out = MODE == 0 ? V.pp : V.ss;
 #pragma unroll
   for (int k = 0; k < NB; k++) out[base + tx + NB * (ty + NB * k)] = (double)z[lds(tx, ty, k)];
@@ -393,6 +406,9 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
     else if (dbl && o.gs == BiOpts::GS32_ARITH64)
       hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float, double>), dim3(nact), dim3(BI_NT), 0, sm, active, blk,
                          cs, st, *o.M32, Vd, count);
+    else if (dbl && o.gs == BiOpts::SGS32_ARITH64)
+      hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float, double, true>), dim3(nact), dim3(BI_NT), 0, sm, active,
+                         blk, cs, st, *o.M32, Vd, count);
     else if (dbl && o.gs == BiOpts::GS64)
       hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, double>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
                          *o.M64, Vd, count);

@@ -610,11 +610,12 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
-  if (method < 0 || method > 8)
+  if (method < 0 || method > 9)
     return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB, 3 Jacobi-BiCGSTAB with reliable updates, "
                     "4 the same with a seeded shadow residual, 5 FP64 BiCGSTAB with an FP32 red-black GS "
                     "preconditioner, 6 the same all in FP64 (reference point), 7 FP64 Jacobi-BiCGSTAB with an FP32 "
-                    "matrix, 8 as 7 with the block red-black GS preconditioner (FP64 arithmetic)");
+                    "matrix, 8 as 7 with the block red-black GS preconditioner (FP64 arithmetic), 9 as 8 with the "
+                    "symmetric red-black-red sweep");
   if (method == 6) {
     const Coef7<double> M64 = coef<double>(c, A);
     BiOpts o;
@@ -630,7 +631,7 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
       if (outer) outer[i] = 0;
     return report(c, c->hst, count, residual);
   }
-  if (method == 5 || method == 7 || method == 8) {  // FP64 recurrence, dots and residual; FP32 matrix
+  if (method == 5 || method == 7 || method == 8 || method == 9) {  // FP64 recurrence, dots and residual; FP32 matrix
     int fa[7];
     for (int q = 0; q < 7; q++) {
       rc = scratch_field(c, 4, R_A + q, &fa[q]);
@@ -646,9 +647,9 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
       o.M32 = &M;
     } else if (method == 7) {
       o.A32 = &M;
-    } else {  // 8: FP32 matrix storage everywhere, GS preconditioner in FP64 arithmetic
+    } else {  // 8 / 9: FP32 matrix storage everywhere, (symmetric) GS preconditioner in FP64 arithmetic
       o.A32 = &M;
-      o.gs = BiOpts::GS32_ARITH64;
+      o.gs = method == 9 ? BiOpts::SGS32_ARITH64 : BiOpts::GS32_ARITH64;
       o.M32 = &M;
     }
     rc = bicgstab_core<double>(c, x, coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega, countMax,
