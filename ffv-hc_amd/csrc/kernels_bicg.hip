@@ -135,7 +135,9 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
                                                               const DevCase* __restrict__ cs,
                                                               const SolveState* __restrict__ st, Coef7<TM> M,
                                                               BiVecs<double> V, int count) {
-  __shared__ TZ z[LP * LP * LP];
+  // z of the block's own cells only: its ghost values are zero, so no ghost ring is staged and
+  // every cell is written (red, then black) before a neighbour reads it
+  __shared__ TZ z[NC];
   const int slot = active[blockIdx.x];
   const DevBlock& B = blk[slot];
   const int c = B.case_id;
@@ -145,63 +147,81 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
   const TZ omega = (TZ)st[c].omega;
   const double ng = -st[c].gamma, beta = st[c].beta, na = -st[c].alpha;
   const bool first = count == 1 || st[c].restart;
-  for (int q = t; q < LP * LP * LP; q += BI_NT) z[q] = (TZ)0;
-  TZ vf[NB];
+  const int par = (tx + ty) & 1;  // red cells: k = 2m + par
+  auto zi = [](int i, int j, int k) { return i + NB * (j + NB * k); };
+  auto zat = [&](int i, int j, int k) -> TZ {
+    return (i < 0 || i >= NB || j < 0 || j >= NB || k < 0 || k >= NB) ? (TZ)0 : z[zi(i, j, k)];
+  };
+  TZ vred[NB / 2];  // v of the red cells (the black cells re-read theirs from p / s)
+  TM apr[NB / 2];
+  const double* vsrc = MODE == 0 ? V.p : V.s;
 #pragma unroll
-  for (int k = 0; k < NB; k++) {
-    const size_t g = base + tx + NB * (ty + NB * k);
-    double v;
-    if (MODE == 0) {
-      if (first) {
-        v = V.r[g];
+  for (int k0 = 0; k0 < NB; k0 += 4) {
+    double rv[4], pv[4], qv[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const int k = k0 + kk;
+      const size_t g = base + tx + NB * (ty + NB * k);
+      rv[kk] = V.r[g];
+      if (MODE == 1 || !first) qv[kk] = V.q[g];
+      if (MODE == 0 && !first) pv[kk] = V.p[g];
+      if ((k & 1) == par) apr[k >> 1] = M.p[0][g];
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const int k = k0 + kk;
+      const size_t g = base + tx + NB * (ty + NB * k);
+      double v;
+      if (MODE == 0) {
+        v = first ? rv[kk] : rv[kk] + beta * (pv[kk] + ng * qv[kk]);
+        V.p[g] = v;
       } else {
-        const double tmp = V.p[g] + ng * V.q[g];
-        v = V.r[g] + beta * tmp;
+        v = na * qv[kk] + rv[kk];
+        V.s[g] = v;
       }
-      V.p[g] = v;
-    } else {
-      v = na * V.q[g] + V.r[g];
-      V.s[g] = v;
+      if ((k & 1) == par) vred[k >> 1] = (TZ)v;
     }
-    vf[k] = (TZ)v;
   }
-  __syncthreads();
-  const int par = (tx + ty) & 1;
 #pragma unroll
-  for (int k = 0; k < NB; k++)
-    if ((k & 1) == par) {  // red: neighbours are zero
-      const size_t g = base + tx + NB * (ty + NB * k);
-      z[lds(tx, ty, k)] = omega * (vf[k] / (TZ)M.p[0][g]);
-    }
-  __syncthreads();
+  for (int m = 0; m < NB / 2; m++) {  // red: every neighbour is still zero
+    const int k = 2 * m + par;
+    z[zi(tx, ty, k)] = omega * (vred[m] / (TZ)apr[m]);
+  }
+  __syncthreads();  // also orders this block's p / s stores before the black cells re-read them
+  // black (then, for the symmetric sweep, red again) from the updated neighbours
+  auto half_sweep = [&](int colour_off) {
+    const bool red = colour_off == par;
 #pragma unroll
-  for (int k = 0; k < NB; k++)
-    if ((k & 1) != par) {  // black: red neighbours inside the block, zero ghosts
-      const size_t g = base + tx + NB * (ty + NB * k);
-      const int li = lds(tx, ty, k);
-      const TZ rs = vf[k] - (TZ)M.p[1][g] * z[li - 1] - (TZ)M.p[2][g] * z[li + 1] - (TZ)M.p[3][g] * z[li - LP] -
-                    (TZ)M.p[4][g] * z[li + LP] - (TZ)M.p[5][g] * z[li - LP * LP] - (TZ)M.p[6][g] * z[li + LP * LP];
-      z[li] = omega * (rs / (TZ)M.p[0][g]);
-    }
-  __syncthreads();
-  if (SYM) {  // backward half: red cells again from the updated black neighbours (symmetric GS)
+    for (int m0 = 0; m0 < NB / 2; m0 += 2) {
+      TM a[2][7];
+      TZ vv[2];
 #pragma unroll
-    for (int k = 0; k < NB; k++)
-      if ((k & 1) == par) {
+      for (int mm = 0; mm < 2; mm++) {
+        const int k = 2 * (m0 + mm) + colour_off;
         const size_t g = base + tx + NB * (ty + NB * k);
-        const int li = lds(tx, ty, k);
-        const TZ zr = z[li];
-        const TZ rs = vf[k] - (TZ)M.p[1][g] * z[li - 1] - (TZ)M.p[2][g] * z[li + 1] - (TZ)M.p[3][g] * z[li - LP] -
-                      (TZ)M.p[4][g] * z[li + LP] - (TZ)M.p[5][g] * z[li - LP * LP] - (TZ)M.p[6][g] * z[li + LP * LP];
-        z[li] = zr + omega * (rs / (TZ)M.p[0][g] - zr);
+#pragma unroll
+        for (int q = 0; q < 7; q++) a[mm][q] = M.p[q][g];
+        vv[mm] = red ? vred[m0 + mm] : (TZ)vsrc[g];
       }
+#pragma unroll
+      for (int mm = 0; mm < 2; mm++) {
+        const int k = 2 * (m0 + mm) + colour_off;
+        const TZ rs = vv[mm] - (TZ)a[mm][1] * zat(tx - 1, ty, k) - (TZ)a[mm][2] * zat(tx + 1, ty, k) -
+                      (TZ)a[mm][3] * zat(tx, ty - 1, k) - (TZ)a[mm][4] * zat(tx, ty + 1, k) -
+                      (TZ)a[mm][5] * zat(tx, ty, k - 1) - (TZ)a[mm][6] * zat(tx, ty, k + 1);
+        const TZ zo = z[zi(tx, ty, k)];
+        z[zi(tx, ty, k)] = red ? zo + omega * (rs / (TZ)a[mm][0] - zo) : omega * (rs / (TZ)a[mm][0]);
+      }
+    }
     __syncthreads();
-  }
+  };
+  half_sweep(1 - par);
+  if (SYM) half_sweep(par);
   double* out = MODE == 0 ? V.pp : V.ss;
 #pragma unroll
-  for (int k = 0; k < NB; k++) out[base + tx + NB * (ty + NB * k)] = (double)z[lds(tx, ty, k)];
+  for (int k = 0; k < NB; k++) out[base + tx + NB * (ty + NB * k)] = (double)z[zi(tx, ty, k)];
   double* h = MODE == 0 ? V.hpp : V.hss;
-  push_halo<double, BI_NT>(B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return (double)z[lds(i, j, k)]; }, h,
+  push_halo<double, BI_NT>(B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return (double)z[zi(i, j, k)]; }, h,
                            h);
 }
 

@@ -652,15 +652,59 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
       o.gs = method == 9 ? BiOpts::SGS32_ARITH64 : BiOpts::GS32_ARITH64;
       o.M32 = &M;
     }
-    rc = bicgstab_core<double>(c, x, coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega, countMax,
-                               eps, c->dst, c->hst, nullptr, nullptr, o);
+    // The Krylov recurrence converges for the FP32-stored matrix.  Its answer is accepted only on the
+    // FP64 residual ||b - A x|| of the FP64 matrix; a case above eps (a matrix whose coefficients are
+    // not all FP32-representable) restarts from that residual, so the reported residual is always
+    // the FP64 one.
+    int fe;
+    rc = scratch_field(c, 4, R_E, &fe);
     if (rc) return rc;
-    rc = sync_state(c, c->dst, c->hst);
-    if (rc) return rc;
+    const Coef7<double> K64 = coef<double>(c, A);
+    const double* b64 = (const double*)c->fields[b].d[0];
     const std::vector<int> ids5 = selected_cases(c);
-    for (size_t i = 0; i < ids5.size(); i++)
-      if (outer) outer[i] = 0;
-    return report(c, c->hst, count, residual);
+    std::vector<int> outers(c->cases.size(), -1);
+    launch_solve_init(c->sm, c->dst2, (int)c->cases.size(), omega, eps, 100000);
+    int o5 = 0, total = 0;
+    bool done = false;
+    auto defect = [&]() -> int {
+      const FieldRec X = c->fields[x];
+      launch_defect(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst2, K64, b64, (const double*)X.d[0],
+                    (const double*)X.h[0], (float*)c->fields[fe].d[0], c->dpart, (size_t)c->cap, o5);
+      fin(c, c->dst2, FIN_DEFECT, o5);
+      HIPCHK(hipGetLastError());
+      int r = read_state(c, c->dst2, c->hst2, ids5, &done);
+      for (int id : ids5)
+        if (c->hst2[id].done && outers[id] < 0) outers[id] = o5;
+      return r;
+    };
+    rc = defect();
+    if (rc) return rc;
+    std::vector<int> its(c->cases.size(), 0);
+    while (!done && total < countMax) {
+      rc = bicgstab_core<double>(c, x, K64, b64, bcg, omega, countMax - total, eps, c->dst, c->hst, c->dst2,
+                                 nullptr, o);
+      if (rc) return rc;
+      bool inner_done;
+      rc = read_state(c, c->dst, c->hst, ids5, &inner_done);
+      if (rc) return rc;
+      for (int id : ids5) {
+        its[id] += c->hst[id].count;
+        total = std::max(total, its[id]);
+      }
+      o5++;
+      rc = defect();
+      if (rc) return rc;
+    }
+    int status = 0;
+    for (size_t i = 0; i < ids5.size(); i++) {
+      const SolveState& s = c->hst2[ids5[i]];
+      if (count) count[i] = its[ids5[i]];
+      if (residual) residual[i] = s.residual;
+      if (outer) outer[i] = outers[ids5[i]] < 0 ? o5 : outers[ids5[i]];
+      if (s.status) status = 1;
+    }
+    if (status) set_error("residual is NaN or Inf");
+    return status;
   }
   if (method >= 3) {
     if (!(inner_eps > 0.0 && inner_eps < 1.0)) return fail(-1, "reliable-update factor must be in (0, 1)");

@@ -315,7 +315,7 @@ def true_residual(og, A, x, b):
     return np.sqrt(rr / bb)
 
 
-@pytest.mark.parametrize("method", [0, 1, 3])
+@pytest.mark.parametrize("method", [0, 1, 3, 5, 6, 7, 8, 9])
 def test_mixed_precision_reaches_fp64_residual(ctx, method):
     tree = trees()["two_level"]
     params = ffvhc.case_params(bc=BC)
@@ -329,10 +329,12 @@ def test_mixed_precision_reaches_fp64_residual(ctx, method):
         x = og.zeros()
         fa, fb, fx = upload_system(ctx, cid, A, b, x)
         eps = 1e-10
-        # methods 0/1: inner relative reduction 1e-4; method 3: reliable update every 1e-2
+        # methods 0/1: inner relative reduction 1e-4; method 3: reliable update every 1e-2; methods 5-9
+        # (FP32-stored matrix / preconditioner) run on the random coefficients here, which FP32 does not
+        # hold exactly, so their acceptance test on the FP64 residual is exercised
         cnt, res, outer, rc = ctx.solve_mixed(method, fx, fa, fb, ffvhc.BC_P, 1.0 if method else 1.2, 100000, eps,
                                               1e-2 if method == 3 else 1e-4, ncases=1)
-        assert rc == 0 and res[0] <= eps and outer[0] >= 2
+        assert rc == 0 and res[0] <= eps and outer[0] >= (2 if method < 5 else 0)
         got = ctx.download(fx, cid, out=x.copy())
         # the FP64 residual the device reports is the true one (computed by the oracle on the result)
         tr = true_residual(og, A, got, b)
@@ -341,7 +343,7 @@ def test_mixed_precision_reaches_fp64_residual(ctx, method):
         ctx.upload(fx, cid, og.zeros())
         if method == 1:
             c64, r64, _ = ctx.cg(fx, fa, fb, ffvhc.BC_P, 1.0, 1, 100000, eps, ncases=1)
-        elif method == 3:
+        elif method >= 3:
             c64, r64, _ = ctx.bicgstab(fx, fa, fb, ffvhc.BC_P, 1.0, 1, 100000, eps, ncases=1)
         else:
             c64, r64, _ = ctx.rbgs(fx, fa, fb, ffvhc.BC_P, 1.2, 100000, eps, ncases=1)
