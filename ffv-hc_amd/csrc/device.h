@@ -97,15 +97,10 @@ struct Coef7 {
   const T* p[7];
 };
 
-// BiCGSTAB work vectors r r0 p p_ q_ s s_ t_ and the ghost layers of p_ and s_.  TP is the
-// storage type of the shadow residual r0 and of the interiors of the preconditioned directions
-// p_ and s_: every kernel reads the same stored values (x += alpha p_ and q_ = A p_ see one p_),
-// so a narrower TP changes the preconditioner and the shadow vector, not the consistency of the
-// recurrence.  The ghost layers of p_ / s_ stay in T, computed from the stored interiors.
-template <class T, class TP = T>
+// BiCGSTAB work vectors r r0 p p_ q_ s s_ t_ and the ghost layers of p_ and s_
+template <class T>
 struct BiVecs {
-  T *r, *p, *q, *s, *t;
-  TP *r0, *pp, *ss;
+  T *r, *r0, *p, *pp, *q, *s, *ss, *t;
   T *hpp, *hss;
 };
 

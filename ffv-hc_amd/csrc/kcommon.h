@@ -77,16 +77,15 @@ constexpr int LP = NB + 2;  // LDS pitch
 __device__ inline int lds(int i, int j, int k) { return (i + 1) + LP * ((j + 1) + LP * (k + 1)); }
 
 // Stage a block's interior and its one-cell ghost layer into LDS (pitch LP); NT threads with
-// thread t owning column (t & 15, t >> 4) of every plane (NT = 256).  TX: storage type of the
-// interior (widened to T on load).
-template <class T, int NT, class TX = T>
-__device__ inline void stage_block(T* s, const TX* __restrict__ x, const T* __restrict__ h, int slot) {
+// thread t owning column (t & 15, t >> 4) of every plane (NT = 256).
+template <class T, int NT>
+__device__ inline void stage_block(T* s, const T* __restrict__ x, const T* __restrict__ h, int slot) {
   static_assert(NT == 256, "one thread per (i, j) column");
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
   T v[NB];
 #pragma unroll
-  for (int k = 0; k < NB; k++) v[k] = (T)x[base + tx + NB * (ty + NB * k)];
+  for (int k = 0; k < NB; k++) v[k] = x[base + tx + NB * (ty + NB * k)];
   T hv[NH / NT];
 #pragma unroll
   for (int q = 0; q < NH / NT; q++) hv[q] = h[(size_t)slot * NH + t + q * NT];

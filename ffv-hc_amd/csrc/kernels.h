@@ -58,10 +58,10 @@ void launch_cg_q(hipStream_t sm, const int* active, int nact, const DevBlock* bl
 template <class T>
 void launch_cg_xr(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs, SolveState* st,
                   Coef7<T> A, T* x, T* r, const T* p, const T* q, double* part, size_t nslots, int count);
-template <class T, class TP = T>
+template <class T>
 void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
-                          SolveState* st, Coef7<T> A, const T* b, const T* x, const T* hx, BiVecs<T, TP> V,
-                          double* part, size_t nslots);
+                          SolveState* st, Coef7<T> A, const T* b, const T* x, const T* hx, BiVecs<T> V, double* part,
+                          size_t nslots);
 // Precision options of an FP64 BiCGSTAB: A32 = FP32 copy of the matrix for the matvecs and the
 // Jacobi preconditioner (widened to FP64 on load); gs = block red-black GS preconditioner with
 // an FP32 matrix and FP32 arithmetic, an FP32 matrix and FP64 arithmetic, all FP64, or the
@@ -72,10 +72,6 @@ struct BiOpts {
   int gs = NONE;
   const Coef7<float>* M32 = nullptr;
   const Coef7<double>* M64 = nullptr;
-  // FP32 r0 / p_ / s_ interiors (FP64 solve only): vec32 asks the solver for them, V32 hands
-  // them to the iteration launcher in place of the FP64 vectors
-  bool vec32 = false;
-  const BiVecs<double, float>* V32 = nullptr;
 };
 
 template <class T>

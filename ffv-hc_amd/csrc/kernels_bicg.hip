@@ -18,11 +18,11 @@ namespace ffvhc {
 
 constexpr int BI_NT = 256;
 
-template <class T, class TP>
+template <class T>
 __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
                                                         const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                         Coef7<T> A, const T* __restrict__ bv, const T* __restrict__ x,
-                                                        const T* __restrict__ hx, BiVecs<T, TP> V,
+                                                        const T* __restrict__ hx, BiVecs<T> V,
                                                         double* __restrict__ part, size_t nslots) {
   __shared__ T s[LP * LP * LP];
   __shared__ double red[BI_NT / 64];
@@ -54,11 +54,10 @@ __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ 
       const T r = b0[kk] - a[kk][1] * s[li - 1] - a[kk][2] * s[li + 1] - a[kk][3] * s[li - LP] -
                   a[kk][4] * s[li + LP] - a[kk][5] * s[li - LP * LP] - a[kk][6] * s[li + LP * LP] -
                   a[kk][0] * s[li];
-      const TP r0 = (TP)r;
       V.r[g] = r;
-      V.r0[g] = r0;
+      V.r0[g] = r;
       bb += (double)b0[kk] * (double)b0[kk];
-      rr1 += (double)r * (double)r0;
+      rr1 += (double)r * (double)r;
     }
   }
   double sb = block_sum<BI_NT>(bb, red);
@@ -71,12 +70,12 @@ __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ 
 
 // P: p update and p_ = M p with its ghost layer (mode 0); S: s = r - alpha q_, s_ = M s (mode 1)
 // TA: storage precision of the matrix (FP32 copies for an FP64 solve are widened on load)
-template <class T, int MODE, class TA = T, class TP = T>
+template <class T, int MODE, class TA = T>
 __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict__ active,
                                                            const DevBlock* __restrict__ blk,
                                                            const DevCase* __restrict__ cs,
-                                                           const SolveState* __restrict__ st, Coef7<TA> A,
-                                                           BiVecs<T, TP> V, int count) {
+                                                           const SolveState* __restrict__ st, Coef7<TA> A, BiVecs<T> V,
+                                                           int count) {
   __shared__ T s[LP * LP * LP];
   const int slot = active[blockIdx.x];
   const DevBlock& B = blk[slot];
@@ -115,10 +114,10 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
         v = na * qv[kk] + rv[kk];  // AXPYZ(s, q_, r, -alpha)
         V.s[g] = v;
       }
-      const TP m = (TP)(omega * (v / ap[kk]));
+      const T m = omega * (v / ap[kk]);
       if (MODE == 0) V.pp[g] = m;
       else V.ss[g] = m;
-      s[lds(tx, ty, k0 + kk)] = (T)m;
+      s[lds(tx, ty, k0 + kk)] = m;
     }
   }
   __syncthreads();
@@ -130,12 +129,12 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
 // z = one SOR sweep of M z = v from z = 0 in the matrix precision TM (FP32 copy of the matrix for
 // the mixed-precision solve, FP64 for its all-FP64 counterpart), red cells first, the block's ghost values zero (a block-Jacobi / red-black GS
 // preconditioner, fixed and linear); p_ (s_) = z widened to FP64, ghost layer pushed.
-template <int MODE, class TM, class TZ = TM, bool SYM = false, class TP = double>
+template <int MODE, class TM, class TZ = TM, bool SYM = false>
 __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restrict__ active,
                                                               const DevBlock* __restrict__ blk,
                                                               const DevCase* __restrict__ cs,
                                                               const SolveState* __restrict__ st, Coef7<TM> M,
-                                                              BiVecs<double, TP> V, int count) {
+                                                              BiVecs<double> V, int count) {
   // z of the block's own cells only: its ghost values are zero, so no ghost ring is staged and
   // every cell is written (red, then black) before a neighbour reads it
   __shared__ TZ z[NC];
@@ -218,21 +217,20 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
   };
   half_sweep(1 - par);
   if (SYM) half_sweep(par);
-  TP* out = MODE == 0 ? V.pp : V.ss;
+  double* out = MODE == 0 ? V.pp : V.ss;
 #pragma unroll
-  for (int k = 0; k < NB; k++) out[base + tx + NB * (ty + NB * k)] = (TP)z[zi(tx, ty, k)];
-  // ghost values from the stored (TP) interior, in FP64
+  for (int k = 0; k < NB; k++) out[base + tx + NB * (ty + NB * k)] = (double)z[zi(tx, ty, k)];
   double* h = MODE == 0 ? V.hpp : V.hss;
-  push_halo<double, BI_NT>(B, cs[c], BC_NULL, slot,
-                           [&](int i, int j, int k) { return (double)(TP)z[zi(i, j, k)]; }, h, h);
+  push_halo<double, BI_NT>(B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return (double)z[zi(i, j, k)]; }, h,
+                           h);
 }
 
 // Q: q_ = A p_, q_.r0 -> alpha (mode 0);  T: t_ = A s_, t_.s and t_.t_ -> gamma (mode 1)
-template <class T, int MODE, class TA = T, class TP = T>
+template <class T, int MODE, class TA = T>
 __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict__ active,
                                                           const DevBlock* __restrict__ blk,
                                                           const DevCase* __restrict__ cs, SolveState* __restrict__ st,
-                                                          Coef7<TA> A, BiVecs<T, TP> V, double* __restrict__ part,
+                                                          Coef7<TA> A, BiVecs<T> V, double* __restrict__ part,
                                                           size_t nslots) {
   __shared__ T s[LP * LP * LP];
   __shared__ double red[BI_NT / 64];
@@ -241,6 +239,7 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
   if (st[c].done) return;
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
+  const T* __restrict__ w = MODE == 0 ? V.r0 : V.s;
   T* __restrict__ y = MODE == 0 ? V.q : V.t;
   constexpr int KB = Batch<T>::K;
   double d0 = 0.0, d1 = 0.0;
@@ -252,10 +251,10 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
 #pragma unroll
       for (int q = 0; q < 7; q++) a[kk][q] = (T)A.p[q][g];
-      wv[kk] = MODE == 0 ? (T)V.r0[g] : V.s[g];
+      wv[kk] = w[g];
     }
     if (k0 == 0) {
-      stage_block<T, BI_NT, TP>(s, MODE == 0 ? V.pp : V.ss, MODE == 0 ? V.hpp : V.hss, slot);
+      stage_block<T, BI_NT>(s, MODE == 0 ? V.pp : V.ss, MODE == 0 ? V.hpp : V.hss, slot);
       __syncthreads();
     }
 #pragma unroll
@@ -278,10 +277,10 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
   }
 }
 
-template <class T, class TP>
+template <class T>
 __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
                                                       const DevCase* __restrict__ cs, SolveState* __restrict__ st,
-                                                      T* __restrict__ x, BiVecs<T, TP> V, double* __restrict__ part,
+                                                      T* __restrict__ x, BiVecs<T> V, double* __restrict__ part,
                                                       size_t nslots, int count) {
   __shared__ double red[BI_NT / 64];
   const int slot = active[blockIdx.x];
@@ -298,12 +297,12 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
-      pv[kk] = (T)V.pp[g];
-      sv_[kk] = (T)V.ss[g];
+      pv[kk] = V.pp[g];
+      sv_[kk] = V.ss[g];
       xv[kk] = x[g];
       tv[kk] = V.t[g];
       sv[kk] = V.s[g];
-      r0v[kk] = (T)V.r0[g];
+      r0v[kk] = V.r0[g];
     }
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
@@ -403,56 +402,51 @@ void launch_bicgstab_restart(hipStream_t sm, const int* active, int nact, const 
   if (nact) hipLaunchKernelGGL(bi_restart_kernel<T>, dim3(nact), dim3(BI_NT), 0, sm, active, blk, st, V);
 }
 
-template <class T, class TP>
+template <class T>
 void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
-                          SolveState* st, Coef7<T> A, const T* b, const T* x, const T* hx, BiVecs<T, TP> V,
-                          double* part, size_t nslots) {
-  if (nact)
-    hipLaunchKernelGGL((bi_init_kernel<T, TP>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, b, x, hx, V,
-                       part, nslots);
+                          SolveState* st, Coef7<T> A, const T* b, const T* x, const T* hx, BiVecs<T> V, double* part,
+                          size_t nslots) {
+  if (nact) hipLaunchKernelGGL(bi_init_kernel<T>, dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, b, x, hx, V,
+                               part, nslots);
 }
 
-// One iteration's launches for vectors stored as BiVecs<T, TP>.
-template <class T, class TP>
-static void bicgstab_iter(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase,
-                          const DevBlock* blk, const DevCase* cs, SolveState* st, Coef7<T> A, T* x, BiVecs<T, TP> V,
+template <class T>
+void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase,
+                          const DevBlock* blk, const DevCase* cs, SolveState* st, Coef7<T> A, T* x, BiVecs<T> V,
                           double* part, size_t nslots, int count, int mask, const BiOpts& o) {
-  constexpr bool D = std::is_same<T, double>::value;
-  const dim3 g(nact), b(BI_NT);
+  // mask bits 0-4 select P, Q, S, T, XR; bit 5 leaves out the per-case finalisations.
+  if (!nact) return;
+  const bool dbl = sizeof(T) == 8;
+  BiVecs<double>& Vd = *reinterpret_cast<BiVecs<double>*>(&V);
   auto precond = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
-    if constexpr (D) {
-      if (o.gs == BiOpts::GS32)
-        hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float, float, false, TP>), g, b, 0, sm, active, blk, cs, st,
-                           *o.M32, V, count);
-      else if (o.gs == BiOpts::GS32_ARITH64)
-        hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float, double, false, TP>), g, b, 0, sm, active, blk, cs, st,
-                           *o.M32, V, count);
-      else if (o.gs == BiOpts::SGS32_ARITH64)
-        hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float, double, true, TP>), g, b, 0, sm, active, blk, cs, st,
-                           *o.M32, V, count);
-      else if (o.gs == BiOpts::GS64)
-        hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, double, double, false, TP>), g, b, 0, sm, active, blk, cs,
-                           st, *o.M64, V, count);
-      else if (o.A32)
-        hipLaunchKernelGGL((bi_precond_kernel<double, MODE, float, TP>), g, b, 0, sm, active, blk, cs, st, *o.A32, V,
-                           count);
-      else
-        hipLaunchKernelGGL((bi_precond_kernel<T, MODE, T, TP>), g, b, 0, sm, active, blk, cs, st, A, V, count);
-    } else {
-      hipLaunchKernelGGL((bi_precond_kernel<T, MODE, T, TP>), g, b, 0, sm, active, blk, cs, st, A, V, count);
-    }
+    if (dbl && o.gs == BiOpts::GS32)
+      hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
+                         *o.M32, Vd, count);
+    else if (dbl && o.gs == BiOpts::GS32_ARITH64)
+      hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float, double>), dim3(nact), dim3(BI_NT), 0, sm, active, blk,
+                         cs, st, *o.M32, Vd, count);
+    else if (dbl && o.gs == BiOpts::SGS32_ARITH64)
+      hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float, double, true>), dim3(nact), dim3(BI_NT), 0, sm, active,
+                         blk, cs, st, *o.M32, Vd, count);
+    else if (dbl && o.gs == BiOpts::GS64)
+      hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, double>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
+                         *o.M64, Vd, count);
+    else if (dbl && o.A32)
+      hipLaunchKernelGGL((bi_precond_kernel<double, MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs,
+                         st, *o.A32, Vd, count);
+    else
+      hipLaunchKernelGGL((bi_precond_kernel<T, MODE>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V,
+                         count);
   };
   auto matvec = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
-    if constexpr (D) {
-      if (o.A32) {
-        hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, TP>), g, b, 0, sm, active, blk, cs, st, *o.A32, V,
-                           part, nslots);
-        return;
-      }
-    }
-    hipLaunchKernelGGL((bi_matvec_kernel<T, MODE, T, TP>), g, b, 0, sm, active, blk, cs, st, A, V, part, nslots);
+    if (dbl && o.A32)
+      hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs,
+                         st, *o.A32, Vd, part, nslots);
+    else
+      hipLaunchKernelGGL((bi_matvec_kernel<T, MODE>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V,
+                         part, nslots);
   };
   using M0 = std::integral_constant<int, 0>;
   using M1 = std::integral_constant<int, 1>;
@@ -467,31 +461,15 @@ static void bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_GAMMA, count);
   }
   if (mask & 16) {
-    hipLaunchKernelGGL((bi_xr_kernel<T, TP>), g, b, 0, sm, active, blk, cs, st, x, V, part, nslots, count);
+    hipLaunchKernelGGL(bi_xr_kernel<T>, dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part, nslots,
+                       count);
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_XR, count);
   }
 }
 
-template <class T>
-void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase,
-                          const DevBlock* blk, const DevCase* cs, SolveState* st, Coef7<T> A, T* x, BiVecs<T> V,
-                          double* part, size_t nslots, int count, int mask, const BiOpts& o) {
-  // mask bits 0-4 select P, Q, S, T, XR; bit 5 leaves out the per-case finalisations.
-  if (!nact) return;
-  if constexpr (std::is_same<T, double>::value) {
-    if (o.V32) {
-      bicgstab_iter<double, float>(sm, active, nact, caseact, ncase, blk, cs, st, A, x, *o.V32, part, nslots, count,
-                                   mask, o);
-      return;
-    }
-  }
-  bicgstab_iter<T, T>(sm, active, nact, caseact, ncase, blk, cs, st, A, x, V, part, nslots, count, mask, o);
-}
-
 #define INST(T)                                                                                                     \
-  template void launch_bicgstab_init<T, T>(hipStream_t, const int*, int, const DevBlock*, const DevCase*,             \
-                                           SolveState*, Coef7<T>, const T*, const T*, const T*, BiVecs<T>, double*,  \
-                                           size_t);                                                                  \
+  template void launch_bicgstab_init<T>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,  \
+                                        Coef7<T>, const T*, const T*, const T*, BiVecs<T>, double*, size_t);         \
   template void launch_bicgstab_iter<T>(hipStream_t, const int*, int, const int*, int, const DevBlock*,            \
                                         const DevCase*, SolveState*, Coef7<T>, T*, BiVecs<T>, double*, size_t, int,  \
                                         int, const BiOpts&);                                                         \
@@ -503,8 +481,5 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
 INST(double)
 INST(float)
 #undef INST
-template void launch_bicgstab_init<double, float>(hipStream_t, const int*, int, const DevBlock*, const DevCase*,
-                                                  SolveState*, Coef7<double>, const double*, const double*,
-                                                  const double*, BiVecs<double, float>, double*, size_t);
 
 }  // namespace ffvhc

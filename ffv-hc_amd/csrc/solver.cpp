@@ -3,7 +3,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <type_traits>
 
 #include "kernels.h"
 #include "runtime.h"
@@ -156,38 +155,10 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
   BiVecs<T> V;
   int rc0 = bicg_vectors<T>(c, &V);
   if (rc0) return rc0;
-  BiOpts o = opts;
-  BiVecs<double, float> V32;
-  launch_solve_init(c->sm, st, (int)c->cases.size(), omega, eps, countMax, outer);
-  if constexpr (std::is_same<T, double>::value) {
-    if (o.vec32) {  // FP32 interiors of r0, p_, s_; the rest (and the ghost layers) FP64
-      int f32[3];
-      const int roles[3] = {R_BI + 1, R_BI + 3, R_BI + 6};
-      for (int q = 0; q < 3; q++) {
-        rc0 = scratch_field(c, 4, roles[q], &f32[q]);
-        if (rc0) return rc0;
-      }
-      V32.r = V.r;
-      V32.p = V.p;
-      V32.q = V.q;
-      V32.s = V.s;
-      V32.t = V.t;
-      V32.hpp = V.hpp;
-      V32.hss = V.hss;
-      V32.r0 = (float*)c->fields[f32[0]].d[0];
-      V32.pp = (float*)c->fields[f32[1]].d[0];
-      V32.ss = (float*)c->fields[f32[2]].d[0];
-      o.V32 = &V32;
-      const FieldRec X0 = c->fields[xf];
-      launch_bicgstab_init<double, float>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv,
-                                          (const double*)X0.d[0], (const double*)X0.h[0], V32, c->dpart,
-                                          (size_t)c->cap);
-    }
-  }
   const FieldRec X = c->fields[xf];  // copy taken after the scratch allocations
-  if (!o.V32)
-    launch_bicgstab_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, (const T*)X.d[0],
-                            (const T*)X.h[0], V, c->dpart, (size_t)c->cap);
+  launch_solve_init(c->sm, st, (int)c->cases.size(), omega, eps, countMax, outer);
+  launch_bicgstab_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, (const T*)X.d[0],
+                          (const T*)X.h[0], V, c->dpart, (size_t)c->cap);
   fin(c, st, FIN_BI_INIT, 0);
   int n = 1, batch = kBatchMin;
   bool done = false;
@@ -195,7 +166,7 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
     int last = std::min(n + batch - 1, countMax);
     for (; n <= last; n++)
       launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase, st, K,
-                              (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, 31, o);
+                              (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, 31, opts);
     HIPCHK(hipGetLastError());
     int rc = read_state(c, st, hst, ids, &done);
     if (rc) return rc;
@@ -639,12 +610,12 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
-  if (method < 0 || method > 11)
+  if (method < 0 || method > 9)
     return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB, 3 Jacobi-BiCGSTAB with reliable updates, "
                     "4 the same with a seeded shadow residual, 5 FP64 BiCGSTAB with an FP32 red-black GS "
                     "preconditioner, 6 the same all in FP64 (reference point), 7 FP64 Jacobi-BiCGSTAB with an FP32 "
                     "matrix, 8 as 7 with the block red-black GS preconditioner (FP64 arithmetic), 9 as 8 with the "
-                    "symmetric red-black-red sweep, 10 / 11 as 8 / 9 with FP32 storage of r0, p_ and s_");
+                    "symmetric red-black-red sweep");
   if (method == 6) {
     const Coef7<double> M64 = coef<double>(c, A);
     BiOpts o;
@@ -660,7 +631,7 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
       if (outer) outer[i] = 0;
     return report(c, c->hst, count, residual);
   }
-  if (method == 5 || method >= 7) {  // FP64 recurrence, dots and residual; FP32 matrix
+  if (method == 5 || method == 7 || method == 8 || method == 9) {  // FP64 recurrence, dots and residual; FP32 matrix
     int fa[7];
     for (int q = 0; q < 7; q++) {
       rc = scratch_field(c, 4, R_A + q, &fa[q]);
@@ -676,11 +647,10 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
       o.M32 = &M;
     } else if (method == 7) {
       o.A32 = &M;
-    } else {  // 8-11: FP32 matrix storage everywhere, (symmetric) GS preconditioner in FP64 arithmetic
+    } else {  // 8 / 9: FP32 matrix storage everywhere, (symmetric) GS preconditioner in FP64 arithmetic
       o.A32 = &M;
-      o.gs = (method & 1) ? BiOpts::SGS32_ARITH64 : BiOpts::GS32_ARITH64;
+      o.gs = method == 9 ? BiOpts::SGS32_ARITH64 : BiOpts::GS32_ARITH64;
       o.M32 = &M;
-      o.vec32 = method >= 10;
     }
     // The Krylov recurrence converges for the FP32-stored matrix.  Its answer is accepted only on the
     // FP64 residual ||b - A x|| of the FP64 matrix; a case above eps (a matrix whose coefficients are

@@ -186,13 +186,12 @@ int ffvhc_residual(ffvhc_ctx* ctx, int x, const int* A, int b, double* residual)
  * 2 Jacobi-BiCGSTAB (each restarted per correction); 3 Jacobi-BiCGSTAB with reliable updates (the
  * FP32 recurrence continues across corrections; inner_eps = residual reduction between updates);
  * 4 as 3 with a seeded random shadow residual r0 instead of r0 = r.
- * Methods 5-11 keep the BiCGSTAB recurrence, dots and state in FP64 and narrow what is stored:
+ * Methods 5-9 keep the BiCGSTAB recurrence, dots and state in FP64 and narrow what is stored:
  * 5 FP32 block red-black GS preconditioner (FP32 arithmetic), 6 the same GS all in FP64,
  * 7 FP32 copy of the matrix for the matvecs and the Jacobi preconditioner, 8 as 7 with the block
- * GS preconditioner in FP64 arithmetic, 9 as 8 with the symmetric (red, black, red) sweep,
- * 10 / 11 as 8 / 9 with the shadow residual r0 and the preconditioned directions p_, s_ stored in
- * FP32.  For 5 and 7-11 the answer is accepted on the FP64 residual of the FP64 matrix; a case
- * above eps restarts from it (inner_eps unused; omega is the GS relaxation factor).
+ * GS preconditioner in FP64 arithmetic, 9 as 8 with the symmetric (red, black, red) sweep.  For 5
+ * and 7-9 the answer is accepted on the FP64 residual of the FP64 matrix; a case above eps
+ * restarts from it (inner_eps unused; omega is the GS relaxation factor).
  * inner_eps: relative reduction per correction; countMax caps each inner solve (a capped inner
  * solve still yields a correction).  count[c] = inner iterations in total, outer[c] = corrections
  * taken, residual[c] = FP64 ||b - Ax|| / ||b|| after the last correction. */
