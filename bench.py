@@ -171,7 +171,13 @@ MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64
                  "FP32 Jacobi-BiCGSTAB in FP64 defect correction",
                  "FP32 Jacobi-BiCGSTAB with FP64 reliable updates", "the same with a seeded shadow residual",
                  "FP64 BiCGSTAB with an FP32 block red-black GS preconditioner",
-                 "FP64 BiCGSTAB with an FP64 block red-black GS preconditioner"]
+                 "FP64 BiCGSTAB with an FP64 block red-black GS preconditioner",
+                 "FP64 Jacobi-BiCGSTAB on an FP32-stored matrix",
+                 "FP64 BiCGSTAB on an FP32-stored matrix, block red-black GS preconditioner",
+                 "FP64 BiCGSTAB on an FP32-stored matrix, symmetric block red-black GS preconditioner"]
+
+# schedule of `bench.py --profile` (the rocprofv3 runs): FP32-stored matrix + block GS, omega 1.2
+PROFILE_SCHEDULE = (8, 0.5, 20000, 1.2)
 
 # kernel names of the timed stages as rocprofv3 reports them
 PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0>", "bi_q": "bi_matvec_kernel<%s, 0>", "bi_s": "bi_precond_kernel<%s, 1>",
@@ -179,15 +185,28 @@ PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0>", "bi_q": "bi_matvec_kernel<%s, 
               "cg_p": "cg_p_kernel<%s>", "cg_q": "cg_q_kernel<%s>", "cg_xr": "cg_xr_kernel<%s>"}
 
 
-def pmc_traffic(stage, w):
-    """HBM bytes per launch of a stage's kernel from the committed rocprofv3 PMC summary of this
+def mixed_prof_name(method, stage):
+    """rocprofv3 name of a stage's kernel in the FP64-recurrence methods 5-9."""
+    mode = {"bi_p": 0, "bi_q": 0, "bi_s": 1, "bi_t": 1}.get(stage)
+    if stage == "bi_xr":
+        return "bi_xr_kernel<double>"
+    if stage in ("bi_q", "bi_t"):
+        return "bi_matvec_kernel<double, %d, float>" % mode if method >= 7 else "bi_matvec_kernel<double, %d>" % mode
+    if method == 7:
+        return "bi_precond_kernel<double, %d, float>" % mode
+    tm, tz, sym = {5: ("float", "float", "false"), 6: ("double", "double", "false"),
+                   8: ("float", "double", "false"), 9: ("float", "double", "true")}[method]
+    return "bi_precond_gs_kernel<%d, %s, %s, %s>" % (mode, tm, tz, sym)
+
+
+def pmc_traffic(name):
+    """HBM bytes per launch of a kernel (rocprofv3 name) from the committed PMC summary of this
     workload (FETCH_SIZE + WRITE_SIZE, calibrated with a plain copy); None if absent."""
     path = os.path.join(ROOT, "profiles", "c2_poisson_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         d = json.load(fh)
-    name = PROF_NAMES[stage] % ("float" if w == 4 else "double")
     v = d.get("bytes_per_launch", {}).get(name)
     return None if v is None else int(round(v))
 
@@ -196,6 +215,21 @@ def kernel_bytes(w):
     h = 0.375
     return {"bi_p": (6 + h) * w, "bi_q": (10 + h) * w, "bi_s": (5 + h) * w, "bi_t": (10 + h) * w, "bi_xr": 8 * w,
             "rbgs": (10 + 2 * h) * w, "cg_p": (4 + h) * w, "cg_q": (9 + h) * w, "cg_xr": 7 * w}
+
+
+def kernel_bytes_mixed(method):
+    """Bytes per cell per launch of the FP64-recurrence methods: FP64 vectors, the matrix bands in
+    FP32 (ma = 4 bytes) or FP64 where the method stores them so; the GS preconditioner reads all
+    seven bands, the Jacobi one the diagonal only."""
+    h = 0.375 * 8                                   # FP64 ghost layer of p_ / s_
+    mv = 4 if method >= 7 else 8                    # matvec bands
+    mp = {5: 4, 6: 8, 7: 4, 8: 4, 9: 4}[method]     # preconditioner bands
+    nb = 1 if method == 7 else 7
+    return {"bi_p": 24 + nb * mp + 16 + h,          # r, p, q in; p, p_ out; p_ ghosts
+            "bi_q": 8 + h + 7 * mv + 8 + 8,         # p_ + ghosts, bands, r0 in; q_ out
+            "bi_s": 16 + nb * mp + 16 + h,          # r, q_ in; s, s_ out; s_ ghosts
+            "bi_t": 8 + h + 7 * mv + 8 + 8,         # s_ + ghosts, bands, s in; t_ out
+            "bi_xr": 64}                            # x, p_, s_, t_, s, r0 in; x, r out
 
 
 def log(msg):
@@ -251,13 +285,25 @@ def main():
 
     # schedule of the mixed-precision solve: picked by the autotuner on this system (untimed)
     import autotune
-    if args.inner_eps > 0 or args.profile:
-        sched = (1, args.inner_eps or 1e-2, 500, 1.0)
+    if args.profile:
+        sched, tuned = PROFILE_SCHEDULE, []
+    elif args.inner_eps > 0:
+        sched = (1, args.inner_eps, 500, 1.0)
         tuned = []
     else:
         log("autotuning the mixed-precision schedule")
         sched, tuned = autotune.tune_mixed(ctx, f64["x"], f64["A"], f64["b"], P, args.eps)
         log("schedule %s; candidates %s" % (sched, [(c, round(t, 4), it, o) for c, t, it, o, r in tuned]))
+
+    if 5 <= sched[0] <= 9:  # stage times of the tuned method's own kernels
+        ctx.fill(f64["x"], 0.0)
+        ctx.bicgstab_timing(f64["x"], f64["A"], f64["b"], sched[3], 5, method=sched[0])
+        ms = ctx.bicgstab_timing(f64["x"], f64["A"], f64["b"], sched[3], nit, method=sched[0])[:5] / nit
+        by = kernel_bytes_mixed(sched[0])
+        kt["mixed"] = {name: {"us": round(t * 1e3, 2), "GBs": round(by[name] * cells / (t * 1e-3) / 1e9, 1),
+                              "kernel": mixed_prof_name(sched[0], name)}
+                       for name, t in zip(("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"), ms)}
+        log("method %d kernels: %s" % (sched[0], json.dumps(kt["mixed"])))
 
     def solve_mixed():
         ctx.fill(f64["x"], 0.0)
@@ -298,10 +344,17 @@ def main():
     # schedule's inner solver
     stage_set = {0: ("32", ("rbgs",)), 1: ("32", ("cg_p", "cg_q", "cg_xr")), 2: ("32", ("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr")),
                  3: ("32", ("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr")), 4: ("32", ("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"))}
-    dtag, dstages = stage_set.get(sched[0], ("64", ("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr")))
+    if "mixed" in kt:
+        dtag, dstages = "mixed", ("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr")
+    else:
+        dtag, dstages = stage_set.get(sched[0], ("64", ("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr")))
     dname = max(dstages, key=lambda k: kt[dtag][k]["us"])
     dom = kt[dtag][dname]
-    dw = 4 if dtag == "32" else 8
+    if dtag == "mixed":
+        dkernel, dbytes = dom["kernel"], kernel_bytes_mixed(sched[0])[dname]
+    else:
+        dw = 4 if dtag == "32" else 8
+        dkernel, dbytes = PROF_NAMES[dname] % ("float" if dw == 4 else "double"), kernel_bytes(dw)[dname]
     out = {
         "metric": "Mcell-steps/s mixed vs FP64 at equal residual; p50/p99 served-case latency",
         "value": round(mix["mcell_steps_s"], 3),
@@ -313,11 +366,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner / f64 defect correction",
+        "dtype": ("f32 inner / f64 defect correction" if sched[0] <= 4 else
+                  "f64 vectors and arithmetic / f32 matrix storage" if sched[0] >= 7 else
+                  "f64 / f32 preconditioner" if sched[0] == 5 else "f64"),
         "data": "synthetic (seeded uniform[-1,1] RHS, mean removed)",
         "config": {"workload": "standalone Poisson, 2-level octree: roots 8^3 of 16^3-cell blocks, level 1 in a "
                                "ball of radius L/4 (1-cell margin)",
-                   "blocks": tree.nblk, "cells": cells, "solver": "Jacobi-BiCGSTAB (PreCount 1, omega 1)",
+                   "blocks": tree.nblk, "cells": cells,
+                   "solver": "FP64 reference path: Jacobi-BiCGSTAB (PreCount 1, omega 1); mixed: the autotuned "
+                             "schedule below, accepted on the FP64 residual",
                    "eps": args.eps},
         "fp64": {"value": round(d64["mcell_steps_s"], 3), "ms_per_step": round(d64["ms_per_step"], 3),
                  "iterations": d64["iters"], "residual": d64["residual"]},
@@ -330,10 +387,9 @@ def main():
                                 "corrections": o} for c, t, it, o, r in tuned]},
         "speedup_mixed_vs_fp64": round(mix["mcell_steps_s"] / d64["mcell_steps_s"], 3),
         "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dname, dw),
-                     "algorithmic_bytes": int(kernel_bytes(dw)[dname] * cells),
-                     "kernel": "%s (FP%d)" % (dname, 8 * dw), "kernel_us": dom["us"],
-                     "bytes_per_cell": kernel_bytes(dw)[dname]},
+                     "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dkernel),
+                     "algorithmic_bytes": int(dbytes * cells), "kernel": dkernel, "kernel_us": dom["us"],
+                     "bytes_per_cell": dbytes},
         "kernels": kt,
     }
     if args.serve_cases > 0:

@@ -293,7 +293,8 @@ int cg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int ni
 
 // niter BiCGSTAB iterations with events between its five kernels; ms[k] = summed device time.
 template <class T>
-int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int niter, double* ms) {
+int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int niter, double* ms,
+                    const BiOpts& o = BiOpts()) {
   BiVecs<T> V;
   int rc = bicg_vectors<T>(c, &V);
   if (rc) return rc;
@@ -316,7 +317,7 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
     for (int k = 0; k < 8; k++) {
       if (bit_of[k])
         launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase, c->dst,
-                                K, (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, bit_of[k] | 32);
+                                K, (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, bit_of[k] | 32, o);
       else
         fin(c, c->dst, fin_of[k], n);
       HIPCHK(hipEventRecord(e[k + 1], c->sm));
@@ -332,6 +333,38 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
       ms[slot_of[k]] += t;
     }
   for (auto& e : ev) (void)hipEventDestroy(e);
+  return 0;
+}
+
+// Storage options of the FP64-recurrence BiCGSTAB methods 5-9: an FP32 copy of the FP64 matrix A
+// (scratch fields, converted here) in *M32 and/or the FP64 matrix in *M64, referenced by *o.
+int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Coef7<double>* M64, BiOpts* o) {
+  *o = BiOpts();
+  *M64 = coef<double>(c, A);
+  if (method == 6) {
+    o->gs = BiOpts::GS64;
+    o->M64 = M64;
+    return 0;
+  }
+  int fa[7];
+  for (int q = 0; q < 7; q++) {
+    int rc = scratch_field(c, 4, R_A + q, &fa[q]);
+    if (rc) return rc;
+  }
+  for (int q = 0; q < 7; q++)
+    launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
+                   (const double*)c->fields[A[q]].d[0]);
+  *M32 = coef<float>(c, fa);
+  if (method == 5) {
+    o->gs = BiOpts::GS32;
+    o->M32 = M32;
+  } else if (method == 7) {
+    o->A32 = M32;
+  } else {  // 8 / 9: FP32 matrix storage everywhere, (symmetric) GS preconditioner in FP64 arithmetic
+    o->A32 = M32;
+    o->gs = method == 9 ? BiOpts::SGS32_ARITH64 : BiOpts::GS32_ARITH64;
+    o->M32 = M32;
+  }
   return 0;
 }
 
@@ -577,6 +610,22 @@ int ffvhc_bicgstab_timing(ffvhc_ctx* c, int x, const int* A, int b, double omega
                    : bicg_timing_run<float>(c, x, A, b, omega, niter, ms5);
 }
 
+int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, double omega, int niter,
+                                double* ms6) {
+  int prec = 8;
+  int rc = prologue(c, x, A, b, BC_NULL, &prec);
+  if (rc) return rc;
+  if (prec != 8) return fail(-1, "methods 5-9 take FP64 x, A, b");
+  if (method < 5 || method > 9) return fail(-1, "method must be 5-9");
+  if (niter < 1 || !ms6) return fail(-1, "niter >= 1 and ms[6] required");
+  Coef7<float> M32;
+  Coef7<double> M64;
+  BiOpts o;
+  rc = fp64_bicg_opts(c, method, A, &M32, &M64, &o);
+  if (rc) return rc;
+  return bicg_timing_run<double>(c, x, A, b, omega, niter, ms6, o);
+}
+
 int ffvhc_bicgstab(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countPre, int countMax,
                    double eps, int* count, double* residual) {
   int prec = 8;
@@ -617,10 +666,11 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
                     "matrix, 8 as 7 with the block red-black GS preconditioner (FP64 arithmetic), 9 as 8 with the "
                     "symmetric red-black-red sweep");
   if (method == 6) {
-    const Coef7<double> M64 = coef<double>(c, A);
+    Coef7<float> M32;
+    Coef7<double> M64;
     BiOpts o;
-    o.gs = BiOpts::GS64;
-    o.M64 = &M64;
+    rc = fp64_bicg_opts(c, method, A, &M32, &M64, &o);
+    if (rc) return rc;
     rc = bicgstab_core<double>(c, x, M64, (const double*)c->fields[b].d[0], bcg, omega, countMax, eps, c->dst,
                                c->hst, nullptr, nullptr, o);
     if (rc) return rc;
@@ -631,27 +681,12 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
       if (outer) outer[i] = 0;
     return report(c, c->hst, count, residual);
   }
-  if (method == 5 || method == 7 || method == 8 || method == 9) {  // FP64 recurrence, dots and residual; FP32 matrix
-    int fa[7];
-    for (int q = 0; q < 7; q++) {
-      rc = scratch_field(c, 4, R_A + q, &fa[q]);
-      if (rc) return rc;
-    }
-    for (int q = 0; q < 7; q++)
-      launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
-                     (const double*)c->fields[A[q]].d[0]);
-    const Coef7<float> M = coef<float>(c, fa);
+  if (method == 5 || method >= 7) {  // FP64 recurrence, dots and residual; FP32 matrix
+    Coef7<float> M32;
+    Coef7<double> K64;
     BiOpts o;
-    if (method == 5) {
-      o.gs = BiOpts::GS32;
-      o.M32 = &M;
-    } else if (method == 7) {
-      o.A32 = &M;
-    } else {  // 8 / 9: FP32 matrix storage everywhere, (symmetric) GS preconditioner in FP64 arithmetic
-      o.A32 = &M;
-      o.gs = method == 9 ? BiOpts::SGS32_ARITH64 : BiOpts::GS32_ARITH64;
-      o.M32 = &M;
-    }
+    rc = fp64_bicg_opts(c, method, A, &M32, &K64, &o);
+    if (rc) return rc;
     // The Krylov recurrence converges for the FP32-stored matrix.  Its answer is accepted only on the
     // FP64 residual ||b - A x|| of the FP64 matrix; a case above eps (a matrix whose coefficients are
     // not all FP32-representable) restarts from that residual, so the reported residual is always
@@ -659,7 +694,6 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
     int fe;
     rc = scratch_field(c, 4, R_E, &fe);
     if (rc) return rc;
-    const Coef7<double> K64 = coef<double>(c, A);
     const double* b64 = (const double*)c->fields[b].d[0];
     const std::vector<int> ids5 = selected_cases(c);
     std::vector<int> outers(c->cases.size(), -1);

@@ -96,6 +96,7 @@ _sig = {
     "ffvhc_cg": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_int, c_double, _vp, _vp]),
     "ffvhc_bicgstab": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_int, c_double, _vp, _vp]),
     "ffvhc_bicgstab_timing": (c_int, [_vp, c_int, _vp, c_int, c_double, c_int, _vp]),
+    "ffvhc_bicgstab_timing_mixed": (c_int, [_vp, c_int, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_cg_timing": (c_int, [_vp, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_jacobi": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
     "ffvhc_step": (c_int, [_vp, _P(StepFields), _P(StepParams), _vp, _vp]),
@@ -375,11 +376,16 @@ class Context:
             check(rc, "ffvhc_bicgstab")
         return cnt, res, rc
 
-    def bicgstab_timing(self, x, A, b, omega, niter):
-        """Device ms of the five BiCGSTAB kernels and of the finalisations, summed over niter."""
+    def bicgstab_timing(self, x, A, b, omega, niter, method=None):
+        """Device ms of the five BiCGSTAB kernels and of the finalisations, summed over niter
+        (method 5-9: the kernels of that solve_mixed method)."""
         ms = np.zeros(6)
         a = np.asarray(A, np.int32)
-        check(_L.ffvhc_bicgstab_timing(self.h, x, _ptr(a), b, omega, niter, _ptr(ms)), "ffvhc_bicgstab_timing")
+        if method is None:
+            check(_L.ffvhc_bicgstab_timing(self.h, x, _ptr(a), b, omega, niter, _ptr(ms)), "ffvhc_bicgstab_timing")
+        else:
+            check(_L.ffvhc_bicgstab_timing_mixed(self.h, method, x, _ptr(a), b, omega, niter, _ptr(ms)),
+                  "ffvhc_bicgstab_timing_mixed")
         return ms
 
     def cg_timing(self, x, A, b, omega, niter):

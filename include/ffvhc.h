@@ -169,6 +169,10 @@ int ffvhc_bicgstab(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, dou
  * P (p update + preconditioner), Q (q_ = A p_), S, T (t_ = A s_), XR; ms[5] = the per-case
  * scalar finalisations (timing entry). */
 int ffvhc_bicgstab_timing(ffvhc_ctx* ctx, int x, const int* A, int b, double omega, int niter, double* ms);
+/* The same stage timing for the FP64-recurrence methods 5-9 of ffvhc_solve_mixed (FP64 x, A, b;
+ * omega = GS relaxation factor): ms[0..4] = P, Q, S, T, XR kernels, ms[5] = finalisations. */
+int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* ctx, int method, int x, const int* A, int b, double omega, int niter,
+                                double* ms);
 /* niter PCG iterations (no convergence test) with HIP events between the launches; ms[0..2] =
  * total device time of the p-update, matvec and x/r-update kernels, ms[3] = the per-case scalar
  * finalisations (timing entry). */
