@@ -305,6 +305,11 @@ def main():
                        for name, t in zip(("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"), ms)}
         log("method %d kernels: %s" % (sched[0], json.dumps(kt["mixed"])))
 
+    def solve64():
+        ctx.fill(f64["x"], 0.0)
+        cnt, res, rc = ctx.bicgstab(f64["x"], f64["A"], f64["b"], P, 1.0, 1, args.count_max, args.eps, ncases=1)
+        return int(cnt[0]), float(res[0]), 1
+
     def solve_mixed():
         ctx.fill(f64["x"], 0.0)
         cnt, res, outer, rc = ctx.solve_mixed(sched[0], f64["x"], f64["A"], f64["b"], P, sched[3], sched[2],

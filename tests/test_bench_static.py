@@ -1,0 +1,52 @@
+"""CPU checks of bench.py that need no GPU: every name its functions read resolves, and the
+per-kernel byte model and rocprofv3 names of the mixed methods are consistent."""
+import builtins
+import os
+import symtable
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _unresolved(src, name):
+    top = symtable.symtable(src, name, "exec")
+    defined = set(top.get_identifiers()) | set(dir(builtins))
+    bad = []
+
+    def walk(t, enclosing):
+        local = {s.get_name() for s in t.get_symbols() if s.is_assigned() or s.is_parameter() or s.is_imported()}
+        for s in t.get_symbols():
+            n = s.get_name()
+            if s.is_referenced() and (s.is_global() or s.is_free()) and n not in defined and n not in enclosing:
+                bad.append((t.get_name(), n))
+        for ch in t.get_children():
+            walk(ch, enclosing | local)
+
+    for ch in top.get_children():
+        walk(ch, set())
+    return bad
+
+
+def test_bench_names_resolve():
+    src = open(os.path.join(ROOT, "bench.py")).read()
+    assert _unresolved(src, "bench.py") == []
+
+
+def test_unresolved_name_is_caught():
+    src = "def f():\n    def g():\n        return h()\n    return g\n"
+    assert _unresolved(src, "t.py") == [("g", "h")]
+
+
+def test_mixed_byte_model_and_names():
+    sys.path[:0] = [ROOT, os.path.join(ROOT, "ffv-hc_amd")]
+    import bench
+    for m in (5, 6, 7, 8, 9):
+        by = bench.kernel_bytes_mixed(m)
+        assert set(by) == {"bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"}
+        assert by["bi_q"] == by["bi_t"] and by["bi_p"] - by["bi_s"] == 8  # p in: the only difference
+        for st in by:
+            assert "<" in bench.mixed_prof_name(m, st)
+    # method 8: FP64 vectors, FP32 bands: 7 bands x 4 bytes in the GS and matvec kernels
+    assert bench.kernel_bytes_mixed(8)["bi_q"] == 8 + 3 + 28 + 16
+    assert bench.mixed_prof_name(8, "bi_p") == "bi_precond_gs_kernel<0, float, double, false>"
+    assert bench.mixed_prof_name(7, "bi_t") == "bi_matvec_kernel<double, 1, float>"
