@@ -87,6 +87,8 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
   const T ng = (T)(-st[c].gamma), beta = (T)st[c].beta, na = (T)(-st[c].alpha);
   const bool first = count == 1 || st[c].restart;
   constexpr int KB = Batch<T>::K;
+  // an FP32 matrix copy of an FP64 solve is in the colour-split order
+  constexpr bool RB = std::is_same<TA, float>::value && std::is_same<T, double>::value;
 #pragma unroll
   for (int k0 = 0; k0 < NB; k0 += KB) {
     T rv[KB], pv[KB], qv[KB], ap[KB];
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
     for (int kk = 0; kk < KB; kk++) {
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
       rv[kk] = V.r[g];
-      ap[kk] = (T)A.p[0][g];
+      ap[kk] = (T)A.p[0][RB ? base + rb_index(tx, ty, k0 + kk) : g];
       if (MODE == 1 || !first) qv[kk] = V.q[g];
       if (MODE == 0 && !first) pv[kk] = V.p[g];
     }
@@ -152,9 +154,11 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
   auto zat = [&](int i, int j, int k) -> TZ {
     return (i < 0 || i >= NB || j < 0 || j >= NB || k < 0 || k >= NB) ? (TZ)0 : z[zi(i, j, k)];
   };
-  TZ vred[NB / 2];  // v of the red cells (the black cells re-read theirs from p / s)
+  // an FP32 matrix is in the colour-split order: each half sweep reads only its colour's lines
+  constexpr bool RB = std::is_same<TM, float>::value;
+  auto mi = [&](int k) { return base + (RB ? rb_index(tx, ty, k) : tx + NB * (ty + NB * k)); };
+  TZ vred[NB / 2], vblk[NB / 2];  // v of the red and of the black cells
   TM apr[NB / 2];
-  const double* vsrc = MODE == 0 ? V.p : V.s;
 #pragma unroll
   for (int k0 = 0; k0 < NB; k0 += 4) {
     double rv[4], pv[4], qv[4];
@@ -165,7 +169,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
       rv[kk] = V.r[g];
       if (MODE == 1 || !first) qv[kk] = V.q[g];
       if (MODE == 0 && !first) pv[kk] = V.p[g];
-      if ((k & 1) == par) apr[k >> 1] = M.p[0][g];
+      if ((k & 1) == par) apr[k >> 1] = M.p[0][mi(k)];
     }
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
@@ -180,6 +184,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
         V.s[g] = v;
       }
       if ((k & 1) == par) vred[k >> 1] = (TZ)v;
+      else vblk[k >> 1] = (TZ)v;
     }
   }
 #pragma unroll
@@ -187,7 +192,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
     const int k = 2 * m + par;
     z[zi(tx, ty, k)] = omega * (vred[m] / (TZ)apr[m]);
   }
-  __syncthreads();  // also orders this block's p / s stores before the black cells re-read them
+  __syncthreads();
   // black (then, for the symmetric sweep, red again) from the updated neighbours
   auto half_sweep = [&](int colour_off) {
     const bool red = colour_off == par;
@@ -198,10 +203,10 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
 #pragma unroll
       for (int mm = 0; mm < 2; mm++) {
         const int k = 2 * (m0 + mm) + colour_off;
-        const size_t g = base + tx + NB * (ty + NB * k);
+        const size_t g = mi(k);
 #pragma unroll
         for (int q = 0; q < 7; q++) a[mm][q] = M.p[q][g];
-        vv[mm] = red ? vred[m0 + mm] : (TZ)vsrc[g];
+        vv[mm] = red ? vred[m0 + mm] : vblk[m0 + mm];
       }
 #pragma unroll
       for (int mm = 0; mm < 2; mm++) {
@@ -242,6 +247,7 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
   const T* __restrict__ w = MODE == 0 ? V.r0 : V.s;
   T* __restrict__ y = MODE == 0 ? V.q : V.t;
   constexpr int KB = Batch<T>::K;
+  constexpr bool RB = std::is_same<TA, float>::value && std::is_same<T, double>::value;  // colour-split copy
   double d0 = 0.0, d1 = 0.0;
 #pragma unroll
   for (int k0 = 0; k0 < NB; k0 += KB) {
@@ -249,8 +255,9 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      const size_t ga = RB ? base + rb_index(tx, ty, k0 + kk) : g;
 #pragma unroll
-      for (int q = 0; q < 7; q++) a[kk][q] = (T)A.p[q][g];
+      for (int q = 0; q < 7; q++) a[kk][q] = (T)A.p[q][ga];
       wv[kk] = w[g];
     }
     if (k0 == 0) {

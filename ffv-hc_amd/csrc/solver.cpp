@@ -352,8 +352,8 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Co
     if (rc) return rc;
   }
   for (int q = 0; q < 7; q++)
-    launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
-                   (const double*)c->fields[A[q]].d[0]);
+    launch_convert_rb(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
+                      (const double*)c->fields[A[q]].d[0]);
   *M32 = coef<float>(c, fa);
   if (method == 5) {
     o->gs = BiOpts::GS32;
