@@ -65,8 +65,10 @@ void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const Dev
 // Precision options of an FP64 BiCGSTAB: A32 = FP32 copy of the matrix for the matvecs and the
 // Jacobi preconditioner (widened to FP64 on load); gs = block red-black GS preconditioner with
 // an FP32 matrix and FP32 arithmetic, an FP32 matrix and FP64 arithmetic, all FP64, or the
-// symmetric (red, black, red) sweep with an FP32 matrix and FP64 arithmetic.  The FP32 copies
-// are in the colour-split cell order (launch_convert_rb).
+// symmetric (red, black, red) sweep with an FP32 matrix and FP64 arithmetic.  A32 is in the
+// natural cell order; M32 in the colour-split order (launch_convert_rb), so a half sweep reads
+// only its colour's lines (the matvec keeps the natural order: there a wavefront's lanes alternate
+// colours, and the split order would send neighbouring lanes to two lines).
 struct BiOpts {
   enum Gs { NONE = 0, GS32, GS32_ARITH64, GS64, SGS32_ARITH64 };
   const Coef7<float>* A32 = nullptr;
@@ -94,8 +96,8 @@ void launch_defect(hipStream_t sm, const int* active, int nact, const DevBlock* 
 void launch_add_correction(hipStream_t sm, const int* active, int nact, const DevBlock* blk, SolveState* st,
                            const SolveState* inner, double* x, const float* e);
 void launch_convert(hipStream_t sm, const int* active, int nact, float* y, const double* x);
-// FP32 copy in the colour-split cell order (rb_index) that the FP64 BiCGSTAB's FP32 matrix
-// copies (BiOpts::A32 / M32) are read in
+// FP32 copy in the colour-split cell order (rb_index) that the GS preconditioner's FP32 matrix
+// (BiOpts::M32) is read in
 void launch_convert_rb(hipStream_t sm, const int* active, int nact, float* y, const double* x);
 
 void launch_predict(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,

@@ -87,8 +87,6 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
   const T ng = (T)(-st[c].gamma), beta = (T)st[c].beta, na = (T)(-st[c].alpha);
   const bool first = count == 1 || st[c].restart;
   constexpr int KB = Batch<T>::K;
-  // an FP32 matrix copy of an FP64 solve is in the colour-split order
-  constexpr bool RB = std::is_same<TA, float>::value && std::is_same<T, double>::value;
 #pragma unroll
   for (int k0 = 0; k0 < NB; k0 += KB) {
     T rv[KB], pv[KB], qv[KB], ap[KB];
@@ -96,7 +94,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
     for (int kk = 0; kk < KB; kk++) {
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
       rv[kk] = V.r[g];
-      ap[kk] = (T)A.p[0][RB ? base + rb_index(tx, ty, k0 + kk) : g];
+      ap[kk] = (T)A.p[0][g];
       if (MODE == 1 || !first) qv[kk] = V.q[g];
       if (MODE == 0 && !first) pv[kk] = V.p[g];
     }
@@ -247,7 +245,6 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
   const T* __restrict__ w = MODE == 0 ? V.r0 : V.s;
   T* __restrict__ y = MODE == 0 ? V.q : V.t;
   constexpr int KB = Batch<T>::K;
-  constexpr bool RB = std::is_same<TA, float>::value && std::is_same<T, double>::value;  // colour-split copy
   double d0 = 0.0, d1 = 0.0;
 #pragma unroll
   for (int k0 = 0; k0 < NB; k0 += KB) {
@@ -255,9 +252,8 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
-      const size_t ga = RB ? base + rb_index(tx, ty, k0 + kk) : g;
 #pragma unroll
-      for (int q = 0; q < 7; q++) a[kk][q] = (T)A.p[q][ga];
+      for (int q = 0; q < 7; q++) a[kk][q] = (T)A.p[q][g];
       wv[kk] = w[g];
     }
     if (k0 == 0) {

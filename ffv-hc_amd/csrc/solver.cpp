@@ -17,7 +17,7 @@ constexpr int kBatchMin = 8;
 constexpr int kBatchMax = 256;
 
 // scratch roles
-enum { R_R = 1, R_P = 2, R_Q = 3, R_E = 4, R_RHS = 5, R_A = 10, R_BI = 20 };
+enum { R_R = 1, R_P = 2, R_Q = 3, R_E = 4, R_RHS = 5, R_A = 10, R_BI = 20, R_ARB = 30 };
 
 template <class T>
 Coef7<T> coef(ffvhc_ctx* c, const int* A) {
@@ -336,9 +336,28 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
   return 0;
 }
 
-// Storage options of the FP64-recurrence BiCGSTAB methods 5-9: an FP32 copy of the FP64 matrix A
-// (scratch fields, converted here) in *M32 and/or the FP64 matrix in *M64, referenced by *o.
-int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Coef7<double>* M64, BiOpts* o) {
+// FP32 copy of the FP64 matrix A in scratch fields role0 .. role0 + 6, natural or colour-split order
+int fp32_matrix(ffvhc_ctx* c, const int* A, int role0, bool colour_split, Coef7<float>* M) {
+  int fa[7];
+  for (int q = 0; q < 7; q++) {
+    int rc = scratch_field(c, 4, role0 + q, &fa[q]);
+    if (rc) return rc;
+  }
+  for (int q = 0; q < 7; q++) {
+    float* y = (float*)c->fields[fa[q]].d[0];
+    const double* x = (const double*)c->fields[A[q]].d[0];
+    if (colour_split) launch_convert_rb(c->sm, c->dactive, c->nactive, y, x);
+    else launch_convert(c->sm, c->dactive, c->nactive, y, x);
+  }
+  *M = coef<float>(c, fa);
+  return 0;
+}
+
+// Storage options of the FP64-recurrence BiCGSTAB methods 5-9: FP32 copies of the FP64 matrix A
+// (*A32 natural order for the matvecs / Jacobi, *G32 colour-split for the GS preconditioner) and
+// the FP64 matrix in *M64, referenced by *o.
+int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* A32, Coef7<float>* G32,
+                   Coef7<double>* M64, BiOpts* o) {
   *o = BiOpts();
   *M64 = coef<double>(c, A);
   if (method == 6) {
@@ -346,25 +365,18 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Co
     o->M64 = M64;
     return 0;
   }
-  int fa[7];
-  for (int q = 0; q < 7; q++) {
-    int rc = scratch_field(c, 4, R_A + q, &fa[q]);
+  int rc = 0;
+  if (method >= 7) {
+    rc = fp32_matrix(c, A, R_A, false, A32);
     if (rc) return rc;
+    o->A32 = A32;
   }
-  for (int q = 0; q < 7; q++)
-    launch_convert_rb(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
-                      (const double*)c->fields[A[q]].d[0]);
-  *M32 = coef<float>(c, fa);
-  if (method == 5) {
-    o->gs = BiOpts::GS32;
-    o->M32 = M32;
-  } else if (method == 7) {
-    o->A32 = M32;
-  } else {  // 8 / 9: FP32 matrix storage everywhere, (symmetric) GS preconditioner in FP64 arithmetic
-    o->A32 = M32;
-    o->gs = method == 9 ? BiOpts::SGS32_ARITH64 : BiOpts::GS32_ARITH64;
-    o->M32 = M32;
-  }
+  if (method == 7) return 0;
+  rc = fp32_matrix(c, A, R_ARB, true, G32);
+  if (rc) return rc;
+  o->M32 = G32;
+  // 5: FP32 GS arithmetic; 8 / 9: FP64 arithmetic, 9 with the symmetric sweep
+  o->gs = method == 5 ? BiOpts::GS32 : method == 9 ? BiOpts::SGS32_ARITH64 : BiOpts::GS32_ARITH64;
   return 0;
 }
 
@@ -618,10 +630,10 @@ int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* c, int method, int x, const int* A, i
   if (prec != 8) return fail(-1, "methods 5-9 take FP64 x, A, b");
   if (method < 5 || method > 9) return fail(-1, "method must be 5-9");
   if (niter < 1 || !ms6) return fail(-1, "niter >= 1 and ms[6] required");
-  Coef7<float> M32;
+  Coef7<float> A32, G32;
   Coef7<double> M64;
   BiOpts o;
-  rc = fp64_bicg_opts(c, method, A, &M32, &M64, &o);
+  rc = fp64_bicg_opts(c, method, A, &A32, &G32, &M64, &o);
   if (rc) return rc;
   return bicg_timing_run<double>(c, x, A, b, omega, niter, ms6, o);
 }
@@ -666,10 +678,10 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
                     "matrix, 8 as 7 with the block red-black GS preconditioner (FP64 arithmetic), 9 as 8 with the "
                     "symmetric red-black-red sweep");
   if (method == 6) {
-    Coef7<float> M32;
+    Coef7<float> A32, G32;
     Coef7<double> M64;
     BiOpts o;
-    rc = fp64_bicg_opts(c, method, A, &M32, &M64, &o);
+    rc = fp64_bicg_opts(c, method, A, &A32, &G32, &M64, &o);
     if (rc) return rc;
     rc = bicgstab_core<double>(c, x, M64, (const double*)c->fields[b].d[0], bcg, omega, countMax, eps, c->dst,
                                c->hst, nullptr, nullptr, o);
@@ -682,10 +694,10 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
     return report(c, c->hst, count, residual);
   }
   if (method == 5 || method >= 7) {  // FP64 recurrence, dots and residual; FP32 matrix
-    Coef7<float> M32;
+    Coef7<float> A32, G32;
     Coef7<double> K64;
     BiOpts o;
-    rc = fp64_bicg_opts(c, method, A, &M32, &K64, &o);
+    rc = fp64_bicg_opts(c, method, A, &A32, &G32, &K64, &o);
     if (rc) return rc;
     // The Krylov recurrence converges for the FP32-stored matrix.  Its answer is accepted only on the
     // FP64 residual ||b - A x|| of the FP64 matrix; a case above eps (a matrix whose coefficients are
