@@ -174,28 +174,40 @@ MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64
                  "FP64 BiCGSTAB with an FP64 block red-black GS preconditioner",
                  "FP64 Jacobi-BiCGSTAB on an FP32-stored matrix",
                  "FP64 BiCGSTAB on an FP32-stored matrix, block red-black GS preconditioner",
-                 "FP64 BiCGSTAB on an FP32-stored matrix, symmetric block red-black GS preconditioner"]
+                 "FP64 BiCGSTAB on an FP32-stored matrix, symmetric block red-black GS preconditioner",
+                 "FP64 BiCGSTAB on an FP32-stored matrix, block red-black GS preconditioner, hashed shadow "
+                 "residual"]
 
 # schedule of `bench.py --profile` (the rocprofv3 runs): FP32-stored matrix + block GS, omega 1.2
 PROFILE_SCHEDULE = (8, 0.5, 20000, 1.2)
 
 # kernel names of the timed stages as rocprofv3 reports them
-PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0>", "bi_q": "bi_matvec_kernel<%s, 0>", "bi_s": "bi_precond_kernel<%s, 1>",
-              "bi_t": "bi_matvec_kernel<%s, 1>", "bi_xr": "bi_xr_kernel<%s>", "rbgs": "rbgs_iter_kernel<%s, 0>",
+PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0, %s>", "bi_q": "bi_matvec_kernel<%s, 0, %s, false>",
+              "bi_s": "bi_precond_kernel<%s, 1, %s>", "bi_t": "bi_matvec_kernel<%s, 1, %s, false>",
+              "bi_xr": "bi_xr_kernel<%s, false>", "rbgs": "rbgs_iter_kernel<%s, 0>",
               "cg_p": "cg_p_kernel<%s>", "cg_q": "cg_q_kernel<%s>", "cg_xr": "cg_xr_kernel<%s>"}
+
+
+def prof_name(stage, w):
+    """rocprofv3 name of a stage's kernel in the plain FP64 (w = 8) / FP32 (w = 4) solvers."""
+    t = "float" if w == 4 else "double"
+    f = PROF_NAMES[stage]
+    return f % ((t,) * f.count("%s"))
 
 
 def mixed_prof_name(method, stage):
     """rocprofv3 name of a stage's kernel in the FP64-recurrence methods 5-9."""
     mode = {"bi_p": 0, "bi_q": 0, "bi_s": 1, "bi_t": 1}.get(stage)
+    hr = "true" if method == 10 else "false"
     if stage == "bi_xr":
-        return "bi_xr_kernel<double>"
+        return "bi_xr_kernel<double, %s>" % hr
     if stage in ("bi_q", "bi_t"):
-        return "bi_matvec_kernel<double, %d, float>" % mode if method >= 7 else "bi_matvec_kernel<double, %d>" % mode
+        return "bi_matvec_kernel<double, %d, %s, %s>" % (mode, "float" if method >= 7 else "double", hr)
     if method == 7:
         return "bi_precond_kernel<double, %d, float>" % mode
     tm, tz, sym = {5: ("float", "float", "false"), 6: ("double", "double", "false"),
-                   8: ("float", "double", "false"), 9: ("float", "double", "true")}[method]
+                   8: ("float", "double", "false"), 9: ("float", "double", "true"),
+                   10: ("float", "double", "false")}[method]
     return "bi_precond_gs_kernel<%d, %s, %s, %s>" % (mode, tm, tz, sym)
 
 
@@ -223,13 +235,14 @@ def kernel_bytes_mixed(method):
     seven bands, the Jacobi one the diagonal only."""
     h = 0.375 * 8                                   # FP64 ghost layer of p_ / s_
     mv = 4 if method >= 7 else 8                    # matvec bands
-    mp = {5: 4, 6: 8, 7: 4, 8: 4, 9: 4}[method]     # preconditioner bands
+    mp = {5: 4, 6: 8, 7: 4, 8: 4, 9: 4, 10: 4}[method]  # preconditioner bands
     nb = 1 if method == 7 else 7
+    r0 = 0 if method == 10 else 8                   # hashed shadow residual: not stored
     return {"bi_p": 24 + nb * mp + 16 + h,          # r, p, q in; p, p_ out; p_ ghosts
-            "bi_q": 8 + h + 7 * mv + 8 + 8,         # p_ + ghosts, bands, r0 in; q_ out
+            "bi_q": 8 + h + 7 * mv + r0 + 8,        # p_ + ghosts, bands, r0 in; q_ out
             "bi_s": 16 + nb * mp + 16 + h,          # r, q_ in; s, s_ out; s_ ghosts
             "bi_t": 8 + h + 7 * mv + 8 + 8,         # s_ + ghosts, bands, s in; t_ out
-            "bi_xr": 64}                            # x, p_, s_, t_, s, r0 in; x, r out
+            "bi_xr": 56 + r0}                       # x, p_, s_, t_, s, r0 in; x, r out
 
 
 def log(msg):
@@ -295,7 +308,7 @@ def main():
         sched, tuned = autotune.tune_mixed(ctx, f64["x"], f64["A"], f64["b"], P, args.eps)
         log("schedule %s; candidates %s" % (sched, [(c, round(t, 4), it, o) for c, t, it, o, r in tuned]))
 
-    if 5 <= sched[0] <= 9:  # stage times of the tuned method's own kernels
+    if 5 <= sched[0] <= 10:  # stage times of the tuned method's own kernels
         ctx.fill(f64["x"], 0.0)
         ctx.bicgstab_timing(f64["x"], f64["A"], f64["b"], sched[3], 5, method=sched[0])
         ms = ctx.bicgstab_timing(f64["x"], f64["A"], f64["b"], sched[3], nit, method=sched[0])[:5] / nit
@@ -359,7 +372,7 @@ def main():
         dkernel, dbytes = dom["kernel"], kernel_bytes_mixed(sched[0])[dname]
     else:
         dw = 4 if dtag == "32" else 8
-        dkernel, dbytes = PROF_NAMES[dname] % ("float" if dw == 4 else "double"), kernel_bytes(dw)[dname]
+        dkernel, dbytes = prof_name(dname, dw), kernel_bytes(dw)[dname]
     out = {
         "metric": "Mcell-steps/s mixed vs FP64 at equal residual; p50/p99 served-case latency",
         "value": round(mix["mcell_steps_s"], 3),

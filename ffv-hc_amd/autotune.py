@@ -25,6 +25,7 @@ DEFAULT_CANDIDATES = (
     (8, 0.5, 20000, 1.0),
     (8, 0.5, 20000, 1.2),
     (9, 0.5, 20000, 1.2),
+    (10, 0.5, 20000, 1.2),
 )
 
 

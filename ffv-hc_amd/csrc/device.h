@@ -106,14 +106,6 @@ struct BiVecs {
 
 __host__ __device__ inline int halo_index(int face, int a, int b) { return face * NF + a + NB * b; }
 
-// Colour-split cell order of a block (the FP32 matrix copies of the FP64 BiCGSTAB): plane k, its
-// red cells ((i + j + k) even) first, then its black ones, each colour as (j, i / 2).  A
-// red-black sweep then reads only the lines of the colour it updates; a wavefront (four rows of
-// a plane) still reads one contiguous 128-byte run per colour.
-__host__ __device__ inline int rb_index(int i, int j, int k) {
-  return (k << 8) | (((i + j + k) & 1) << 7) | (j << 3) | (i >> 1);
-}
-
 // cell (i,j,k) that touches face `face` at face position (a,b), layer `l` (0 = adjacent)
 __host__ __device__ inline void face_cell(int face, int a, int b, int l, int& i, int& j, int& k) {
   const bool plus = face & 1;

@@ -61,20 +61,20 @@ void launch_cg_xr(hipStream_t sm, const int* active, int nact, const DevBlock* b
 template <class T>
 void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                           SolveState* st, Coef7<T> A, const T* b, const T* x, const T* hx, BiVecs<T> V, double* part,
-                          size_t nslots);
+                          size_t nslots, bool hash_r0 = false);
 // Precision options of an FP64 BiCGSTAB: A32 = FP32 copy of the matrix for the matvecs and the
 // Jacobi preconditioner (widened to FP64 on load); gs = block red-black GS preconditioner with
 // an FP32 matrix and FP32 arithmetic, an FP32 matrix and FP64 arithmetic, all FP64, or the
-// symmetric (red, black, red) sweep with an FP32 matrix and FP64 arithmetic.  A32 is in the
-// natural cell order; M32 in the colour-split order (launch_convert_rb), so a half sweep reads
-// only its colour's lines (the matvec keeps the natural order: there a wavefront's lanes alternate
-// colours, and the split order would send neighbouring lanes to two lines).
+// symmetric (red, black, red) sweep with an FP32 matrix and FP64 arithmetic.
 struct BiOpts {
   enum Gs { NONE = 0, GS32, GS32_ARITH64, GS64, SGS32_ARITH64 };
   const Coef7<float>* A32 = nullptr;
   int gs = NONE;
   const Coef7<float>* M32 = nullptr;
   const Coef7<double>* M64 = nullptr;
+  // shadow residual r0 = the seeded hash vector computed in the kernels that read it (nothing
+  // stored or read for r0; the matvec and XR hashed variants exist for the A32 path)
+  bool hash_r0 = false;
 };
 
 template <class T>
@@ -96,9 +96,6 @@ void launch_defect(hipStream_t sm, const int* active, int nact, const DevBlock* 
 void launch_add_correction(hipStream_t sm, const int* active, int nact, const DevBlock* blk, SolveState* st,
                            const SolveState* inner, double* x, const float* e);
 void launch_convert(hipStream_t sm, const int* active, int nact, float* y, const double* x);
-// FP32 copy in the colour-split cell order (rb_index) that the GS preconditioner's FP32 matrix
-// (BiOpts::M32) is read in
-void launch_convert_rb(hipStream_t sm, const int* active, int nact, float* y, const double* x);
 
 void launch_predict(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                     const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc);

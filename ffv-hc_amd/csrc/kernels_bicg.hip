@@ -18,7 +18,24 @@ namespace ffvhc {
 
 constexpr int BI_NT = 256;
 
-template <class T>
+// Seeded shadow residual for the mixed-precision solver: r0 = uniform(-1, 1) from a hash of the
+// block's index within its case and the cell (so a case draws the same r0 whatever slots it
+// holds).  With r0 = r (the reference's choice) rho = r.r0 decays far below FP32 resolution on
+// the coarse/fine octree operator; a random shadow keeps it near |r||r0|/sqrt(N).
+__device__ inline float shadow_value(uint32_t blk, uint32_t cell) {
+  uint32_t h = blk * 0x9E3779B1u ^ (cell + 0x7F4A7C15u) * 0x85EBCA77u;  // integer mix (murmur-style)
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return (float)((double)h * (2.0 / 4294967296.0) - 1.0);
+}
+
+
+// HR: the shadow residual r0 is the seeded hash vector (shadow_value), computed where it is read
+// instead of stored
+template <class T, bool HR = false>
 __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
                                                         const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                         Coef7<T> A, const T* __restrict__ bv, const T* __restrict__ x,
@@ -31,6 +48,7 @@ __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ 
   if (st[c].done) return;
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
+  const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
   constexpr int KB = Batch<T>::K;
   double bb = 0.0, rr1 = 0.0;
 #pragma unroll
@@ -55,9 +73,13 @@ __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ 
                   a[kk][4] * s[li + LP] - a[kk][5] * s[li - LP * LP] - a[kk][6] * s[li + LP * LP] -
                   a[kk][0] * s[li];
       V.r[g] = r;
-      V.r0[g] = r;
       bb += (double)b0[kk] * (double)b0[kk];
-      rr1 += (double)r * (double)r;
+      if (HR) {
+        rr1 += (double)r * (double)shadow_value(local, (uint32_t)(g - base));
+      } else {
+        V.r0[g] = r;
+        rr1 += (double)r * (double)r;
+      }
     }
   }
   double sb = block_sum<BI_NT>(bb, red);
@@ -152,9 +174,6 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
   auto zat = [&](int i, int j, int k) -> TZ {
     return (i < 0 || i >= NB || j < 0 || j >= NB || k < 0 || k >= NB) ? (TZ)0 : z[zi(i, j, k)];
   };
-  // an FP32 matrix is in the colour-split order: each half sweep reads only its colour's lines
-  constexpr bool RB = std::is_same<TM, float>::value;
-  auto mi = [&](int k) { return base + (RB ? rb_index(tx, ty, k) : tx + NB * (ty + NB * k)); };
   TZ vred[NB / 2], vblk[NB / 2];  // v of the red and of the black cells
   TM apr[NB / 2];
 #pragma unroll
@@ -167,7 +186,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
       rv[kk] = V.r[g];
       if (MODE == 1 || !first) qv[kk] = V.q[g];
       if (MODE == 0 && !first) pv[kk] = V.p[g];
-      if ((k & 1) == par) apr[k >> 1] = M.p[0][mi(k)];
+      if ((k & 1) == par) apr[k >> 1] = M.p[0][g];
     }
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
@@ -201,7 +220,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
 #pragma unroll
       for (int mm = 0; mm < 2; mm++) {
         const int k = 2 * (m0 + mm) + colour_off;
-        const size_t g = mi(k);
+        const size_t g = base + tx + NB * (ty + NB * k);
 #pragma unroll
         for (int q = 0; q < 7; q++) a[mm][q] = M.p[q][g];
         vv[mm] = red ? vred[m0 + mm] : vblk[m0 + mm];
@@ -229,7 +248,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
 }
 
 // Q: q_ = A p_, q_.r0 -> alpha (mode 0);  T: t_ = A s_, t_.s and t_.t_ -> gamma (mode 1)
-template <class T, int MODE, class TA = T>
+template <class T, int MODE, class TA = T, bool HR = false>
 __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict__ active,
                                                           const DevBlock* __restrict__ blk,
                                                           const DevCase* __restrict__ cs, SolveState* __restrict__ st,
@@ -244,6 +263,7 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
   const size_t base = (size_t)slot * NC;
   const T* __restrict__ w = MODE == 0 ? V.r0 : V.s;
   T* __restrict__ y = MODE == 0 ? V.q : V.t;
+  const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
   constexpr int KB = Batch<T>::K;
   double d0 = 0.0, d1 = 0.0;
 #pragma unroll
@@ -254,7 +274,7 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
 #pragma unroll
       for (int q = 0; q < 7; q++) a[kk][q] = (T)A.p[q][g];
-      wv[kk] = w[g];
+      wv[kk] = (MODE == 0 && HR) ? (T)shadow_value(local, (uint32_t)(g - base)) : w[g];
     }
     if (k0 == 0) {
       stage_block<T, BI_NT>(s, MODE == 0 ? V.pp : V.ss, MODE == 0 ? V.hpp : V.hss, slot);
@@ -280,7 +300,7 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
   }
 }
 
-template <class T>
+template <class T, bool HR = false>
 __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
                                                       const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                       T* __restrict__ x, BiVecs<T> V, double* __restrict__ part,
@@ -292,6 +312,7 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
   const T alpha = (T)st[c].alpha, gamma = (T)st[c].gamma, ng = (T)(-st[c].gamma);
+  const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
   constexpr int KB = Batch<T>::K;
   double rr = 0.0, rr1 = 0.0;
 #pragma unroll
@@ -305,7 +326,7 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
       xv[kk] = x[g];
       tv[kk] = V.t[g];
       sv[kk] = V.s[g];
-      r0v[kk] = V.r0[g];
+      r0v[kk] = HR ? (T)shadow_value(local, (uint32_t)(g - base)) : V.r0[g];
     }
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
@@ -347,20 +368,6 @@ __global__ __launch_bounds__(BI_NT) void bi_reanchor_kernel(const int* __restric
     part[slot] = s0;
     part[nslots + slot] = s1;
   }
-}
-
-// Seeded shadow residual for the mixed-precision solver: r0 = uniform(-1, 1) from a hash of the
-// block's index within its case and the cell (so a case draws the same r0 whatever slots it
-// holds).  With r0 = r (the reference's choice) rho = r.r0 decays far below FP32 resolution on
-// the coarse/fine octree operator; a random shadow keeps it near |r||r0|/sqrt(N).
-__device__ inline float shadow_value(uint32_t blk, uint32_t cell) {
-  uint32_t h = blk * 0x9E3779B1u ^ (cell + 0x7F4A7C15u) * 0x85EBCA77u;  // integer mix (murmur-style)
-  h ^= h >> 16;
-  h *= 0x7FEB352Du;
-  h ^= h >> 15;
-  h *= 0x846CA68Bu;
-  h ^= h >> 16;
-  return (float)((double)h * (2.0 / 4294967296.0) - 1.0);
 }
 
 template <class T>
@@ -408,9 +415,14 @@ void launch_bicgstab_restart(hipStream_t sm, const int* active, int nact, const 
 template <class T>
 void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                           SolveState* st, Coef7<T> A, const T* b, const T* x, const T* hx, BiVecs<T> V, double* part,
-                          size_t nslots) {
-  if (nact) hipLaunchKernelGGL(bi_init_kernel<T>, dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, b, x, hx, V,
-                               part, nslots);
+                          size_t nslots, bool hash_r0) {
+  if (!nact) return;
+  if (hash_r0)
+    hipLaunchKernelGGL((bi_init_kernel<T, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, b, x, hx, V,
+                       part, nslots);
+  else
+    hipLaunchKernelGGL((bi_init_kernel<T, false>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, b, x, hx,
+                       V, part, nslots);
 }
 
 template <class T>
@@ -444,7 +456,10 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   };
   auto matvec = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
-    if (dbl && o.A32)
+    if (dbl && o.A32 && o.hash_r0)
+      hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk,
+                         cs, st, *o.A32, Vd, part, nslots);
+    else if (dbl && o.A32)
       hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs,
                          st, *o.A32, Vd, part, nslots);
     else
@@ -464,15 +479,19 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_GAMMA, count);
   }
   if (mask & 16) {
-    hipLaunchKernelGGL(bi_xr_kernel<T>, dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part, nslots,
-                       count);
+    if (o.hash_r0)
+      hipLaunchKernelGGL((bi_xr_kernel<T, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
+                         nslots, count);
+    else
+      hipLaunchKernelGGL((bi_xr_kernel<T, false>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
+                         nslots, count);
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_XR, count);
   }
 }
 
 #define INST(T)                                                                                                     \
   template void launch_bicgstab_init<T>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,  \
-                                        Coef7<T>, const T*, const T*, const T*, BiVecs<T>, double*, size_t);         \
+                                        Coef7<T>, const T*, const T*, const T*, BiVecs<T>, double*, size_t, bool);   \
   template void launch_bicgstab_iter<T>(hipStream_t, const int*, int, const int*, int, const DevBlock*,            \
                                         const DevCase*, SolveState*, Coef7<T>, T*, BiVecs<T>, double*, size_t, int,  \
                                         int, const BiOpts&);                                                         \

@@ -327,14 +327,6 @@ __global__ __launch_bounds__(256) void convert_kernel(const int* __restrict__ ac
   for (int q = threadIdx.x; q < NC; q += 256) y[base + q] = (float)x[base + q];
 }
 
-// FP32 copy in the colour-split order of rb_index
-__global__ __launch_bounds__(256) void convert_rb_kernel(const int* __restrict__ active, float* __restrict__ y,
-                                                         const double* __restrict__ x) {
-  const int slot = active[blockIdx.x];
-  const size_t base = (size_t)slot * NC;
-  for (int q = threadIdx.x; q < NC; q += 256) y[base + rb_index(q & 15, (q >> 4) & 15, q >> 8)] = (float)x[base + q];
-}
-
 void launch_defect(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                    SolveState* st, Coef7<double> A, const double* b, const double* x, const double* hx, float* r32,
                    double* part, size_t nslots, int outer) {
@@ -347,9 +339,6 @@ void launch_add_correction(hipStream_t sm, const int* active, int nact, const De
 }
 void launch_convert(hipStream_t sm, const int* active, int nact, float* y, const double* x) {
   if (nact) hipLaunchKernelGGL(convert_kernel, dim3(nact), dim3(256), 0, sm, active, y, x);
-}
-void launch_convert_rb(hipStream_t sm, const int* active, int nact, float* y, const double* x) {
-  if (nact) hipLaunchKernelGGL(convert_rb_kernel, dim3(nact), dim3(256), 0, sm, active, y, x);
 }
 
 }  // namespace ffvhc

@@ -17,7 +17,7 @@ constexpr int kBatchMin = 8;
 constexpr int kBatchMax = 256;
 
 // scratch roles
-enum { R_R = 1, R_P = 2, R_Q = 3, R_E = 4, R_RHS = 5, R_A = 10, R_BI = 20, R_ARB = 30 };
+enum { R_R = 1, R_P = 2, R_Q = 3, R_E = 4, R_RHS = 5, R_A = 10, R_BI = 20 };
 
 template <class T>
 Coef7<T> coef(ffvhc_ctx* c, const int* A) {
@@ -158,7 +158,7 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
   const FieldRec X = c->fields[xf];  // copy taken after the scratch allocations
   launch_solve_init(c->sm, st, (int)c->cases.size(), omega, eps, countMax, outer);
   launch_bicgstab_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, (const T*)X.d[0],
-                          (const T*)X.h[0], V, c->dpart, (size_t)c->cap);
+                          (const T*)X.h[0], V, c->dpart, (size_t)c->cap, opts.hash_r0);
   fin(c, st, FIN_BI_INIT, 0);
   int n = 1, batch = kBatchMin;
   bool done = false;
@@ -303,7 +303,7 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
   launch_solve_init(c->sm, c->dst, (int)c->cases.size(), omega, -1.0, niter);
   launch_bicgstab_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K,
                           (const T*)c->fields[b].d[0], (const T*)X.d[0], (const T*)X.h[0], V, c->dpart,
-                          (size_t)c->cap);
+                          (size_t)c->cap, o.hash_r0);
   fin(c, c->dst, FIN_BI_INIT, 0);
   // stages in launch order: P, Q, fin(alpha), S, T, fin(gamma), XR, fin(convergence, beta)
   static const int bit_of[8] = {1, 2, 0, 4, 8, 0, 16, 0};
@@ -336,28 +336,9 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
   return 0;
 }
 
-// FP32 copy of the FP64 matrix A in scratch fields role0 .. role0 + 6, natural or colour-split order
-int fp32_matrix(ffvhc_ctx* c, const int* A, int role0, bool colour_split, Coef7<float>* M) {
-  int fa[7];
-  for (int q = 0; q < 7; q++) {
-    int rc = scratch_field(c, 4, role0 + q, &fa[q]);
-    if (rc) return rc;
-  }
-  for (int q = 0; q < 7; q++) {
-    float* y = (float*)c->fields[fa[q]].d[0];
-    const double* x = (const double*)c->fields[A[q]].d[0];
-    if (colour_split) launch_convert_rb(c->sm, c->dactive, c->nactive, y, x);
-    else launch_convert(c->sm, c->dactive, c->nactive, y, x);
-  }
-  *M = coef<float>(c, fa);
-  return 0;
-}
-
-// Storage options of the FP64-recurrence BiCGSTAB methods 5-9: FP32 copies of the FP64 matrix A
-// (*A32 natural order for the matvecs / Jacobi, *G32 colour-split for the GS preconditioner) and
-// the FP64 matrix in *M64, referenced by *o.
-int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* A32, Coef7<float>* G32,
-                   Coef7<double>* M64, BiOpts* o) {
+// Storage options of the FP64-recurrence BiCGSTAB methods 5-9: an FP32 copy of the FP64 matrix A
+// (scratch fields, converted here) in *M32 and/or the FP64 matrix in *M64, referenced by *o.
+int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Coef7<double>* M64, BiOpts* o) {
   *o = BiOpts();
   *M64 = coef<double>(c, A);
   if (method == 6) {
@@ -365,18 +346,26 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* A32, Co
     o->M64 = M64;
     return 0;
   }
-  int rc = 0;
-  if (method >= 7) {
-    rc = fp32_matrix(c, A, R_A, false, A32);
+  int fa[7];
+  for (int q = 0; q < 7; q++) {
+    int rc = scratch_field(c, 4, R_A + q, &fa[q]);
     if (rc) return rc;
-    o->A32 = A32;
   }
-  if (method == 7) return 0;
-  rc = fp32_matrix(c, A, R_ARB, true, G32);
-  if (rc) return rc;
-  o->M32 = G32;
-  // 5: FP32 GS arithmetic; 8 / 9: FP64 arithmetic, 9 with the symmetric sweep
-  o->gs = method == 5 ? BiOpts::GS32 : method == 9 ? BiOpts::SGS32_ARITH64 : BiOpts::GS32_ARITH64;
+  for (int q = 0; q < 7; q++)
+    launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
+                   (const double*)c->fields[A[q]].d[0]);
+  *M32 = coef<float>(c, fa);
+  if (method == 5) {
+    o->gs = BiOpts::GS32;
+    o->M32 = M32;
+  } else if (method == 7) {
+    o->A32 = M32;
+  } else {  // 8-10: FP32 matrix storage everywhere, (symmetric) GS preconditioner in FP64 arithmetic
+    o->A32 = M32;
+    o->gs = method == 9 ? BiOpts::SGS32_ARITH64 : BiOpts::GS32_ARITH64;
+    o->M32 = M32;
+    o->hash_r0 = method == 10;
+  }
   return 0;
 }
 
@@ -628,12 +617,12 @@ int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* c, int method, int x, const int* A, i
   int rc = prologue(c, x, A, b, BC_NULL, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "methods 5-9 take FP64 x, A, b");
-  if (method < 5 || method > 9) return fail(-1, "method must be 5-9");
+  if (method < 5 || method > 10) return fail(-1, "method must be 5-10");
   if (niter < 1 || !ms6) return fail(-1, "niter >= 1 and ms[6] required");
-  Coef7<float> A32, G32;
+  Coef7<float> M32;
   Coef7<double> M64;
   BiOpts o;
-  rc = fp64_bicg_opts(c, method, A, &A32, &G32, &M64, &o);
+  rc = fp64_bicg_opts(c, method, A, &M32, &M64, &o);
   if (rc) return rc;
   return bicg_timing_run<double>(c, x, A, b, omega, niter, ms6, o);
 }
@@ -671,17 +660,17 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
-  if (method < 0 || method > 9)
+  if (method < 0 || method > 10)
     return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB, 3 Jacobi-BiCGSTAB with reliable updates, "
                     "4 the same with a seeded shadow residual, 5 FP64 BiCGSTAB with an FP32 red-black GS "
                     "preconditioner, 6 the same all in FP64 (reference point), 7 FP64 Jacobi-BiCGSTAB with an FP32 "
                     "matrix, 8 as 7 with the block red-black GS preconditioner (FP64 arithmetic), 9 as 8 with the "
-                    "symmetric red-black-red sweep");
+                    "symmetric red-black-red sweep, 10 as 8 with the seeded hash vector as shadow residual");
   if (method == 6) {
-    Coef7<float> A32, G32;
+    Coef7<float> M32;
     Coef7<double> M64;
     BiOpts o;
-    rc = fp64_bicg_opts(c, method, A, &A32, &G32, &M64, &o);
+    rc = fp64_bicg_opts(c, method, A, &M32, &M64, &o);
     if (rc) return rc;
     rc = bicgstab_core<double>(c, x, M64, (const double*)c->fields[b].d[0], bcg, omega, countMax, eps, c->dst,
                                c->hst, nullptr, nullptr, o);
@@ -694,10 +683,10 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
     return report(c, c->hst, count, residual);
   }
   if (method == 5 || method >= 7) {  // FP64 recurrence, dots and residual; FP32 matrix
-    Coef7<float> A32, G32;
+    Coef7<float> M32;
     Coef7<double> K64;
     BiOpts o;
-    rc = fp64_bicg_opts(c, method, A, &A32, &G32, &K64, &o);
+    rc = fp64_bicg_opts(c, method, A, &M32, &K64, &o);
     if (rc) return rc;
     // The Krylov recurrence converges for the FP32-stored matrix.  Its answer is accepted only on the
     // FP64 residual ||b - A x|| of the FP64 matrix; a case above eps (a matrix whose coefficients are

@@ -169,7 +169,7 @@ int ffvhc_bicgstab(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, dou
  * P (p update + preconditioner), Q (q_ = A p_), S, T (t_ = A s_), XR; ms[5] = the per-case
  * scalar finalisations (timing entry). */
 int ffvhc_bicgstab_timing(ffvhc_ctx* ctx, int x, const int* A, int b, double omega, int niter, double* ms);
-/* The same stage timing for the FP64-recurrence methods 5-9 of ffvhc_solve_mixed (FP64 x, A, b;
+/* The same stage timing for the FP64-recurrence methods 5-10 of ffvhc_solve_mixed (FP64 x, A, b;
  * omega = GS relaxation factor): ms[0..4] = P, Q, S, T, XR kernels, ms[5] = finalisations. */
 int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* ctx, int method, int x, const int* A, int b, double omega, int niter,
                                 double* ms);
@@ -193,9 +193,10 @@ int ffvhc_residual(ffvhc_ctx* ctx, int x, const int* A, int b, double* residual)
  * Methods 5-9 keep the BiCGSTAB recurrence, dots and state in FP64 and narrow what is stored:
  * 5 FP32 block red-black GS preconditioner (FP32 arithmetic), 6 the same GS all in FP64,
  * 7 FP32 copy of the matrix for the matvecs and the Jacobi preconditioner, 8 as 7 with the block
- * GS preconditioner in FP64 arithmetic, 9 as 8 with the symmetric (red, black, red) sweep.  For 5
- * and 7-9 the answer is accepted on the FP64 residual of the FP64 matrix; a case above eps
- * restarts from it (inner_eps unused; omega is the GS relaxation factor).
+ * GS preconditioner in FP64 arithmetic, 9 as 8 with the symmetric (red, black, red) sweep, 10 as 8
+ * with a seeded hash vector as the shadow residual r0 (computed where it is read, never stored).
+ * For 5 and 7-10 the answer is accepted on the FP64 residual of the FP64 matrix; a case above
+ * eps restarts from it (inner_eps unused; omega is the GS relaxation factor).
  * inner_eps: relative reduction per correction; countMax caps each inner solve (a capped inner
  * solve still yields a correction).  count[c] = inner iterations in total, outer[c] = corrections
  * taken, residual[c] = FP64 ||b - Ax|| / ||b|| after the last correction. */

@@ -40,13 +40,17 @@ def test_unresolved_name_is_caught():
 def test_mixed_byte_model_and_names():
     sys.path[:0] = [ROOT, os.path.join(ROOT, "ffv-hc_amd")]
     import bench
-    for m in (5, 6, 7, 8, 9):
+    for m in (5, 6, 7, 8, 9, 10):
         by = bench.kernel_bytes_mixed(m)
         assert set(by) == {"bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"}
-        assert by["bi_q"] == by["bi_t"] and by["bi_p"] - by["bi_s"] == 8  # p in: the only difference
+        # q_ and t_ differ by r0 (hashed in 10); p_ and s_ by the p read
+        assert by["bi_q"] == by["bi_t"] - (8 if m == 10 else 0) and by["bi_p"] - by["bi_s"] == 8
         for st in by:
             assert "<" in bench.mixed_prof_name(m, st)
     # method 8: FP64 vectors, FP32 bands: 7 bands x 4 bytes in the GS and matvec kernels
     assert bench.kernel_bytes_mixed(8)["bi_q"] == 8 + 3 + 28 + 16
     assert bench.mixed_prof_name(8, "bi_p") == "bi_precond_gs_kernel<0, float, double, false>"
-    assert bench.mixed_prof_name(7, "bi_t") == "bi_matvec_kernel<double, 1, float>"
+    assert bench.mixed_prof_name(7, "bi_t") == "bi_matvec_kernel<double, 1, float, false>"
+    assert bench.prof_name("bi_q", 4) == "bi_matvec_kernel<float, 0, float, false>"
+    assert bench.mixed_prof_name(10, "bi_xr") == "bi_xr_kernel<double, true>"
+    assert bench.kernel_bytes_mixed(10)["bi_xr"] == 56
