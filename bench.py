@@ -176,7 +176,8 @@ MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64
                  "FP64 BiCGSTAB on an FP32-stored matrix, block red-black GS preconditioner",
                  "FP64 BiCGSTAB on an FP32-stored matrix, symmetric block red-black GS preconditioner",
                  "FP64 BiCGSTAB on an FP32-stored matrix, block red-black GS preconditioner, hashed shadow "
-                 "residual"]
+                 "residual"] + ["FP64 BiCGSTAB on an FP32-stored matrix, %d block red-black SOR sweep%s on a compact "
+                                "FP32/FP16 preconditioner matrix" % (n, "" if n == 1 else "s") for n in (1, 2, 3)]
 
 # schedule of `bench.py --profile` (the rocprofv3 runs): FP32-stored matrix + block GS, omega 1.2
 PROFILE_SCHEDULE = (8, 0.5, 20000, 1.2)
@@ -205,6 +206,8 @@ def mixed_prof_name(method, stage):
         return "bi_matvec_kernel<double, %d, %s, %s>" % (mode, "float" if method >= 7 else "double", hr)
     if method == 7:
         return "bi_precond_kernel<double, %d, float>" % mode
+    if method >= 11:
+        return "bi_precond_gsk_kernel<%d, %d>" % (mode, method - 10)
     tm, tz, sym = {5: ("float", "float", "false"), 6: ("double", "double", "false"),
                    8: ("float", "double", "false"), 9: ("float", "double", "true"),
                    10: ("float", "double", "false")}[method]
@@ -235,8 +238,8 @@ def kernel_bytes_mixed(method):
     seven bands, the Jacobi one the diagonal only."""
     h = 0.375 * 8                                   # FP64 ghost layer of p_ / s_
     mv = 4 if method >= 7 else 8                    # matvec bands
-    mp = {5: 4, 6: 8, 7: 4, 8: 4, 9: 4, 10: 4}[method]  # preconditioner bands
-    nb = 1 if method == 7 else 7
+    mp = {5: 4, 6: 8, 7: 4, 8: 4, 9: 4, 10: 4}.get(method, 16 / 7)  # preconditioner bands
+    nb = 1 if method == 7 else 7                    # (11-13: one 16-byte record per cell)
     r0 = 0 if method == 10 else 8                   # hashed shadow residual: not stored
     return {"bi_p": 24 + nb * mp + 16 + h,          # r, p, q in; p, p_ out; p_ ghosts
             "bi_q": 8 + h + 7 * mv + r0 + 8,        # p_ + ghosts, bands, r0 in; q_ out
@@ -308,7 +311,7 @@ def main():
         sched, tuned = autotune.tune_mixed(ctx, f64["x"], f64["A"], f64["b"], P, args.eps)
         log("schedule %s; candidates %s" % (sched, [(c, round(t, 4), it, o) for c, t, it, o, r in tuned]))
 
-    if 5 <= sched[0] <= 10:  # stage times of the tuned method's own kernels
+    if sched[0] >= 5:  # stage times of the tuned method's own kernels
         ctx.fill(f64["x"], 0.0)
         ctx.bicgstab_timing(f64["x"], f64["A"], f64["b"], sched[3], 5, method=sched[0])
         ms = ctx.bicgstab_timing(f64["x"], f64["A"], f64["b"], sched[3], nit, method=sched[0])[:5] / nit

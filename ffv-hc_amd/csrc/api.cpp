@@ -21,7 +21,9 @@ int fail(int code, const std::string& msg) {
 }
 
 bool field_ok(ffvhc_ctx* ctx, int f, int prec) {
-  return f >= 0 && f < (int)ctx->fields.size() && ctx->fields[f].used && (prec == 0 || ctx->fields[f].prec == prec);
+  // 16-byte records are solver scratch, never a caller's field
+  return f >= 0 && f < (int)ctx->fields.size() && ctx->fields[f].used &&
+         (prec == 0 ? ctx->fields[f].prec != 16 : ctx->fields[f].prec == prec);
 }
 
 std::vector<int> selected_cases(ffvhc_ctx* ctx) {
@@ -61,7 +63,8 @@ int refresh_active(ffvhc_ctx* ctx) {
 }
 
 int alloc_field(ffvhc_ctx* ctx, int prec, int* field) {
-  if (prec != 8 && prec != 4) return fail(-1, "precision must be 8 or 4");
+  // 8 / 4: FP64 / FP32 fields; 16: one 16-byte record per cell (solver scratch only)
+  if (prec != 8 && prec != 4 && prec != 16) return fail(-1, "precision must be 8 or 4");
   int id = -1;
   for (int i = 0; i < (int)ctx->fields.size(); i++)
     if (!ctx->fields[i].used) {
@@ -434,6 +437,7 @@ int ffvhc_case_set_bc(ffvhc_ctx* c, int id, int group, int face, int type, doubl
 
 int ffvhc_field_alloc(ffvhc_ctx* c, int prec, int* field) {
   if (!c || !field) return fail(-1, "null argument");
+  if (prec != 8 && prec != 4) return fail(-1, "precision must be 8 or 4");
   return alloc_field(c, prec, field);
 }
 

@@ -67,7 +67,8 @@ void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const Dev
 // an FP32 matrix and FP32 arithmetic, an FP32 matrix and FP64 arithmetic, all FP64, or the
 // symmetric (red, black, red) sweep with an FP32 matrix and FP64 arithmetic.
 struct BiOpts {
-  enum Gs { NONE = 0, GS32, GS32_ARITH64, GS64, SGS32_ARITH64 };
+  // GSC: `sweeps` (1-3) red-black SOR sweeps on the compact per-cell records of launch_gs_pack
+  enum Gs { NONE = 0, GS32, GS32_ARITH64, GS64, SGS32_ARITH64, GSC };
   const Coef7<float>* A32 = nullptr;
   int gs = NONE;
   const Coef7<float>* M32 = nullptr;
@@ -75,6 +76,8 @@ struct BiOpts {
   // shadow residual r0 = the seeded hash vector computed in the kernels that read it (nothing
   // stored or read for r0; the matvec and XR hashed variants exist for the A32 path)
   bool hash_r0 = false;
+  const uint4* rec = nullptr;
+  int sweeps = 1;
 };
 
 template <class T>
@@ -96,6 +99,8 @@ void launch_defect(hipStream_t sm, const int* active, int nact, const DevBlock* 
 void launch_add_correction(hipStream_t sm, const int* active, int nact, const DevBlock* blk, SolveState* st,
                            const SolveState* inner, double* x, const float* e);
 void launch_convert(hipStream_t sm, const int* active, int nact, float* y, const double* x);
+// Compact GS preconditioner records (16 bytes per cell: 1 / Ap in FP32, A_q / Ap in FP16)
+void launch_gs_pack(hipStream_t sm, const int* active, int nact, Coef7<double> A, uint4* rec);
 
 void launch_predict(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                     const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc);

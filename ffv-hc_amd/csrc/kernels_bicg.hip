@@ -247,6 +247,149 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
                            h);
 }
 
+// Compact matrix of the multi-sweep GS preconditioner, one 16-byte record per cell:
+// x = 1 / Ap as FP32 bits, y / z / w = the normalised couplings A_q / Ap (q = 1..6: w e, s n, b t)
+// as FP16 pairs (low half = odd q).  Any rounding here only defines a slightly different, still
+// fixed and linear, preconditioner; the sweeps run in FP64.
+__device__ inline uint32_t half_bits(double v) {
+  const double lim = 65504.0;  // largest finite FP16
+  const _Float16 h = (_Float16)(v > lim ? lim : v < -lim ? -lim : v);
+  return (uint32_t)__builtin_bit_cast(uint16_t, h);
+}
+__device__ inline double half_lo(uint32_t u) { return (double)__builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu)); }
+__device__ inline double half_hi(uint32_t u) { return (double)__builtin_bit_cast(_Float16, (uint16_t)(u >> 16)); }
+
+__global__ __launch_bounds__(256) void gs_pack_kernel(const int* __restrict__ active, Coef7<double> A,
+                                                      uint4* __restrict__ rec) {
+  const size_t base = (size_t)active[blockIdx.x] * NC;
+  for (int q = threadIdx.x; q < NC; q += 256) {
+    const size_t g = base + q;
+    const double d = A.p[0][g];
+    const double inv = d != 0.0 ? 1.0 / d : 0.0;  // a zero pivot decouples the cell in M
+    uint32_t w[6];
+#pragma unroll
+    for (int b = 0; b < 6; b++) w[b] = half_bits(A.p[b + 1][g] * inv);
+    rec[g] = make_uint4(__float_as_uint((float)inv), w[0] | (w[1] << 16), w[2] | (w[3] << 16), w[4] | (w[5] << 16));
+  }
+}
+
+void launch_gs_pack(hipStream_t sm, const int* active, int nact, Coef7<double> A, uint4* rec) {
+  if (nact) hipLaunchKernelGGL(gs_pack_kernel, dim3(nact), dim3(256), 0, sm, active, A, rec);
+}
+
+// P / S with NS red-black SOR sweeps of the block-local system (ghost values zero) from z = 0,
+// on the compact matrix above, in FP64: u = v / Ap; red cells first; each update
+// z <- z + omega (u - sum_q w_q z_q - z).  The block's z lives in LDS; each thread keeps u and
+// the couplings of its own (i, j) column in registers, so the sweeps after the first read no HBM.
+template <int MODE, int NS>
+__global__ __launch_bounds__(BI_NT) void bi_precond_gsk_kernel(const int* __restrict__ active,
+                                                               const DevBlock* __restrict__ blk,
+                                                               const DevCase* __restrict__ cs,
+                                                               const SolveState* __restrict__ st,
+                                                               const uint4* __restrict__ rec, BiVecs<double> V,
+                                                               int count) {
+  __shared__ double z[NC];
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const int c = B.case_id;
+  if (st[c].done) return;
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const size_t base = (size_t)slot * NC;
+  const double omega = st[c].omega;
+  const double ng = -st[c].gamma, beta = st[c].beta, na = -st[c].alpha;
+  const bool first = count == 1 || st[c].restart;
+  const int par = (tx + ty) & 1;  // red cells of this column: k = 2m + par
+  auto zi = [](int i, int j, int k) { return i + NB * (j + NB * k); };
+  // per colour (index m: k = 2m + par for red, 2m + 1 - par for black), compile-time indexed so
+  // that they stay in registers
+  double ur[NB / 2], ub[NB / 2];
+  uint32_t wr[NB / 2][3], wb[NB / 2][3];
+#pragma unroll
+  for (int k0 = 0; k0 < NB; k0 += 4) {
+    double rv[4], pv[4], qv[4];
+    uint4 rc[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      rv[kk] = V.r[g];
+      if (MODE == 1 || !first) qv[kk] = V.q[g];
+      if (MODE == 0 && !first) pv[kk] = V.p[g];
+      rc[kk] = rec[g];
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const int k = k0 + kk;
+      const size_t g = base + tx + NB * (ty + NB * k);
+      double v;
+      if (MODE == 0) {
+        v = first ? rv[kk] : rv[kk] + beta * (pv[kk] + ng * qv[kk]);
+        V.p[g] = v;
+      } else {
+        v = na * qv[kk] + rv[kk];
+        V.s[g] = v;
+      }
+      const double uk = v * (double)__uint_as_float(rc[kk].x);
+      const int m = k >> 1;
+      if ((k & 1) == par) {
+        ur[m] = uk;
+        wr[m][0] = rc[kk].y;
+        wr[m][1] = rc[kk].z;
+        wr[m][2] = rc[kk].w;
+      } else {
+        ub[m] = uk;
+        wb[m][0] = rc[kk].y;
+        wb[m][1] = rc[kk].z;
+        wb[m][2] = rc[kk].w;
+      }
+    }
+  }
+  // first red half sweep: every neighbour is still zero
+#pragma unroll
+  for (int m = 0; m < NB / 2; m++) z[zi(tx, ty, 2 * m + par)] = omega * ur[m];
+  __syncthreads();
+  auto zat = [&](int i, int j, int k) -> double {
+    return (i < 0 || i >= NB || j < 0 || j >= NB || k < 0 || k >= NB) ? 0.0 : z[zi(i, j, k)];
+  };
+  // one half sweep over this column's cells of one colour; from_zero: the cells' old values are
+  // zero (first visit)
+  auto half = [&](auto red_tag, bool from_zero) {
+    constexpr bool RED = decltype(red_tag)::value;
+    const int kp = RED ? par : 1 - par;
+#pragma unroll
+    for (int m = 0; m < NB / 2; m++) {
+      const int k = 2 * m + kp;
+      uint32_t* w = RED ? wr[m] : wb[m];
+      // keep the couplings packed across sweeps (an empty asm hides that the unpacked values
+      // of the last sweep could be reused, which would cost 12 more registers per cell)
+      asm volatile("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]));
+      const double sum = half_lo(w[0]) * zat(tx - 1, ty, k) + half_hi(w[0]) * zat(tx + 1, ty, k) +
+                         half_lo(w[1]) * zat(tx, ty - 1, k) + half_hi(w[1]) * zat(tx, ty + 1, k) +
+                         half_lo(w[2]) * zat(tx, ty, k - 1) + half_hi(w[2]) * zat(tx, ty, k + 1);
+      const double zn = (RED ? ur[m] : ub[m]) - sum;
+      if (from_zero) {
+        z[zi(tx, ty, k)] = omega * zn;
+      } else {
+        const double zo = z[zi(tx, ty, k)];
+        z[zi(tx, ty, k)] = zo + omega * (zn - zo);
+      }
+    }
+    __syncthreads();
+  };
+  using Red = std::true_type;
+  using Black = std::false_type;
+  half(Black{}, true);
+#pragma unroll
+  for (int sweep = 1; sweep < NS; sweep++) {
+    half(Red{}, false);
+    half(Black{}, false);
+  }
+  double* out = MODE == 0 ? V.pp : V.ss;
+#pragma unroll
+  for (int k = 0; k < NB; k++) out[base + tx + NB * (ty + NB * k)] = z[zi(tx, ty, k)];
+  double* h = MODE == 0 ? V.hpp : V.hss;
+  push_halo<double, BI_NT>(B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return z[zi(i, j, k)]; }, h, h);
+}
+
 // Q: q_ = A p_, q_.r0 -> alpha (mode 0);  T: t_ = A s_, t_.s and t_.t_ -> gamma (mode 1)
 template <class T, int MODE, class TA = T, bool HR = false>
 __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict__ active,
@@ -435,7 +578,16 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   BiVecs<double>& Vd = *reinterpret_cast<BiVecs<double>*>(&V);
   auto precond = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
-    if (dbl && o.gs == BiOpts::GS32)
+    if (dbl && o.gs == BiOpts::GSC && o.sweeps == 2)
+      hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 2>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
+                         o.rec, Vd, count);
+    else if (dbl && o.gs == BiOpts::GSC && o.sweeps == 3)
+      hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 3>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
+                         o.rec, Vd, count);
+    else if (dbl && o.gs == BiOpts::GSC)
+      hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 1>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
+                         o.rec, Vd, count);
+    else if (dbl && o.gs == BiOpts::GS32)
       hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
                          *o.M32, Vd, count);
     else if (dbl && o.gs == BiOpts::GS32_ARITH64)
