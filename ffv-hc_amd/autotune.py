@@ -15,17 +15,15 @@ import numpy as np
 DEFAULT_CANDIDATES = (
     # (method, inner_eps or reliable-update factor, inner iteration cap, omega); see
     # ffvhc_solve_mixed for the methods
-    (1, 1e-2, 300, 1.0),
-    (1, 1e-2, 500, 1.0),
-    (1, 3e-2, 400, 1.0),
-    (2, 1e-1, 250, 1.0),
-    (3, 0.1, 100000, 1.0),
-    (5, 0.5, 100000, 1.0),
-    # FP64 recurrence on an FP32-stored matrix with the block GS preconditioner (omega = SOR factor)
-    (8, 0.5, 20000, 1.0),
-    (8, 0.5, 20000, 1.2),
-    (9, 0.5, 20000, 1.2),
-    (10, 0.5, 20000, 1.2),
+    (1, 1e-2, 500, 1.0),                 # FP32 Jacobi-PCG in FP64 defect correction
+    (2, 1e-1, 250, 1.0),                 # FP32 Jacobi-BiCGSTAB in FP64 defect correction
+    (8, 0.5, 20000, 1.2),                # FP64 recurrence, FP32 matrix, one block GS sweep
+    # FP64 recurrence, FP32 matrix, 2 / 3 block SOR sweeps on the compact preconditioner matrix
+    (12, 0.5, 20000, 1.2),
+    (12, 0.5, 20000, 1.4),
+    (13, 0.5, 20000, 1.2),
+    (13, 0.5, 20000, 1.4),
+    (13, 0.5, 20000, 1.6),
 )
 
 
