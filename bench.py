@@ -184,7 +184,7 @@ MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64
                  "FP64 BiCGSTAB on an FP32-stored matrix, symmetric block red-black GS preconditioner",
                  "FP64 BiCGSTAB on an FP32-stored matrix, block red-black GS preconditioner, hashed shadow "
                  "residual"] + ["FP64 BiCGSTAB on an FP32-stored matrix, %d block red-black SOR sweep%s on a compact "
-                                "FP32/FP16 preconditioner matrix" % (n, "" if n == 1 else "s") for n in (1, 2, 3)]
+                                "FP32/FP16 preconditioner matrix" % (n, "" if n == 1 else "s") for n in range(1, 7)]
 
 # schedule of `bench.py --profile` (the rocprofv3 runs): FP32-stored matrix, 3 SOR sweeps, omega 1.6
 PROFILE_SCHEDULE = (13, 0.5, 20000, 1.6)

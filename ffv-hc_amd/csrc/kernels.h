@@ -67,7 +67,7 @@ void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const Dev
 // an FP32 matrix and FP32 arithmetic, an FP32 matrix and FP64 arithmetic, all FP64, or the
 // symmetric (red, black, red) sweep with an FP32 matrix and FP64 arithmetic.
 struct BiOpts {
-  // GSC: `sweeps` (1-3) red-black SOR sweeps on the compact per-cell records of launch_gs_pack
+  // GSC: `sweeps` (1-6) red-black SOR sweeps on the compact per-cell records of launch_gs_pack
   enum Gs { NONE = 0, GS32, GS32_ARITH64, GS64, SGS32_ARITH64, GSC };
   const Coef7<float>* A32 = nullptr;
   int gs = NONE;

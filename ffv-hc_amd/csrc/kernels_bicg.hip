@@ -279,10 +279,13 @@ void launch_gs_pack(hipStream_t sm, const int* active, int nact, Coef7<double> A
 
 // P / S with NS red-black SOR sweeps of the block-local system (ghost values zero) from z = 0,
 // on the compact matrix above, in FP64: u = v / Ap; red cells first; each update
-// z <- z + omega (u - sum_q w_q z_q - z).  The block's z lives in LDS; each thread keeps u and
-// the couplings of its own (i, j) column in registers, so the sweeps after the first read no HBM.
+// z <- z + omega (u - sum_q w_q z_q - z).  The block's z lives in LDS; each of the GSK_NT threads
+// keeps u and the couplings of half an (i, j) column (8 cells) in registers, so the sweeps after
+// the first read no HBM.
+constexpr int GSK_NT = 512;
+constexpr int GSK_NK = NB * 256 / GSK_NT;  // cells of a column per thread
 template <int MODE, int NS>
-__global__ __launch_bounds__(BI_NT) void bi_precond_gsk_kernel(const int* __restrict__ active,
+__global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __restrict__ active,
                                                                const DevBlock* __restrict__ blk,
                                                                const DevCase* __restrict__ cs,
                                                                const SolveState* __restrict__ st,
@@ -293,24 +296,25 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gsk_kernel(const int* __rest
   const DevBlock& B = blk[slot];
   const int c = B.case_id;
   if (st[c].done) return;
-  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const int t = threadIdx.x, tx = t & 15, ty = (t >> 4) & 15;
+  const int kb = (t >> 8) * GSK_NK;  // this thread's cells: k = kb .. kb + GSK_NK - 1
   const size_t base = (size_t)slot * NC;
   const double omega = st[c].omega;
   const double ng = -st[c].gamma, beta = st[c].beta, na = -st[c].alpha;
   const bool first = count == 1 || st[c].restart;
-  const int par = (tx + ty) & 1;  // red cells of this column: k = 2m + par
+  const int par = (tx + ty) & 1;  // red cells of this column: k = kb + 2m + par (kb even)
   auto zi = [](int i, int j, int k) { return i + NB * (j + NB * k); };
-  // per colour (index m: k = 2m + par for red, 2m + 1 - par for black), compile-time indexed so
-  // that they stay in registers
-  double ur[NB / 2], ub[NB / 2];
-  uint32_t wr[NB / 2][3], wb[NB / 2][3];
+  // per colour (index m: k = kb + 2m + par for red, kb + 2m + 1 - par for black), compile-time
+  // indexed so that they stay in registers
+  double ur[GSK_NK / 2], ub[GSK_NK / 2];
+  uint32_t wr[GSK_NK / 2][3], wb[GSK_NK / 2][3];
 #pragma unroll
-  for (int k0 = 0; k0 < NB; k0 += 4) {
+  for (int k0 = 0; k0 < GSK_NK; k0 += 4) {
     double rv[4], pv[4], qv[4];
     uint4 rc[4];
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
-      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      const size_t g = base + tx + NB * (ty + NB * (kb + k0 + kk));
       rv[kk] = V.r[g];
       if (MODE == 1 || !first) qv[kk] = V.q[g];
       if (MODE == 0 && !first) pv[kk] = V.p[g];
@@ -318,8 +322,8 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gsk_kernel(const int* __rest
     }
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
-      const int k = k0 + kk;
-      const size_t g = base + tx + NB * (ty + NB * k);
+      const int k = k0 + kk;  // relative to kb
+      const size_t g = base + tx + NB * (ty + NB * (kb + k));
       double v;
       if (MODE == 0) {
         v = first ? rv[kk] : rv[kk] + beta * (pv[kk] + ng * qv[kk]);
@@ -345,7 +349,7 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gsk_kernel(const int* __rest
   }
   // first red half sweep: every neighbour is still zero
 #pragma unroll
-  for (int m = 0; m < NB / 2; m++) z[zi(tx, ty, 2 * m + par)] = omega * ur[m];
+  for (int m = 0; m < GSK_NK / 2; m++) z[zi(tx, ty, kb + 2 * m + par)] = omega * ur[m];
   __syncthreads();
   auto zat = [&](int i, int j, int k) -> double {
     return (i < 0 || i >= NB || j < 0 || j >= NB || k < 0 || k >= NB) ? 0.0 : z[zi(i, j, k)];
@@ -354,9 +358,9 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gsk_kernel(const int* __rest
   // zero (first visit)
   auto half = [&](auto red_tag, bool from_zero) {
     constexpr bool RED = decltype(red_tag)::value;
-    const int kp = RED ? par : 1 - par;
+    const int kp = kb + (RED ? par : 1 - par);
 #pragma unroll
-    for (int m = 0; m < NB / 2; m++) {
+    for (int m = 0; m < GSK_NK / 2; m++) {
       const int k = 2 * m + kp;
       uint32_t* w = RED ? wr[m] : wb[m];
       // keep the couplings packed across sweeps (an empty asm hides that the unpacked values
@@ -385,9 +389,23 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gsk_kernel(const int* __rest
   }
   double* out = MODE == 0 ? V.pp : V.ss;
 #pragma unroll
-  for (int k = 0; k < NB; k++) out[base + tx + NB * (ty + NB * k)] = z[zi(tx, ty, k)];
+  for (int k = kb; k < kb + GSK_NK; k++) out[base + tx + NB * (ty + NB * k)] = z[zi(tx, ty, k)];
   double* h = MODE == 0 ? V.hpp : V.hss;
-  push_halo<double, BI_NT>(B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return z[zi(i, j, k)]; }, h, h);
+  push_halo<double, GSK_NT>(B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return z[zi(i, j, k)]; }, h, h);
+}
+
+template <int MODE>
+void launch_gsk(int sweeps, hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                const SolveState* st, const uint4* rec, BiVecs<double> V, int count) {
+  const dim3 g(nact), b(GSK_NT);
+  switch (sweeps) {
+    case 1: hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 1>), g, b, 0, sm, active, blk, cs, st, rec, V, count); break;
+    case 2: hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 2>), g, b, 0, sm, active, blk, cs, st, rec, V, count); break;
+    case 3: hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 3>), g, b, 0, sm, active, blk, cs, st, rec, V, count); break;
+    case 4: hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 4>), g, b, 0, sm, active, blk, cs, st, rec, V, count); break;
+    case 5: hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 5>), g, b, 0, sm, active, blk, cs, st, rec, V, count); break;
+    default: hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 6>), g, b, 0, sm, active, blk, cs, st, rec, V, count);
+  }
 }
 
 // Q: q_ = A p_, q_.r0 -> alpha (mode 0);  T: t_ = A s_, t_.s and t_.t_ -> gamma (mode 1)
@@ -578,15 +596,8 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   BiVecs<double>& Vd = *reinterpret_cast<BiVecs<double>*>(&V);
   auto precond = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
-    if (dbl && o.gs == BiOpts::GSC && o.sweeps == 2)
-      hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 2>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
-                         o.rec, Vd, count);
-    else if (dbl && o.gs == BiOpts::GSC && o.sweeps == 3)
-      hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 3>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
-                         o.rec, Vd, count);
-    else if (dbl && o.gs == BiOpts::GSC)
-      hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 1>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
-                         o.rec, Vd, count);
+    if (dbl && o.gs == BiOpts::GSC)
+      launch_gsk<MODE>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count);
     else if (dbl && o.gs == BiOpts::GS32)
       hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
                          *o.M32, Vd, count);

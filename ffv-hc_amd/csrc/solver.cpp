@@ -365,7 +365,7 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Co
     o->gs = method == 9 ? BiOpts::SGS32_ARITH64 : BiOpts::GS32_ARITH64;
     o->M32 = M32;
     o->hash_r0 = method == 10;
-  } else {  // 11-13: FP32 matrix for the matvecs, 1-3 GS sweeps on the compact records
+  } else {  // 11-16: FP32 matrix for the matvecs, 1-6 SOR sweeps on the compact records
     int fr;
     const int rc = scratch_field(c, 16, R_REC, &fr);
     if (rc) return rc;
@@ -626,7 +626,7 @@ int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* c, int method, int x, const int* A, i
   int rc = prologue(c, x, A, b, BC_NULL, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "methods 5-9 take FP64 x, A, b");
-  if (method < 5 || method > 13) return fail(-1, "method must be 5-13");
+  if (method < 5 || method > 16) return fail(-1, "method must be 5-16");
   if (niter < 1 || !ms6) return fail(-1, "niter >= 1 and ms[6] required");
   Coef7<float> M32;
   Coef7<double> M64;
@@ -669,13 +669,13 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
-  if (method < 0 || method > 13)
+  if (method < 0 || method > 16)
     return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB, 3 Jacobi-BiCGSTAB with reliable updates, "
                     "4 the same with a seeded shadow residual, 5 FP64 BiCGSTAB with an FP32 red-black GS "
                     "preconditioner, 6 the same all in FP64 (reference point), 7 FP64 Jacobi-BiCGSTAB with an FP32 "
                     "matrix, 8 as 7 with the block red-black GS preconditioner (FP64 arithmetic), 9 as 8 with the "
                     "symmetric red-black-red sweep, 10 as 8 with the seeded hash vector as shadow residual, "
-                    "11-13 as 8 with 1-3 GS sweeps on a compact FP32/FP16 copy of the matrix");
+                    "11-16 as 8 with 1-6 red-black SOR sweeps on a compact FP32/FP16 copy of the matrix");
   if (method == 6) {
     Coef7<float> M32;
     Coef7<double> M64;

@@ -169,7 +169,7 @@ int ffvhc_bicgstab(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, dou
  * P (p update + preconditioner), Q (q_ = A p_), S, T (t_ = A s_), XR; ms[5] = the per-case
  * scalar finalisations (timing entry). */
 int ffvhc_bicgstab_timing(ffvhc_ctx* ctx, int x, const int* A, int b, double omega, int niter, double* ms);
-/* The same stage timing for the FP64-recurrence methods 5-10 of ffvhc_solve_mixed (FP64 x, A, b;
+/* The same stage timing for the FP64-recurrence methods 5-16 of ffvhc_solve_mixed (FP64 x, A, b;
  * omega = GS relaxation factor): ms[0..4] = P, Q, S, T, XR kernels, ms[5] = finalisations. */
 int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* ctx, int method, int x, const int* A, int b, double omega, int niter,
                                 double* ms);
@@ -190,13 +190,15 @@ int ffvhc_residual(ffvhc_ctx* ctx, int x, const int* A, int b, double* residual)
  * 2 Jacobi-BiCGSTAB (each restarted per correction); 3 Jacobi-BiCGSTAB with reliable updates (the
  * FP32 recurrence continues across corrections; inner_eps = residual reduction between updates);
  * 4 as 3 with a seeded random shadow residual r0 instead of r0 = r.
- * Methods 5-9 keep the BiCGSTAB recurrence, dots and state in FP64 and narrow what is stored:
+ * Methods 5-16 keep the BiCGSTAB recurrence, dots and state in FP64 and narrow what is stored:
  * 5 FP32 block red-black GS preconditioner (FP32 arithmetic), 6 the same GS all in FP64,
  * 7 FP32 copy of the matrix for the matvecs and the Jacobi preconditioner, 8 as 7 with the block
  * GS preconditioner in FP64 arithmetic, 9 as 8 with the symmetric (red, black, red) sweep, 10 as 8
- * with a seeded hash vector as the shadow residual r0 (computed where it is read, never stored).
- * For 5 and 7-10 the answer is accepted on the FP64 residual of the FP64 matrix; a case above
- * eps restarts from it (inner_eps unused; omega is the GS relaxation factor).
+ * with a seeded hash vector as the shadow residual r0 (computed where it is read, never stored),
+ * 11-16 as 8 with 1-6 red-black SOR sweeps (block-local, from zero, FP64 arithmetic) on a compact
+ * copy of the matrix: 1 / Ap in FP32 and A_q / Ap in FP16, 16 bytes per cell.
+ * For 5 and 7-16 the answer is accepted on the FP64 residual of the FP64 matrix; a case above
+ * eps restarts from it (inner_eps unused; omega is the GS / SOR relaxation factor).
  * inner_eps: relative reduction per correction; countMax caps each inner solve (a capped inner
  * solve still yields a correction).  count[c] = inner iterations in total, outer[c] = corrections
  * taken, residual[c] = FP64 ||b - Ax|| / ||b|| after the last correction. */

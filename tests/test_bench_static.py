@@ -40,7 +40,7 @@ def test_unresolved_name_is_caught():
 def test_mixed_byte_model_and_names():
     sys.path[:0] = [ROOT, os.path.join(ROOT, "ffv-hc_amd")]
     import bench
-    for m in (5, 6, 7, 8, 9, 10, 11, 12, 13):
+    for m in range(5, 17):
         by = bench.kernel_bytes_mixed(m)
         assert set(by) == {"bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"}
         # q_ and t_ differ by r0 (hashed in 10); p_ and s_ by the p read
@@ -56,4 +56,4 @@ def test_mixed_byte_model_and_names():
     assert bench.kernel_bytes_mixed(10)["bi_xr"] == 56
     assert abs(bench.kernel_bytes_mixed(12)["bi_p"] - (24 + 16 + 16 + 3)) < 1e-9
     assert bench.mixed_prof_name(13, "bi_s") == "bi_precond_gsk_kernel<1, 3>"
-    assert len(bench.MIXED_METHODS) == 14
+    assert len(bench.MIXED_METHODS) == 17
