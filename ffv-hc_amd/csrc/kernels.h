@@ -117,6 +117,10 @@ void launch_matrix_bc(hipStream_t sm, const int* active, int nact, const DevBloc
 void launch_force_p(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase, const DevBlock* blk,
                     const DevCase* cs, const double* p, const double* hp, const uint8_t* cid8, const uint32_t* flag,
                     size_t capnc, int target, double* part, size_t nslots, double* out);
+void launch_force_v(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase, const DevBlock* blk,
+                    const DevCase* cs, const double* const* u, const double* const* hu, const float* cut,
+                    const uint8_t* cid8, const uint32_t* flag, size_t capnc, int target, double* part, size_t nslots,
+                    double* out);
 void launch_stats(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase, const DevCase* cs,
                   const double* x, double* part, size_t nslots, double* out);
 void launch_corr(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,

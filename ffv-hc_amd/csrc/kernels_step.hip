@@ -525,6 +525,78 @@ __global__ __launch_bounds__(ST_NT) void force_p_kernel(const int* __restrict__ 
   }
 }
 
+// bcut_calc_f_v (bcut.f90:4840-5032): viscous force on the faces cut by `target`.  A cut face
+// takes the wall-extrapolated neighbour (1 - 1/d) u_p; per component the doubled normal gradient
+// on its own axis and, on the two others, the gradient plus half the cross difference of the
+// tangential component; terms summed in the reference's order.  Triples per block as above.
+__global__ __launch_bounds__(ST_NT) void force_v_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
+                                                        const DevCase* __restrict__ cs, const double* __restrict__ ux,
+                                                        const double* __restrict__ hux, const double* __restrict__ uy,
+                                                        const double* __restrict__ huy, const double* __restrict__ uz,
+                                                        const double* __restrict__ huz, const float* __restrict__ cut,
+                                                        const uint8_t* __restrict__ cid8,
+                                                        const uint32_t* __restrict__ flag, size_t capnc, int target,
+                                                        double* __restrict__ part, size_t nslots) {
+  __shared__ double red[ST_NT / 64];
+  const int slot = active[blockIdx.x];
+  const DevBlock& B = blk[slot];
+  const double mu = cs[B.case_id].mu;
+  const int t = threadIdx.x, i = t & 15, j = t >> 4;
+  const double dx = B.dx;
+  const double* u[3] = {ux, uy, uz};
+  const double* hu[3] = {hux, huy, huz};
+  double fs[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < NB; k++) {
+    const size_t g = (size_t)slot * NC + i + NB * (j + NB * k);
+    const int ni[6] = {i - 1, i + 1, i, i, i, i}, nj[6] = {j, j, j - 1, j + 1, j, j}, nk[6] = {k, k, k, k, k - 1, k + 1};
+    double up[3], un[3][6], mf[6];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      up[a] = u[a][g];
+#pragma unroll
+      for (int f = 0; f < 6; f++) un[a][f] = at(u[a], hu[a], slot, ni[f], nj[f], nk[f]);
+    }
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+      mf[f] = 0.0;
+      if (cid8[f * capnc + g] == target) {
+        const double w = 1.0 - 1.0 / (double)cut[f * capnc + g];
+#pragma unroll
+        for (int a = 0; a < 3; a++) un[a][f] = w * up[a];
+        mf[f] = 1.0;
+      }
+    }
+    if (!((flag[g] >> 6) & 1u))
+#pragma unroll
+      for (int f = 0; f < 6; f++) mf[f] = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int ax = 0; ax < 3; ax++) {
+        const int fm = 2 * ax, fp = 2 * ax + 1;
+        double lo, hi;
+        if (ax == a) {
+          lo = mu * (up[a] - un[a][fm]) / dx * mf[fm] * 2.0;
+          hi = mu * (un[a][fp] - up[a]) / dx * mf[fp] * 2.0;
+        } else {
+          const double cross = (un[ax][2 * a + 1] - un[ax][2 * a]) * 0.5;
+          lo = mu * ((up[a] - un[a][fm]) + cross) / dx * mf[fm];
+          hi = mu * ((un[a][fp] - up[a]) + cross) / dx * mf[fp];
+        }
+        acc = ax == 0 ? lo - hi : acc + lo - hi;
+      }
+      fs[a] += acc * dx * dx;
+    }
+  }
+  const double sx = block_sum<ST_NT>(fs[0], red), sy = block_sum<ST_NT>(fs[1], red), sz = block_sum<ST_NT>(fs[2], red);
+  if (t == 0) {
+    part[slot] = sx;
+    part[nslots + slot] = sy;
+    part[2 * nslots + slot] = sz;
+  }
+}
+
 __global__ __launch_bounds__(ST_NT) void force_sum_kernel(const int* __restrict__ caseact,
                                                           const DevCase* __restrict__ cs,
                                                           const double* __restrict__ part, size_t nslots,
@@ -637,6 +709,15 @@ void launch_force_p(hipStream_t sm, const int* active, int nact, const int* case
   if (!nact || !ncase) return;
   hipLaunchKernelGGL(force_p_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, blk, p, hp, cid8, flag, capnc, target,
                      part, nslots);
+  hipLaunchKernelGGL(force_sum_kernel, dim3(ncase), dim3(ST_NT), 0, sm, caseact, cs, part, nslots, out);
+}
+void launch_force_v(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase, const DevBlock* blk,
+                    const DevCase* cs, const double* const* u, const double* const* hu, const float* cut,
+                    const uint8_t* cid8, const uint32_t* flag, size_t capnc, int target, double* part, size_t nslots,
+                    double* out) {
+  if (!nact || !ncase) return;
+  hipLaunchKernelGGL(force_v_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, blk, cs, u[0], hu[0], u[1], hu[1], u[2],
+                     hu[2], cut, cid8, flag, capnc, target, part, nslots);
   hipLaunchKernelGGL(force_sum_kernel, dim3(ncase), dim3(ST_NT), 0, sm, caseact, cs, part, nslots, out);
 }
 void launch_stats(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase, const DevCase* cs,

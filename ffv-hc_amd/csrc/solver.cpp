@@ -846,17 +846,17 @@ int ffvhc_residual(ffvhc_ctx* c, int x, const int* A, int b, double* residual) {
   return 0;
 }
 
-int ffvhc_force_p(ffvhc_ctx* c, int p, int cut_id, double* f) {
-  if (!c || !f) return fail(-1, "null argument");
-  if (!field_ok(c, p, 8)) return fail(-1, "p must be an FP64 field");
-  if (cut_id < 1 || cut_id > 255) return fail(-1, "cut_id must be in 1..255");
+}  // extern "C"
+
+// Per-case force triples into f[3 * selected case] (0 for a case without blocks); `launch`
+// writes the triples of the selected cases with blocks, in selection order, to its device array.
+template <class Launch>
+static int case_triples(ffvhc_ctx* c, double* f, Launch launch) {
   int rc = refresh_active(c);
   if (rc) return rc;
   double* dout = nullptr;
   HIPCHK(hipMallocAsync((void**)&dout, sizeof(double) * 3 * std::max(c->ncaseact, 1), c->sm));
-  launch_force_p(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase,
-                 (const double*)c->fields[p].d[0], (const double*)c->fields[p].h[0], c->dcid, c->dflag,
-                 (size_t)c->cap * NC, cut_id, c->dpart, (size_t)c->cap, dout);
+  launch(dout);
   HIPCHK(hipGetLastError());
   std::vector<double> h(3 * std::max(c->ncaseact, 1), 0.0);
   HIPCHK(hipMemcpyAsync(h.data(), dout, sizeof(double) * 3 * c->ncaseact, hipMemcpyDeviceToHost, c->sm));
@@ -874,6 +874,34 @@ int ffvhc_force_p(ffvhc_ctx* c, int p, int cut_id, double* f) {
     }
   }
   return 0;
+}
+
+extern "C" {
+
+int ffvhc_force_p(ffvhc_ctx* c, int p, int cut_id, double* f) {
+  if (!c || !f) return fail(-1, "null argument");
+  if (!field_ok(c, p, 8)) return fail(-1, "p must be an FP64 field");
+  if (cut_id < 1 || cut_id > 255) return fail(-1, "cut_id must be in 1..255");
+  return case_triples(c, f, [&](double* dout) {
+    launch_force_p(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase,
+                   (const double*)c->fields[p].d[0], (const double*)c->fields[p].h[0], c->dcid, c->dflag,
+                   (size_t)c->cap * NC, cut_id, c->dpart, (size_t)c->cap, dout);
+  });
+}
+
+int ffvhc_force_v(ffvhc_ctx* c, int ux, int uy, int uz, int cut_id, double* f) {
+  if (!c || !f) return fail(-1, "null argument");
+  if (!field_ok(c, ux, 8) || !field_ok(c, uy, 8) || !field_ok(c, uz, 8))
+    return fail(-1, "ux, uy, uz must be FP64 fields");
+  if (cut_id < 1 || cut_id > 255) return fail(-1, "cut_id must be in 1..255");
+  const double* u[3] = {(const double*)c->fields[ux].d[0], (const double*)c->fields[uy].d[0],
+                        (const double*)c->fields[uz].d[0]};
+  const double* hu[3] = {(const double*)c->fields[ux].h[0], (const double*)c->fields[uy].h[0],
+                         (const double*)c->fields[uz].h[0]};
+  return case_triples(c, f, [&](double* dout) {
+    launch_force_v(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase, u, hu, c->dcut,
+                   c->dcid, c->dflag, (size_t)c->cap * NC, cut_id, c->dpart, (size_t)c->cap, dout);
+  });
 }
 
 int ffvhc_field_stats(ffvhc_ctx* c, int f, double* out) {

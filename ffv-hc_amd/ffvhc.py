@@ -437,6 +437,12 @@ class Context:
         check(_L.ffvhc_force_p(self.h, p, cut_id, _ptr(f)), "ffvhc_force_p")
         return f
 
+    def force_v(self, ux, uy, uz, cut_id, ncases=None):
+        """Viscous force (fx, fy, fz) on the faces with boundary id cut_id, per selected case."""
+        f = np.zeros((ncases or self._nsel(), 3))
+        check(_L.ffvhc_force_v(self.h, ux, uy, uz, cut_id, _ptr(f)), "ffvhc_force_v")
+        return f
+
     def stats(self, f, ncases=None):
         """(sum, max, min, |max|, |min|) over the interior of each selected case."""
         out = np.zeros((ncases or self._nsel(), 5))

@@ -263,6 +263,10 @@ int ffvhc_step(ffvhc_ctx* ctx, ffvhc_step_fields* fields, const ffvhc_step_param
  * by Solver::PrintForceCID, Solver.cpp:2994-3160): f[3*i .. 3*i+2] for the i-th selected case.
  * p's ghost layer must be current. */
 int ffvhc_force_p(ffvhc_ctx* ctx, int p, int cut_id, double* f);
+/* Viscous force on the faces with boundary id cut_id (bcut_calc_f_v, bcut.f90:4840-5032, as summed
+ * by Solver::PrintForceCID with the case's mu): f[3*i .. 3*i+2] for the i-th selected case.  The
+ * ghost layers of ux, uy, uz must be current. */
+int ffvhc_force_v(ffvhc_ctx* ctx, int ux, int uy, int uz, int cut_id, double* f);
 
 /* LocalScalar3D::CalcStats (LocalScalar3D.cpp:9-80): out[5*i ..] = {sum, max, min, |max|, |min|}
  * over the interior cells of the i-th selected case. */

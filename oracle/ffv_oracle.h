@@ -108,6 +108,9 @@ void orc_bcut_set_reference_value(real* const* A, real* b, real xr, real yr, rea
 void orc_bcut_calc_f_p(real* fspx, real* fspy, real* fspz, real* fsp, int cid_target, const real* p,
                        const real* const* c, const int* const* cid, const int* pid, real dx, real dt,
                        const int* sz, int g);
+void orc_bcut_calc_f_v(real* fsvx, real* fsvy, real* fsvz, real* fsv, int cid_target, const real* ux,
+                       const real* uy, const real* uz, const real* const* c, const int* const* cid,
+                       const int* pid, real rho, real mu, real dx, real dt, real Uc, const int* sz, int g);
 /* bstl.f90 (setup) */
 void orc_bstl_cutoff(real* const* c, int* const* cid, real eps, const int* sz, int g);
 int orc_bstl_detect_zerocut(real* const* c, int* const* cid, const int* sz, int g);

@@ -846,6 +846,64 @@ void orc_bcut_calc_f_p(real* fspx, real* fspy, real* fspz, real* fsp, int cid_ta
   fsp[0] = fsx; fsp[1] = fsy; fsp[2] = fsz;
 }
 
+/* bcut.f90:4840-5032 bcut_calc_f_v: viscous force on the faces cut by boundary id cid_target.
+ * A cut face replaces the neighbour velocity by the wall-extrapolated (1 - 1/d) u_p (no-slip wall
+ * at distance d); solid cells contribute nothing.  Per component the six face terms are summed
+ * left to right as written in the reference; rho, dt and Uc are accepted and unused there. */
+void orc_bcut_calc_f_v(real* fsvx, real* fsvy, real* fsvz, real* fsv, int cid_target, const real* ux,
+                       const real* uy, const real* uz, const real* const* c, const int* const* cid,
+                       const int* pid, real rho, real mu, real dx, real dt, real Uc, const int* sz, int g) {
+  DECL_SZ
+  (void)rho; (void)dt; (void)Uc;
+  real fsx = 0.0, fsy = 0.0, fsz = 0.0;
+  for (int k = 1; k <= kx; k++)
+    for (int j = 1; j <= jx; j++)
+      for (int i = 1; i <= ix; i++) {
+        ptrdiff_t m = IX(i, j, k);
+        const ptrdiff_t nb[6] = {IX(i - 1, j, k), IX(i + 1, j, k), IX(i, j - 1, k),
+                                 IX(i, j + 1, k), IX(i, j, k - 1), IX(i, j, k + 1)};
+        real up[3] = {ux[m], uy[m], uz[m]};
+        real un[3][6];  /* component, face (w e s n b t) */
+        const real* u3[3] = {ux, uy, uz};
+        for (int a = 0; a < 3; a++)
+          for (int f = 0; f < 6; f++) un[a][f] = u3[a][nb[f]];
+        real mf[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int f = 0; f < 6; f++)
+          if (cid[f][m] == cid_target) {
+            const real w = 1.0 - 1.0 / c[f][m];
+            for (int a = 0; a < 3; a++) un[a][f] = w * up[a];
+            mf[f] = 1.0;
+          }
+        if (pid[m] != 1)
+          for (int f = 0; f < 6; f++) mf[f] = 0.0;
+        /* component a: its own axis uses the doubled normal gradient; the two other axes add half
+         * the cross difference of the tangential component along a */
+        real* out[3] = {fsvx, fsvy, fsvz};
+        for (int a = 0; a < 3; a++) {
+          real acc = 0.0;
+          for (int ax = 0; ax < 3; ax++) {
+            const int fm = 2 * ax, fp = 2 * ax + 1;
+            real lo, hi;
+            if (ax == a) {
+              lo = mu * ((up[a] - un[a][fm])) / dx * mf[fm] * 2.0;
+              hi = mu * ((un[a][fp] - up[a])) / dx * mf[fp] * 2.0;
+            } else {
+              /* d u_ax / d x_a from the two neighbours along a */
+              const real cross = (un[ax][2 * a + 1] - un[ax][2 * a]) * 0.5;
+              lo = mu * ((up[a] - un[a][fm]) + cross) / dx * mf[fm];
+              hi = mu * ((un[a][fp] - up[a]) + cross) / dx * mf[fp];
+            }
+            acc = (ax == 0) ? +lo - hi : acc + lo - hi;
+          }
+          out[a][m] = acc;
+        }
+        fsx = fsx + fsvx[m] * dx * dx;
+        fsy = fsy + fsvy[m] * dx * dx;
+        fsz = fsz + fsvz[m] * dx * dx;
+      }
+  fsv[0] = fsx; fsv[1] = fsy; fsv[2] = fsz;
+}
+
 /* ------------------------------------------------------------------ bstl.f90 */
 
 /* bstl.f90:1306-1372 bstl_cutoff */

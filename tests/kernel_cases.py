@@ -356,6 +356,17 @@ def _run_bcut(name, G, L, F, s3, g, A, b, x, y, c, cid, pid):
             F.bcut_calc_f_p_(P(fx), P(fy), P(fz), P(fsp), _i(1), P(x), *cp, *ip_, P(pid), _r(dx), _r(dt), s3,
                              _i(g))
         out.update(fx=fx, fy=fy, fz=fz, fsp=fsp)
+    elif name == "calc_f_v":
+        ux, uy, uz = G.real(), G.real(), G.real()
+        fx, fy, fz = G.real(), G.real(), G.real()
+        fsv = np.zeros(3)
+        if F is None:
+            L.orc_bcut_calc_f_v(P(fx), P(fy), P(fz), P(fsv), I(1), P(ux), P(uy), P(uz), cpp, ipp, P(pid), D(1.0),
+                                D(0.01), D(dx), D(dt), D(0.0), s3, I(g))
+        else:
+            F.bcut_calc_f_v_(P(fx), P(fy), P(fz), P(fsv), _i(1), P(ux), P(uy), P(uz), *cp, *ip_, P(pid), _r(1.0),
+                             _r(0.01), _r(dx), _r(dt), _r(0.0), s3, _i(g))
+        out.update(fx=fx, fy=fy, fz=fz, fsv=fsv)
     elif name == "bstl":
         cc = [a.copy() for a in c]
         ii = [a.copy() for a in cid]
@@ -389,4 +400,4 @@ def _run_bcut(name, G, L, F, s3, g, A, b, x, y, c, cid, pid):
 
 CASES = ["jacobi_smoother", "rbgs_smoother", "calc_ax", "calc_r", "calc_r2", "blas", "sup_copy", "sup_c2f",
          "sup_f2c", "bc_x", "bc_a", "calc_c_f_c2", "calc_d_u", "update_u", "calc_abd_u", "add_g_f", "remove_p",
-         "calc_ab_p", "corr_u", "set_reference_value", "calc_f_p", "bstl"]
+         "calc_ab_p", "corr_u", "set_reference_value", "calc_f_p", "calc_f_v", "bstl"]

@@ -245,6 +245,51 @@ def test_force_p_matches_oracle(ctx, name):
         ctx.remove_case(cid)
 
 
+@pytest.mark.parametrize("name", ["flat", "two_level"])
+def test_force_v_matches_oracle(ctx, name):
+    """bcut_calc_f_v summed over the blocks (Solver::PrintForceCID) for each boundary id, with the
+    case's mu and velocity ghosts filled by Scalar3DUpdater5."""
+    tree = trees()[name]
+    og = OracleGrid(tree)
+    rng = np.random.default_rng(22)
+    geo = geometry(og, rng)
+    params = ffvhc.case_params(mu=0.013, bc=BC)
+    cid = ctx.add_case(tree, params, geometry=(geo[0], geo[1], geo[2]))
+    uf = [ctx.field(8) for _ in range(3)]
+    try:
+        u = [rng.standard_normal(og.shape) for _ in range(3)]
+        for d in range(3):
+            og.ghost_fill(u[d], 5, [params.bct[d][f] for f in range(6)], [params.bcv[d][f] for f in range(6)])
+            ctx.upload(uf[d], cid, u[d])
+        ctx.select([cid])
+        sz = (ctypes.c_int * 3)(N, N, N)
+        for target in (1, 2):
+            got = ctx.force_v(uf[0], uf[1], uf[2], target, ncases=1)[0]
+            want = np.zeros(3)
+            npad = (N + 2 * G) ** 3
+            for b in range(og.nb):
+                fs = [np.zeros(npad) for _ in range(3)]
+                out = np.zeros(3)
+                cb = [np.ascontiguousarray(geo[0][f][b]) for f in range(6)]
+                ib = [np.ascontiguousarray(geo[1][f][b]) for f in range(6)]
+                ub = [np.ascontiguousarray(u[d][b]) for d in range(3)]
+                og.L.orc_bcut_calc_f_v(O.ptr(fs[0]), O.ptr(fs[1]), O.ptr(fs[2]), O.ptr(out), target,
+                                       O.ptr(ub[0]), O.ptr(ub[1]), O.ptr(ub[2]), O.parr(cb),
+                                       O.parr(ib, ctypes.c_int), O.ptr(np.ascontiguousarray(geo[2][b])),
+                                       ctypes.c_double(params.rho), ctypes.c_double(params.mu),
+                                       ctypes.c_double(og.tb["dx"][b]), ctypes.c_double(0.01),
+                                       ctypes.c_double(0.0), sz, G)
+                want += out
+            assert np.any(want != 0.0)
+            # per-block order differs from the reference's serial loop: rounding-level tolerance
+            assert np.all(np.abs(got - want) <= 1e-12 * np.max(np.abs(want)) + 1e-300), (got, want)
+    finally:
+        ctx.select(None)
+        for f in uf:
+            ctx.free(f)
+        ctx.remove_case(cid)
+
+
 def test_checkpoint_round_trip(ctx, tmp_path):
     """Dump2 of a field, then Load2 into another: interior and first ghost layer identical; the
     file has the reference's layout (header + padded blocks)."""
