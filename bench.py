@@ -186,8 +186,8 @@ MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64
                  "residual"] + ["FP64 BiCGSTAB on an FP32-stored matrix, %d block red-black SOR sweep%s on a compact "
                                 "FP32/FP16 preconditioner matrix" % (n, "" if n == 1 else "s") for n in range(1, 7)]
 
-# schedule of `bench.py --profile` (the rocprofv3 runs): FP32-stored matrix, 3 SOR sweeps, omega 1.6
-PROFILE_SCHEDULE = (13, 0.5, 20000, 1.6)
+# schedule of `bench.py --profile` (the rocprofv3 runs): FP32-stored matrix, 5 SOR sweeps, omega 1.6
+PROFILE_SCHEDULE = (15, 0.5, 20000, 1.6)
 
 # kernel names of the timed stages as rocprofv3 reports them
 PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0, %s>", "bi_q": "bi_matvec_kernel<%s, 0, %s, false>",

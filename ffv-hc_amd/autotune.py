@@ -18,12 +18,13 @@ DEFAULT_CANDIDATES = (
     (1, 1e-2, 500, 1.0),                 # FP32 Jacobi-PCG in FP64 defect correction
     (2, 1e-1, 250, 1.0),                 # FP32 Jacobi-BiCGSTAB in FP64 defect correction
     (8, 0.5, 20000, 1.2),                # FP64 recurrence, FP32 matrix, one block GS sweep
-    # FP64 recurrence, FP32 matrix, 2 / 3 block SOR sweeps on the compact preconditioner matrix
-    (12, 0.5, 20000, 1.2),
+    # FP64 recurrence, FP32 matrix, 2-6 block SOR sweeps on the compact preconditioner matrix
     (12, 0.5, 20000, 1.4),
-    (13, 0.5, 20000, 1.2),
     (13, 0.5, 20000, 1.4),
     (13, 0.5, 20000, 1.6),
+    (14, 0.5, 20000, 1.4),
+    (15, 0.5, 20000, 1.6),
+    (16, 0.5, 20000, 1.6),
 )
 
 

@@ -88,7 +88,7 @@ struct SolveState {
   int parity;       // buffer that holds the result (RBGS ping-pong)
   int iter;         // iterations launched so far
   int restart;      // Krylov restart pending: the next P step takes p = r (reliable update)
-  int pad;
+  int used;         // outer state: inner iterations spent so far (budget of a restarted inner solve)
 };
 
 // the seven bands of a 7-point matrix {Ap, Aw, Ae, As, An, Ab, At}

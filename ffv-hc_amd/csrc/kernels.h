@@ -126,7 +126,11 @@ void launch_stats(hipStream_t sm, const int* active, int nact, const int* caseac
 void launch_corr(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                  const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc);
 
+// outer (optional): inner solve of a defect-correction / restart loop: cases done there are
+// skipped, and each case's countMax is reduced by the iterations its outer state has used
 void launch_solve_init(hipStream_t sm, SolveState* st, int ncase, double omega, double eps, int countMax,
                        const SolveState* outer = nullptr);
+// outer[c].used += inner[c].count
+void launch_add_used(hipStream_t sm, SolveState* outer, const SolveState* inner, int ncase);
 
 }  // namespace ffvhc

@@ -243,7 +243,9 @@ __global__ void solve_init_kernel(SolveState* st, int ncase, double omega, doubl
   if (outer && outer[c].done) s.done = 1;  // inner solve of a case whose defect correction finished
   s.omega = omega;
   s.eps = eps;
-  s.countMax = countMax;
+  // a restarted inner solve gets what is left of the case's own budget
+  s.countMax = outer ? countMax - outer[c].used : countMax;
+  if (s.countMax <= 0) s.done = 1;
   s.count = 0;
   s.rr0 = 1.0;
   st[c] = s;
@@ -289,6 +291,15 @@ void launch_select(hipStream_t sm, const int* active, int nact, const DevBlock* 
                    const T* x1, T* h0, const T* h1) {
   if (nact == 0) return;
   hipLaunchKernelGGL(select_kernel<T>, dim3(nact), dim3(256), 0, sm, active, blk, st, x0, x1, h0, h1);
+}
+
+__global__ void add_used_kernel(SolveState* outer, const SolveState* inner, int ncase) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncase) outer[c].used += inner[c].count;
+}
+
+void launch_add_used(hipStream_t sm, SolveState* outer, const SolveState* inner, int ncase) {
+  if (ncase > 0) hipLaunchKernelGGL(add_used_kernel, dim3((ncase + 63) / 64), dim3(64), 0, sm, outer, inner, ncase);
 }
 
 void launch_solve_init(hipStream_t sm, SolveState* st, int ncase, double omega, double eps, int countMax,

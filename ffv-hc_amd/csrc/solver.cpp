@@ -709,7 +709,7 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
     const std::vector<int> ids5 = selected_cases(c);
     std::vector<int> outers(c->cases.size(), -1);
     launch_solve_init(c->sm, c->dst2, (int)c->cases.size(), omega, eps, 100000);
-    int o5 = 0, total = 0;
+    int o5 = 0;
     bool done = false;
     auto defect = [&]() -> int {
       const FieldRec X = c->fields[x];
@@ -725,17 +725,21 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
     rc = defect();
     if (rc) return rc;
     std::vector<int> its(c->cases.size(), 0);
-    while (!done && total < countMax) {
-      rc = bicgstab_core<double>(c, x, K64, b64, bcg, omega, countMax - total, eps, c->dst, c->hst, c->dst2,
-                                 nullptr, o);
+    // another round while a case is above eps and has budget left (a case's rounds depend on it alone)
+    auto more = [&]() {
+      for (int id : ids5)
+        if (!c->hst2[id].done && its[id] < countMax) return true;
+      return false;
+    };
+    while (more()) {
+      // each case's budget: countMax less its own iterations so far (dst2 used)
+      rc = bicgstab_core<double>(c, x, K64, b64, bcg, omega, countMax, eps, c->dst, c->hst, c->dst2, nullptr, o);
       if (rc) return rc;
+      launch_add_used(c->sm, c->dst2, c->dst, (int)c->cases.size());
       bool inner_done;
       rc = read_state(c, c->dst, c->hst, ids5, &inner_done);
       if (rc) return rc;
-      for (int id : ids5) {
-        its[id] += c->hst[id].count;
-        total = std::max(total, its[id]);
-      }
+      for (int id : ids5) its[id] += c->hst[id].count;
       o5++;
       rc = defect();
       if (rc) return rc;
@@ -976,7 +980,7 @@ int ffvhc_step(ffvhc_ctx* c, ffvhc_step_fields* F, const ffvhc_step_params* P, f
   if (P->solver < 0 || P->solver > 3)
     return fail(-1, "solver: 0 RBGS, 1 RBGS_WithFixedAverage, 2 Jacobi-PCG, 3 Jacobi-BiCGSTAB");
   if (P->solver >= 2 && P->countPre != 1) return fail(-1, "countPre must be 1");
-  if (P->mixed && P->solver == 1) return fail(-1, "mixed precision takes RBGS, PCG or BiCGSTAB");
+  if (P->mixed && P->mixed_method < 0 && P->solver == 1) return fail(-1, "mixed precision takes RBGS, PCG or BiCGSTAB");
   if (P->u_implicit && P->countMax_u < 0) return fail(-1, "countMax_u must be >= 0");
   int rc = refresh_active(c);
   if (rc) return rc;
@@ -1066,7 +1070,7 @@ int ffvhc_step(ffvhc_ctx* c, ffvhc_step_fields* F, const ffvhc_step_params* P, f
   std::vector<double> res(nc, 0.0);
   int st = 0;
   if (P->mixed) {
-    const int method = P->solver == 0 ? 0 : (P->solver == 2 ? 1 : 2);
+    const int method = P->mixed_method >= 0 ? P->mixed_method : P->solver == 0 ? 0 : (P->solver == 2 ? 1 : 2);
     st = ffvhc_solve_mixed(c, method, F->p, F->A, F->b, BC_P, P->omega, P->countMax, P->eps, P->inner_eps,
                            cnt.data(), res.data(), outer.data());
   } else if (P->solver == 0) {

@@ -57,7 +57,7 @@ class StepParams(ctypes.Structure):
     _fields_ = [("solver", c_int), ("mixed", c_int), ("omega", c_double), ("countMax", c_int), ("eps", c_double),
                 ("countPre", c_int), ("inner_eps", c_double), ("ref_active", c_int), ("ref", c_double * 3),
                 ("ref_p", c_double), ("u_implicit", c_int), ("alpha_u", c_double), ("omega_u", c_double),
-                ("countMax_u", c_int), ("eps_u", c_double)]
+                ("countMax_u", c_int), ("eps_u", c_double), ("mixed_method", c_int)]
 
 
 class StepInfo(ctypes.Structure):
@@ -412,11 +412,14 @@ class Context:
         return F
 
     def step(self, F, solver=0, omega=1.2, count_max=100, eps=1e-8, count_pre=1, mixed=False, inner_eps=1e-5,
-             ref=None, implicit_u=False, alpha_u=0.5, omega_u=1.0, count_max_u=100, eps_u=1e-8, ncases=None):
+             ref=None, implicit_u=False, alpha_u=0.5, omega_u=1.0, count_max_u=100, eps_u=1e-8, ncases=None,
+             mixed_method=-1):
         """One Solver::Update step of the selected cases; F's uc/ucp, ud/udp swap in place.
+        mixed_method (with mixed): the solve_mixed method for P (-1: chosen by solver).
         Returns (per-case StepInfo list, stage ms [7], status)."""
         P = StepParams()
         P.solver, P.mixed, P.omega, P.countMax, P.eps = solver, int(mixed), omega, count_max, eps
+        P.mixed_method = mixed_method
         P.countPre, P.inner_eps = count_pre, inner_eps
         P.u_implicit, P.alpha_u, P.omega_u, P.countMax_u, P.eps_u = int(implicit_u), alpha_u, omega_u, count_max_u, eps_u
         if ref is not None:

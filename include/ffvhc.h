@@ -241,6 +241,9 @@ typedef struct ffvhc_step_params {
   double omega_u;
   int countMax_u;
   double eps_u;
+  /* with mixed = 1: the ffvhc_solve_mixed method for P (-1: by `solver`: RBGS 0, PCG 1,
+   * BiCGSTAB 2); omega is then that method's relaxation factor */
+  int mixed_method;
 } ffvhc_step_params;
 
 typedef struct ffvhc_step_info {
