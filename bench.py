@@ -184,15 +184,18 @@ MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64
                  "FP64 BiCGSTAB on an FP32-stored matrix, symmetric block red-black GS preconditioner",
                  "FP64 BiCGSTAB on an FP32-stored matrix, block red-black GS preconditioner, hashed shadow "
                  "residual"] + ["FP64 BiCGSTAB on an FP32-stored matrix, %d block red-black SOR sweep%s on a compact "
-                                "FP32/FP16 preconditioner matrix" % (n, "" if n == 1 else "s") for n in range(1, 7)]
+                                "FP32/FP16 preconditioner matrix" % (n, "" if n == 1 else "s") for n in range(1, 7)] + [
+                     "FP64 BiCGSTAB on an FP32-stored matrix, %d block red-black SOR sweep%s on a compact "
+                     "FP32/FP16 preconditioner matrix + coarse (one value per block) correction"
+                     % (n, "" if n == 1 else "s") for n in range(1, 7)]
 
 # schedule of `bench.py --profile` (the rocprofv3 runs): FP32-stored matrix, 5 SOR sweeps, omega 1.6
 PROFILE_SCHEDULE = (15, 0.5, 20000, 1.6)
 
 # kernel names of the timed stages as rocprofv3 reports them
-PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0, %s>", "bi_q": "bi_matvec_kernel<%s, 0, %s, false>",
-              "bi_s": "bi_precond_kernel<%s, 1, %s>", "bi_t": "bi_matvec_kernel<%s, 1, %s, false>",
-              "bi_xr": "bi_xr_kernel<%s, false>", "rbgs": "rbgs_iter_kernel<%s, 0>",
+PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0, %s>", "bi_q": "bi_matvec_kernel<%s, 0, %s, false, false>",
+              "bi_s": "bi_precond_kernel<%s, 1, %s>", "bi_t": "bi_matvec_kernel<%s, 1, %s, false, false>",
+              "bi_xr": "bi_xr_kernel<%s, false, false>", "rbgs": "rbgs_iter_kernel<%s, 0>",
               "cg_p": "cg_p_kernel<%s>", "cg_q": "cg_q_kernel<%s>", "cg_xr": "cg_xr_kernel<%s>"}
 
 
@@ -208,13 +211,16 @@ def mixed_prof_name(method, stage):
     mode = {"bi_p": 0, "bi_q": 0, "bi_s": 1, "bi_t": 1}.get(stage)
     hr = "true" if method == 10 else "false"
     if stage == "bi_xr":
-        return "bi_xr_kernel<double, %s>" % hr
+        return "bi_xr_kernel<double, %s, %s>" % (hr, "true" if method >= 17 else "false")
     if stage in ("bi_q", "bi_t"):
-        return "bi_matvec_kernel<double, %d, %s, %s>" % (mode, "float" if method >= 7 else "double", hr)
+        if method >= 17:
+            return "bi_matvec_kernel<double, %d, float, false, true>" % mode
+        return "bi_matvec_kernel<double, %d, %s, %s, false>" % (mode, "float" if method >= 7 else "double", hr)
     if method == 7:
         return "bi_precond_kernel<double, %d, float>" % mode
     if method >= 11:
-        return "bi_precond_gsk_kernel<%d, %d>" % (mode, method - 10)
+        return "bi_precond_gsk_kernel<%d, %d, %s>" % (mode, (method - 11) % 6 + 1,
+                                                      "true" if method >= 17 else "false")
     tm, tz, sym = {5: ("float", "float", "false"), 6: ("double", "double", "false"),
                    8: ("float", "double", "false"), 9: ("float", "double", "true"),
                    10: ("float", "double", "false")}[method]
@@ -250,6 +256,7 @@ def kernel_bytes_mixed(method):
     mp = {5: 4, 6: 8, 7: 4, 8: 4, 9: 4, 10: 4}.get(method, 16 / 7)  # preconditioner bands
     nb = 1 if method == 7 else 7                    # (11-13: one 16-byte record per cell)
     r0 = 0 if method == 10 else 8                   # hashed shadow residual: not stored
+    # 17-22 add 8 B per cell (the block sum needs no extra read; the corrections are per block)
     return {"bi_p": 24 + nb * mp + 16 + h,          # r, p, q in; p, p_ out; p_ ghosts
             "bi_q": 8 + h + 7 * mv + r0 + 8,        # p_ + ghosts, bands, r0 in; q_ out
             "bi_s": 16 + nb * mp + 16 + h,          # r, q_ in; s, s_ out; s_ ghosts

@@ -270,6 +270,11 @@ void ffvhc_ctx_destroy(ffvhc_ctx* c) {
   (void)hipFree(c->dst);
   (void)hipFree(c->dst2);
   (void)hipFree(c->dpart);
+  (void)hipFree(c->dcm);
+  (void)hipFree(c->dci);
+  (void)hipFree(c->dcoff);
+  (void)hipFree(c->dcsnap);
+  (void)hipFree(c->dce);
   (void)hipFree(c->dcut);
   (void)hipFree(c->dflag);
   (void)hipFree(c->dcid);

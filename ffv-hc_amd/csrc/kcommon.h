@@ -120,6 +120,22 @@ __device__ inline void converged(SolveState& s, double rr, int count) {
 }
 
 
+// Neighbour slot a face position (a, b) of face f takes its ghost value from.
+__device__ inline int ghost_source(const DevBlock& B, int slot, int f, int a, int b) {
+  switch (B.ftype[f]) {
+    case FACE_SAME:
+    case FACE_COARSER:
+      return B.nbr[f][0];
+    case FACE_FINER: {
+      int u0, u1;
+      face_ab_to_o(f, a, b, u0, u1);
+      return B.nbr[f][(u0 >= NB / 2 ? 1 : 0) + (u1 >= NB / 2 ? 2 : 0)];
+    }
+    default:
+      return slot;  // outer face: the Neumann ghost of the block's own cell
+  }
+}
+
 // Push this block's ghost-layer contributions into the next halo buffer: its own outer faces
 // (bcx.f90), the facing layer of same-level neighbours, F2C means into a coarser neighbour's
 // quarter face and C2F trilinear values into each finer neighbour (Scalar3DUpdater1 semantics).

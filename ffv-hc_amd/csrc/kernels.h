@@ -78,6 +78,11 @@ struct BiOpts {
   bool hash_r0 = false;
   const uint4* rec = nullptr;
   int sweeps = 1;
+  // coarse correction (GSC only): FP32 inverse of A_c per case at offsets coff[case], the block
+  // sums of v in part[4 * nslots + slot], corrections ce[mode * nslots + slot]
+  const float* cinv = nullptr;
+  const long long* coff = nullptr;
+  double* ce = nullptr;
 };
 
 template <class T>
@@ -101,6 +106,13 @@ void launch_add_correction(hipStream_t sm, const int* active, int nact, const De
 void launch_convert(hipStream_t sm, const int* active, int nact, float* y, const double* x);
 // Compact GS preconditioner records (16 bytes per cell: 1 / Ap in FP32, A_q / Ap in FP16)
 void launch_gs_pack(hipStream_t sm, const int* active, int nact, Coef7<double> A, uint4* rec);
+// coarse operator rows (25 doubles per slot), dense Gauss-Jordan inversion, FP32 copy, apply
+void launch_coarse_rows(hipStream_t sm, const int* active, int nact, const DevBlock* blk, Coef7<double> A,
+                        double* rows);
+void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* snap);
+void launch_to_f32(hipStream_t sm, const double* x, float* y, size_t n);
+void launch_coarse_apply(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                         const SolveState* st, const float* inv, const long long* off, const double* c, double* e);
 
 void launch_predict(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                     const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc);

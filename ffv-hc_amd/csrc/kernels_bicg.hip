@@ -284,14 +284,17 @@ void launch_gs_pack(hipStream_t sm, const int* active, int nact, Coef7<double> A
 // the first read no HBM.
 constexpr int GSK_NT = 512;
 constexpr int GSK_NK = NB * 256 / GSK_NT;  // cells of a column per thread
-template <int MODE, int NS>
+// CC: also write the block sum of v to part[4 * nslots + slot] (restriction P^T v of the coarse
+// correction).
+template <int MODE, int NS, bool CC>
 __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __restrict__ active,
                                                                const DevBlock* __restrict__ blk,
                                                                const DevCase* __restrict__ cs,
                                                                const SolveState* __restrict__ st,
                                                                const uint4* __restrict__ rec, BiVecs<double> V,
-                                                               int count) {
+                                                               int count, double* __restrict__ part, size_t nslots) {
   __shared__ double z[NC];
+  __shared__ double red[GSK_NT / 64];
   const int slot = active[blockIdx.x];
   const DevBlock& B = blk[slot];
   const int c = B.case_id;
@@ -308,6 +311,7 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
   // indexed so that they stay in registers
   double ur[GSK_NK / 2], ub[GSK_NK / 2];
   uint32_t wr[GSK_NK / 2][3], wb[GSK_NK / 2][3];
+  double vsum = 0.0;
 #pragma unroll
   for (int k0 = 0; k0 < GSK_NK; k0 += 4) {
     double rv[4], pv[4], qv[4];
@@ -332,6 +336,7 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
         v = na * qv[kk] + rv[kk];
         V.s[g] = v;
       }
+      if (CC) vsum += v;
       const double uk = v * (double)__uint_as_float(rc[kk].x);
       const int m = k >> 1;
       if ((k & 1) == par) {
@@ -346,6 +351,10 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
         wb[m][2] = rc[kk].w;
       }
     }
+  }
+  if (CC) {
+    const double sv = block_sum<GSK_NT>(vsum, red);
+    if (t == 0) part[4 * nslots + slot] = sv;
   }
   // first red half sweep: every neighbour is still zero
 #pragma unroll
@@ -394,27 +403,32 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
   push_halo<double, GSK_NT>(B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return z[zi(i, j, k)]; }, h, h);
 }
 
-template <int MODE>
+template <int MODE, bool CC>
 void launch_gsk(int sweeps, hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
-                const SolveState* st, const uint4* rec, BiVecs<double> V, int count) {
+                const SolveState* st, const uint4* rec, BiVecs<double> V, int count, double* part, size_t nslots) {
   const dim3 g(nact), b(GSK_NT);
+#define GSK(NS) \
+  hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, NS, CC>), g, b, 0, sm, active, blk, cs, st, rec, V, count, part, nslots)
   switch (sweeps) {
-    case 1: hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 1>), g, b, 0, sm, active, blk, cs, st, rec, V, count); break;
-    case 2: hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 2>), g, b, 0, sm, active, blk, cs, st, rec, V, count); break;
-    case 3: hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 3>), g, b, 0, sm, active, blk, cs, st, rec, V, count); break;
-    case 4: hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 4>), g, b, 0, sm, active, blk, cs, st, rec, V, count); break;
-    case 5: hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 5>), g, b, 0, sm, active, blk, cs, st, rec, V, count); break;
-    default: hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, 6>), g, b, 0, sm, active, blk, cs, st, rec, V, count);
+    case 1: GSK(1); break;
+    case 2: GSK(2); break;
+    case 3: GSK(3); break;
+    case 4: GSK(4); break;
+    case 5: GSK(5); break;
+    default: GSK(6);
   }
+#undef GSK
 }
 
 // Q: q_ = A p_, q_.r0 -> alpha (mode 0);  T: t_ = A s_, t_.s and t_.t_ -> gamma (mode 1)
-template <class T, int MODE, class TA = T, bool HR = false>
+// CC: the staged p_ / s_ (interior and ghosts) get the coarse correction ce[MODE * nslots + the
+// source block] added (two-level preconditioners).
+template <class T, int MODE, class TA = T, bool HR = false, bool CC = false>
 __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict__ active,
                                                           const DevBlock* __restrict__ blk,
                                                           const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                           Coef7<TA> A, BiVecs<T> V, double* __restrict__ part,
-                                                          size_t nslots) {
+                                                          size_t nslots, const double* __restrict__ ce = nullptr) {
   __shared__ T s[LP * LP * LP];
   __shared__ double red[BI_NT / 64];
   const int slot = active[blockIdx.x];
@@ -439,6 +453,20 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
     }
     if (k0 == 0) {
       stage_block<T, BI_NT>(s, MODE == 0 ? V.pp : V.ss, MODE == 0 ? V.hpp : V.hss, slot);
+      if constexpr (CC) {  // each thread corrects the LDS values it staged
+        const DevBlock& Bk = blk[slot];
+        const double* e = ce + MODE * nslots;
+        const T eo = (T)e[slot];
+#pragma unroll
+        for (int k = 0; k < NB; k++) s[lds(tx, ty, k)] += eo;
+#pragma unroll
+        for (int q = 0; q < NH / BI_NT; q++) {
+          const int w = t + q * BI_NT, f = w >> 8, pa = w & 15, pb = (w >> 4) & 15;
+          int i, j, k;
+          face_cell(f, pa, pb, -1, i, j, k);
+          s[lds(i, j, k)] += (T)e[ghost_source(Bk, slot, f, pa, pb)];
+        }
+      }
       __syncthreads();
     }
 #pragma unroll
@@ -461,11 +489,13 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
   }
 }
 
-template <class T, bool HR = false>
+// CC: p_ and s_ carry the coarse corrections ce[slot] / ce[nslots + slot] (two-level
+// preconditioners), added as the matvecs added them.
+template <class T, bool HR = false, bool CC = false>
 __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
                                                       const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                       T* __restrict__ x, BiVecs<T> V, double* __restrict__ part,
-                                                      size_t nslots, int count) {
+                                                      size_t nslots, int count, const double* __restrict__ ce = nullptr) {
   __shared__ double red[BI_NT / 64];
   const int slot = active[blockIdx.x];
   const int c = blk[slot].case_id;
@@ -484,6 +514,10 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
       pv[kk] = V.pp[g];
       sv_[kk] = V.ss[g];
+      if constexpr (CC) {
+        pv[kk] += (T)ce[slot];
+        sv_[kk] += (T)ce[nslots + slot];
+      }
       xv[kk] = x[g];
       tv[kk] = V.t[g];
       sv[kk] = V.s[g];
@@ -596,8 +630,11 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   BiVecs<double>& Vd = *reinterpret_cast<BiVecs<double>*>(&V);
   auto precond = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
-    if (dbl && o.gs == BiOpts::GSC)
-      launch_gsk<MODE>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count);
+    if (dbl && o.gs == BiOpts::GSC && o.cinv) {
+      launch_gsk<MODE, true>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count, part, nslots);
+      launch_coarse_apply(sm, active, nact, blk, cs, st, o.cinv, o.coff, part + 4 * nslots, o.ce + MODE * nslots);
+    } else if (dbl && o.gs == BiOpts::GSC)
+      launch_gsk<MODE, false>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count, part, nslots);
     else if (dbl && o.gs == BiOpts::GS32)
       hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
                          *o.M32, Vd, count);
@@ -619,7 +656,10 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   };
   auto matvec = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
-    if (dbl && o.A32 && o.hash_r0)
+    if (dbl && o.A32 && o.cinv)
+      hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, false, true>), dim3(nact), dim3(BI_NT), 0, sm, active,
+                         blk, cs, st, *o.A32, Vd, part, nslots, (const double*)o.ce);
+    else if (dbl && o.A32 && o.hash_r0)
       hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk,
                          cs, st, *o.A32, Vd, part, nslots);
     else if (dbl && o.A32)
@@ -642,7 +682,10 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_GAMMA, count);
   }
   if (mask & 16) {
-    if (o.hash_r0)
+    if (o.cinv)
+      hipLaunchKernelGGL((bi_xr_kernel<T, false, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V,
+                         part, nslots, count, (const double*)o.ce);
+    else if (o.hash_r0)
       hipLaunchKernelGGL((bi_xr_kernel<T, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
                          nslots, count);
     else

@@ -81,6 +81,15 @@ struct ffvhc_ctx {
   // internally allocated scratch fields, keyed by precision * 100 + role
   std::vector<std::pair<int, int>> scratch_ids;
   double* dpart = nullptr;  // 6 * cap per-block partials
+  // coarse correction of the two-level preconditioners: dense A_c per case (inverted in place),
+  // its FP32 copy, per-case offsets, Gauss-Jordan snapshots, corrections e_p / e_s per slot
+  double* dcm = nullptr;
+  size_t dcm_n = 0;
+  float* dci = nullptr;
+  long long* dcoff = nullptr;
+  double* dcsnap = nullptr;
+  size_t dcsnap_n = 0;
+  double* dce = nullptr;
   std::vector<ffvhc::FieldRec> fields;
   long long bytes = 0;
   // free slot intervals [begin, end)

@@ -336,6 +336,88 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
   return 0;
 }
 
+// Coarse correction of the two-level methods: A_c = P^T A P per selected case (rows from the
+// device, assembled densely here), inverted on the device (Gauss-Jordan, FP64), kept as FP32.
+// A case whose A_c has zero row sums throughout (no Dirichlet face: A_c singular) has its last
+// coarse unknown decoupled, which keeps the coarse correction a fixed linear map.
+int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o) {
+  int fr;
+  int rc = scratch_field(c, 8, R_REC, &fr);
+  if (rc) return rc;
+  double* drows = (double*)c->fields[fr].d[0];  // 25 doubles per slot (cap * NC doubles available)
+  launch_coarse_rows(c->sm, c->dactive, c->nactive, c->dblk, K, drows);
+  HIPCHK(hipGetLastError());
+  std::vector<double> rows((size_t)c->cap * 25);
+  HIPCHK(hipMemcpyAsync(rows.data(), drows, rows.size() * sizeof(double), hipMemcpyDeviceToHost, c->sm));
+  HIPCHK(hipStreamSynchronize(c->sm));
+  const std::vector<int> ids = selected_cases(c);
+  std::vector<long long> off(c->cases.size(), 0);
+  size_t total = 0, maxn = 0;
+  for (int id : ids) {
+    off[id] = (long long)total;
+    const size_t n = (size_t)c->cases[id].nblk;
+    total += n * n;
+    maxn = std::max(maxn, n);
+  }
+  if (total > c->dcm_n) {
+    (void)hipFree(c->dcm);
+    (void)hipFree(c->dci);
+    c->dcm = nullptr;
+    c->dci = nullptr;
+    HIPCHK(hipMalloc(&c->dcm, total * sizeof(double)));
+    HIPCHK(hipMalloc(&c->dci, total * sizeof(float)));
+    c->dcm_n = total;
+  }
+  if (4 * maxn > c->dcsnap_n) {
+    (void)hipFree(c->dcsnap);
+    c->dcsnap = nullptr;
+    HIPCHK(hipMalloc(&c->dcsnap, 4 * maxn * sizeof(double)));
+    c->dcsnap_n = 4 * maxn;
+  }
+  if (!c->dcoff) HIPCHK(hipMalloc(&c->dcoff, MAX_CASES * sizeof(long long)));
+  if (!c->dce) HIPCHK(hipMalloc(&c->dce, 2 * (size_t)c->cap * sizeof(double)));
+  std::vector<double> M(total, 0.0);
+  for (int id : ids) {
+    const CaseRec& cr = c->cases[id];
+    const int n = cr.nblk, b0 = cr.slot_begin;
+    double* m = M.data() + off[id];
+    double dmax = 0.0, smax = 0.0;
+    for (int a = 0; a < n; a++) {
+      const int slot = b0 + a;
+      const DevBlock& B = c->hblk[slot];
+      m[(size_t)a * n + a] += rows[(size_t)slot * 25];
+      for (int f = 0; f < 6; f++) {
+        if (B.ftype[f] == FACE_OUTER) continue;
+        const int ns = B.ftype[f] == FACE_FINER ? 4 : 1;
+        for (int s = 0; s < ns; s++) m[(size_t)a * n + (B.nbr[f][s] - b0)] += rows[(size_t)slot * 25 + 1 + 4 * f + s];
+      }
+    }
+    for (int a = 0; a < n; a++) {
+      double rs = 0.0;
+      for (int b = 0; b < n; b++) rs += m[(size_t)a * n + b];
+      dmax = std::max(dmax, std::fabs(m[(size_t)a * n + a]));
+      smax = std::max(smax, std::fabs(rs));
+    }
+    if (smax <= 1e-12 * dmax) {  // singular (all-Neumann): decouple the last coarse unknown
+      for (int b = 0; b < n - 1; b++) {
+        m[(size_t)(n - 1) * n + b] = 0.0;
+        m[(size_t)b * n + (n - 1)] = 0.0;
+      }
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->dcm, M.data(), total * sizeof(double), hipMemcpyHostToDevice, c->sm));
+  HIPCHK(hipMemcpyAsync(c->dcoff, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice, c->sm));
+  for (int id : ids) gauss_jordan_inverse(c->sm, c->dcm + off[id], c->cases[id].nblk, c->dcsnap);
+  launch_to_f32(c->sm, c->dcm, c->dci, total);
+  HIPCHK(hipGetLastError());
+  // the host vectors above must outlive the asynchronous copies
+  HIPCHK(hipStreamSynchronize(c->sm));
+  o->cinv = c->dci;
+  o->coff = c->dcoff;
+  o->ce = c->dce;
+  return 0;
+}
+
 // Storage options of the FP64-recurrence BiCGSTAB methods 5-9: an FP32 copy of the FP64 matrix A
 // (scratch fields, converted here) in *M32 and/or the FP64 matrix in *M64, referenced by *o.
 int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Coef7<double>* M64, BiOpts* o) {
@@ -365,15 +447,17 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Co
     o->gs = method == 9 ? BiOpts::SGS32_ARITH64 : BiOpts::GS32_ARITH64;
     o->M32 = M32;
     o->hash_r0 = method == 10;
-  } else {  // 11-16: FP32 matrix for the matvecs, 1-6 SOR sweeps on the compact records
+  } else {  // 11-16: FP32 matrix for the matvecs, 1-6 SOR sweeps on the compact records;
+            // 17-22: the same plus the coarse correction
     int fr;
     const int rc = scratch_field(c, 16, R_REC, &fr);
     if (rc) return rc;
     o->A32 = M32;
     o->gs = BiOpts::GSC;
     o->rec = (const uint4*)c->fields[fr].d[0];
-    o->sweeps = method - 10;
+    o->sweeps = (method - 11) % 6 + 1;
     launch_gs_pack(c->sm, c->dactive, c->nactive, *M64, (uint4*)c->fields[fr].d[0]);
+    if (method >= 17) return coarse_setup(c, *M64, o);
   }
   return 0;
 }
@@ -626,7 +710,7 @@ int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* c, int method, int x, const int* A, i
   int rc = prologue(c, x, A, b, BC_NULL, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "methods 5-9 take FP64 x, A, b");
-  if (method < 5 || method > 16) return fail(-1, "method must be 5-16");
+  if (method < 5 || method > 22) return fail(-1, "method must be 5-22");
   if (niter < 1 || !ms6) return fail(-1, "niter >= 1 and ms[6] required");
   Coef7<float> M32;
   Coef7<double> M64;
@@ -669,13 +753,14 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
-  if (method < 0 || method > 16)
+  if (method < 0 || method > 22)
     return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB, 3 Jacobi-BiCGSTAB with reliable updates, "
                     "4 the same with a seeded shadow residual, 5 FP64 BiCGSTAB with an FP32 red-black GS "
                     "preconditioner, 6 the same all in FP64 (reference point), 7 FP64 Jacobi-BiCGSTAB with an FP32 "
                     "matrix, 8 as 7 with the block red-black GS preconditioner (FP64 arithmetic), 9 as 8 with the "
                     "symmetric red-black-red sweep, 10 as 8 with the seeded hash vector as shadow residual, "
-                    "11-16 as 8 with 1-6 red-black SOR sweeps on a compact FP32/FP16 copy of the matrix");
+                    "11-16 as 8 with 1-6 red-black SOR sweeps on a compact FP32/FP16 copy of the matrix, "
+                    "17-22 as 11-16 plus a coarse (one value per block) correction");
   if (method == 6) {
     Coef7<float> M32;
     Coef7<double> M64;

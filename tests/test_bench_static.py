@@ -40,7 +40,7 @@ def test_unresolved_name_is_caught():
 def test_mixed_byte_model_and_names():
     sys.path[:0] = [ROOT, os.path.join(ROOT, "ffv-hc_amd")]
     import bench
-    for m in range(5, 17):
+    for m in range(5, 23):
         by = bench.kernel_bytes_mixed(m)
         assert set(by) == {"bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"}
         # q_ and t_ differ by r0 (hashed in 10); p_ and s_ by the p read
@@ -50,10 +50,12 @@ def test_mixed_byte_model_and_names():
     # method 8: FP64 vectors, FP32 bands: 7 bands x 4 bytes in the GS and matvec kernels
     assert bench.kernel_bytes_mixed(8)["bi_q"] == 8 + 3 + 28 + 16
     assert bench.mixed_prof_name(8, "bi_p") == "bi_precond_gs_kernel<0, float, double, false>"
-    assert bench.mixed_prof_name(7, "bi_t") == "bi_matvec_kernel<double, 1, float, false>"
-    assert bench.prof_name("bi_q", 4) == "bi_matvec_kernel<float, 0, float, false>"
-    assert bench.mixed_prof_name(10, "bi_xr") == "bi_xr_kernel<double, true>"
+    assert bench.mixed_prof_name(7, "bi_t") == "bi_matvec_kernel<double, 1, float, false, false>"
+    assert bench.prof_name("bi_q", 4) == "bi_matvec_kernel<float, 0, float, false, false>"
+    assert bench.mixed_prof_name(10, "bi_xr") == "bi_xr_kernel<double, true, false>"
     assert bench.kernel_bytes_mixed(10)["bi_xr"] == 56
     assert abs(bench.kernel_bytes_mixed(12)["bi_p"] - (24 + 16 + 16 + 3)) < 1e-9
-    assert bench.mixed_prof_name(13, "bi_s") == "bi_precond_gsk_kernel<1, 3>"
-    assert len(bench.MIXED_METHODS) == 17
+    assert bench.mixed_prof_name(13, "bi_s") == "bi_precond_gsk_kernel<1, 3, false>"
+    assert bench.mixed_prof_name(21, "bi_p") == "bi_precond_gsk_kernel<0, 5, true>"
+    assert bench.mixed_prof_name(21, "bi_q") == "bi_matvec_kernel<double, 0, float, false, true>"
+    assert len(bench.MIXED_METHODS) == 23

@@ -315,7 +315,7 @@ def true_residual(og, A, x, b):
     return np.sqrt(rr / bb)
 
 
-@pytest.mark.parametrize("method", [0, 1, 3, 5, 6, 7, 8, 9, 10, 11, 12, 13, 16])
+@pytest.mark.parametrize("method", [0, 1, 3, 5, 6, 7, 8, 9, 10, 11, 12, 13, 16, 17, 19, 21])
 def test_mixed_precision_reaches_fp64_residual(ctx, method):
     tree = trees()["two_level"]
     params = ffvhc.case_params(bc=BC)
