@@ -62,41 +62,100 @@ void launch_coarse_rows(hipStream_t sm, const int* active, int nact, const DevBl
   if (nact) hipLaunchKernelGGL(coarse_rows_kernel, dim3(nact), dim3(CR_NT), 0, sm, active, blk, A, rows);
 }
 
-// In-place Gauss-Jordan inversion of a dense n x n matrix (row-major), one launch per pivot k.
-// snap holds the pivot row and column of step k in buffer (k & 1): each launch reads only the
-// snapshot of its step and its own element, and the threads owning row / column k + 1 write the
-// next snapshot, so no element is read after another thread of the launch wrote it.
-//   snap layout per buffer: row[n], col[n]
-__global__ void gj_snap_kernel(const double* __restrict__ M, int n, double* __restrict__ snap) {
+// In-place blocked Gauss-Jordan inversion of a dense n x n matrix (row-major, no pivoting: the
+// coarse operators are diagonally dominant).  For each panel P of GJB pivots, with D = M_PP:
+//   M_PP <- D^-1,  M_Pj <- D^-1 M_Pj,  M_iP <- -M_iP D^-1,  M_ij <- M_ij - M_iP D^-1 M_Pj
+// (i, j outside P) -- the scalar Gauss-Jordan step applied to a block of pivots.  Five launches
+// per panel; every launch reads only values no other thread of it writes.
+constexpr int GJB = 32;
+
+// D <- D^-1 in LDS by unblocked Gauss-Jordan; one workgroup of GJB * GJB threads
+__global__ __launch_bounds__(GJB * GJB) void gjb_diag_kernel(double* __restrict__ M, int n, int k0, int kb) {
+  __shared__ double d[GJB][GJB + 1];
+  const int r = threadIdx.x / GJB, c = threadIdx.x % GJB;
+  const bool in = r < kb && c < kb;
+  if (in) d[r][c] = M[(size_t)(k0 + r) * n + k0 + c];
+  __syncthreads();
+  for (int k = 0; k < kb; k++) {
+    double v = 0.0;
+    if (in) {
+      const double p = d[k][k], rk = d[k][c], ck = d[r][k];
+      if (r == k && c == k) v = 1.0 / p;
+      else if (r == k) v = rk / p;
+      else if (c == k) v = -ck / p;
+      else v = d[r][c] - ck * rk / p;
+    }
+    __syncthreads();
+    if (in) d[r][c] = v;
+    __syncthreads();
+  }
+  if (in) M[(size_t)(k0 + r) * n + k0 + c] = d[r][c];
+}
+
+// T[r][j] = (D^-1 M_P)[r][j] for the columns j outside the panel
+__global__ __launch_bounds__(256) void gjb_row_kernel(const double* __restrict__ M, int n, int k0, int kb,
+                                                      double* __restrict__ T) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (j >= n || (j >= k0 && j < k0 + kb)) return;
+  const double* dinv = M + (size_t)(k0 + r) * n + k0;
+  double s = 0.0;
+  for (int c = 0; c < kb; c++) s += dinv[c] * M[(size_t)(k0 + c) * n + j];
+  T[(size_t)r * n + j] = s;
+}
+
+// M_ij -= sum_c M_i,k0+c T[c][j] for i, j outside the panel; 16 x 16 tiles staged in LDS
+__global__ __launch_bounds__(256) void gjb_update_kernel(double* __restrict__ M, int n, int k0, int kb,
+                                                         const double* __restrict__ T) {
+  __shared__ double a[16][GJB + 1], t[GJB][17];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int i = blockIdx.y * 16 + ty, j = blockIdx.x * 16 + tx;
+  for (int q = threadIdx.x; q < 16 * GJB; q += 256) {
+    const int rr = q / GJB, cc = q % GJB, ii = blockIdx.y * 16 + rr;
+    a[rr][cc] = (ii < n && cc < kb) ? M[(size_t)ii * n + k0 + cc] : 0.0;
+    const int cr = q / 16, jc = q % 16, jj = blockIdx.x * 16 + jc;
+    t[cr][jc] = (jj < n && cr < kb) ? T[(size_t)cr * n + jj] : 0.0;
+  }
+  __syncthreads();
+  if (i >= n || j >= n || (i >= k0 && i < k0 + kb) || (j >= k0 && j < k0 + kb)) return;
+  double s = 0.0;
+#pragma unroll 8
+  for (int c = 0; c < GJB; c++) s += a[ty][c] * t[c][tx];
+  M[(size_t)i * n + j] -= s;
+}
+
+// M_iP <- -M_iP D^-1 for the rows outside the panel (one thread per row owns its panel segment)
+__global__ __launch_bounds__(256) void gjb_col_kernel(double* __restrict__ M, int n, int k0, int kb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  snap[i] = M[i];                // row 0
-  snap[n + i] = M[(size_t)i * n];  // column 0
+  if (i >= n || (i >= k0 && i < k0 + kb)) return;
+  double* row = M + (size_t)i * n + k0;
+  double a[GJB];
+  for (int c = 0; c < GJB; c++) a[c] = c < kb ? row[c] : 0.0;
+  for (int c2 = 0; c2 < kb; c2++) {
+    double s = 0.0;
+    for (int c = 0; c < kb; c++) s += a[c] * M[(size_t)(k0 + c) * n + k0 + c2];
+    row[c2] = -s;
+  }
 }
 
-__global__ void gj_step_kernel(double* __restrict__ M, int n, int k, double* __restrict__ snap) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)n * n) return;
-  const int i = (int)(e / n), j = (int)(e % n);
-  const double* cur = snap + (size_t)(k & 1) * 2 * n;
-  double* nxt = snap + (size_t)((k + 1) & 1) * 2 * n;
-  const double p = cur[k];  // row[k] = M[k][k]
-  double v;
-  if (i == k && j == k) v = 1.0 / p;
-  else if (i == k) v = cur[j] / p;
-  else if (j == k) v = -cur[n + i] / p;
-  else v = M[e] - cur[n + i] * cur[j] / p;
-  M[e] = v;
-  if (i == k + 1) nxt[j] = v;
-  if (j == k + 1) nxt[n + i] = v;
+// panel rows <- T outside the panel columns
+__global__ __launch_bounds__(256) void gjb_copy_kernel(double* __restrict__ M, int n, int k0, int kb,
+                                                       const double* __restrict__ T) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (j >= n || (j >= k0 && j < k0 + kb)) return;
+  M[(size_t)(k0 + r) * n + j] = T[(size_t)r * n + j];
 }
 
-void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* snap) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(gj_snap_kernel, dim3((n + 255) / 256), dim3(256), 0, sm, (const double*)M, n, snap);
-  const size_t nn = (size_t)n * n;
-  const dim3 g((unsigned)((nn + 255) / 256));
-  for (int k = 0; k < n; k++) hipLaunchKernelGGL(gj_step_kernel, g, dim3(256), 0, sm, M, n, k, snap);
+// T: scratch of GJB * n doubles
+void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* T) {
+  for (int k0 = 0; k0 < n; k0 += GJB) {
+    const int kb = n - k0 < GJB ? n - k0 : GJB;
+    const unsigned nt = (unsigned)((n + 255) / 256), nt16 = (unsigned)((n + 15) / 16);
+    hipLaunchKernelGGL(gjb_diag_kernel, dim3(1), dim3(GJB * GJB), 0, sm, M, n, k0, kb);
+    hipLaunchKernelGGL(gjb_row_kernel, dim3(nt, kb), dim3(256), 0, sm, (const double*)M, n, k0, kb, T);
+    hipLaunchKernelGGL(gjb_update_kernel, dim3(nt16, nt16), dim3(256), 0, sm, M, n, k0, kb, (const double*)T);
+    hipLaunchKernelGGL(gjb_col_kernel, dim3(nt), dim3(256), 0, sm, M, n, k0, kb);
+    hipLaunchKernelGGL(gjb_copy_kernel, dim3(nt, kb), dim3(256), 0, sm, M, n, k0, kb, (const double*)T);
+  }
 }
 
 __global__ void to_f32_kernel(const double* __restrict__ x, float* __restrict__ y, size_t n) {
@@ -128,7 +187,9 @@ __global__ __launch_bounds__(256) void coarse_apply_kernel(const int* __restrict
   const int n = C.nblk, a = slot - C.slot_begin;
   const float* row = inv + off[cid] + (size_t)a * n;
   double s = 0.0;
-  for (int b = lane; b < n; b += 64) s += (double)row[b] * c[C.slot_begin + b];
+  const double* cc = c + C.slot_begin;
+#pragma unroll 8
+  for (int b = lane; b < n; b += 64) s += (double)row[b] * cc[b];
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if (lane == 0) e[slot] = s;
 }

@@ -82,7 +82,7 @@ struct ffvhc_ctx {
   std::vector<std::pair<int, int>> scratch_ids;
   double* dpart = nullptr;  // 6 * cap per-block partials
   // coarse correction of the two-level preconditioners: dense A_c per case (inverted in place),
-  // its FP32 copy, per-case offsets, Gauss-Jordan snapshots, corrections e_p / e_s per slot
+  // its FP32 copy, per-case offsets, Gauss-Jordan panel scratch, corrections e_p / e_s per slot
   double* dcm = nullptr;
   size_t dcm_n = 0;
   float* dci = nullptr;

@@ -340,7 +340,8 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
 // device, assembled densely here), inverted on the device (Gauss-Jordan, FP64), kept as FP32.
 // A case whose A_c has zero row sums throughout (no Dirichlet face: A_c singular) has its last
 // coarse unknown decoupled, which keeps the coarse correction a fixed linear map.
-int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o) {
+int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o, double* ac_out = nullptr,
+                 double* inv_out = nullptr) {
   int fr;
   int rc = scratch_field(c, 8, R_REC, &fr);
   if (rc) return rc;
@@ -368,11 +369,11 @@ int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o) {
     HIPCHK(hipMalloc(&c->dci, total * sizeof(float)));
     c->dcm_n = total;
   }
-  if (4 * maxn > c->dcsnap_n) {
+  if (32 * maxn > c->dcsnap_n) {  // panel rows of the blocked Gauss-Jordan
     (void)hipFree(c->dcsnap);
     c->dcsnap = nullptr;
-    HIPCHK(hipMalloc(&c->dcsnap, 4 * maxn * sizeof(double)));
-    c->dcsnap_n = 4 * maxn;
+    HIPCHK(hipMalloc(&c->dcsnap, 32 * maxn * sizeof(double)));
+    c->dcsnap_n = 32 * maxn;
   }
   if (!c->dcoff) HIPCHK(hipMalloc(&c->dcoff, MAX_CASES * sizeof(long long)));
   if (!c->dce) HIPCHK(hipMalloc(&c->dce, 2 * (size_t)c->cap * sizeof(double)));
@@ -405,11 +406,16 @@ int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o) {
       }
     }
   }
+  if (ac_out && !ids.empty()) std::copy(M.begin(), M.begin() + (size_t)c->cases[ids[0]].nblk * c->cases[ids[0]].nblk, ac_out);
   HIPCHK(hipMemcpyAsync(c->dcm, M.data(), total * sizeof(double), hipMemcpyHostToDevice, c->sm));
   HIPCHK(hipMemcpyAsync(c->dcoff, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice, c->sm));
   for (int id : ids) gauss_jordan_inverse(c->sm, c->dcm + off[id], c->cases[id].nblk, c->dcsnap);
   launch_to_f32(c->sm, c->dcm, c->dci, total);
   HIPCHK(hipGetLastError());
+  if (inv_out && !ids.empty())
+    HIPCHK(hipMemcpyAsync(inv_out, c->dcm + off[ids[0]],
+                          (size_t)c->cases[ids[0]].nblk * c->cases[ids[0]].nblk * sizeof(double),
+                          hipMemcpyDeviceToHost, c->sm));
   // the host vectors above must outlive the asynchronous copies
   HIPCHK(hipStreamSynchronize(c->sm));
   o->cinv = c->dci;
@@ -718,6 +724,21 @@ int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* c, int method, int x, const int* A, i
   rc = fp64_bicg_opts(c, method, A, &M32, &M64, &o);
   if (rc) return rc;
   return bicg_timing_run<double>(c, x, A, b, omega, niter, ms6, o);
+}
+
+int ffvhc_coarse_operator(ffvhc_ctx* c, const int* A, int case_id, double* ac, double* ainv) {
+  if (!c || !A || !ac || !ainv) return fail(-1, "null argument");
+  for (int q = 0; q < 7; q++)
+    if (!field_ok(c, A[q], 8)) return fail(-1, "A bands must be FP64 fields");
+  if (case_id < 0 || case_id >= (int)c->cases.size() || !c->cases[case_id].used) return fail(-4, "no such case");
+  const std::vector<int> saved = c->sel;
+  int rc = ffvhc_ctx_select(c, &case_id, 1);
+  if (rc) return rc;
+  rc = refresh_active(c);
+  BiOpts o;
+  if (!rc) rc = coarse_setup(c, coef<double>(c, A), &o, ac, ainv);
+  const int rc2 = ffvhc_ctx_select(c, saved.empty() ? nullptr : saved.data(), (int)saved.size());
+  return rc ? rc : rc2;
 }
 
 int ffvhc_bicgstab(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countPre, int countMax,

@@ -97,6 +97,7 @@ _sig = {
     "ffvhc_bicgstab": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_int, c_double, _vp, _vp]),
     "ffvhc_bicgstab_timing": (c_int, [_vp, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_bicgstab_timing_mixed": (c_int, [_vp, c_int, c_int, _vp, c_int, c_double, c_int, _vp]),
+    "ffvhc_coarse_operator": (c_int, [_vp, _vp, c_int, _vp, _vp]),
     "ffvhc_cg_timing": (c_int, [_vp, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_jacobi": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
     "ffvhc_step": (c_int, [_vp, _P(StepFields), _P(StepParams), _vp, _vp]),
@@ -439,6 +440,14 @@ class Context:
         f = np.zeros((ncases or self._nsel(), 3))
         check(_L.ffvhc_force_p(self.h, p, cut_id, _ptr(f)), "ffvhc_force_p")
         return f
+
+    def coarse_operator(self, A, case_id):
+        """(A_c, its device inverse) of the two-level preconditioner for one case."""
+        n = self.cases[case_id].nblk
+        ac, ainv = np.zeros((n, n)), np.zeros((n, n))
+        a = np.asarray(A, np.int32)
+        check(_L.ffvhc_coarse_operator(self.h, _ptr(a), case_id, _ptr(ac), _ptr(ainv)), "ffvhc_coarse_operator")
+        return ac, ainv
 
     def force_v(self, ux, uy, uz, cut_id, ncases=None):
         """Viscous force (fx, fy, fz) on the faces with boundary id cut_id, per selected case."""

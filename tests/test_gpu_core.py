@@ -356,6 +356,46 @@ def test_mixed_precision_reaches_fp64_residual(ctx, method):
         ctx.remove_case(cid)
 
 
+@pytest.mark.parametrize("name", ["two_level", "three_level"])
+def test_coarse_operator_is_ptap(ctx, name):
+    """The two-level preconditioner's coarse operator equals P^T A P by its definition (A applied
+    by the oracle to each block's indicator, ghosts by the oracle's Updater1 with the preconditioned
+    vectors' Neumann-0 outer BC, block sums), and the device's blocked Gauss-Jordan inverts it."""
+    tree = trees()[name]
+    params = ffvhc.case_params(bc=BC)
+    cid = ctx.add_case(tree, params)
+    try:
+        og = OracleGrid(tree)
+        rng = np.random.default_rng(31)
+        bct = [params.bct[ffvhc.BC_P][q] for q in range(6)]
+        bcv = [params.bcv[ffvhc.BC_P][q] for q in range(6)]
+        A, b = make_system(og, rng, bct, bcv)
+        fa, fb, fx = upload_system(ctx, cid, A, b, og.zeros())
+        ac, ainv = ctx.coarse_operator(fa, cid)
+        n = og.nb
+        inner = (slice(G, G + N),) * 3
+        sz = (ctypes.c_int * 3)(N, N, N)
+        want = np.zeros((n, n))
+        ax = np.zeros((N + 2 * G,) * 3)
+        Ab = [[np.ascontiguousarray(A[q][a]) for q in range(7)] for a in range(n)]
+        for blk in range(n):
+            x = og.zeros()
+            x[(blk,) + inner] = 1.0
+            og.ghost_fill(x, 1, [1] * 6, [0.0] * 6)
+            for a in range(n):
+                og.L.orc_calc_ax(O.ptr(ax), *[O.ptr(Ab[a][q]) for q in range(7)],
+                                 O.ptr(np.ascontiguousarray(x[a])), sz, G)
+                want[a, blk] = ax[inner].sum()
+        scale = np.max(np.abs(want))
+        assert np.max(np.abs(ac - want)) <= 1e-12 * scale
+        assert np.count_nonzero(np.abs(want) > 1e-12 * scale) > 2 * n  # couplings between blocks exist
+        assert np.max(np.abs(ainv @ ac - np.eye(n))) <= 1e-8
+        for f in fa + [fb, fx]:
+            ctx.free(f)
+    finally:
+        ctx.remove_case(cid)
+
+
 def run_oracle_bicgstab(og, x, A, b, bct, bcv, omega, cmax, eps):
     cnt, res = ctypes.c_int(), ctypes.c_double()
     sc = [og.zeros() for _ in range(9)]
