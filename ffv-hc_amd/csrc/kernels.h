@@ -109,7 +109,11 @@ void launch_gs_pack(hipStream_t sm, const int* active, int nact, Coef7<double> A
 // coarse operator rows (25 doubles per slot), dense Gauss-Jordan inversion, FP32 copy, apply
 void launch_coarse_rows(hipStream_t sm, const int* active, int nact, const DevBlock* blk, Coef7<double> A,
                         double* rows);
-void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* snap);
+void launch_coarse_assemble(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                            const long long* off, const double* rows, double* M, size_t total);
+void launch_coarse_decouple(hipStream_t sm, double* M, int n);
+// T: scratch of 32 * n doubles
+void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* T);
 void launch_to_f32(hipStream_t sm, const double* x, float* y, size_t n);
 void launch_coarse_apply(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                          const SolveState* st, const float* inv, const long long* off, const double* c, double* e);

@@ -62,6 +62,54 @@ void launch_coarse_rows(hipStream_t sm, const int* active, int nact, const DevBl
   if (nact) hipLaunchKernelGGL(coarse_rows_kernel, dim3(nact), dim3(CR_NT), 0, sm, active, blk, A, rows);
 }
 
+// Dense A_c of every selected case from the rows: M (zeroed before) gets, in the row of block a,
+// the self coupling on the diagonal and each face coupling in its neighbour's column.  One thread
+// per block writes its whole row (repeated neighbours add up in order).
+__global__ void coarse_scatter_kernel(const int* __restrict__ active, int nact, const DevBlock* __restrict__ blk,
+                                      const DevCase* __restrict__ cs, const long long* __restrict__ off,
+                                      const double* __restrict__ rows, double* __restrict__ M) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nact) return;
+  const int slot = active[w];
+  const DevBlock& B = blk[slot];
+  const DevCase& C = cs[B.case_id];
+  const int n = C.nblk, b0 = C.slot_begin;
+  double* row = M + off[B.case_id] + (size_t)(slot - b0) * n;
+  const double* r = rows + (size_t)slot * 25;
+  row[slot - b0] += r[0];
+  for (int f = 0; f < 6; f++) {
+    if (B.ftype[f] == FACE_OUTER) continue;
+    const int ns = B.ftype[f] == FACE_FINER ? 4 : 1;
+    for (int s = 0; s < ns; s++) row[B.nbr[f][s] - b0] += r[1 + 4 * f + s];
+  }
+}
+
+// Decouple the last coarse unknown of a singular (all-Neumann) case: zero its row and column
+// outside the diagonal.
+__global__ void coarse_decouple_kernel(double* __restrict__ M, int n) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n - 1) return;
+  M[(size_t)(n - 1) * n + b] = 0.0;
+  M[(size_t)b * n + (n - 1)] = 0.0;
+}
+
+__global__ void zero_kernel(double* __restrict__ x, size_t n) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) x[e] = 0.0;
+}
+
+void launch_coarse_assemble(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                            const long long* off, const double* rows, double* M, size_t total) {
+  if (!nact) return;
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, sm, M, total);
+  hipLaunchKernelGGL(coarse_scatter_kernel, dim3((nact + 255) / 256), dim3(256), 0, sm, active, nact, blk, cs, off,
+                     rows, M);
+}
+
+void launch_coarse_decouple(hipStream_t sm, double* M, int n) {
+  if (n > 1) hipLaunchKernelGGL(coarse_decouple_kernel, dim3((n + 254) / 256), dim3(256), 0, sm, M, n);
+}
+
 // In-place blocked Gauss-Jordan inversion of a dense n x n matrix (row-major, no pivoting: the
 // coarse operators are diagonally dominant).  For each panel P of GJB pivots, with D = M_PP:
 //   M_PP <- D^-1,  M_Pj <- D^-1 M_Pj,  M_iP <- -M_iP D^-1,  M_ij <- M_ij - M_iP D^-1 M_Pj

@@ -377,38 +377,33 @@ int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o, double* ac_out
   }
   if (!c->dcoff) HIPCHK(hipMalloc(&c->dcoff, MAX_CASES * sizeof(long long)));
   if (!c->dce) HIPCHK(hipMalloc(&c->dce, 2 * (size_t)c->cap * sizeof(double)));
-  std::vector<double> M(total, 0.0);
+  HIPCHK(hipMemcpyAsync(c->dcoff, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice, c->sm));
+  launch_coarse_assemble(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dcoff, drows, c->dcm, total);
   for (int id : ids) {
+    // singular A_c (zero row sums throughout: no Dirichlet face) -> decouple the last unknown
     const CaseRec& cr = c->cases[id];
-    const int n = cr.nblk, b0 = cr.slot_begin;
-    double* m = M.data() + off[id];
     double dmax = 0.0, smax = 0.0;
-    for (int a = 0; a < n; a++) {
-      const int slot = b0 + a;
+    for (int a = 0; a < cr.nblk; a++) {
+      const int slot = cr.slot_begin + a;
       const DevBlock& B = c->hblk[slot];
-      m[(size_t)a * n + a] += rows[(size_t)slot * 25];
+      double rs = rows[(size_t)slot * 25];
       for (int f = 0; f < 6; f++) {
         if (B.ftype[f] == FACE_OUTER) continue;
         const int ns = B.ftype[f] == FACE_FINER ? 4 : 1;
-        for (int s = 0; s < ns; s++) m[(size_t)a * n + (B.nbr[f][s] - b0)] += rows[(size_t)slot * 25 + 1 + 4 * f + s];
+        for (int s = 0; s < ns; s++) {
+          rs += rows[(size_t)slot * 25 + 1 + 4 * f + s];
+          if (B.nbr[f][s] == slot) dmax = std::max(dmax, std::fabs(rows[(size_t)slot * 25 + 1 + 4 * f + s]));
+        }
       }
-    }
-    for (int a = 0; a < n; a++) {
-      double rs = 0.0;
-      for (int b = 0; b < n; b++) rs += m[(size_t)a * n + b];
-      dmax = std::max(dmax, std::fabs(m[(size_t)a * n + a]));
+      dmax = std::max(dmax, std::fabs(rows[(size_t)slot * 25]));
       smax = std::max(smax, std::fabs(rs));
     }
-    if (smax <= 1e-12 * dmax) {  // singular (all-Neumann): decouple the last coarse unknown
-      for (int b = 0; b < n - 1; b++) {
-        m[(size_t)(n - 1) * n + b] = 0.0;
-        m[(size_t)b * n + (n - 1)] = 0.0;
-      }
-    }
+    if (smax <= 1e-12 * dmax) launch_coarse_decouple(c->sm, c->dcm + off[id], cr.nblk);
   }
-  if (ac_out && !ids.empty()) std::copy(M.begin(), M.begin() + (size_t)c->cases[ids[0]].nblk * c->cases[ids[0]].nblk, ac_out);
-  HIPCHK(hipMemcpyAsync(c->dcm, M.data(), total * sizeof(double), hipMemcpyHostToDevice, c->sm));
-  HIPCHK(hipMemcpyAsync(c->dcoff, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice, c->sm));
+  if (ac_out && !ids.empty())
+    HIPCHK(hipMemcpyAsync(ac_out, c->dcm + off[ids[0]],
+                          (size_t)c->cases[ids[0]].nblk * c->cases[ids[0]].nblk * sizeof(double),
+                          hipMemcpyDeviceToHost, c->sm));
   for (int id : ids) gauss_jordan_inverse(c->sm, c->dcm + off[id], c->cases[id].nblk, c->dcsnap);
   launch_to_f32(c->sm, c->dcm, c->dci, total);
   HIPCHK(hipGetLastError());
