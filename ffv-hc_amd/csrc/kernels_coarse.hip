@@ -113,7 +113,7 @@ void launch_coarse_decouple(hipStream_t sm, double* M, int n) {
 // In-place blocked Gauss-Jordan inversion of a dense n x n matrix (row-major, no pivoting: the
 // coarse operators are diagonally dominant).  For each panel P of GJB pivots, with D = M_PP:
 //   M_PP <- D^-1,  M_Pj <- D^-1 M_Pj,  M_iP <- -M_iP D^-1,  M_ij <- M_ij - M_iP D^-1 M_Pj
-// (i, j outside P) -- the scalar Gauss-Jordan step applied to a block of pivots.  Five launches
+// (i, j outside P) -- the scalar Gauss-Jordan step applied to a block of pivots.  Six launches
 // per panel; every launch reads only values no other thread of it writes.
 constexpr int GJB = 32;
 
@@ -171,18 +171,24 @@ __global__ __launch_bounds__(256) void gjb_update_kernel(double* __restrict__ M,
   M[(size_t)i * n + j] -= s;
 }
 
-// M_iP <- -M_iP D^-1 for the rows outside the panel (one thread per row owns its panel segment)
-__global__ __launch_bounds__(256) void gjb_col_kernel(double* __restrict__ M, int n, int k0, int kb) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || (i >= k0 && i < k0 + kb)) return;
-  double* row = M + (size_t)i * n + k0;
-  double a[GJB];
-  for (int c = 0; c < GJB; c++) a[c] = c < kb ? row[c] : 0.0;
-  for (int c2 = 0; c2 < kb; c2++) {
-    double s = 0.0;
-    for (int c = 0; c < kb; c++) s += a[c] * M[(size_t)(k0 + c) * n + k0 + c2];
-    row[c2] = -s;
-  }
+// T[i][c2] = -(M_iP D^-1)[i][c2] for the rows outside the panel (T reused after the panel rows
+// took their values); 8 rows x GJB columns per workgroup
+__global__ __launch_bounds__(256) void gjb_col_kernel(const double* __restrict__ M, int n, int k0, int kb,
+                                                      double* __restrict__ T) {
+  const int c2 = threadIdx.x % GJB, i = blockIdx.x * (256 / GJB) + threadIdx.x / GJB;
+  if (i >= n || c2 >= kb || (i >= k0 && i < k0 + kb)) return;
+  const double* a = M + (size_t)i * n + k0;
+  double s = 0.0;
+  for (int c = 0; c < kb; c++) s += a[c] * M[(size_t)(k0 + c) * n + k0 + c2];
+  T[(size_t)i * GJB + c2] = -s;
+}
+
+// M_iP <- T[i] for the rows outside the panel
+__global__ __launch_bounds__(256) void gjb_colcopy_kernel(double* __restrict__ M, int n, int k0, int kb,
+                                                          const double* __restrict__ T) {
+  const int c2 = threadIdx.x % GJB, i = blockIdx.x * (256 / GJB) + threadIdx.x / GJB;
+  if (i >= n || c2 >= kb || (i >= k0 && i < k0 + kb)) return;
+  M[(size_t)i * n + k0 + c2] = T[(size_t)i * GJB + c2];
 }
 
 // panel rows <- T outside the panel columns
@@ -193,16 +199,19 @@ __global__ __launch_bounds__(256) void gjb_copy_kernel(double* __restrict__ M, i
   M[(size_t)(k0 + r) * n + j] = T[(size_t)r * n + j];
 }
 
-// T: scratch of GJB * n doubles
+// T: scratch of GJB * n doubles.  Order per panel: D^-1; T = D^-1 M_P; the trailing update
+// (old M_iP, T); panel rows <- T; T <- -M_iP D^-1 (old M_iP); M_iP <- T.
 void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* T) {
   for (int k0 = 0; k0 < n; k0 += GJB) {
     const int kb = n - k0 < GJB ? n - k0 : GJB;
     const unsigned nt = (unsigned)((n + 255) / 256), nt16 = (unsigned)((n + 15) / 16);
+    const unsigned nr = (unsigned)((n + 256 / GJB - 1) / (256 / GJB));
     hipLaunchKernelGGL(gjb_diag_kernel, dim3(1), dim3(GJB * GJB), 0, sm, M, n, k0, kb);
     hipLaunchKernelGGL(gjb_row_kernel, dim3(nt, kb), dim3(256), 0, sm, (const double*)M, n, k0, kb, T);
     hipLaunchKernelGGL(gjb_update_kernel, dim3(nt16, nt16), dim3(256), 0, sm, M, n, k0, kb, (const double*)T);
-    hipLaunchKernelGGL(gjb_col_kernel, dim3(nt), dim3(256), 0, sm, M, n, k0, kb);
     hipLaunchKernelGGL(gjb_copy_kernel, dim3(nt, kb), dim3(256), 0, sm, M, n, k0, kb, (const double*)T);
+    hipLaunchKernelGGL(gjb_col_kernel, dim3(nr), dim3(256), 0, sm, (const double*)M, n, k0, kb, T);
+    hipLaunchKernelGGL(gjb_colcopy_kernel, dim3(nr), dim3(256), 0, sm, M, n, k0, kb, (const double*)T);
   }
 }
 
