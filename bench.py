@@ -141,9 +141,10 @@ def cpu_baseline(tree, A, b, bct, iters_per_solve, budget_s=15.0):
                                                                            iters_per_solve)}
 
 
-def serve_leg(ncases, seed, mixed, budget_s):
+def serve_leg(ncases, seed, budget_s, method=None, omega=1.0):
     """Served stream: flow cases of uneven block counts arrive as a Poisson process and run a
-    number of time steps each under continuous batching; p50/p99 time-to-solution."""
+    number of time steps each under continuous batching; p50/p99 time-to-solution.  The pressure
+    solve is the reference's FP64 Jacobi-BiCGSTAB, or ffvhc_solve_mixed `method` with `omega`."""
     import ffvhc
     from serve import Request, Server, uniform_flow_init
     rng = np.random.default_rng(seed)
@@ -163,11 +164,15 @@ def serve_leg(ncases, seed, mixed, budget_s):
     cap = 4 * max(r.tree.nblk for r in reqs)
     ctx = ffvhc.Context(max_blocks=cap)
     F = ctx.step_fields()
-    kw = dict(solver=3, omega=1.0, count_max=2000, eps=1e-8, mixed=mixed, inner_eps=1e-2)
+    if method is None:
+        kw = dict(solver=3, omega=1.0, count_max=2000, eps=1e-8)
+    else:
+        kw = dict(solver=3, omega=omega, count_max=2000, eps=1e-8, mixed=True, mixed_method=method)
     srv = Server(ctx, F, kw)
     srv.run(reqs, t_limit=budget_s)
     rep = srv.report()
-    rep.update(solver="Jacobi-BiCGSTAB to 1e-8" + (" (FP32 inner + FP64 correction)" if mixed else " (FP64)"),
+    rep.update(solver=("FP64 Jacobi-BiCGSTAB" if method is None else
+                       "solve_mixed method %d, omega %.1f (%s)" % (method, omega, MIXED_METHODS[method])) + " to 1e-8",
                blocks_capacity=cap, blocks_per_case=[int(r.tree.nblk) for r in reqs],
                arrival_rate_per_s=round(ncases / t, 2))
     ctx.close()
@@ -189,8 +194,9 @@ MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64
                      "FP32/FP16 preconditioner matrix + coarse (one value per block) correction"
                      % (n, "" if n == 1 else "s") for n in range(1, 7)]
 
-# schedule of `bench.py --profile` (the rocprofv3 runs): FP32-stored matrix, 5 SOR sweeps, omega 1.6
-PROFILE_SCHEDULE = (15, 0.5, 20000, 1.6)
+# schedule of `bench.py --profile` (the rocprofv3 runs): FP32-stored matrix, 3 SOR sweeps + coarse
+# correction, omega 1.4
+PROFILE_SCHEDULE = (19, 0.5, 20000, 1.4)
 
 # kernel names of the timed stages as rocprofv3 reports them
 PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0, %s>", "bi_q": "bi_matvec_kernel<%s, 0, %s, false, false>",
@@ -435,8 +441,11 @@ def main():
     }
     if args.serve_cases > 0:
         log("served stream of %d cases" % args.serve_cases)
-        out["serving"] = serve_leg(args.serve_cases, 7, False, 60.0)
+        out["serving"] = serve_leg(args.serve_cases, 7, 60.0, method=sched[0], omega=sched[3])
         log("serving: " + json.dumps({k: v for k, v in out["serving"].items() if k != "blocks_per_case"}))
+        out["serving_fp64"] = serve_leg(args.serve_cases, 7, 60.0)
+        log("serving (FP64 reference solver): " +
+            json.dumps({k: v for k, v in out["serving_fp64"].items() if k != "blocks_per_case"}))
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(tree, A, b, bct, d64["iters"], args.cpu_seconds)

@@ -16,15 +16,15 @@ DEFAULT_CANDIDATES = (
     # (method, inner_eps or reliable-update factor, inner iteration cap, omega); see
     # ffvhc_solve_mixed for the methods
     (1, 1e-2, 500, 1.0),                 # FP32 Jacobi-PCG in FP64 defect correction
-    (2, 1e-1, 250, 1.0),                 # FP32 Jacobi-BiCGSTAB in FP64 defect correction
     (8, 0.5, 20000, 1.2),                # FP64 recurrence, FP32 matrix, one block GS sweep
-    # FP64 recurrence, FP32 matrix, 2-6 block SOR sweeps on the compact preconditioner matrix
-    (12, 0.5, 20000, 1.4),
-    (13, 0.5, 20000, 1.4),
-    (13, 0.5, 20000, 1.6),
-    (14, 0.5, 20000, 1.4),
-    (15, 0.5, 20000, 1.6),
-    (16, 0.5, 20000, 1.6),
+    (15, 0.5, 20000, 1.6),               # 5 block SOR sweeps on the compact preconditioner matrix
+    # the same with the coarse (one value per block) correction: 2, 3, 4, 5 sweeps
+    (18, 0.5, 20000, 1.4),
+    (19, 0.5, 20000, 1.2),
+    (19, 0.5, 20000, 1.4),
+    (19, 0.5, 20000, 1.6),
+    (20, 0.5, 20000, 1.4),
+    (21, 0.5, 20000, 1.6),
 )
 
 
