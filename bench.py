@@ -340,11 +340,13 @@ def main():
     if sched[0] >= 5:  # stage times of the tuned method's own kernels
         ctx.fill(f64["x"], 0.0)
         ctx.bicgstab_timing(f64["x"], f64["A"], f64["b"], sched[3], 5, method=sched[0])
-        ms = ctx.bicgstab_timing(f64["x"], f64["A"], f64["b"], sched[3], nit, method=sched[0])[:5] / nit
+        msall = ctx.bicgstab_timing(f64["x"], f64["A"], f64["b"], sched[3], nit, method=sched[0]) / nit
+        ms = msall[:5]
         by = kernel_bytes_mixed(sched[0])
         kt["mixed"] = {name: {"us": round(t * 1e3, 2), "GBs": round(by[name] * cells / (t * 1e-3) / 1e9, 1),
                               "kernel": mixed_prof_name(sched[0], name)}
                        for name, t in zip(("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"), ms)}
+        kt["mixed_other_us"] = {"finalisations": round(msall[5] * 1e3, 2), "coarse_corrections": round(msall[6] * 1e3, 2)}
         log("method %d kernels: %s" % (sched[0], json.dumps(kt["mixed"])))
 
     def solve64():

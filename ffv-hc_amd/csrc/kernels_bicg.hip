@@ -624,16 +624,16 @@ template <class T>
 void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int* caseact, int ncase,
                           const DevBlock* blk, const DevCase* cs, SolveState* st, Coef7<T> A, T* x, BiVecs<T> V,
                           double* part, size_t nslots, int count, int mask, const BiOpts& o) {
-  // mask bits 0-4 select P, Q, S, T, XR; bit 5 leaves out the per-case finalisations.
+  // mask bits 0-4 select P, Q, S, T, XR; bit 5 leaves out the per-case finalisations; bit 8
+  // splits the coarse corrections of the two-level methods into stages of their own (bits 6, 7).
   if (!nact) return;
   const bool dbl = sizeof(T) == 8;
   BiVecs<double>& Vd = *reinterpret_cast<BiVecs<double>*>(&V);
   auto precond = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
-    if (dbl && o.gs == BiOpts::GSC && o.cinv) {
+    if (dbl && o.gs == BiOpts::GSC && o.cinv)
       launch_gsk<MODE, true>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count, part, nslots);
-      launch_coarse_apply(sm, active, nact, blk, cs, st, o.cinv, o.coff, part + 4 * nslots, o.ce + MODE * nslots);
-    } else if (dbl && o.gs == BiOpts::GSC)
+    else if (dbl && o.gs == BiOpts::GSC)
       launch_gsk<MODE, false>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count, part, nslots);
     else if (dbl && o.gs == BiOpts::GS32)
       hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
@@ -671,12 +671,23 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   };
   using M0 = std::integral_constant<int, 0>;
   using M1 = std::integral_constant<int, 1>;
+  // the coarse correction of p_ / s_ follows P / S; with bit 8 (stage timing) it runs only as
+  // its own stage, bits 6 / 7
+  auto coarse = [&](int mode) {
+    if (dbl && o.gs == BiOpts::GSC && o.cinv)
+      launch_coarse_apply(sm, active, nact, blk, cs, st, o.cinv, o.coff, part + 4 * nslots, o.ce + mode * nslots);
+  };
+  const bool split = mask & 256;
   if (mask & 1) precond(M0{});
+  if ((mask & 1) && !split) coarse(0);
+  if (split && (mask & 64)) coarse(0);
   if (mask & 2) {
     matvec(M0{});
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_ALPHA, count);
   }
   if (mask & 4) precond(M1{});
+  if ((mask & 4) && !split) coarse(1);
+  if (split && (mask & 128)) coarse(1);
   if (mask & 8) {
     matvec(M1{});
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_GAMMA, count);

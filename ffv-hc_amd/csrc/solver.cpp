@@ -305,19 +305,21 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
                           (const T*)c->fields[b].d[0], (const T*)X.d[0], (const T*)X.h[0], V, c->dpart,
                           (size_t)c->cap, o.hash_r0);
   fin(c, c->dst, FIN_BI_INIT, 0);
-  // stages in launch order: P, Q, fin(alpha), S, T, fin(gamma), XR, fin(convergence, beta)
-  static const int bit_of[8] = {1, 2, 0, 4, 8, 0, 16, 0};
-  static const int fin_of[8] = {-1, -1, FIN_BI_ALPHA, -1, -1, FIN_BI_GAMMA, -1, FIN_BI_XR};
-  static const int slot_of[8] = {0, 1, 5, 2, 3, 5, 4, 5};
-  std::vector<hipEvent_t> ev(9 * niter);
+  // stages in launch order: P, coarse(p_), Q, fin(alpha), S, coarse(s_), T, fin(gamma), XR,
+  // fin(convergence, beta); the coarse applies (two-level methods only) time as their own stage
+  constexpr int NS = 10;
+  static const int bit_of[NS] = {1, 64, 2, 0, 4, 128, 8, 0, 16, 0};
+  static const int fin_of[NS] = {-1, -1, -1, FIN_BI_ALPHA, -1, -1, -1, FIN_BI_GAMMA, -1, FIN_BI_XR};
+  static const int slot_of[NS] = {0, 6, 1, 5, 2, 6, 3, 5, 4, 5};
+  std::vector<hipEvent_t> ev((NS + 1) * niter);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   for (int n = 1; n <= niter; n++) {
-    hipEvent_t* e = &ev[9 * (n - 1)];
+    hipEvent_t* e = &ev[(NS + 1) * (n - 1)];
     HIPCHK(hipEventRecord(e[0], c->sm));
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < NS; k++) {
       if (bit_of[k])
         launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase, c->dst,
-                                K, (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, bit_of[k] | 32, o);
+                                K, (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, bit_of[k] | 32 | 256, o);
       else
         fin(c, c->dst, fin_of[k], n);
       HIPCHK(hipEventRecord(e[k + 1], c->sm));
@@ -325,11 +327,11 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->sm));
-  for (int k = 0; k < 6; k++) ms[k] = 0.0;
+  for (int k = 0; k < 7; k++) ms[k] = 0.0;
   for (int n = 0; n < niter; n++)
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < NS; k++) {
       float t = 0.f;
-      HIPCHK(hipEventElapsedTime(&t, ev[9 * n + k], ev[9 * n + k + 1]));
+      HIPCHK(hipEventElapsedTime(&t, ev[(NS + 1) * n + k], ev[(NS + 1) * n + k + 1]));
       ms[slot_of[k]] += t;
     }
   for (auto& e : ev) (void)hipEventDestroy(e);
@@ -700,7 +702,7 @@ int ffvhc_bicgstab_timing(ffvhc_ctx* c, int x, const int* A, int b, double omega
   int prec = 8;
   int rc = prologue(c, x, A, b, BC_NULL, &prec);
   if (rc) return rc;
-  if (niter < 1 || !ms5) return fail(-1, "niter >= 1 and ms[5] required");
+  if (niter < 1 || !ms5) return fail(-1, "niter >= 1 and ms[7] required");
   return prec == 8 ? bicg_timing_run<double>(c, x, A, b, omega, niter, ms5)
                    : bicg_timing_run<float>(c, x, A, b, omega, niter, ms5);
 }
@@ -712,7 +714,7 @@ int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* c, int method, int x, const int* A, i
   if (rc) return rc;
   if (prec != 8) return fail(-1, "methods 5-9 take FP64 x, A, b");
   if (method < 5 || method > 22) return fail(-1, "method must be 5-22");
-  if (niter < 1 || !ms6) return fail(-1, "niter >= 1 and ms[6] required");
+  if (niter < 1 || !ms6) return fail(-1, "niter >= 1 and ms[7] required");
   Coef7<float> M32;
   Coef7<double> M64;
   BiOpts o;

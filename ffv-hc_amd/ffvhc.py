@@ -378,9 +378,9 @@ class Context:
         return cnt, res, rc
 
     def bicgstab_timing(self, x, A, b, omega, niter, method=None):
-        """Device ms of the five BiCGSTAB kernels and of the finalisations, summed over niter
-        (method 5-9: the kernels of that solve_mixed method)."""
-        ms = np.zeros(6)
+        """Device ms of the five BiCGSTAB kernels, of the finalisations and of the coarse
+        corrections, summed over niter (method 5-22: the kernels of that solve_mixed method)."""
+        ms = np.zeros(7)
         a = np.asarray(A, np.int32)
         if method is None:
             check(_L.ffvhc_bicgstab_timing(self.h, x, _ptr(a), b, omega, niter, _ptr(ms)), "ffvhc_bicgstab_timing")

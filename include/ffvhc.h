@@ -167,7 +167,8 @@ int ffvhc_bicgstab(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, dou
                    int countMax, double eps, int* count, double* residual);
 /* niter BiCGSTAB iterations with HIP events between the launches; ms[0..4] = total device time of
  * P (p update + preconditioner), Q (q_ = A p_), S, T (t_ = A s_), XR; ms[5] = the per-case
- * scalar finalisations (timing entry). */
+ * scalar finalisations; ms[6] = the coarse corrections of the two-level methods (else 0).
+ * ms holds 7 doubles (timing entry). */
 int ffvhc_bicgstab_timing(ffvhc_ctx* ctx, int x, const int* A, int b, double omega, int niter, double* ms);
 /* Coarse operator of the two-level methods (17-22) for one case, as the solver builds it:
  * ac = P^T A P (P = block indicator; n x n, n = the case's blocks, row-major, block order) and
@@ -175,7 +176,8 @@ int ffvhc_bicgstab_timing(ffvhc_ctx* ctx, int x, const int* A, int b, double ome
  * exported so the coarse operator can be checked against its definition. */
 int ffvhc_coarse_operator(ffvhc_ctx* ctx, const int* A, int case_id, double* ac, double* ainv);
 /* The same stage timing for the FP64-recurrence methods 5-22 of ffvhc_solve_mixed (FP64 x, A, b;
- * omega = GS relaxation factor): ms[0..4] = P, Q, S, T, XR kernels, ms[5] = finalisations. */
+ * omega = GS relaxation factor): ms[0..4] = P, Q, S, T, XR kernels, ms[5] = finalisations,
+ * ms[6] = coarse corrections (methods 17-22). */
 int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* ctx, int method, int x, const int* A, int b, double omega, int niter,
                                 double* ms);
 /* niter PCG iterations (no convergence test) with HIP events between the launches; ms[0..2] =
