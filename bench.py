@@ -179,6 +179,52 @@ def serve_leg(ncases, seed, budget_s, method=None, omega=1.0):
     return rep
 
 
+def time_step_leg(steps, warmup, method, omega):
+    """Fractional time steps of the flow past a sphere on a single-level grid: 8x4x4 blocks of 16^3
+    cells (128x64x64, h = D/8, domain 16D x 8D x 8D), sphere D = 1 at (4D, 4D, 4D) as cut cells,
+    uniform inflow U = 1 on X-, Re = 100, dt = 0.01; pressure to relative residual 1e-8 each
+    step with the reference's RBGS (omega 1.2, FP64) and with solve_mixed `method`."""
+    import ffvhc
+    from serve import uniform_flow_init
+    tree = ffvhc.Tree(root=(8, 4, 4), root_length=2.0)
+    c, cid, pid, counts = tree.geometry_sphere(center=(4.0, 4.0, 4.0), radius=0.5, cut_id=1)
+    bc = {ffvhc.BC_UX: [(0, 1.0)] + [(1, 0.0)] * 5,
+          ffvhc.BC_UY: [(0, 0.0)] + [(1, 0.0)] * 5,
+          ffvhc.BC_UZ: [(0, 0.0)] + [(1, 0.0)] * 5,
+          ffvhc.BC_P: [(1, 0.0), (0, 0.0)] + [(1, 0.0)] * 4}
+    params = ffvhc.case_params(rho=1.0, mu=0.01, dt=0.01, bc=bc)
+    cells = tree.nblk * N ** 3
+    out = {"workload": "flow past a sphere, 8x4x4 blocks of 16^3 (524,288 cells), Re 100, dt 0.01, "
+                       "pressure to 1e-8 per step", "fluid_cells": int(counts[0]), "solid_cells": int(counts[1])}
+    for tag, kw in (("fp64_rbgs", dict(solver=0, omega=1.2, count_max=20000, eps=1e-8)),
+                    ("mixed", dict(solver=3, omega=omega, count_max=20000, eps=1e-8, mixed=True,
+                                   mixed_method=method))):
+        ctx = ffvhc.Context(max_blocks=tree.nblk)
+        case = ctx.add_case(tree, params, geometry=(c, cid, pid))
+        F = ctx.step_fields()
+        ctx.select([case])
+        uniform_flow_init(0, u0=1.0, noise=0.0)(ctx, case, F)
+        for _ in range(warmup):
+            ctx.step(F, ncases=1, **kw)
+        ctx.sync()
+        t0 = time.perf_counter()
+        its = []
+        for _ in range(steps):
+            info, _ms, rc = ctx.step(F, ncases=1, **kw)
+            if rc:
+                raise RuntimeError("time step failed with status %d" % rc)
+            its.append(int(info[0].countP))
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        st = ctx.stats(F.ux, ncases=1)[0]
+        out[tag] = {"ms_per_step": round(dt / steps * 1e3, 3), "mcell_steps_s": round(cells * steps / dt / 1e6, 3),
+                    "poisson_iterations_mean": round(float(np.mean(its)), 1), "ux_max": float(st[1])}
+        ctx.close()
+    out["speedup"] = round(out["mixed"]["mcell_steps_s"] / out["fp64_rbgs"]["mcell_steps_s"], 3)
+    out["solver_mixed"] = "solve_mixed method %d, omega %.1f" % (method, omega)
+    return out
+
+
 MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64 defect correction",
                  "FP32 Jacobi-BiCGSTAB in FP64 defect correction",
                  "FP32 Jacobi-BiCGSTAB with FP64 reliable updates", "the same with a seeded shadow residual",
@@ -286,6 +332,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--serve-cases", type=int, default=16)
+    ap.add_argument("--step-steps", type=int, default=10, help="timed steps of the time-step leg (0: skip)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: timing loops and one solve per precision")
     args = ap.parse_args()
@@ -448,6 +495,10 @@ def main():
         out["serving_fp64"] = serve_leg(args.serve_cases, 7, 60.0)
         log("serving (FP64 reference solver): " +
             json.dumps({k: v for k, v in out["serving_fp64"].items() if k != "blocks_per_case"}))
+    if args.step_steps > 0:
+        log("time steps of the flow past a sphere")
+        out["time_step"] = time_step_leg(args.step_steps, 2, sched[0], sched[3])
+        log("time step: " + json.dumps(out["time_step"]))
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(tree, A, b, bct, d64["iters"], args.cpu_seconds)

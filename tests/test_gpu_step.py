@@ -290,6 +290,44 @@ def test_force_v_matches_oracle(ctx, name):
         ctx.remove_case(cid)
 
 
+@pytest.mark.parametrize("method", [15, 19])
+def test_step_with_fast_poisson_matches_fp64(ctx, method):
+    """Two time steps of a flow past a cut-cell sphere: the pressure solved by the FP32-matrix
+    BiCGSTAB with block SOR (+ coarse correction) gives the same fields as the FP64 BiCGSTAB at
+    the same 1e-10 residual, within the solver tolerance; the cut-cell matrix is not FP32-exact, so
+    the FP64 acceptance check takes part."""
+    from serve import uniform_flow_init
+    tree = ffvhc.Tree(root=(4, 2, 2), root_length=2.0)
+    c, cid, pid, counts = tree.geometry_sphere(center=(2.0, 2.0, 2.0), radius=0.5, cut_id=1)
+    assert counts[1] > 0
+    bc = {ffvhc.BC_UX: [(0, 1.0)] + [(1, 0.0)] * 5, ffvhc.BC_UY: [(0, 0.0)] + [(1, 0.0)] * 5,
+          ffvhc.BC_UZ: [(0, 0.0)] + [(1, 0.0)] * 5, ffvhc.BC_P: [(1, 0.0), (0, 0.0)] + [(1, 0.0)] * 4}
+    params = ffvhc.case_params(mu=0.01, dt=0.01, bc=bc)
+    out = []
+    for kw in (dict(solver=3, omega=1.0, count_max=5000, eps=1e-10),
+               dict(solver=3, omega=1.6, count_max=5000, eps=1e-10, mixed=True, mixed_method=method)):
+        case = ctx.add_case(tree, params, geometry=(c, cid, pid))
+        F = ctx.step_fields()
+        try:
+            ctx.select([case])
+            uniform_flow_init(0, u0=1.0, noise=0.0)(ctx, case, F)
+            for _ in range(2):
+                info, _ms, rc = ctx.step(F, ncases=1, **kw)
+                assert rc == 0 and info[0].residualP <= 1e-10
+            shp = (tree.nblk, 20, 20, 20)
+            out.append([ctx.download(f, case, out=np.zeros(shp)) for f in (F.ux, F.uy, F.uz, F.p)])
+        finally:
+            ctx.select(None)
+            for f in [F.ux, F.uy, F.uz, F.p, F.b, F.lapp] + list(F.uc) + list(F.ucp) + list(F.ud) + list(F.udp) + \
+                    list(F.v) + list(F.A):
+                ctx.free(f)
+            ctx.remove_case(case)
+    inner = (slice(None),) + (slice(2, 18),) * 3
+    for a, b in zip(out[0], out[1]):
+        scale = np.max(np.abs(a[inner]))
+        assert np.max(np.abs(a[inner] - b[inner])) <= 1e-6 * scale
+
+
 def test_checkpoint_round_trip(ctx, tmp_path):
     """Dump2 of a field, then Load2 into another: interior and first ghost layer identical; the
     file has the reference's layout (header + padded blocks)."""
