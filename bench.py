@@ -183,7 +183,7 @@ def time_step_leg(steps, warmup, method, omega):
     """Fractional time steps of the flow past a sphere on a single-level grid: 8x4x4 blocks of 16^3
     cells (128x64x64, h = D/8, domain 16D x 8D x 8D), sphere D = 1 at (4D, 4D, 4D) as cut cells,
     uniform inflow U = 1 on X-, Re = 100, dt = 0.01; pressure to relative residual 1e-8 each
-    step with the reference's RBGS (omega 1.2, FP64) and with solve_mixed `method`."""
+    step with the reference's FP64 Jacobi-BiCGSTAB and with solve_mixed `method`."""
     import ffvhc
     from serve import uniform_flow_init
     tree = ffvhc.Tree(root=(8, 4, 4), root_length=2.0)
@@ -196,7 +196,9 @@ def time_step_leg(steps, warmup, method, omega):
     cells = tree.nblk * N ** 3
     out = {"workload": "flow past a sphere, 8x4x4 blocks of 16^3 (524,288 cells), Re 100, dt 0.01, "
                        "pressure to 1e-8 per step", "fluid_cells": int(counts[0]), "solid_cells": int(counts[1])}
-    for tag, kw in (("fp64_rbgs", dict(solver=0, omega=1.2, count_max=20000, eps=1e-8)),
+    # FP64 reference path: Jacobi-BiCGSTAB (the reference's RBGS does not reach 1e-8 on this grid
+    # within 20,000 sweeps per step)
+    for tag, kw in (("fp64", dict(solver=3, omega=1.0, count_max=20000, eps=1e-8)),
                     ("mixed", dict(solver=3, omega=omega, count_max=20000, eps=1e-8, mixed=True,
                                    mixed_method=method))):
         ctx = ffvhc.Context(max_blocks=tree.nblk)
@@ -208,19 +210,21 @@ def time_step_leg(steps, warmup, method, omega):
             ctx.step(F, ncases=1, **kw)
         ctx.sync()
         t0 = time.perf_counter()
-        its = []
+        its, res = [], []
         for _ in range(steps):
             info, _ms, rc = ctx.step(F, ncases=1, **kw)
             if rc:
                 raise RuntimeError("time step failed with status %d" % rc)
             its.append(int(info[0].countP))
+            res.append(float(info[0].residualP))
         ctx.sync()
         dt = time.perf_counter() - t0
         st = ctx.stats(F.ux, ncases=1)[0]
         out[tag] = {"ms_per_step": round(dt / steps * 1e3, 3), "mcell_steps_s": round(cells * steps / dt / 1e6, 3),
-                    "poisson_iterations_mean": round(float(np.mean(its)), 1), "ux_max": float(st[1])}
+                    "poisson_iterations_mean": round(float(np.mean(its)), 1),
+                    "poisson_residual_max": max(res), "ux_max": float(st[1])}
         ctx.close()
-    out["speedup"] = round(out["mixed"]["mcell_steps_s"] / out["fp64_rbgs"]["mcell_steps_s"], 3)
+    out["speedup"] = round(out["mixed"]["mcell_steps_s"] / out["fp64"]["mcell_steps_s"], 3)
     out["solver_mixed"] = "solve_mixed method %d, omega %.1f" % (method, omega)
     return out
 
