@@ -1,0 +1,70 @@
+"""The benchmark's own system at full size (C2: 1,128 blocks of 16^3, two levels), checked through
+properties that hold at any size: every solver reaches the FP64 relative residual it reports
+(recomputed by ffvhc_residual), the fast solvers' answers equal the FP64 BiCGSTAB answer to the
+solver tolerance, and the solves are homogeneous (b -> 2b gives exactly 2x, bit for bit: scaling
+by two is exact in binary floating point and every step of the solvers is linear in b)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import ffvhc
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def c2():
+    tree = bench.build_c2()
+    A, b, bct = bench.poisson_system(tree)
+    ctx, cid, f64, f32 = bench.setup(tree, A, b, bct)
+    yield ctx, cid, f64, tree, b
+    ctx.close()
+
+
+def solve(ctx, f64, method):
+    ctx.fill(f64["x"], 0.0)
+    if method is None:
+        cnt, res, rc = ctx.bicgstab(f64["x"], f64["A"], f64["b"], ffvhc.BC_P, 1.0, 1, 20000, 1e-10, ncases=1)
+    else:
+        cnt, res, _o, rc = ctx.solve_mixed(method, f64["x"], f64["A"], f64["b"], ffvhc.BC_P, 1.6, 20000, 1e-10,
+                                           0.5, ncases=1)
+    assert rc == 0
+    return int(cnt[0]), float(res[0])
+
+
+def test_fullsize_solvers_reach_and_agree(c2):
+    ctx, cid, f64, tree, b = c2
+    shp = (tree.nblk, 20, 20, 20)
+    inner = (slice(None),) + (slice(2, 18),) * 3
+    sols = {}
+    for method in (None, 15, 19):
+        cnt, res = solve(ctx, f64, method)
+        true = ctx.residual(f64["x"], f64["A"], f64["b"], ncases=1)[0]
+        assert res <= 1e-10 and true <= 1e-10 * 1.0001, (method, cnt, res, true)
+        sols[method] = ctx.download(f64["x"], cid, out=np.zeros(shp))[inner]
+    scale = np.max(np.abs(sols[None]))
+    for m in (15, 19):
+        assert np.max(np.abs(sols[m] - sols[None])) <= 1e-7 * scale
+
+
+@pytest.mark.parametrize("method", [None, 19])
+def test_fullsize_solve_is_homogeneous(c2, method):
+    ctx, cid, f64, tree, b = c2
+    shp = (tree.nblk, 20, 20, 20)
+    inner = (slice(None),) + (slice(2, 18),) * 3
+    c1, _ = solve(ctx, f64, method)
+    x1 = ctx.download(f64["x"], cid, out=np.zeros(shp))[inner]
+    ctx.upload(f64["b"], cid, 2.0 * b)
+    try:
+        c2_, _ = solve(ctx, f64, method)
+        x2 = ctx.download(f64["x"], cid, out=np.zeros(shp))[inner]
+    finally:
+        ctx.upload(f64["b"], cid, b)
+    assert c1 == c2_
+    assert np.array_equal(x2, 2.0 * x1)
