@@ -107,7 +107,9 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_kernel(const int* __restrict
   const size_t base = (size_t)slot * NC;
   const T omega = (T)st[c].omega;
   const T ng = (T)(-st[c].gamma), beta = (T)st[c].beta, na = (T)(-st[c].alpha);
-  const bool first = count == 1 || st[c].restart;
+  // count 0: the iteration number comes from the case state (a replayed graph has no per-launch
+  // arguments); FIN_BI_XR of the previous iteration stored it in iter
+  const bool first = (count > 0 ? count : st[c].iter + 1) == 1 || st[c].restart;
   constexpr int KB = Batch<T>::K;
 #pragma unroll
   for (int k0 = 0; k0 < NB; k0 += KB) {
@@ -168,7 +170,9 @@ __global__ __launch_bounds__(BI_NT) void bi_precond_gs_kernel(const int* __restr
   const size_t base = (size_t)slot * NC;
   const TZ omega = (TZ)st[c].omega;
   const double ng = -st[c].gamma, beta = st[c].beta, na = -st[c].alpha;
-  const bool first = count == 1 || st[c].restart;
+  // count 0: the iteration number comes from the case state (a replayed graph has no per-launch
+  // arguments); FIN_BI_XR of the previous iteration stored it in iter
+  const bool first = (count > 0 ? count : st[c].iter + 1) == 1 || st[c].restart;
   const int par = (tx + ty) & 1;  // red cells: k = 2m + par
   auto zi = [](int i, int j, int k) { return i + NB * (j + NB * k); };
   auto zat = [&](int i, int j, int k) -> TZ {
@@ -304,7 +308,9 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
   const size_t base = (size_t)slot * NC;
   const double omega = st[c].omega;
   const double ng = -st[c].gamma, beta = st[c].beta, na = -st[c].alpha;
-  const bool first = count == 1 || st[c].restart;
+  // count 0: the iteration number comes from the case state (a replayed graph has no per-launch
+  // arguments); FIN_BI_XR of the previous iteration stored it in iter
+  const bool first = (count > 0 ? count : st[c].iter + 1) == 1 || st[c].restart;
   const int par = (tx + ty) & 1;  // red cells of this column: k = kb + 2m + par (kb even)
   auto zi = [](int i, int j, int k) { return i + NB * (j + NB * k); };
   // per colour (index m: k = kb + 2m + par for red, kb + 2m + 1 - par for black), compile-time

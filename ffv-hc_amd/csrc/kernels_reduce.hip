@@ -57,9 +57,10 @@ __global__ __launch_bounds__(FIN_NT) void finalize_kernel(const int* __restrict_
     case FIN_BI_GAMMA:  // FFVILS.h:905-915: gamma = (t_.s) / (t_.t_)
       S.gamma = t0 / t1;
       break;
-    case FIN_BI_XR:  // FFVILS.h:925-965: t0 = r.r (IsConverged), t1 = next r.r0
-      S.iter = arg;
-      converged(S, t0, arg);
+    case FIN_BI_XR: {  // FFVILS.h:925-965: t0 = r.r (IsConverged), t1 = next r.r0
+      const int n = arg > 0 ? arg : S.iter + 1;  // arg 0: count kept on the device
+      S.iter = n;
+      converged(S, t0, n);
       S.rr0 = S.rr1;  // this iteration's rho becomes the previous one (also kept for a reliable update)
       if (!S.done) {
         if (fabs(t1) < FLT_MIN) {  // next iteration's r.r0 test (FFVILS.h:829-834)
@@ -72,6 +73,7 @@ __global__ __launch_bounds__(FIN_NT) void finalize_kernel(const int* __restrict_
         }
       }
       break;
+    }
     case FIN_CG_INIT:  // FFVILS.h:679-722: t0 = b.b, t1 = r.z
       S.bb = t0;
       S.rr0 = 1.0;

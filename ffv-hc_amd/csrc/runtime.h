@@ -90,6 +90,8 @@ struct ffvhc_ctx {
   double* dcsnap = nullptr;
   size_t dcsnap_n = 0;
   double* dce = nullptr;
+  // replay the BiCGSTAB iteration as a captured HIP graph (ffvhc_ctx_set_option)
+  bool graphs = true;
   std::vector<ffvhc::FieldRec> fields;
   long long bytes = 0;
   // free slot intervals [begin, end)

@@ -160,19 +160,46 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
   launch_bicgstab_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, (const T*)X.d[0],
                           (const T*)X.h[0], V, c->dpart, (size_t)c->cap, opts.hash_r0);
   fin(c, st, FIN_BI_INIT, 0);
+  // One iteration's launches (count 0: the iteration number lives in the case state) captured
+  // once per solve and replayed: one graph launch per iteration instead of up to ten kernel launches.
+  hipGraphExec_t exec = nullptr;
+  if (c->graphs) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->sm, hipStreamCaptureModeThreadLocal));
+    launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase, st, K,
+                            (T*)X.d[0], V, c->dpart, (size_t)c->cap, 0, 31, opts);
+    const hipError_t ec = hipStreamEndCapture(c->sm, &graph);
+    if (ec != hipSuccess || !graph) return fail(-2, std::string("graph capture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(-2, std::string("graph instantiate: ") + hipGetErrorString(ei));
+  }
   int n = 1, batch = kBatchMin;
   bool done = false;
   while (!done) {
     int last = std::min(n + batch - 1, countMax);
-    for (; n <= last; n++)
-      launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase, st, K,
-                              (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, 31, opts);
+    for (; n <= last; n++) {
+      if (exec) {
+        const hipError_t e = hipGraphLaunch(exec, c->sm);
+        if (e != hipSuccess) {
+          (void)hipGraphExecDestroy(exec);
+          return fail(-2, std::string("graph launch: ") + hipGetErrorString(e));
+        }
+      } else {
+        launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase, st, K,
+                                (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, 31, opts);
+      }
+    }
     HIPCHK(hipGetLastError());
     int rc = read_state(c, st, hst, ids, &done);
-    if (rc) return rc;
+    if (rc) {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      return rc;
+    }
     if (n > countMax) done = true;
     batch = std::min(batch * 2, kBatchMax);
   }
+  if (exec) HIPCHK(hipGraphExecDestroy(exec));
   if (launched) *launched = n - 1;
   launch_ghost_pull<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, (const T*)X.d[0], (T*)X.h[0], bcg, 1,
                        c->multilevel);
@@ -721,6 +748,15 @@ int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* c, int method, int x, const int* A, i
   rc = fp64_bicg_opts(c, method, A, &M32, &M64, &o);
   if (rc) return rc;
   return bicg_timing_run<double>(c, x, A, b, omega, niter, ms6, o);
+}
+
+int ffvhc_ctx_set_option(ffvhc_ctx* c, int option, int value) {
+  if (!c) return fail(-1, "null context");
+  if (option == FFVHC_OPTION_GRAPHS) {
+    c->graphs = value != 0;
+    return 0;
+  }
+  return fail(-1, "unknown option");
 }
 
 int ffvhc_coarse_operator(ffvhc_ctx* c, const int* A, int case_id, double* ac, double* ainv) {

@@ -78,6 +78,7 @@ _sig = {
     "ffvhc_ctx_destroy": (None, [_vp]),
     "ffvhc_ctx_sync": (c_int, [_vp]),
     "ffvhc_ctx_device_bytes": (c_ll, [_vp]),
+    "ffvhc_ctx_set_option": (c_int, [_vp, c_int, c_int]),
     "ffvhc_ctx_select": (c_int, [_vp, _vp, c_int]),
     "ffvhc_case_add": (c_int, [_vp, _vp, _P(CaseParams), c_int, _vp, _vp, _vp, _P(c_int)]),
     "ffvhc_case_remove": (c_int, [_vp, c_int]),
@@ -276,6 +277,10 @@ class Context:
     def remove_case(self, case_id):
         check(_L.ffvhc_case_remove(self.h, int(case_id)), "ffvhc_case_remove")
         self.cases.pop(case_id, None)
+
+    def set_graphs(self, on):
+        """Replay the BiCGSTAB iteration as a captured HIP graph (default) or launch one by one."""
+        check(_L.ffvhc_ctx_set_option(self.h, 1, int(bool(on))), "ffvhc_ctx_set_option")
 
     def select(self, ids=None):
         if ids is None:

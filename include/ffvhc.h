@@ -94,6 +94,11 @@ void ffvhc_ctx_destroy(ffvhc_ctx* ctx);
 int ffvhc_ctx_sync(ffvhc_ctx* ctx);
 /* bytes of device memory the context holds */
 long long ffvhc_ctx_device_bytes(const ffvhc_ctx* ctx);
+/* Context options (no counterpart in the reference).  FFVHC_OPTION_GRAPHS: 1 (default) replays
+ * the BiCGSTAB iteration as a HIP graph captured once per solve, 0 launches its kernels one by
+ * one; the results are identical. */
+#define FFVHC_OPTION_GRAPHS 1
+int ffvhc_ctx_set_option(ffvhc_ctx* ctx, int option, int value);
 
 /* Physical parameters and outer BCs of one flow case.  BC groups: 0 UX, 1 UY, 2 UZ, 3 P,
  * 4 Neumann-0 fields (boundaryTypeNULL, Solver.cpp:2024-2034), 5 zero Dirichlet.  Types
