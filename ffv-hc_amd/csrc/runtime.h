@@ -90,8 +90,9 @@ struct ffvhc_ctx {
   double* dcsnap = nullptr;
   size_t dcsnap_n = 0;
   double* dce = nullptr;
-  // replay the BiCGSTAB iteration as a captured HIP graph (ffvhc_ctx_set_option)
-  bool graphs = true;
+  // replay the BiCGSTAB iteration as a captured HIP graph (ffvhc_ctx_set_option; off by default:
+  // measured 1-11% slower per iteration than direct launches on ROCm 7.2, 64-960 blocks)
+  bool graphs = false;
   std::vector<ffvhc::FieldRec> fields;
   long long bytes = 0;
   // free slot intervals [begin, end)

@@ -279,7 +279,7 @@ class Context:
         self.cases.pop(case_id, None)
 
     def set_graphs(self, on):
-        """Replay the BiCGSTAB iteration as a captured HIP graph (default) or launch one by one."""
+        """Replay the BiCGSTAB iteration as a captured HIP graph, or launch one by one (default)."""
         check(_L.ffvhc_ctx_set_option(self.h, 1, int(bool(on))), "ffvhc_ctx_set_option")
 
     def select(self, ids=None):

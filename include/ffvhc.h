@@ -94,9 +94,9 @@ void ffvhc_ctx_destroy(ffvhc_ctx* ctx);
 int ffvhc_ctx_sync(ffvhc_ctx* ctx);
 /* bytes of device memory the context holds */
 long long ffvhc_ctx_device_bytes(const ffvhc_ctx* ctx);
-/* Context options (no counterpart in the reference).  FFVHC_OPTION_GRAPHS: 1 (default) replays
- * the BiCGSTAB iteration as a HIP graph captured once per solve, 0 launches its kernels one by
- * one; the results are identical. */
+/* Context options (no counterpart in the reference).  FFVHC_OPTION_GRAPHS: 1 replays the
+ * BiCGSTAB iteration as a HIP graph captured once per solve, 0 (default) launches its kernels
+ * one by one; the results are identical (the graph replay measured slower on ROCm 7.2). */
 #define FFVHC_OPTION_GRAPHS 1
 int ffvhc_ctx_set_option(ffvhc_ctx* ctx, int option, int value);
 

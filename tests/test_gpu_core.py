@@ -429,7 +429,7 @@ def test_graph_replay_equals_launches(ctx, method):
         for f in fa + [fb, fx]:
             ctx.free(f)
     finally:
-        ctx.set_graphs(True)
+        ctx.set_graphs(False)
         ctx.remove_case(cid)
 
 
