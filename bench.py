@@ -242,6 +242,9 @@ MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64
                                 "FP32/FP16 preconditioner matrix" % (n, "" if n == 1 else "s") for n in range(1, 7)] + [
                      "FP64 BiCGSTAB on an FP32-stored matrix, %d block red-black SOR sweep%s on a compact "
                      "FP32/FP16 preconditioner matrix + coarse (one value per block) correction"
+                     % (n, "" if n == 1 else "s") for n in range(1, 7)] + [
+                     "FP64 BiCGSTAB on an FP32-stored matrix, coarse (one value per block) correction, then "
+                     "%d block red-black SOR sweep%s on the remaining residual (multiplicative two-level)"
                      % (n, "" if n == 1 else "s") for n in range(1, 7)]
 
 # schedule of `bench.py --profile` (the rocprofv3 runs): FP32-stored matrix, 3 SOR sweeps + coarse
@@ -276,7 +279,7 @@ def mixed_prof_name(method, stage):
         return "bi_precond_kernel<double, %d, float>" % mode
     if method >= 11:
         return "bi_precond_gsk_kernel<%d, %d, %s>" % (mode, (method - 11) % 6 + 1,
-                                                      "true" if method >= 17 else "false")
+                                                      "true" if method >= 23 else "false")
     tm, tz, sym = {5: ("float", "float", "false"), 6: ("double", "double", "false"),
                    8: ("float", "double", "false"), 9: ("float", "double", "true"),
                    10: ("float", "double", "false")}[method]

@@ -61,7 +61,7 @@ void launch_cg_xr(hipStream_t sm, const int* active, int nact, const DevBlock* b
 template <class T>
 void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                           SolveState* st, Coef7<T> A, const T* b, const T* x, const T* hx, BiVecs<T> V, double* part,
-                          size_t nslots, bool hash_r0 = false);
+                          size_t nslots, bool hash_r0 = false, double* bsr = nullptr);
 // Precision options of an FP64 BiCGSTAB: A32 = FP32 copy of the matrix for the matvecs and the
 // Jacobi preconditioner (widened to FP64 on load); gs = block red-black GS preconditioner with
 // an FP32 matrix and FP32 arithmetic, an FP32 matrix and FP64 arithmetic, all FP64, or the
@@ -79,10 +79,12 @@ struct BiOpts {
   const uint4* rec = nullptr;
   int sweeps = 1;
   // coarse correction (GSC only): FP32 inverse of A_c per case at offsets coff[case], the block
-  // sums of v in part[4 * nslots + slot], corrections ce[mode * nslots + slot]
+  // sums of v in part[4 * nslots + slot], corrections ce[mode * nslots + slot]; mult: applied
+  // before the sweeps (multiplicative), from the block sums of r, q_, p in ce[2..4][slot]
   const float* cinv = nullptr;
   const long long* coff = nullptr;
   double* ce = nullptr;
+  bool mult = false;
 };
 
 template <class T>
@@ -117,6 +119,9 @@ void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* T);
 void launch_to_f32(hipStream_t sm, const double* x, float* y, size_t n);
 void launch_coarse_apply(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                          const SolveState* st, const float* inv, const long long* off, const double* c, double* e);
+void launch_coarse_apply_mul(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                             const SolveState* st, const float* inv, const long long* off, double* ce, size_t nslots,
+                             int mode);
 
 void launch_predict(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                     const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc);

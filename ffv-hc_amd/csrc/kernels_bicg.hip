@@ -40,7 +40,8 @@ __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ 
                                                         const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                         Coef7<T> A, const T* __restrict__ bv, const T* __restrict__ x,
                                                         const T* __restrict__ hx, BiVecs<T> V,
-                                                        double* __restrict__ part, size_t nslots) {
+                                                        double* __restrict__ part, size_t nslots,
+                                                        double* __restrict__ bsr) {
   __shared__ T s[LP * LP * LP];
   __shared__ double red[BI_NT / 64];
   const int slot = active[blockIdx.x];
@@ -50,7 +51,7 @@ __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ 
   const size_t base = (size_t)slot * NC;
   const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
   constexpr int KB = Batch<T>::K;
-  double bb = 0.0, rr1 = 0.0;
+  double bb = 0.0, rr1 = 0.0, rs = 0.0;
 #pragma unroll
   for (int k0 = 0; k0 < NB; k0 += KB) {
     T a[KB][7], b0[KB];
@@ -73,6 +74,7 @@ __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ 
                   a[kk][4] * s[li + LP] - a[kk][5] * s[li - LP * LP] - a[kk][6] * s[li + LP * LP] -
                   a[kk][0] * s[li];
       V.r[g] = r;
+      rs += (double)r;
       bb += (double)b0[kk] * (double)b0[kk];
       if (HR) {
         rr1 += (double)r * (double)shadow_value(local, (uint32_t)(g - base));
@@ -87,6 +89,10 @@ __global__ __launch_bounds__(BI_NT) void bi_init_kernel(const int* __restrict__ 
   if (t == 0) {
     part[slot] = sb;
     part[nslots + slot] = sr;
+  }
+  if (bsr) {  // block sums of r for the multiplicative coarse correction
+    const double v = block_sum<BI_NT>(rs, red);
+    if (t == 0) bsr[slot] = v;
   }
 }
 
@@ -288,15 +294,19 @@ void launch_gs_pack(hipStream_t sm, const int* active, int nact, Coef7<double> A
 // the first read no HBM.
 constexpr int GSK_NT = 512;
 constexpr int GSK_NK = NB * 256 / GSK_NT;  // cells of a column per thread
-// CC: also write the block sum of v to part[4 * nslots + slot] (restriction P^T v of the coarse
-// correction).
-template <int MODE, int NS, bool CC>
+// vs_out (optional): the block sum of v (restriction P^T v of the coarse correction).
+// MUL (multiplicative two-level): e (this mode's coarse correction per slot, computed before this
+// kernel) is subtracted first, the sweeps smooth r1 = v - A P e with A the compact matrix
+// (A_0 = 1 / dinv, A_q = w_q A_0): u1 = u - (e_a + sum_q w_q e of the block each neighbour
+// lies in).
+template <int MODE, int NS, bool MUL>
 __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __restrict__ active,
                                                                const DevBlock* __restrict__ blk,
                                                                const DevCase* __restrict__ cs,
                                                                const SolveState* __restrict__ st,
                                                                const uint4* __restrict__ rec, BiVecs<double> V,
-                                                               int count, double* __restrict__ part, size_t nslots) {
+                                                               int count, double* __restrict__ vs_out,
+                                                               const double* __restrict__ ce) {
   __shared__ double z[NC];
   __shared__ double red[GSK_NT / 64];
   const int slot = active[blockIdx.x];
@@ -318,6 +328,19 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
   double ur[GSK_NK / 2], ub[GSK_NK / 2];
   uint32_t wr[GSK_NK / 2][3], wb[GSK_NK / 2][3];
   double vsum = 0.0;
+  // MUL: e of the block each neighbour lies in -- x-, x+, y-, y+ (the same for the thread's 8
+  // cells: a half column never straddles a subface boundary), z- (k = 0), z+ (k = 15), own
+  double en[7];
+  if constexpr (MUL) {
+    const double eo = ce[slot];
+    en[0] = tx > 0 ? eo : ce[ghost_source(B, slot, 0, ty, kb)];
+    en[1] = tx < NB - 1 ? eo : ce[ghost_source(B, slot, 1, ty, kb)];
+    en[2] = ty > 0 ? eo : ce[ghost_source(B, slot, 2, tx, kb)];
+    en[3] = ty < NB - 1 ? eo : ce[ghost_source(B, slot, 3, tx, kb)];
+    en[4] = ce[ghost_source(B, slot, 4, tx, ty)];
+    en[5] = ce[ghost_source(B, slot, 5, tx, ty)];
+    en[6] = eo;
+  }
 #pragma unroll
   for (int k0 = 0; k0 < GSK_NK; k0 += 4) {
     double rv[4], pv[4], qv[4];
@@ -342,8 +365,14 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
         v = na * qv[kk] + rv[kk];
         V.s[g] = v;
       }
-      if (CC) vsum += v;
-      const double uk = v * (double)__uint_as_float(rc[kk].x);
+      vsum += v;
+      double uk = v * (double)__uint_as_float(rc[kk].x);
+      if constexpr (MUL) {
+        const int ka = kb + k;
+        uk -= en[6] + half_lo(rc[kk].y) * en[0] + half_hi(rc[kk].y) * en[1] + half_lo(rc[kk].z) * en[2] +
+              half_hi(rc[kk].z) * en[3] + half_lo(rc[kk].w) * (ka > 0 ? en[6] : en[4]) +
+              half_hi(rc[kk].w) * (ka < NB - 1 ? en[6] : en[5]);
+      }
       const int m = k >> 1;
       if ((k & 1) == par) {
         ur[m] = uk;
@@ -358,9 +387,9 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
       }
     }
   }
-  if (CC) {
+  if (vs_out) {
     const double sv = block_sum<GSK_NT>(vsum, red);
-    if (t == 0) part[4 * nslots + slot] = sv;
+    if (t == 0) vs_out[slot] = sv;
   }
   // first red half sweep: every neighbour is still zero
 #pragma unroll
@@ -409,12 +438,14 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
   push_halo<double, GSK_NT>(B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return z[zi(i, j, k)]; }, h, h);
 }
 
-template <int MODE, bool CC>
+template <int MODE, bool MUL>
 void launch_gsk(int sweeps, hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
-                const SolveState* st, const uint4* rec, BiVecs<double> V, int count, double* part, size_t nslots) {
+                const SolveState* st, const uint4* rec, BiVecs<double> V, int count, double* vs_out,
+                const double* ce) {
   const dim3 g(nact), b(GSK_NT);
 #define GSK(NS) \
-  hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, NS, CC>), g, b, 0, sm, active, blk, cs, st, rec, V, count, part, nslots)
+  hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, NS, MUL>), g, b, 0, sm, active, blk, cs, st, rec, V, count, vs_out, \
+                     ce)
   switch (sweeps) {
     case 1: GSK(1); break;
     case 2: GSK(2); break;
@@ -434,7 +465,8 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
                                                           const DevBlock* __restrict__ blk,
                                                           const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                           Coef7<TA> A, BiVecs<T> V, double* __restrict__ part,
-                                                          size_t nslots, const double* __restrict__ ce = nullptr) {
+                                                          size_t nslots, const double* __restrict__ ce = nullptr,
+                                                          double* __restrict__ bsy = nullptr) {
   __shared__ T s[LP * LP * LP];
   __shared__ double red[BI_NT / 64];
   const int slot = active[blockIdx.x];
@@ -446,7 +478,7 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
   T* __restrict__ y = MODE == 0 ? V.q : V.t;
   const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
   constexpr int KB = Batch<T>::K;
-  double d0 = 0.0, d1 = 0.0;
+  double d0 = 0.0, d1 = 0.0, ys = 0.0;
 #pragma unroll
   for (int k0 = 0; k0 < NB; k0 += KB) {
     T a[KB][7], wv[KB];
@@ -483,6 +515,7 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
       const T v = a[kk][1] * s[li - 1] + a[kk][2] * s[li + 1] + a[kk][3] * s[li - LP] + a[kk][4] * s[li + LP] +
                   a[kk][5] * s[li - LP * LP] + a[kk][6] * s[li + LP * LP] + a[kk][0] * s[li];
       y[g] = v;
+      ys += (double)v;
       d0 += (double)v * (double)wv[kk];
       if (MODE == 1) d1 += (double)v * (double)v;
     }
@@ -493,6 +526,10 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
     part[slot] = s0;
     part[nslots + slot] = s1;
   }
+  if (bsy) {  // block sums of q_ / t_ (multiplicative coarse correction)
+    const double v = block_sum<BI_NT>(ys, red);
+    if (t == 0) bsy[slot] = v;
+  }
 }
 
 // CC: p_ and s_ carry the coarse corrections ce[slot] / ce[nslots + slot] (two-level
@@ -501,7 +538,8 @@ template <class T, bool HR = false, bool CC = false>
 __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
                                                       const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                       T* __restrict__ x, BiVecs<T> V, double* __restrict__ part,
-                                                      size_t nslots, int count, const double* __restrict__ ce = nullptr) {
+                                                      size_t nslots, int count, const double* __restrict__ ce = nullptr,
+                                                      double* __restrict__ bsr = nullptr) {
   __shared__ double red[BI_NT / 64];
   const int slot = active[blockIdx.x];
   const int c = blk[slot].case_id;
@@ -511,7 +549,7 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
   const T alpha = (T)st[c].alpha, gamma = (T)st[c].gamma, ng = (T)(-st[c].gamma);
   const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
   constexpr int KB = Batch<T>::K;
-  double rr = 0.0, rr1 = 0.0;
+  double rr = 0.0, rr1 = 0.0, rs = 0.0;
 #pragma unroll
   for (int k0 = 0; k0 < NB; k0 += KB) {
     T pv[KB], sv_[KB], xv[KB], tv[KB], sv[KB], r0v[KB];
@@ -535,6 +573,7 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
       x[g] = alpha * pv[kk] + gamma * sv_[kk] + xv[kk];  // AXPBYPZ(x, p_, s_, alpha, gamma)
       const T r = ng * tv[kk] + sv[kk];                  // AXPYZ(r, t_, s, -gamma)
       V.r[g] = r;
+      rs += (double)r;
       rr += (double)r * (double)r;
       rr1 += (double)r * (double)r0v[kk];
     }
@@ -544,6 +583,10 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
   if (t == 0) {
     part[slot] = s0;
     part[nslots + slot] = s1;
+  }
+  if (bsr) {
+    const double v = block_sum<BI_NT>(rs, red);
+    if (t == 0) bsr[slot] = v;
   }
 }
 
@@ -616,14 +659,14 @@ void launch_bicgstab_restart(hipStream_t sm, const int* active, int nact, const 
 template <class T>
 void launch_bicgstab_init(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                           SolveState* st, Coef7<T> A, const T* b, const T* x, const T* hx, BiVecs<T> V, double* part,
-                          size_t nslots, bool hash_r0) {
+                          size_t nslots, bool hash_r0, double* bsr) {
   if (!nact) return;
   if (hash_r0)
     hipLaunchKernelGGL((bi_init_kernel<T, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, b, x, hx, V,
-                       part, nslots);
+                       part, nslots, bsr);
   else
     hipLaunchKernelGGL((bi_init_kernel<T, false>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, b, x, hx,
-                       V, part, nslots);
+                       V, part, nslots, bsr);
 }
 
 template <class T>
@@ -637,10 +680,13 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   BiVecs<double>& Vd = *reinterpret_cast<BiVecs<double>*>(&V);
   auto precond = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
-    if (dbl && o.gs == BiOpts::GSC && o.cinv)
-      launch_gsk<MODE, true>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count, part, nslots);
+    if (dbl && o.gs == BiOpts::GSC && o.cinv && o.mult)  // block sum of p kept for the next coarse rhs
+      launch_gsk<MODE, true>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count,
+                             MODE == 0 ? o.ce + 4 * nslots : nullptr, o.ce + MODE * nslots);
+    else if (dbl && o.gs == BiOpts::GSC && o.cinv)
+      launch_gsk<MODE, false>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count, part + 4 * nslots, nullptr);
     else if (dbl && o.gs == BiOpts::GSC)
-      launch_gsk<MODE, false>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count, part, nslots);
+      launch_gsk<MODE, false>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count, nullptr, nullptr);
     else if (dbl && o.gs == BiOpts::GS32)
       hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
                          *o.M32, Vd, count);
@@ -664,7 +710,8 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
     constexpr int MODE = decltype(mode_tag)::value;
     if (dbl && o.A32 && o.cinv)
       hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, false, true>), dim3(nact), dim3(BI_NT), 0, sm, active,
-                         blk, cs, st, *o.A32, Vd, part, nslots, (const double*)o.ce);
+                         blk, cs, st, *o.A32, Vd, part, nslots, (const double*)o.ce,
+                         (o.mult && MODE == 0) ? o.ce + 3 * nslots : nullptr);
     else if (dbl && o.A32 && o.hash_r0)
       hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk,
                          cs, st, *o.A32, Vd, part, nslots);
@@ -679,20 +726,27 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   using M1 = std::integral_constant<int, 1>;
   // the coarse correction of p_ / s_ follows P / S; with bit 8 (stage timing) it runs only as
   // its own stage, bits 6 / 7
+  // additive: after P / S from the block sums that kernel wrote; multiplicative: before P / S from
+  // the block sums of r, p, q the previous kernels wrote
   auto coarse = [&](int mode) {
-    if (dbl && o.gs == BiOpts::GSC && o.cinv)
+    if (!(dbl && o.gs == BiOpts::GSC && o.cinv)) return;
+    if (o.mult)
+      launch_coarse_apply_mul(sm, active, nact, blk, cs, st, o.cinv, o.coff, o.ce, nslots, mode);
+    else
       launch_coarse_apply(sm, active, nact, blk, cs, st, o.cinv, o.coff, part + 4 * nslots, o.ce + mode * nslots);
   };
-  const bool split = mask & 256;
+  const bool split = mask & 256, pre = o.mult;
+  if ((mask & 1) && !split && pre) coarse(0);
   if (mask & 1) precond(M0{});
-  if ((mask & 1) && !split) coarse(0);
+  if ((mask & 1) && !split && !pre) coarse(0);
   if (split && (mask & 64)) coarse(0);
   if (mask & 2) {
     matvec(M0{});
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_ALPHA, count);
   }
+  if ((mask & 4) && !split && pre) coarse(1);
   if (mask & 4) precond(M1{});
-  if ((mask & 4) && !split) coarse(1);
+  if ((mask & 4) && !split && !pre) coarse(1);
   if (split && (mask & 128)) coarse(1);
   if (mask & 8) {
     matvec(M1{});
@@ -701,7 +755,7 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   if (mask & 16) {
     if (o.cinv)
       hipLaunchKernelGGL((bi_xr_kernel<T, false, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V,
-                         part, nslots, count, (const double*)o.ce);
+                         part, nslots, count, (const double*)o.ce, o.mult ? o.ce + 2 * nslots : nullptr);
     else if (o.hash_r0)
       hipLaunchKernelGGL((bi_xr_kernel<T, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
                          nslots, count);
@@ -714,7 +768,8 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
 
 #define INST(T)                                                                                                     \
   template void launch_bicgstab_init<T>(hipStream_t, const int*, int, const DevBlock*, const DevCase*, SolveState*,  \
-                                        Coef7<T>, const T*, const T*, const T*, BiVecs<T>, double*, size_t, bool);   \
+                                        Coef7<T>, const T*, const T*, const T*, BiVecs<T>, double*, size_t, bool,    \
+                                        double*);                                                                    \
   template void launch_bicgstab_iter<T>(hipStream_t, const int*, int, const int*, int, const DevBlock*,            \
                                         const DevCase*, SolveState*, Coef7<T>, T*, BiVecs<T>, double*, size_t, int,  \
                                         int, const BiOpts&);                                                         \

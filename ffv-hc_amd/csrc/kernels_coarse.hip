@@ -251,6 +251,48 @@ __global__ __launch_bounds__(256) void coarse_apply_kernel(const int* __restrict
   if (lane == 0) e[slot] = s;
 }
 
+// Multiplicative variant: the restriction c = P^T v is formed from block sums the previous
+// kernels wrote (ce layout per slot: [2] r, [3] q_, [4] p), by the vector updates themselves:
+// p = r (first iteration) or r + beta (p - gamma q_), s = r - alpha q_.
+__global__ __launch_bounds__(256) void coarse_apply_mul_kernel(const int* __restrict__ active, int nact,
+                                                               const DevBlock* __restrict__ blk,
+                                                               const DevCase* __restrict__ cs,
+                                                               const SolveState* __restrict__ st,
+                                                               const float* __restrict__ inv,
+                                                               const long long* __restrict__ off,
+                                                               double* __restrict__ ce, size_t nslots, int mode) {
+  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (w >= nact) return;
+  const int slot = active[w];
+  const int cid = blk[slot].case_id;
+  const SolveState& S = st[cid];
+  if (S.done) return;
+  const DevCase& C = cs[cid];
+  const int n = C.nblk, a = slot - C.slot_begin;
+  const float* row = inv + off[cid] + (size_t)a * n;
+  const double* br = ce + 2 * nslots + C.slot_begin;
+  const double* bq = ce + 3 * nslots + C.slot_begin;
+  const double* bp = ce + 4 * nslots + C.slot_begin;
+  const bool first = S.iter == 0 || S.restart;
+  const double beta = S.beta, ng = -S.gamma, na = -S.alpha;
+  double s = 0.0;
+#pragma unroll 4
+  for (int b = lane; b < n; b += 64) {
+    const double cb = mode == 0 ? (first ? br[b] : br[b] + beta * (bp[b] + ng * bq[b])) : na * bq[b] + br[b];
+    s += (double)row[b] * cb;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) ce[mode * nslots + slot] = s;
+}
+
+void launch_coarse_apply_mul(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                             const SolveState* st, const float* inv, const long long* off, double* ce, size_t nslots,
+                             int mode) {
+  if (nact)
+    hipLaunchKernelGGL(coarse_apply_mul_kernel, dim3((nact + 3) / 4), dim3(256), 0, sm, active, nact, blk, cs, st,
+                       inv, off, ce, nslots, mode);
+}
+
 void launch_coarse_apply(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                          const SolveState* st, const float* inv, const long long* off, const double* c, double* e) {
   if (nact)

@@ -158,7 +158,8 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
   const FieldRec X = c->fields[xf];  // copy taken after the scratch allocations
   launch_solve_init(c->sm, st, (int)c->cases.size(), omega, eps, countMax, outer);
   launch_bicgstab_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, st, K, bv, (const T*)X.d[0],
-                          (const T*)X.h[0], V, c->dpart, (size_t)c->cap, opts.hash_r0);
+                          (const T*)X.h[0], V, c->dpart, (size_t)c->cap, opts.hash_r0,
+                          opts.mult ? opts.ce + 2 * c->cap : nullptr);
   fin(c, st, FIN_BI_INIT, 0);
   // One iteration's launches (count 0: the iteration number lives in the case state) captured
   // once per solve and replayed: one graph launch per iteration instead of up to ten kernel launches.
@@ -330,14 +331,19 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
   launch_solve_init(c->sm, c->dst, (int)c->cases.size(), omega, -1.0, niter);
   launch_bicgstab_init<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst, K,
                           (const T*)c->fields[b].d[0], (const T*)X.d[0], (const T*)X.h[0], V, c->dpart,
-                          (size_t)c->cap, o.hash_r0);
+                          (size_t)c->cap, o.hash_r0, o.mult ? o.ce + 2 * c->cap : nullptr);
   fin(c, c->dst, FIN_BI_INIT, 0);
   // stages in launch order: P, coarse(p_), Q, fin(alpha), S, coarse(s_), T, fin(gamma), XR,
   // fin(convergence, beta); the coarse applies (two-level methods only) time as their own stage
+  // (the multiplicative coarse correction runs before P / S instead)
   constexpr int NS = 10;
-  static const int bit_of[NS] = {1, 64, 2, 0, 4, 128, 8, 0, 16, 0};
+  static const int bit_add[NS] = {1, 64, 2, 0, 4, 128, 8, 0, 16, 0};
+  static const int slot_add[NS] = {0, 6, 1, 5, 2, 6, 3, 5, 4, 5};
+  static const int bit_mul[NS] = {64, 1, 2, 0, 128, 4, 8, 0, 16, 0};
+  static const int slot_mul[NS] = {6, 0, 1, 5, 6, 2, 3, 5, 4, 5};
   static const int fin_of[NS] = {-1, -1, -1, FIN_BI_ALPHA, -1, -1, -1, FIN_BI_GAMMA, -1, FIN_BI_XR};
-  static const int slot_of[NS] = {0, 6, 1, 5, 2, 6, 3, 5, 4, 5};
+  const int* bit_of = o.mult ? bit_mul : bit_add;
+  const int* slot_of = o.mult ? slot_mul : slot_add;
   std::vector<hipEvent_t> ev((NS + 1) * niter);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   for (int n = 1; n <= niter; n++) {
@@ -405,7 +411,7 @@ int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o, double* ac_out
     c->dcsnap_n = 32 * maxn;
   }
   if (!c->dcoff) HIPCHK(hipMalloc(&c->dcoff, MAX_CASES * sizeof(long long)));
-  if (!c->dce) HIPCHK(hipMalloc(&c->dce, 2 * (size_t)c->cap * sizeof(double)));
+  if (!c->dce) HIPCHK(hipMalloc(&c->dce, 5 * (size_t)c->cap * sizeof(double)));
   HIPCHK(hipMemcpyAsync(c->dcoff, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice, c->sm));
   launch_coarse_assemble(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dcoff, drows, c->dcm, total);
   for (int id : ids) {
@@ -478,7 +484,7 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Co
     o->M32 = M32;
     o->hash_r0 = method == 10;
   } else {  // 11-16: FP32 matrix for the matvecs, 1-6 SOR sweeps on the compact records;
-            // 17-22: the same plus the coarse correction
+            // 17-22: the same plus the additive coarse correction, 23-28 the multiplicative one
     int fr;
     const int rc = scratch_field(c, 16, R_REC, &fr);
     if (rc) return rc;
@@ -487,6 +493,7 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Co
     o->rec = (const uint4*)c->fields[fr].d[0];
     o->sweeps = (method - 11) % 6 + 1;
     launch_gs_pack(c->sm, c->dactive, c->nactive, *M64, (uint4*)c->fields[fr].d[0]);
+    o->mult = method >= 23;
     if (method >= 17) return coarse_setup(c, *M64, o);
   }
   return 0;
@@ -740,7 +747,7 @@ int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* c, int method, int x, const int* A, i
   int rc = prologue(c, x, A, b, BC_NULL, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "methods 5-9 take FP64 x, A, b");
-  if (method < 5 || method > 22) return fail(-1, "method must be 5-22");
+  if (method < 5 || method > 28) return fail(-1, "method must be 5-28");
   if (niter < 1 || !ms6) return fail(-1, "niter >= 1 and ms[7] required");
   Coef7<float> M32;
   Coef7<double> M64;
@@ -807,14 +814,15 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
-  if (method < 0 || method > 22)
+  if (method < 0 || method > 28)
     return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB, 3 Jacobi-BiCGSTAB with reliable updates, "
                     "4 the same with a seeded shadow residual, 5 FP64 BiCGSTAB with an FP32 red-black GS "
                     "preconditioner, 6 the same all in FP64 (reference point), 7 FP64 Jacobi-BiCGSTAB with an FP32 "
                     "matrix, 8 as 7 with the block red-black GS preconditioner (FP64 arithmetic), 9 as 8 with the "
                     "symmetric red-black-red sweep, 10 as 8 with the seeded hash vector as shadow residual, "
                     "11-16 as 8 with 1-6 red-black SOR sweeps on a compact FP32/FP16 copy of the matrix, "
-                    "17-22 as 11-16 plus a coarse (one value per block) correction");
+                    "17-22 as 11-16 plus a coarse (one value per block) correction, 23-28 the same correction "
+                    "applied before the sweeps (multiplicative)");
   if (method == 6) {
     Coef7<float> M32;
     Coef7<double> M64;

@@ -175,14 +175,14 @@ int ffvhc_bicgstab(ffvhc_ctx* ctx, int x, const int* A, int b, int bc_group, dou
  * scalar finalisations; ms[6] = the coarse corrections of the two-level methods (else 0).
  * ms holds 7 doubles (timing entry). */
 int ffvhc_bicgstab_timing(ffvhc_ctx* ctx, int x, const int* A, int b, double omega, int niter, double* ms);
-/* Coarse operator of the two-level methods (17-22) for one case, as the solver builds it:
+/* Coarse operator of the two-level methods (17-28) for one case, as the solver builds it:
  * ac = P^T A P (P = block indicator; n x n, n = the case's blocks, row-major, block order) and
  * ainv = its inverse from the device's blocked Gauss-Jordan.  No counterpart in the reference;
  * exported so the coarse operator can be checked against its definition. */
 int ffvhc_coarse_operator(ffvhc_ctx* ctx, const int* A, int case_id, double* ac, double* ainv);
-/* The same stage timing for the FP64-recurrence methods 5-22 of ffvhc_solve_mixed (FP64 x, A, b;
+/* The same stage timing for the FP64-recurrence methods 5-28 of ffvhc_solve_mixed (FP64 x, A, b;
  * omega = GS relaxation factor): ms[0..4] = P, Q, S, T, XR kernels, ms[5] = finalisations,
- * ms[6] = coarse corrections (methods 17-22). */
+ * ms[6] = coarse corrections (methods 17-28). */
 int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* ctx, int method, int x, const int* A, int b, double omega, int niter,
                                 double* ms);
 /* niter PCG iterations (no convergence test) with HIP events between the launches; ms[0..2] =
@@ -209,8 +209,10 @@ int ffvhc_residual(ffvhc_ctx* ctx, int x, const int* A, int b, double* residual)
  * with a seeded hash vector as the shadow residual r0 (computed where it is read, never stored),
  * 11-16 as 8 with 1-6 red-black SOR sweeps (block-local, from zero, FP64 arithmetic) on a compact
  * copy of the matrix: 1 / Ap in FP32 and A_q / Ap in FP16, 16 bytes per cell, 17-22 as 11-16
- * plus a coarse correction P A_c^-1 P^T (one value per block, A_c = P^T A P inverted densely).
- * For 5 and 7-22 the answer is accepted on the FP64 residual of the FP64 matrix; a case above
+ * plus a coarse correction P A_c^-1 P^T (one value per block, A_c = P^T A P inverted densely),
+ * 23-28 the same correction applied first and the sweeps on the remaining residual
+ * (multiplicative two-level).
+ * For 5 and 7-28 the answer is accepted on the FP64 residual of the FP64 matrix; a case above
  * eps restarts from it (inner_eps unused; omega is the GS / SOR relaxation factor).
  * inner_eps: relative reduction per correction; countMax caps each inner solve (a capped inner
  * solve still yields a correction).  count[c] = inner iterations in total, outer[c] = corrections

@@ -40,7 +40,7 @@ def test_unresolved_name_is_caught():
 def test_mixed_byte_model_and_names():
     sys.path[:0] = [ROOT, os.path.join(ROOT, "ffv-hc_amd")]
     import bench
-    for m in range(5, 23):
+    for m in range(5, 29):
         by = bench.kernel_bytes_mixed(m)
         assert set(by) == {"bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"}
         # q_ and t_ differ by r0 (hashed in 10); p_ and s_ by the p read
@@ -56,6 +56,7 @@ def test_mixed_byte_model_and_names():
     assert bench.kernel_bytes_mixed(10)["bi_xr"] == 56
     assert abs(bench.kernel_bytes_mixed(12)["bi_p"] - (24 + 16 + 16 + 3)) < 1e-9
     assert bench.mixed_prof_name(13, "bi_s") == "bi_precond_gsk_kernel<1, 3, false>"
-    assert bench.mixed_prof_name(21, "bi_p") == "bi_precond_gsk_kernel<0, 5, true>"
+    assert bench.mixed_prof_name(21, "bi_p") == "bi_precond_gsk_kernel<0, 5, false>"
+    assert bench.mixed_prof_name(25, "bi_s") == "bi_precond_gsk_kernel<1, 3, true>"
     assert bench.mixed_prof_name(21, "bi_q") == "bi_matvec_kernel<double, 0, float, false, true>"
-    assert len(bench.MIXED_METHODS) == 23
+    assert len(bench.MIXED_METHODS) == 29

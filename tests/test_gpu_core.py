@@ -315,7 +315,7 @@ def true_residual(og, A, x, b):
     return np.sqrt(rr / bb)
 
 
-@pytest.mark.parametrize("method", [0, 1, 3, 5, 6, 7, 8, 9, 10, 11, 12, 13, 16, 17, 19, 21])
+@pytest.mark.parametrize("method", [0, 1, 3, 5, 6, 7, 8, 9, 10, 11, 12, 13, 16, 17, 19, 21, 23, 25, 27])
 def test_mixed_precision_reaches_fp64_residual(ctx, method):
     tree = trees()["two_level"]
     params = ffvhc.case_params(bc=BC)
@@ -396,7 +396,7 @@ def test_coarse_operator_is_ptap(ctx, name):
         ctx.remove_case(cid)
 
 
-@pytest.mark.parametrize("method", [None, 8, 19])
+@pytest.mark.parametrize("method", [None, 8, 19, 25])
 def test_graph_replay_equals_launches(ctx, method):
     """BiCGSTAB with its iteration replayed as a captured HIP graph gives bit-identical results to
     launching the kernels one by one, over repeated solves (each solve captures and replays its
