@@ -247,9 +247,9 @@ MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64
                      "%d block red-black SOR sweep%s on the remaining residual (multiplicative two-level)"
                      % (n, "" if n == 1 else "s") for n in range(1, 7)]
 
-# schedule of `bench.py --profile` (the rocprofv3 runs): FP32-stored matrix, 3 SOR sweeps + coarse
-# correction, omega 1.4
-PROFILE_SCHEDULE = (19, 0.5, 20000, 1.4)
+# schedule of `bench.py --profile` (the rocprofv3 runs): FP32-stored matrix, coarse correction then
+# 4 SOR sweeps, omega 1.6
+PROFILE_SCHEDULE = (26, 0.5, 20000, 1.6)
 
 # kernel names of the timed stages as rocprofv3 reports them
 PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0, %s>", "bi_q": "bi_matvec_kernel<%s, 0, %s, false, false>",

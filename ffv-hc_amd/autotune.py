@@ -18,13 +18,14 @@ DEFAULT_CANDIDATES = (
     (1, 1e-2, 500, 1.0),                 # FP32 Jacobi-PCG in FP64 defect correction
     (8, 0.5, 20000, 1.2),                # FP64 recurrence, FP32 matrix, one block GS sweep
     (15, 0.5, 20000, 1.6),               # 5 block SOR sweeps on the compact preconditioner matrix
-    # the same with the coarse (one value per block) correction: 2, 3, 4, 5 sweeps
-    (18, 0.5, 20000, 1.4),
-    (19, 0.5, 20000, 1.2),
-    (19, 0.5, 20000, 1.4),
-    (19, 0.5, 20000, 1.6),
-    (20, 0.5, 20000, 1.4),
-    (21, 0.5, 20000, 1.6),
+    (19, 0.5, 20000, 1.6),               # 3 sweeps + additive coarse correction
+    # coarse correction first, then 2-6 sweeps on the remaining residual (multiplicative)
+    (24, 0.5, 20000, 1.6),
+    (25, 0.5, 20000, 1.4),
+    (25, 0.5, 20000, 1.6),
+    (26, 0.5, 20000, 1.6),
+    (27, 0.5, 20000, 1.6),
+    (28, 0.5, 20000, 1.6),
 )
 
 

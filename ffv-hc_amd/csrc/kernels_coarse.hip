@@ -253,15 +253,46 @@ __global__ __launch_bounds__(256) void coarse_apply_kernel(const int* __restrict
 
 // Multiplicative variant: the restriction c = P^T v is formed from block sums the previous
 // kernels wrote (ce layout per slot: [2] r, [3] q_, [4] p), by the vector updates themselves:
-// p = r (first iteration) or r + beta (p - gamma q_), s = r - alpha q_.
-__global__ __launch_bounds__(256) void coarse_apply_mul_kernel(const int* __restrict__ active, int nact,
-                                                               const DevBlock* __restrict__ blk,
-                                                               const DevCase* __restrict__ cs,
-                                                               const SolveState* __restrict__ st,
-                                                               const float* __restrict__ inv,
-                                                               const long long* __restrict__ off,
-                                                               double* __restrict__ ce, size_t nslots, int mode) {
-  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+// p = r (first iteration) or r + beta (p - gamma q_), s = r - alpha q_.  A workgroup of CA_NT
+// threads handles CA_NT / 64 rows of one case; it forms c once in LDS (up to CA_MAXN blocks per
+// case, larger cases read it from the block sums per row).
+constexpr int CA_NT = 1024, CA_MAXN = 4096;
+
+// c_b of block b of a case from the block sums
+struct CoarseRhs {
+  const double *br, *bq, *bp;
+  bool first;
+  double beta, ng, na;
+  int mode;
+  __device__ CoarseRhs(const double* ce, size_t nslots, const DevCase& C, const SolveState& S, int m)
+      : br(ce + 2 * nslots + C.slot_begin), bq(ce + 3 * nslots + C.slot_begin), bp(ce + 4 * nslots + C.slot_begin),
+        first(S.iter == 0 || S.restart), beta(S.beta), ng(-S.gamma), na(-S.alpha), mode(m) {}
+  __device__ double operator()(int b) const {
+    return mode == 0 ? (first ? br[b] : br[b] + beta * (bp[b] + ng * bq[b])) : na * bq[b] + br[b];
+  }
+};
+
+__global__ __launch_bounds__(CA_NT) void coarse_apply_mul_kernel(const int* __restrict__ active, int nact,
+                                                                 const DevBlock* __restrict__ blk,
+                                                                 const DevCase* __restrict__ cs,
+                                                                 const SolveState* __restrict__ st,
+                                                                 const float* __restrict__ inv,
+                                                                 const long long* __restrict__ off,
+                                                                 double* __restrict__ ce, size_t nslots, int mode) {
+  __shared__ double cl[CA_MAXN];
+  constexpr int R = CA_NT / 64;  // rows per workgroup
+  const int first_row = blockIdx.x * R;
+  const int last_row = min(first_row + R - 1, nact - 1);
+  // the active list holds each case's blocks contiguously: a workgroup whose first and last rows
+  // share a case stages that case's c once in LDS
+  const int c0 = blk[active[first_row]].case_id;
+  const bool uni = c0 == blk[active[last_row]].case_id && cs[c0].nblk <= CA_MAXN && !st[c0].done;
+  if (uni) {
+    const CoarseRhs cb(ce, nslots, cs[c0], st[c0], mode);
+    for (int b = threadIdx.x; b < cs[c0].nblk; b += CA_NT) cl[b] = cb(b);
+  }
+  __syncthreads();
+  const int w = first_row + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (w >= nact) return;
   const int slot = active[w];
   const int cid = blk[slot].case_id;
@@ -269,17 +300,14 @@ __global__ __launch_bounds__(256) void coarse_apply_mul_kernel(const int* __rest
   if (S.done) return;
   const DevCase& C = cs[cid];
   const int n = C.nblk, a = slot - C.slot_begin;
+  const CoarseRhs cb(ce, nslots, C, S, mode);
   const float* row = inv + off[cid] + (size_t)a * n;
-  const double* br = ce + 2 * nslots + C.slot_begin;
-  const double* bq = ce + 3 * nslots + C.slot_begin;
-  const double* bp = ce + 4 * nslots + C.slot_begin;
-  const bool first = S.iter == 0 || S.restart;
-  const double beta = S.beta, ng = -S.gamma, na = -S.alpha;
   double s = 0.0;
-#pragma unroll 4
-  for (int b = lane; b < n; b += 64) {
-    const double cb = mode == 0 ? (first ? br[b] : br[b] + beta * (bp[b] + ng * bq[b])) : na * bq[b] + br[b];
-    s += (double)row[b] * cb;
+  if (uni) {
+#pragma unroll 8
+    for (int b = lane; b < n; b += 64) s += (double)row[b] * cl[b];
+  } else {
+    for (int b = lane; b < n; b += 64) s += (double)row[b] * cb(b);
   }
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if (lane == 0) ce[mode * nslots + slot] = s;
@@ -289,8 +317,8 @@ void launch_coarse_apply_mul(hipStream_t sm, const int* active, int nact, const 
                              const SolveState* st, const float* inv, const long long* off, double* ce, size_t nslots,
                              int mode) {
   if (nact)
-    hipLaunchKernelGGL(coarse_apply_mul_kernel, dim3((nact + 3) / 4), dim3(256), 0, sm, active, nact, blk, cs, st,
-                       inv, off, ce, nslots, mode);
+    hipLaunchKernelGGL(coarse_apply_mul_kernel, dim3((nact + CA_NT / 64 - 1) / (CA_NT / 64)), dim3(CA_NT), 0, sm,
+                       active, nact, blk, cs, st, inv, off, ce, nslots, mode);
 }
 
 void launch_coarse_apply(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,

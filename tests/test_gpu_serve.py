@@ -55,9 +55,9 @@ def serve(plan, step_kwargs, cap=2048):
     return out, rep, reqs
 
 
-@pytest.mark.parametrize("solver", [0, 3, "m15", "m19"])
+@pytest.mark.parametrize("solver", [0, 3, "m15", "m19", "m26"])
 def test_case_alone_equals_case_in_mix(solver):
-    if solver in ("m15", "m19"):  # FP64 BiCGSTAB on the FP32 matrix, SOR sweeps (+ coarse correction)
+    if solver in ("m15", "m19", "m26"):  # FP64 BiCGSTAB on the FP32 matrix, SOR sweeps (+ coarse correction)
         kw = dict(solver=3, omega=1.6, count_max=300, eps=1e-8, mixed=True, mixed_method=int(solver[1:]))
     else:
         kw = dict(solver=solver, omega=1.2 if solver == 0 else 1.0, count_max=300, eps=1e-6)

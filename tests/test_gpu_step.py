@@ -290,7 +290,7 @@ def test_force_v_matches_oracle(ctx, name):
         ctx.remove_case(cid)
 
 
-@pytest.mark.parametrize("method", [15, 19])
+@pytest.mark.parametrize("method", [15, 19, 26])
 def test_step_with_fast_poisson_matches_fp64(ctx, method):
     """Two time steps of a flow past a cut-cell sphere: the pressure solved by the FP32-matrix
     BiCGSTAB with block SOR (+ coarse correction) gives the same fields as the FP64 BiCGSTAB at
