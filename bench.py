@@ -266,14 +266,17 @@ def prof_name(stage, w):
 
 
 def mixed_prof_name(method, stage):
-    """rocprofv3 name of a stage's kernel in the FP64-recurrence methods 5-9."""
+    """rocprofv3 name of a stage's kernel in the FP64-recurrence methods 5-28."""
     mode = {"bi_p": 0, "bi_q": 0, "bi_s": 1, "bi_t": 1}.get(stage)
     hr = "true" if method == 10 else "false"
+    # additive coarse correction (17-22): the matvec and XR add e to p_ / s_; multiplicative
+    # (23-28): the sweep kernel stores p_ / s_ with e already added
+    cc = 17 <= method <= 22
     if stage == "bi_xr":
-        return "bi_xr_kernel<double, %s, %s>" % (hr, "true" if method >= 17 else "false")
+        return "bi_xr_kernel<double, %s, %s>" % (hr, "true" if cc else "false")
     if stage in ("bi_q", "bi_t"):
         if method >= 17:
-            return "bi_matvec_kernel<double, %d, float, false, true>" % mode
+            return "bi_matvec_kernel<double, %d, float, false, %s>" % (mode, "true" if cc else "false")
         return "bi_matvec_kernel<double, %d, %s, %s, false>" % (mode, "float" if method >= 7 else "double", hr)
     if method == 7:
         return "bi_precond_kernel<double, %d, float>" % mode

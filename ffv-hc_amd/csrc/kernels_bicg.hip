@@ -291,7 +291,11 @@ void launch_gs_pack(hipStream_t sm, const int* active, int nact, Coef7<double> A
 // on the compact matrix above, in FP64: u = v / Ap; red cells first; each update
 // z <- z + omega (u - sum_q w_q z_q - z).  The block's z lives in LDS; each of the GSK_NT threads
 // keeps u and the couplings of half an (i, j) column (8 cells) in registers, so the sweeps after
-// the first read no HBM.
+// the first read no HBM.  The couplings to cells outside the block are zeroed in the registers
+// (ghost values are zero in the block-local system), so a sweep reads its six LDS neighbours at
+// fixed offsets with no bounds test: the x / y offsets of an edge cell then wrap to a cell of
+// the same colour of another column (multiplied by zero) and the k offsets of the end planes to
+// a zero plane below and above the block.
 constexpr int GSK_NT = 512;
 constexpr int GSK_NK = NB * 256 / GSK_NT;  // cells of a column per thread
 // vs_out (optional): the block sum of v (restriction P^T v of the coarse correction).
@@ -307,15 +311,25 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
                                                                const uint4* __restrict__ rec, BiVecs<double> V,
                                                                int count, double* __restrict__ vs_out,
                                                                const double* __restrict__ ce) {
-  __shared__ double z[NC];
+  __shared__ double zpad[NC + 2 * NF];  // z with a zero plane below (k = -1) and above (k = NB)
   __shared__ double red[GSK_NT / 64];
   const int slot = active[blockIdx.x];
   const DevBlock& B = blk[slot];
   const int c = B.case_id;
   if (st[c].done) return;
+  double* const z = zpad + NF;
   const int t = threadIdx.x, tx = t & 15, ty = (t >> 4) & 15;
   const int kb = (t >> 8) * GSK_NK;  // this thread's cells: k = kb .. kb + GSK_NK - 1
+  static_assert(GSK_NT == 2 * NF, "one pad plane entry per thread");
+  zpad[t < NF ? t : NC + t] = 0.0;
   const size_t base = (size_t)slot * NC;
+  // the block's vectors from uniform base pointers, indexed by 32-bit cell offsets (one offset
+  // register serves every vector)
+  const unsigned o0 = tx + NB * ty + NF * kb;
+  const double* const rb = V.r + base;
+  const double* const qb = V.q + base;
+  double* const vb = (MODE == 0 ? V.p : V.s) + base;  // MODE 0 reads the old p here too
+  const uint4* const recb = rec + base;
   const double omega = st[c].omega;
   const double ng = -st[c].gamma, beta = st[c].beta, na = -st[c].alpha;
   // count 0: the iteration number comes from the case state (a replayed graph has no per-launch
@@ -328,18 +342,23 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
   double ur[GSK_NK / 2], ub[GSK_NK / 2];
   uint32_t wr[GSK_NK / 2][3], wb[GSK_NK / 2][3];
   double vsum = 0.0;
-  // MUL: e of the block each neighbour lies in -- x-, x+, y-, y+ (the same for the thread's 8
-  // cells: a half column never straddles a subface boundary), z- (k = 0), z+ (k = 15), own
-  double en[7];
+  // couplings kept for the sweeps: those to cells outside the block zeroed (per half word)
+  const uint32_t mx = (tx == 0 ? 0xffff0000u : ~0u) & (tx == NB - 1 ? 0x0000ffffu : ~0u);
+  const uint32_t my = (ty == 0 ? 0xffff0000u : ~0u) & (ty == NB - 1 ? 0x0000ffffu : ~0u);
+  // MUL: e of this block (eo) and of the block the ghost value of the x, y and z face this
+  // thread's cells touch comes from (ex: face x- when tx = 0, else x+; ey likewise; ez: z- for the
+  // lower half column, z+ for the upper).  A half column never straddles a subface boundary, so
+  // one value per axis covers the thread's 8 cells; the indices are formed for every lane (a lane
+  // off the face does not use its value) so that the loads issue together.
+  double eo = 0.0, ex = 0.0, ey = 0.0, ez = 0.0;
   if constexpr (MUL) {
-    const double eo = ce[slot];
-    en[0] = tx > 0 ? eo : ce[ghost_source(B, slot, 0, ty, kb)];
-    en[1] = tx < NB - 1 ? eo : ce[ghost_source(B, slot, 1, ty, kb)];
-    en[2] = ty > 0 ? eo : ce[ghost_source(B, slot, 2, tx, kb)];
-    en[3] = ty < NB - 1 ? eo : ce[ghost_source(B, slot, 3, tx, kb)];
-    en[4] = ce[ghost_source(B, slot, 4, tx, ty)];
-    en[5] = ce[ghost_source(B, slot, 5, tx, ty)];
-    en[6] = eo;
+    const int sx = ghost_source(B, slot, tx == 0 ? 0 : 1, ty, kb);
+    const int sy = ghost_source(B, slot, ty == 0 ? 2 : 3, tx, kb);
+    const int sz = ghost_source(B, slot, kb == 0 ? 4 : 5, tx, ty);
+    eo = ce[slot];
+    ex = ce[sx];
+    ey = ce[sy];
+    ez = ce[sz];
   }
 #pragma unroll
   for (int k0 = 0; k0 < GSK_NK; k0 += 4) {
@@ -347,43 +366,41 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
     uint4 rc[4];
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
-      const size_t g = base + tx + NB * (ty + NB * (kb + k0 + kk));
-      rv[kk] = V.r[g];
-      if (MODE == 1 || !first) qv[kk] = V.q[g];
-      if (MODE == 0 && !first) pv[kk] = V.p[g];
-      rc[kk] = rec[g];
+      const unsigned o = o0 + NF * (k0 + kk);
+      rv[kk] = rb[o];
+      if (MODE == 1 || !first) qv[kk] = qb[o];
+      if (MODE == 0 && !first) pv[kk] = vb[o];
+      rc[kk] = recb[o];
     }
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
       const int k = k0 + kk;  // relative to kb
-      const size_t g = base + tx + NB * (ty + NB * (kb + k));
       double v;
-      if (MODE == 0) {
+      if (MODE == 0)
         v = first ? rv[kk] : rv[kk] + beta * (pv[kk] + ng * qv[kk]);
-        V.p[g] = v;
-      } else {
+      else
         v = na * qv[kk] + rv[kk];
-        V.s[g] = v;
-      }
+      vb[o0 + NF * k] = v;
       vsum += v;
       double uk = v * (double)__uint_as_float(rc[kk].x);
+      const int ka = kb + k;
       if constexpr (MUL) {
-        const int ka = kb + k;
-        uk -= en[6] + half_lo(rc[kk].y) * en[0] + half_hi(rc[kk].y) * en[1] + half_lo(rc[kk].z) * en[2] +
-              half_hi(rc[kk].z) * en[3] + half_lo(rc[kk].w) * (ka > 0 ? en[6] : en[4]) +
-              half_hi(rc[kk].w) * (ka < NB - 1 ? en[6] : en[5]);
+        uk -= eo + half_lo(rc[kk].y) * (tx > 0 ? eo : ex) + half_hi(rc[kk].y) * (tx < NB - 1 ? eo : ex) +
+              half_lo(rc[kk].z) * (ty > 0 ? eo : ey) + half_hi(rc[kk].z) * (ty < NB - 1 ? eo : ey) +
+              half_lo(rc[kk].w) * (ka > 0 ? eo : ez) + half_hi(rc[kk].w) * (ka < NB - 1 ? eo : ez);
       }
       const int m = k >> 1;
+      const uint32_t mz = (ka == 0 ? 0xffff0000u : ~0u) & (ka == NB - 1 ? 0x0000ffffu : ~0u);
       if ((k & 1) == par) {
         ur[m] = uk;
-        wr[m][0] = rc[kk].y;
-        wr[m][1] = rc[kk].z;
-        wr[m][2] = rc[kk].w;
+        wr[m][0] = rc[kk].y & mx;
+        wr[m][1] = rc[kk].z & my;
+        wr[m][2] = rc[kk].w & mz;
       } else {
         ub[m] = uk;
-        wb[m][0] = rc[kk].y;
-        wb[m][1] = rc[kk].z;
-        wb[m][2] = rc[kk].w;
+        wb[m][0] = rc[kk].y & mx;
+        wb[m][1] = rc[kk].z & my;
+        wb[m][2] = rc[kk].w & mz;
       }
     }
   }
@@ -391,13 +408,14 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
     const double sv = block_sum<GSK_NT>(vsum, red);
     if (t == 0) vs_out[slot] = sv;
   }
-  // first red half sweep: every neighbour is still zero
+  // first red half sweep: every neighbour is still zero; the black cells start at zero (the
+  // wrapped reads of the first black half sweep must see finite values)
 #pragma unroll
-  for (int m = 0; m < GSK_NK / 2; m++) z[zi(tx, ty, kb + 2 * m + par)] = omega * ur[m];
+  for (int m = 0; m < GSK_NK / 2; m++) {
+    z[zi(tx, ty, kb + 2 * m + par)] = omega * ur[m];
+    z[zi(tx, ty, kb + 2 * m + 1 - par)] = 0.0;
+  }
   __syncthreads();
-  auto zat = [&](int i, int j, int k) -> double {
-    return (i < 0 || i >= NB || j < 0 || j >= NB || k < 0 || k >= NB) ? 0.0 : z[zi(i, j, k)];
-  };
   // one half sweep over this column's cells of one colour; from_zero: the cells' old values are
   // zero (first visit)
   auto half = [&](auto red_tag, bool from_zero) {
@@ -410,15 +428,19 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
       // keep the couplings packed across sweeps (an empty asm hides that the unpacked values
       // of the last sweep could be reused, which would cost 12 more registers per cell)
       asm volatile("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]));
-      const double sum = half_lo(w[0]) * zat(tx - 1, ty, k) + half_hi(w[0]) * zat(tx + 1, ty, k) +
-                         half_lo(w[1]) * zat(tx, ty - 1, k) + half_hi(w[1]) * zat(tx, ty + 1, k) +
-                         half_lo(w[2]) * zat(tx, ty, k - 1) + half_hi(w[2]) * zat(tx, ty, k + 1);
+      const double* zc = z + zi(tx, ty, k);
+      double sum = half_lo(w[0]) * zc[-1];
+      sum = __builtin_fma(half_hi(w[0]), zc[1], sum);
+      sum = __builtin_fma(half_lo(w[1]), zc[-NB], sum);
+      sum = __builtin_fma(half_hi(w[1]), zc[NB], sum);
+      sum = __builtin_fma(half_lo(w[2]), zc[-NB * NB], sum);
+      sum = __builtin_fma(half_hi(w[2]), zc[NB * NB], sum);
       const double zn = (RED ? ur[m] : ub[m]) - sum;
       if (from_zero) {
         z[zi(tx, ty, k)] = omega * zn;
       } else {
         const double zo = z[zi(tx, ty, k)];
-        z[zi(tx, ty, k)] = zo + omega * (zn - zo);
+        z[zi(tx, ty, k)] = __builtin_fma(omega, zn - zo, zo);
       }
     }
     __syncthreads();
@@ -431,11 +453,15 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
     half(Red{}, false);
     half(Black{}, false);
   }
-  double* out = MODE == 0 ? V.pp : V.ss;
+  // MUL: the stored direction is z + P e (e added to this block's values and to the ghost values
+  // it pushes, which then come from z + e through every face kind)
+  double* const out = (MODE == 0 ? V.pp : V.ss) + base;
 #pragma unroll
-  for (int k = kb; k < kb + GSK_NK; k++) out[base + tx + NB * (ty + NB * k)] = z[zi(tx, ty, k)];
+  for (int k = 0; k < GSK_NK; k++) out[o0 + NF * k] = MUL ? z[o0 + NF * k] + eo : z[o0 + NF * k];
   double* h = MODE == 0 ? V.hpp : V.hss;
-  push_halo<double, GSK_NT>(B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return z[zi(i, j, k)]; }, h, h);
+  push_halo<double, GSK_NT>(
+      B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return MUL ? z[zi(i, j, k)] + eo : z[zi(i, j, k)]; }, h,
+      h);
 }
 
 template <int MODE, bool MUL>
@@ -458,8 +484,11 @@ void launch_gsk(int sweeps, hipStream_t sm, const int* active, int nact, const D
 }
 
 // Q: q_ = A p_, q_.r0 -> alpha (mode 0);  T: t_ = A s_, t_.s and t_.t_ -> gamma (mode 1)
-// CC: the staged p_ / s_ (interior and ghosts) get the coarse correction ce[MODE * nslots + the
+// CC: the operand's values (interior and ghosts) get the coarse correction ce[MODE * nslots + the
 // source block] added (two-level preconditioners).
+// LDS holds the operand's interior (NC values) and the ghost layers of the four side faces
+// (4 NF values, same layout as the halo); the two z ghosts of a column stay in registers.  In
+// FP64 that is 40 KiB, so four workgroups fit a CU.
 template <class T, int MODE, class TA = T, bool HR = false, bool CC = false>
 __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict__ active,
                                                           const DevBlock* __restrict__ blk,
@@ -467,8 +496,10 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
                                                           Coef7<TA> A, BiVecs<T> V, double* __restrict__ part,
                                                           size_t nslots, const double* __restrict__ ce = nullptr,
                                                           double* __restrict__ bsy = nullptr) {
-  __shared__ T s[LP * LP * LP];
-  __shared__ double red[BI_NT / 64];
+  __shared__ T s[NC + 4 * NF];
+  double* const red = reinterpret_cast<double*>(s);  // after the last read of s (block_sum syncs first)
+  static_assert(NC * sizeof(T) >= BI_NT / 64 * sizeof(double), "reduction scratch fits");
+  static_assert(BI_NT == NF, "one thread per (i, j) column and per face position");
   const int slot = active[blockIdx.x];
   const int c = blk[slot].case_id;
   if (st[c].done) return;
@@ -477,43 +508,62 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
   const T* __restrict__ w = MODE == 0 ? V.r0 : V.s;
   T* __restrict__ y = MODE == 0 ? V.q : V.t;
   const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
+  const int o0 = tx + NB * ty;
   constexpr int KB = Batch<T>::K;
   double d0 = 0.0, d1 = 0.0, ys = 0.0;
+  T zm, zp;  // ghosts below k = 0 / above k = NB - 1
 #pragma unroll
   for (int k0 = 0; k0 < NB; k0 += KB) {
     T a[KB][7], wv[KB];
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
-      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      const size_t g = base + o0 + NF * (k0 + kk);
 #pragma unroll
       for (int q = 0; q < 7; q++) a[kk][q] = (T)A.p[q][g];
       wv[kk] = (MODE == 0 && HR) ? (T)shadow_value(local, (uint32_t)(g - base)) : w[g];
     }
     if (k0 == 0) {
-      stage_block<T, BI_NT>(s, MODE == 0 ? V.pp : V.ss, MODE == 0 ? V.hpp : V.hss, slot);
-      if constexpr (CC) {  // each thread corrects the LDS values it staged
+      const T* __restrict__ xo = (MODE == 0 ? V.pp : V.ss) + base;
+      const T* __restrict__ h = (MODE == 0 ? V.hpp : V.hss) + (size_t)slot * NH;
+      T v[NB], gv[4];
+#pragma unroll
+      for (int k = 0; k < NB; k++) v[k] = xo[o0 + NF * k];
+#pragma unroll
+      for (int f = 0; f < 4; f++) gv[f] = h[f * NF + t];
+      zm = h[halo_index(4, tx, ty)];
+      zp = h[halo_index(5, tx, ty)];
+      if constexpr (CC) {  // each thread corrects the values it stages
         const DevBlock& Bk = blk[slot];
         const double* e = ce + MODE * nslots;
         const T eo = (T)e[slot];
 #pragma unroll
-        for (int k = 0; k < NB; k++) s[lds(tx, ty, k)] += eo;
+        for (int k = 0; k < NB; k++) v[k] += eo;
 #pragma unroll
-        for (int q = 0; q < NH / BI_NT; q++) {
-          const int w = t + q * BI_NT, f = w >> 8, pa = w & 15, pb = (w >> 4) & 15;
-          int i, j, k;
-          face_cell(f, pa, pb, -1, i, j, k);
-          s[lds(i, j, k)] += (T)e[ghost_source(Bk, slot, f, pa, pb)];
-        }
+        for (int f = 0; f < 4; f++) gv[f] += (T)e[ghost_source(Bk, slot, f, t & 15, t >> 4)];
+        zm += (T)e[ghost_source(Bk, slot, 4, tx, ty)];
+        zp += (T)e[ghost_source(Bk, slot, 5, tx, ty)];
       }
+#pragma unroll
+      for (int k = 0; k < NB; k++) s[o0 + NF * k] = v[k];
+#pragma unroll
+      for (int f = 0; f < 4; f++) s[NC + f * NF + t] = gv[f];
       __syncthreads();
     }
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
-      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
-      const int li = lds(tx, ty, k0 + kk);
+      const int k = k0 + kk;
+      const int li = o0 + NF * k;
+      const size_t g = base + li;
+      // a face lane reads the ghost layer (halo layout: face f at (a, b) = NC + f NF + a + NB b)
+      const T vxm = s[tx > 0 ? li - 1 : NC + halo_index(0, ty, k)];
+      const T vxp = s[tx < NB - 1 ? li + 1 : NC + halo_index(1, ty, k)];
+      const T vym = s[ty > 0 ? li - NB : NC + halo_index(2, tx, k)];
+      const T vyp = s[ty < NB - 1 ? li + NB : NC + halo_index(3, tx, k)];
+      const T vzm = k > 0 ? s[li - NF] : zm;
+      const T vzp = k < NB - 1 ? s[li + NF] : zp;
       // calc_ax order (bils.f90:195-202)
-      const T v = a[kk][1] * s[li - 1] + a[kk][2] * s[li + 1] + a[kk][3] * s[li - LP] + a[kk][4] * s[li + LP] +
-                  a[kk][5] * s[li - LP * LP] + a[kk][6] * s[li + LP * LP] + a[kk][0] * s[li];
+      const T v = a[kk][1] * vxm + a[kk][2] * vxp + a[kk][3] * vym + a[kk][4] * vyp + a[kk][5] * vzm +
+                  a[kk][6] * vzp + a[kk][0] * s[li];
       y[g] = v;
       ys += (double)v;
       d0 += (double)v * (double)wv[kk];
@@ -708,10 +758,12 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   };
   auto matvec = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
-    if (dbl && o.A32 && o.cinv)
+    if (dbl && o.A32 && o.cinv && o.mult)  // p_ / s_ already hold P e; block sum of q_ kept
+      hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs,
+                         st, *o.A32, Vd, part, nslots, nullptr, MODE == 0 ? o.ce + 3 * nslots : nullptr);
+    else if (dbl && o.A32 && o.cinv)
       hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, false, true>), dim3(nact), dim3(BI_NT), 0, sm, active,
-                         blk, cs, st, *o.A32, Vd, part, nslots, (const double*)o.ce,
-                         (o.mult && MODE == 0) ? o.ce + 3 * nslots : nullptr);
+                         blk, cs, st, *o.A32, Vd, part, nslots, (const double*)o.ce, nullptr);
     else if (dbl && o.A32 && o.hash_r0)
       hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk,
                          cs, st, *o.A32, Vd, part, nslots);
@@ -753,9 +805,12 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_GAMMA, count);
   }
   if (mask & 16) {
-    if (o.cinv)
+    if (o.cinv && o.mult)  // p_ / s_ already hold P e; block sum of r kept
+      hipLaunchKernelGGL((bi_xr_kernel<T, false>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
+                         nslots, count, nullptr, o.ce + 2 * nslots);
+    else if (o.cinv)
       hipLaunchKernelGGL((bi_xr_kernel<T, false, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V,
-                         part, nslots, count, (const double*)o.ce, o.mult ? o.ce + 2 * nslots : nullptr);
+                         part, nslots, count, (const double*)o.ce, nullptr);
     else if (o.hash_r0)
       hipLaunchKernelGGL((bi_xr_kernel<T, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
                          nslots, count);

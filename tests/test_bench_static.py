@@ -59,4 +59,8 @@ def test_mixed_byte_model_and_names():
     assert bench.mixed_prof_name(21, "bi_p") == "bi_precond_gsk_kernel<0, 5, false>"
     assert bench.mixed_prof_name(25, "bi_s") == "bi_precond_gsk_kernel<1, 3, true>"
     assert bench.mixed_prof_name(21, "bi_q") == "bi_matvec_kernel<double, 0, float, false, true>"
+    # multiplicative methods store p_ / s_ with the coarse correction added: plain matvec and XR
+    assert bench.mixed_prof_name(26, "bi_t") == "bi_matvec_kernel<double, 1, float, false, false>"
+    assert bench.mixed_prof_name(26, "bi_xr") == "bi_xr_kernel<double, false, false>"
+    assert bench.mixed_prof_name(19, "bi_xr") == "bi_xr_kernel<double, false, true>"
     assert len(bench.MIXED_METHODS) == 29
