@@ -3,8 +3,8 @@
 Workload (the configuration the headline metric is quoted on): standalone pressure Poisson on a
 2-level block octree -- roots 8x8x8 of 16^3-cell blocks, level 1 inside a ball of radius L/4 at the
 centre (1-cell margin: 1,128 blocks, 4.62 M cells); 7-band matrix of bcut_calc_ab_p without cut
-cells (bcut.f90:608-832), outer faces Neumann except X+ Dirichlet p = 0 (the
-all-Neumann 2-level operator is singular and not symmetric, so no solver reaches 1e-10 on it);
+cells (bcut.f90:608-832), outer faces Neumann except X+ Dirichlet p = 0 (the all-Neumann 2-level
+operator is singular and not symmetric, so no solver reaches 1e-10 on it);
 seeded uniform[-1,1] RHS (PCG64 seed 0) with the mean removed.
 
 A step is one Poisson solve from x = 0 to relative residual ||b-Ax||/||b|| <= 1e-10 (FFVILS.h:24-47),
@@ -252,8 +252,8 @@ MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64
 PROFILE_SCHEDULE = (26, 0.5, 20000, 1.6)
 
 # kernel names of the timed stages as rocprofv3 reports them
-PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0, %s>", "bi_q": "bi_matvec_kernel<%s, 0, %s, false, false>",
-              "bi_s": "bi_precond_kernel<%s, 1, %s>", "bi_t": "bi_matvec_kernel<%s, 1, %s, false, false>",
+PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0, %s>", "bi_q": "bi_matvec_kernel<%s, 0, %s, false, false, false>",
+              "bi_s": "bi_precond_kernel<%s, 1, %s>", "bi_t": "bi_matvec_kernel<%s, 1, %s, false, false, false>",
               "bi_xr": "bi_xr_kernel<%s, false, false>", "rbgs": "rbgs_iter_kernel<%s, 0>",
               "cg_p": "cg_p_kernel<%s>", "cg_q": "cg_q_kernel<%s>", "cg_xr": "cg_xr_kernel<%s>"}
 
@@ -265,24 +265,26 @@ def prof_name(stage, w):
     return f % ((t,) * f.count("%s"))
 
 
-def mixed_prof_name(method, stage):
-    """rocprofv3 name of a stage's kernel in the FP64-recurrence methods 5-28."""
+def mixed_prof_name(method, stage, uniform=False):
+    """rocprofv3 name of a stage's kernel in the FP64-recurrence methods 5-28; uniform: the
+    variant of the class-uniform blocks (methods 11-28 read their bands from the class rows)."""
     mode = {"bi_p": 0, "bi_q": 0, "bi_s": 1, "bi_t": 1}.get(stage)
     hr = "true" if method == 10 else "false"
+    tf = lambda v: "true" if v else "false"
+    reg = tf(uniform and method >= 11)
     # additive coarse correction (17-22): the matvec and XR add e to p_ / s_; multiplicative
     # (23-28): the sweep kernel stores p_ / s_ with e already added
     cc = 17 <= method <= 22
     if stage == "bi_xr":
-        return "bi_xr_kernel<double, %s, %s>" % (hr, "true" if cc else "false")
+        return "bi_xr_kernel<double, %s, %s>" % (hr, tf(cc))
     if stage in ("bi_q", "bi_t"):
-        if method >= 17:
-            return "bi_matvec_kernel<double, %d, float, false, %s>" % (mode, "true" if cc else "false")
-        return "bi_matvec_kernel<double, %d, %s, %s, false>" % (mode, "float" if method >= 7 else "double", hr)
+        if method >= 11:
+            return "bi_matvec_kernel<double, %d, float, false, %s, %s>" % (mode, tf(cc), reg)
+        return "bi_matvec_kernel<double, %d, %s, %s, false, false>" % (mode, "float" if method >= 7 else "double", hr)
     if method == 7:
         return "bi_precond_kernel<double, %d, float>" % mode
     if method >= 11:
-        return "bi_precond_gsk_kernel<%d, %d, %s>" % (mode, (method - 11) % 6 + 1,
-                                                      "true" if method >= 23 else "false")
+        return "bi_precond_gsk_kernel<%d, %d, %s, %s>" % (mode, (method - 11) % 6 + 1, tf(method >= 23), reg)
     tm, tz, sym = {5: ("float", "float", "false"), 6: ("double", "double", "false"),
                    8: ("float", "double", "false"), 9: ("float", "double", "true"),
                    10: ("float", "double", "false")}[method]
@@ -309,13 +311,16 @@ def kernel_bytes(w):
             "rbgs": (10 + 2 * h) * w, "cg_p": (4 + h) * w, "cg_q": (9 + h) * w, "cg_xr": 7 * w}
 
 
-def kernel_bytes_mixed(method):
+def kernel_bytes_mixed(method, uniform=False):
     """Bytes per cell per launch of the FP64-recurrence methods: FP64 vectors, the matrix bands in
     FP32 (ma = 4 bytes) or FP64 where the method stores them so; the GS preconditioner reads all
-    seven bands, the Jacobi one the diagonal only."""
+    seven bands, the Jacobi one the diagonal only.  uniform: every block is class-uniform, so the
+    compact-record methods (11-28) read no per-cell band or record (27 rows per block)."""
     h = 0.375 * 8                                   # FP64 ghost layer of p_ / s_
     mv = 4 if method >= 7 else 8                    # matvec bands
     mp = {5: 4, 6: 8, 7: 4, 8: 4, 9: 4, 10: 4}.get(method, 16 / 7)  # preconditioner bands
+    if uniform and method >= 11:
+        mv, mp = 0, 0
     nb = 1 if method == 7 else 7                    # (11-13: one 16-byte record per cell)
     r0 = 0 if method == 10 else 8                   # hashed shadow residual: not stored
     # 17-22 add 8 B per cell (the block sum needs no extra read; the corrections are per block)
@@ -347,14 +352,18 @@ def main():
                     help="short run for rocprofv3: timing loops and one solve per precision")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: every box in this run has one GPU; multi-GPU runs are not part of this build "
-              "(the sharding design is in DESIGN.md)", file=sys.stderr)
+        print("bench.py: this benchmark runs on one GPU only; multi-GPU runs are not implemented (the "
+              "block octree would shard by contiguous Hilbert-ordered block ranges per GPU)", file=sys.stderr)
         sys.exit(2)
     import ffvhc
     tree = build_c2()
     cells = tree.nblk * N ** 3
     A, b, bct = poisson_system(tree)
     ctx, cid, f64, f32 = setup(tree, A, b, bct)
+    # blocks whose bands are uniform per stencil position class (no cut cells here: all of them)
+    n_uniform, n_other = ctx.stencil_classes(f64["A"])
+    uniform = n_other == 0
+    log("stencil classes: %d class-uniform blocks, %d others" % (n_uniform, n_other))
     P = ffvhc.BC_P
 
     log("%d blocks, %d cells resident; timing kernels" % (tree.nblk, cells))
@@ -399,9 +408,9 @@ def main():
         ctx.bicgstab_timing(f64["x"], f64["A"], f64["b"], sched[3], 5, method=sched[0])
         msall = ctx.bicgstab_timing(f64["x"], f64["A"], f64["b"], sched[3], nit, method=sched[0]) / nit
         ms = msall[:5]
-        by = kernel_bytes_mixed(sched[0])
+        by = kernel_bytes_mixed(sched[0], uniform)
         kt["mixed"] = {name: {"us": round(t * 1e3, 2), "GBs": round(by[name] * cells / (t * 1e-3) / 1e9, 1),
-                              "kernel": mixed_prof_name(sched[0], name)}
+                              "kernel": mixed_prof_name(sched[0], name, uniform)}
                        for name, t in zip(("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"), ms)}
         kt["mixed_other_us"] = {"finalisations": round(msall[5] * 1e3, 2), "coarse_corrections": round(msall[6] * 1e3, 2)}
         log("method %d kernels: %s" % (sched[0], json.dumps(kt["mixed"])))
@@ -457,7 +466,7 @@ def main():
     dname = max(dstages, key=lambda k: kt[dtag][k]["us"])
     dom = kt[dtag][dname]
     if dtag == "mixed":
-        dkernel, dbytes = dom["kernel"], kernel_bytes_mixed(sched[0])[dname]
+        dkernel, dbytes = dom["kernel"], kernel_bytes_mixed(sched[0], uniform)[dname]
     else:
         dw = 4 if dtag == "32" else 8
         dkernel, dbytes = prof_name(dname, dw), kernel_bytes(dw)[dname]
@@ -479,6 +488,7 @@ def main():
         "config": {"workload": "standalone Poisson, 2-level octree: roots 8^3 of 16^3-cell blocks, level 1 in a "
                                "ball of radius L/4 (1-cell margin)",
                    "blocks": tree.nblk, "cells": cells,
+                   "stencil_classes": {"uniform_blocks": n_uniform, "other_blocks": n_other},
                    "solver": "FP64 reference path: Jacobi-BiCGSTAB (PreCount 1, omega 1); mixed: the autotuned "
                              "schedule below, accepted on the FP64 residual",
                    "eps": args.eps},

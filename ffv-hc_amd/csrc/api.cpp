@@ -275,6 +275,9 @@ void ffvhc_ctx_destroy(ffvhc_ctx* c) {
   (void)hipFree(c->dcoff);
   (void)hipFree(c->dcsnap);
   (void)hipFree(c->dce);
+  (void)hipFree(c->dcuni);
+  (void)hipFree(c->dctab);
+  (void)hipFree(c->dctrec);
   (void)hipFree(c->dcut);
   (void)hipFree(c->dflag);
   (void)hipFree(c->dcid);

@@ -85,6 +85,13 @@ struct BiOpts {
   const long long* coff = nullptr;
   double* ce = nullptr;
   bool mult = false;
+  // stencil classes (GSC only): class rows of every block as FP32 bands tab[slot][27][8] and as
+  // records trec[slot][27], uni[slot] = 1 for a class-uniform block; nreg / nirr count the active
+  // class-uniform blocks and the others (trec null: every block uses the per-cell arrays)
+  const float* tab = nullptr;
+  const uint4* trec = nullptr;
+  const int* uni = nullptr;
+  int nreg = 0, nirr = 0;
 };
 
 template <class T>
@@ -108,6 +115,10 @@ void launch_add_correction(hipStream_t sm, const int* active, int nact, const De
 void launch_convert(hipStream_t sm, const int* active, int nact, float* y, const double* x);
 // Compact GS preconditioner records (16 bytes per cell: 1 / Ap in FP32, A_q / Ap in FP16)
 void launch_gs_pack(hipStream_t sm, const int* active, int nact, Coef7<double> A, uint4* rec);
+// Stencil position classes of each block (uniform[slot] = 1 when the block is class-uniform) and
+// its 27 class rows: FP32 bands tab[slot][27][8] and preconditioner records trec[slot][27]
+void launch_stencil_classes(hipStream_t sm, const int* active, int nact, Coef7<double> A, int* uniform, float* tab,
+                            uint4* trec);
 // coarse operator rows (25 doubles per slot), dense Gauss-Jordan inversion, FP32 copy, apply
 void launch_coarse_rows(hipStream_t sm, const int* active, int nact, const DevBlock* blk, Coef7<double> A,
                         double* rows);

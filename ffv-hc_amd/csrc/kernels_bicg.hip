@@ -269,22 +269,65 @@ __device__ inline uint32_t half_bits(double v) {
 __device__ inline double half_lo(uint32_t u) { return (double)__builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu)); }
 __device__ inline double half_hi(uint32_t u) { return (double)__builtin_bit_cast(_Float16, (uint16_t)(u >> 16)); }
 
+__device__ inline uint4 gs_record(const Coef7<double>& A, size_t g) {
+  const double d = A.p[0][g];
+  const double inv = d != 0.0 ? 1.0 / d : 0.0;  // a zero pivot decouples the cell in M
+  uint32_t w[6];
+#pragma unroll
+  for (int b = 0; b < 6; b++) w[b] = half_bits(A.p[b + 1][g] * inv);
+  return make_uint4(__float_as_uint((float)inv), w[0] | (w[1] << 16), w[2] | (w[3] << 16), w[4] | (w[5] << 16));
+}
+
 __global__ __launch_bounds__(256) void gs_pack_kernel(const int* __restrict__ active, Coef7<double> A,
                                                       uint4* __restrict__ rec) {
   const size_t base = (size_t)active[blockIdx.x] * NC;
-  for (int q = threadIdx.x; q < NC; q += 256) {
-    const size_t g = base + q;
-    const double d = A.p[0][g];
-    const double inv = d != 0.0 ? 1.0 / d : 0.0;  // a zero pivot decouples the cell in M
-    uint32_t w[6];
-#pragma unroll
-    for (int b = 0; b < 6; b++) w[b] = half_bits(A.p[b + 1][g] * inv);
-    rec[g] = make_uint4(__float_as_uint((float)inv), w[0] | (w[1] << 16), w[2] | (w[3] << 16), w[4] | (w[5] << 16));
-  }
+  for (int q = threadIdx.x; q < NC; q += 256) rec[base + q] = gs_record(A, base + q);
 }
 
 void launch_gs_pack(hipStream_t sm, const int* active, int nact, Coef7<double> A, uint4* rec) {
   if (nact) hipLaunchKernelGGL(gs_pack_kernel, dim3(nact), dim3(256), 0, sm, active, A, rec);
+}
+
+// Stencil position classes: a cell's class is (cx, cy, cz), each 0 on the block's low face layer,
+// 2 on its high one and 1 inside (27 classes, index cx + 3 cy + 9 cz).  A block is class-uniform
+// when every cell has bitwise the same seven coefficients as its class's representative cell
+// (the lowest-index cell of the class); a block without cut cells is, whatever its outer faces
+// and level interfaces (those change the coefficients of a whole face layer alike).  For such a
+// block the 27 rows -- as FP32 bands (8 floats each, the eighth unused) and as preconditioner
+// records -- stand for the per-cell arrays, bit for bit.
+__device__ inline int stencil_class1(int i) { return i == 0 ? 0 : i == NB - 1 ? 2 : 1; }
+__device__ inline int stencil_rep1(int c) { return c == 0 ? 0 : c == 2 ? NB - 1 : 1; }
+
+__global__ __launch_bounds__(256) void stencil_class_kernel(const int* __restrict__ active, Coef7<double> A,
+                                                            int* __restrict__ uniform, float* __restrict__ tab,
+                                                            uint4* __restrict__ trec) {
+  const int slot = active[blockIdx.x];
+  const size_t base = (size_t)slot * NC;
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const int rx = stencil_rep1(stencil_class1(tx)), ry = stencil_rep1(stencil_class1(ty));
+  bool same = true;
+  for (int k = 0; k < NB; k++) {
+    const size_t g = base + tx + NB * (ty + NB * k);
+    const size_t r = base + rx + NB * (ry + NB * stencil_rep1(stencil_class1(k)));
+#pragma unroll
+    for (int q = 0; q < 7; q++)
+      same = same && __double_as_longlong(A.p[q][g]) == __double_as_longlong(A.p[q][r]);
+  }
+  const int all = __syncthreads_and(same ? 1 : 0);
+  if (t < 27) {
+    const size_t r = base + stencil_rep1(t % 3) + NB * (stencil_rep1((t / 3) % 3) + NB * stencil_rep1(t / 9));
+    float* row = tab + ((size_t)slot * 27 + t) * 8;
+#pragma unroll
+    for (int q = 0; q < 7; q++) row[q] = (float)A.p[q][r];
+    row[7] = 0.0f;
+    trec[(size_t)slot * 27 + t] = gs_record(A, r);
+  }
+  if (t == 0) uniform[slot] = all;
+}
+
+void launch_stencil_classes(hipStream_t sm, const int* active, int nact, Coef7<double> A, int* uniform, float* tab,
+                            uint4* trec) {
+  if (nact) hipLaunchKernelGGL(stencil_class_kernel, dim3(nact), dim3(256), 0, sm, active, A, uniform, tab, trec);
 }
 
 // P / S with NS red-black SOR sweeps of the block-local system (ghost values zero) from z = 0,
@@ -303,14 +346,21 @@ constexpr int GSK_NK = NB * 256 / GSK_NT;  // cells of a column per thread
 // kernel) is subtracted first, the sweeps smooth r1 = v - A P e with A the compact matrix
 // (A_0 = 1 / dinv, A_q = w_q A_0): u1 = u - (e_a + sum_q w_q e of the block each neighbour
 // lies in).
-template <int MODE, int NS, bool MUL>
-__global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __restrict__ active,
+// REG 1 (every block of the launch class-uniform): the records are the block's class records
+// trec[slot][27] instead of the per-cell array, and the sweeps keep one set of couplings for the
+// thread's inner cells and one for its end cell (k = 0 or NB - 1) instead of one per cell, which
+// leaves registers for three workgroups per CU.  REG 2 (mixed launch): each block takes its
+// records from the class rows when uni[slot] is set, else per cell.
+template <int MODE, int NS, bool MUL, int REG>
+__global__ __launch_bounds__(GSK_NT) __attribute__((amdgpu_waves_per_eu(REG == 1 ? 6 : 1))) void bi_precond_gsk_kernel(const int* __restrict__ active,
                                                                const DevBlock* __restrict__ blk,
                                                                const DevCase* __restrict__ cs,
                                                                const SolveState* __restrict__ st,
                                                                const uint4* __restrict__ rec, BiVecs<double> V,
                                                                int count, double* __restrict__ vs_out,
-                                                               const double* __restrict__ ce) {
+                                                               const double* __restrict__ ce,
+                                                               const uint4* __restrict__ trec,
+                                                               const int* __restrict__ uni) {
   __shared__ double zpad[NC + 2 * NF];  // z with a zero plane below (k = -1) and above (k = NB)
   __shared__ double red[GSK_NT / 64];
   const int slot = active[blockIdx.x];
@@ -345,6 +395,23 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
   // couplings kept for the sweeps: those to cells outside the block zeroed (per half word)
   const uint32_t mx = (tx == 0 ? 0xffff0000u : ~0u) & (tx == NB - 1 ? 0x0000ffffu : ~0u);
   const uint32_t my = (ty == 0 ? 0xffff0000u : ~0u) & (ty == NB - 1 ? 0x0000ffffu : ~0u);
+  // REG: the class records of the thread's inner cells and of its end cell (k = 0 for the lower
+  // half column, NB - 1 for the upper), and their couplings as kept for the sweeps
+  const bool cls = REG == 1 || (REG == 2 && uni[slot] != 0);  // uniform over the workgroup
+  uint4 rin = {}, rend = {};
+  uint32_t win[3] = {}, wend[3] = {};
+  if (REG != 0 && cls) {
+    const uint4* tr = trec + (size_t)slot * 27 + stencil_class1(tx) + 3 * stencil_class1(ty);
+    rin = tr[9];
+    rend = tr[kb == 0 ? 0 : 18];
+    win[0] = rin.y & mx;
+    win[1] = rin.z & my;
+    win[2] = rin.w;
+    wend[0] = rend.y & mx;
+    wend[1] = rend.z & my;
+    wend[2] = rend.w & (kb == 0 ? 0xffff0000u : 0x0000ffffu);
+  }
+  const bool lower = kb == 0;
   // MUL: e of this block (eo) and of the block the ghost value of the x, y and z face this
   // thread's cells touch comes from (ex: face x- when tx = 0, else x+; ey likewise; ez: z- for the
   // lower half column, z+ for the upper).  A half column never straddles a subface boundary, so
@@ -370,11 +437,16 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
       rv[kk] = rb[o];
       if (MODE == 1 || !first) qv[kk] = qb[o];
       if (MODE == 0 && !first) pv[kk] = vb[o];
-      rc[kk] = recb[o];
+      if (REG == 0 || (REG == 2 && !cls)) rc[kk] = recb[o];
     }
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
       const int k = k0 + kk;  // relative to kb
+      if (REG != 0 && cls) {  // the end cell of the half column is k = 0 (lower) or GSK_NK - 1 (upper)
+        if (k == 0) rc[kk] = lower ? rend : rin;
+        else if (k == GSK_NK - 1) rc[kk] = lower ? rin : rend;
+        else rc[kk] = rin;
+      }
       double v;
       if (MODE == 0)
         v = first ? rv[kk] : rv[kk] + beta * (pv[kk] + ng * qv[kk]);
@@ -393,14 +465,18 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
       const uint32_t mz = (ka == 0 ? 0xffff0000u : ~0u) & (ka == NB - 1 ? 0x0000ffffu : ~0u);
       if ((k & 1) == par) {
         ur[m] = uk;
-        wr[m][0] = rc[kk].y & mx;
-        wr[m][1] = rc[kk].z & my;
-        wr[m][2] = rc[kk].w & mz;
+        if constexpr (REG != 1) {
+          wr[m][0] = rc[kk].y & mx;
+          wr[m][1] = rc[kk].z & my;
+          wr[m][2] = rc[kk].w & mz;
+        }
       } else {
         ub[m] = uk;
-        wb[m][0] = rc[kk].y & mx;
-        wb[m][1] = rc[kk].z & my;
-        wb[m][2] = rc[kk].w & mz;
+        if constexpr (REG != 1) {
+          wb[m][0] = rc[kk].y & mx;
+          wb[m][1] = rc[kk].z & my;
+          wb[m][2] = rc[kk].w & mz;
+        }
       }
     }
   }
@@ -420,11 +496,21 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
   // zero (first visit)
   auto half = [&](auto red_tag, bool from_zero) {
     constexpr bool RED = decltype(red_tag)::value;
-    const int kp = kb + (RED ? par : 1 - par);
+    const int cp = RED ? par : 1 - par;  // this colour's cells: k = kb + 2m + cp
+    const int kp = kb + cp;
 #pragma unroll
     for (int m = 0; m < GSK_NK / 2; m++) {
       const int k = 2 * m + kp;
-      uint32_t* w = RED ? wr[m] : wb[m];
+      uint32_t w[3];
+      if constexpr (REG == 1) {
+        // the end cell: m = 0 with cp = 0 (lower half), m = last with cp = 1 (upper half)
+        const bool end = m == 0 ? (lower && cp == 0) : m == GSK_NK / 2 - 1 ? (!lower && cp == 1) : false;
+#pragma unroll
+        for (int b = 0; b < 3; b++) w[b] = end ? wend[b] : win[b];
+      } else {
+#pragma unroll
+        for (int b = 0; b < 3; b++) w[b] = RED ? wr[m][b] : wb[m][b];
+      }
       // keep the couplings packed across sweeps (an empty asm hides that the unpacked values
       // of the last sweep could be reused, which would cost 12 more registers per cell)
       asm volatile("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]));
@@ -464,15 +550,26 @@ __global__ __launch_bounds__(GSK_NT) void bi_precond_gsk_kernel(const int* __res
       h);
 }
 
+// Launches the sweep kernel; with the class rows (o.trec) the class-uniform variant when every
+// active block is class-uniform, the mixed one when some are, else the per-cell one.
 template <int MODE, bool MUL>
-void launch_gsk(int sweeps, hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
-                const SolveState* st, const uint4* rec, BiVecs<double> V, int count, double* vs_out,
-                const double* ce) {
+void launch_gsk(const BiOpts& o, hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
+                const SolveState* st, BiVecs<double> V, int count, double* vs_out, const double* ce) {
   const dim3 g(nact), b(GSK_NT);
-#define GSK(NS) \
-  hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, NS, MUL>), g, b, 0, sm, active, blk, cs, st, rec, V, count, vs_out, \
-                     ce)
-  switch (sweeps) {
+  const int reg = !o.trec || o.nreg == 0 ? 0 : o.nirr == 0 ? 1 : 2;
+#define GSK1(NS, R)                                                                                                 \
+  hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, NS, MUL, R>), g, b, 0, sm, active, blk, cs, st, o.rec, V, count, \
+                     vs_out, ce, o.trec, o.uni)
+#define GSK(NS)            \
+  do {                     \
+    if (reg == 1)          \
+      GSK1(NS, 1);         \
+    else if (reg == 2)     \
+      GSK1(NS, 2);         \
+    else                   \
+      GSK1(NS, 0);         \
+  } while (0)
+  switch (o.sweeps) {
     case 1: GSK(1); break;
     case 2: GSK(2); break;
     case 3: GSK(3); break;
@@ -481,6 +578,7 @@ void launch_gsk(int sweeps, hipStream_t sm, const int* active, int nact, const D
     default: GSK(6);
   }
 #undef GSK
+#undef GSK1
 }
 
 // Q: q_ = A p_, q_.r0 -> alpha (mode 0);  T: t_ = A s_, t_.s and t_.t_ -> gamma (mode 1)
@@ -489,13 +587,18 @@ void launch_gsk(int sweeps, hipStream_t sm, const int* active, int nact, const D
 // LDS holds the operand's interior (NC values) and the ghost layers of the four side faces
 // (4 NF values, same layout as the halo); the two z ghosts of a column stay in registers.  In
 // FP64 that is 40 KiB, so four workgroups fit a CU.
-template <class T, int MODE, class TA = T, bool HR = false, bool CC = false>
+// REG 1 (FP32 bands, every block of the launch class-uniform): the bands come from the block's 27
+// class rows tab[slot][27][8] (the thread's column needs three: k = 0, inner, NB - 1), so no
+// per-cell band is read; REG 2 (mixed launch): so for the blocks with uni[slot] set.
+template <class T, int MODE, class TA = T, bool HR = false, bool CC = false, int REG = 0>
 __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict__ active,
                                                           const DevBlock* __restrict__ blk,
                                                           const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                           Coef7<TA> A, BiVecs<T> V, double* __restrict__ part,
                                                           size_t nslots, const double* __restrict__ ce = nullptr,
-                                                          double* __restrict__ bsy = nullptr) {
+                                                          double* __restrict__ bsy = nullptr,
+                                                          const float* __restrict__ tab = nullptr,
+                                                          const int* __restrict__ uni = nullptr) {
   __shared__ T s[NC + 4 * NF];
   double* const red = reinterpret_cast<double*>(s);  // after the last read of s (block_sum syncs first)
   static_assert(NC * sizeof(T) >= BI_NT / 64 * sizeof(double), "reduction scratch fits");
@@ -512,14 +615,33 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
   constexpr int KB = Batch<T>::K;
   double d0 = 0.0, d1 = 0.0, ys = 0.0;
   T zm, zp;  // ghosts below k = 0 / above k = NB - 1
+  // REG: the bands of the column's classes (k = 0, inner, NB - 1)
+  const bool cls = REG == 1 || (REG == 2 && uni[slot] != 0);  // uniform over the workgroup
+  TA al[7], ai[7], ah[7];
+  if (REG != 0 && cls) {
+    static_assert(REG == 0 || sizeof(TA) == 4, "class rows are FP32 bands");
+    const float* tr = tab + ((size_t)slot * 27 + stencil_class1(tx) + 3 * stencil_class1(ty)) * 8;
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+      al[q] = tr[q];
+      ai[q] = tr[9 * 8 + q];
+      ah[q] = tr[18 * 8 + q];
+    }
+  }
 #pragma unroll
   for (int k0 = 0; k0 < NB; k0 += KB) {
     T a[KB][7], wv[KB];
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
-      const size_t g = base + o0 + NF * (k0 + kk);
+      const int k = k0 + kk;
+      const size_t g = base + o0 + NF * k;
+      if (REG != 0 && cls) {
 #pragma unroll
-      for (int q = 0; q < 7; q++) a[kk][q] = (T)A.p[q][g];
+        for (int q = 0; q < 7; q++) a[kk][q] = (T)(k == 0 ? al[q] : k == NB - 1 ? ah[q] : ai[q]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 7; q++) a[kk][q] = (T)A.p[q][g];
+      }
       wv[kk] = (MODE == 0 && HR) ? (T)shadow_value(local, (uint32_t)(g - base)) : w[g];
     }
     if (k0 == 0) {
@@ -731,12 +853,12 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   auto precond = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
     if (dbl && o.gs == BiOpts::GSC && o.cinv && o.mult)  // block sum of p kept for the next coarse rhs
-      launch_gsk<MODE, true>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count,
-                             MODE == 0 ? o.ce + 4 * nslots : nullptr, o.ce + MODE * nslots);
+      launch_gsk<MODE, true>(o, sm, active, nact, blk, cs, st, Vd, count, MODE == 0 ? o.ce + 4 * nslots : nullptr,
+                             o.ce + MODE * nslots);
     else if (dbl && o.gs == BiOpts::GSC && o.cinv)
-      launch_gsk<MODE, false>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count, part + 4 * nslots, nullptr);
+      launch_gsk<MODE, false>(o, sm, active, nact, blk, cs, st, Vd, count, part + 4 * nslots, nullptr);
     else if (dbl && o.gs == BiOpts::GSC)
-      launch_gsk<MODE, false>(o.sweeps, sm, active, nact, blk, cs, st, o.rec, Vd, count, nullptr, nullptr);
+      launch_gsk<MODE, false>(o, sm, active, nact, blk, cs, st, Vd, count, nullptr, nullptr);
     else if (dbl && o.gs == BiOpts::GS32)
       hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
                          *o.M32, Vd, count);
@@ -758,13 +880,32 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   };
   auto matvec = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
-    if (dbl && o.A32 && o.cinv && o.mult)  // p_ / s_ already hold P e; block sum of q_ kept
-      hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs,
-                         st, *o.A32, Vd, part, nslots, nullptr, MODE == 0 ? o.ce + 3 * nslots : nullptr);
-    else if (dbl && o.A32 && o.cinv)
-      hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, false, true>), dim3(nact), dim3(BI_NT), 0, sm, active,
-                         blk, cs, st, *o.A32, Vd, part, nslots, (const double*)o.ce, nullptr);
-    else if (dbl && o.A32 && o.hash_r0)
+    if (dbl && o.A32 && o.gs == BiOpts::GSC) {
+      // 11-28: the additive correction (17-22) adds e to the operand; the multiplicative one
+      // (23-28) finds it in p_ / s_ already and keeps the block sum of q_.  Class-uniform blocks
+      // take their bands from the class rows.
+      double* const bsy = (o.cinv && o.mult && MODE == 0) ? o.ce + 3 * nslots : nullptr;
+      const int reg = !o.trec || o.nreg == 0 ? 0 : o.nirr == 0 ? 1 : 2;
+      auto go = [&](auto cc_tag, auto reg_tag) {
+        constexpr bool CC = decltype(cc_tag)::value;
+        constexpr int REG = decltype(reg_tag)::value;
+        hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, false, CC, REG>), dim3(nact), dim3(BI_NT), 0, sm,
+                           active, blk, cs, st, *o.A32, Vd, part, nslots, CC ? (const double*)o.ce : nullptr, bsy,
+                           o.tab, o.uni);
+      };
+      auto regs = [&](auto cc_tag) {
+        if (reg == 1)
+          go(cc_tag, std::integral_constant<int, 1>{});
+        else if (reg == 2)
+          go(cc_tag, std::integral_constant<int, 2>{});
+        else
+          go(cc_tag, std::integral_constant<int, 0>{});
+      };
+      if (o.cinv && !o.mult)
+        regs(std::true_type{});
+      else
+        regs(std::false_type{});
+    } else if (dbl && o.A32 && o.hash_r0)
       hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk,
                          cs, st, *o.A32, Vd, part, nslots);
     else if (dbl && o.A32)

@@ -90,6 +90,12 @@ struct ffvhc_ctx {
   double* dcsnap = nullptr;
   size_t dcsnap_n = 0;
   double* dce = nullptr;
+  // stencil classes of the compact-record methods (FFVHC_OPTION_CLASSES): per slot the
+  // class-uniform flag and the 27 class rows as FP32 bands and as 16-byte preconditioner records
+  int* dcuni = nullptr;
+  float* dctab = nullptr;
+  void* dctrec = nullptr;
+  bool classes = true;
   // replay the BiCGSTAB iteration as a captured HIP graph (ffvhc_ctx_set_option; off by default:
   // measured 1-11% slower per iteration than direct launches on ROCm 7.2, 64-960 blocks)
   bool graphs = false;

@@ -454,6 +454,33 @@ int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o, double* ac_out
   return 0;
 }
 
+// Stencil classes of the active blocks for the compact-record methods: the 27 class rows and the
+// class-uniform flag of every block (device) and the counts of uniform and other blocks, which
+// pick the kernel variant.  FFVHC_OPTION_CLASSES 0 leaves o without them (every block then reads
+// the per-cell arrays; the results are the same bit for bit).
+int class_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o) {
+  if (!c->classes || !c->nactive) return 0;
+  const size_t cap = (size_t)c->cap;
+  if (!c->dcuni) {
+    HIPCHK(hipMalloc(&c->dcuni, cap * sizeof(int)));
+    HIPCHK(hipMalloc(&c->dctab, cap * 27 * 8 * sizeof(float)));
+    HIPCHK(hipMalloc(&c->dctrec, cap * 27 * sizeof(uint4)));
+  }
+  launch_stencil_classes(c->sm, c->dactive, c->nactive, K, c->dcuni, c->dctab, (uint4*)c->dctrec);
+  HIPCHK(hipGetLastError());
+  std::vector<int> uni(cap);
+  HIPCHK(hipMemcpyAsync(uni.data(), c->dcuni, cap * sizeof(int), hipMemcpyDeviceToHost, c->sm));
+  HIPCHK(hipStreamSynchronize(c->sm));
+  int nreg = 0;
+  for (int s : c->hactive) nreg += uni[s] != 0;
+  o->tab = c->dctab;
+  o->trec = (const uint4*)c->dctrec;
+  o->uni = c->dcuni;
+  o->nreg = nreg;
+  o->nirr = c->nactive - nreg;
+  return 0;
+}
+
 // Storage options of the FP64-recurrence BiCGSTAB methods 5-9: an FP32 copy of the FP64 matrix A
 // (scratch fields, converted here) in *M32 and/or the FP64 matrix in *M64, referenced by *o.
 int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Coef7<double>* M64, BiOpts* o) {
@@ -494,6 +521,8 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Co
     o->sweeps = (method - 11) % 6 + 1;
     launch_gs_pack(c->sm, c->dactive, c->nactive, *M64, (uint4*)c->fields[fr].d[0]);
     o->mult = method >= 23;
+    const int rcc = class_setup(c, *M64, o);
+    if (rcc) return rcc;
     if (method >= 17) return coarse_setup(c, *M64, o);
   }
   return 0;
@@ -704,7 +733,7 @@ int ffvhc_cg(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, in
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (countPre != 1)
-    return fail(-1, "the device CG implements countPreConditioner = 1 (z = omega r / Ap); see DESIGN.md");
+    return fail(-1, "the device CG implements countPreConditioner = 1 (z = omega r / Ap)");
   if (countMax < 1) {
     const std::vector<int> ids = selected_cases(c);
     for (size_t i = 0; i < ids.size(); i++)
@@ -763,6 +792,10 @@ int ffvhc_ctx_set_option(ffvhc_ctx* c, int option, int value) {
     c->graphs = value != 0;
     return 0;
   }
+  if (option == FFVHC_OPTION_CLASSES) {
+    c->classes = value != 0;
+    return 0;
+  }
   return fail(-1, "unknown option");
 }
 
@@ -781,13 +814,29 @@ int ffvhc_coarse_operator(ffvhc_ctx* c, const int* A, int case_id, double* ac, d
   return rc ? rc : rc2;
 }
 
+int ffvhc_stencil_classes(ffvhc_ctx* c, const int* A, int* n_uniform, int* n_other) {
+  if (!c || !A || !n_uniform || !n_other) return fail(-1, "null argument");
+  for (int q = 0; q < 7; q++)
+    if (!field_ok(c, A[q], 8)) return fail(-1, "A bands must be FP64 fields");
+  int rc = refresh_active(c);
+  if (rc) return rc;
+  const bool saved = c->classes;
+  c->classes = true;
+  BiOpts o;
+  rc = class_setup(c, coef<double>(c, A), &o);
+  c->classes = saved;
+  *n_uniform = o.nreg;
+  *n_other = o.nirr;
+  return rc;
+}
+
 int ffvhc_bicgstab(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countPre, int countMax,
                    double eps, int* count, double* residual) {
   int prec = 8;
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (countPre != 1)
-    return fail(-1, "the device BiCGSTAB implements countPreConditioner = 1 (M v = omega v / Ap); see DESIGN.md");
+    return fail(-1, "the device BiCGSTAB implements countPreConditioner = 1 (M v = omega v / Ap)");
   if (countMax < 1) {
     const std::vector<int> ids = selected_cases(c);
     for (size_t i = 0; i < ids.size(); i++)

@@ -99,6 +99,7 @@ _sig = {
     "ffvhc_bicgstab_timing": (c_int, [_vp, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_bicgstab_timing_mixed": (c_int, [_vp, c_int, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_coarse_operator": (c_int, [_vp, _vp, c_int, _vp, _vp]),
+    "ffvhc_stencil_classes": (c_int, [_vp, _vp, _vp, _vp]),
     "ffvhc_cg_timing": (c_int, [_vp, c_int, _vp, c_int, c_double, c_int, _vp]),
     "ffvhc_jacobi": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
     "ffvhc_step": (c_int, [_vp, _P(StepFields), _P(StepParams), _vp, _vp]),
@@ -282,6 +283,11 @@ class Context:
         """Replay the BiCGSTAB iteration as a captured HIP graph, or launch one by one (default)."""
         check(_L.ffvhc_ctx_set_option(self.h, 1, int(bool(on))), "ffvhc_ctx_set_option")
 
+    def set_classes(self, on):
+        """Let the compact-record solve methods (11-28) read the bands of blocks that are uniform
+        per stencil position class from 27 rows (default), or read every block per cell."""
+        check(_L.ffvhc_ctx_set_option(self.h, 2, int(bool(on))), "ffvhc_ctx_set_option")
+
     def select(self, ids=None):
         if ids is None:
             check(_L.ffvhc_ctx_select(self.h, None, 0), "ffvhc_ctx_select")
@@ -453,6 +459,13 @@ class Context:
         a = np.asarray(A, np.int32)
         check(_L.ffvhc_coarse_operator(self.h, _ptr(a), case_id, _ptr(ac), _ptr(ainv)), "ffvhc_coarse_operator")
         return ac, ainv
+
+    def stencil_classes(self, A):
+        """(class-uniform blocks, other blocks) of the selected cases for the FP64 bands A."""
+        a = np.asarray(A, np.int32)
+        nu, no = c_int(), c_int()
+        check(_L.ffvhc_stencil_classes(self.h, _ptr(a), ctypes.byref(nu), ctypes.byref(no)), "ffvhc_stencil_classes")
+        return nu.value, no.value
 
     def force_v(self, ux, uy, uz, cut_id, ncases=None):
         """Viscous force (fx, fy, fz) on the faces with boundary id cut_id, per selected case."""

@@ -98,6 +98,12 @@ long long ffvhc_ctx_device_bytes(const ffvhc_ctx* ctx);
  * BiCGSTAB iteration as a HIP graph captured once per solve, 0 (default) launches its kernels
  * one by one; the results are identical (the graph replay measured slower on ROCm 7.2). */
 #define FFVHC_OPTION_GRAPHS 1
+/* FFVHC_OPTION_CLASSES: 1 (default) lets the compact-record methods (ffvhc_solve_mixed 11-28)
+ * find the blocks whose seven bands are the same within each of the 27 stencil position classes
+ * (face layers, edges, corners, inside: every block without cut cells) and read those blocks'
+ * bands and preconditioner records from 27 rows instead of per cell; 0 reads every block per
+ * cell.  The results are identical. */
+#define FFVHC_OPTION_CLASSES 2
 int ffvhc_ctx_set_option(ffvhc_ctx* ctx, int option, int value);
 
 /* Physical parameters and outer BCs of one flow case.  BC groups: 0 UX, 1 UY, 2 UZ, 3 P,
@@ -180,6 +186,10 @@ int ffvhc_bicgstab_timing(ffvhc_ctx* ctx, int x, const int* A, int b, double ome
  * ainv = its inverse from the device's blocked Gauss-Jordan.  No counterpart in the reference;
  * exported so the coarse operator can be checked against its definition. */
 int ffvhc_coarse_operator(ffvhc_ctx* ctx, const int* A, int case_id, double* ac, double* ainv);
+/* Stencil classes (no counterpart in the reference): of the selected cases' blocks, how many have
+ * the same seven FP64 bands A throughout each of the 27 position classes (n_uniform; the compact-
+ * record methods read their bands from 27 rows, see FFVHC_OPTION_CLASSES) and how many not. */
+int ffvhc_stencil_classes(ffvhc_ctx* ctx, const int* A, int* n_uniform, int* n_other);
 /* The same stage timing for the FP64-recurrence methods 5-28 of ffvhc_solve_mixed (FP64 x, A, b;
  * omega = GS relaxation factor): ms[0..4] = P, Q, S, T, XR kernels, ms[5] = finalisations,
  * ms[6] = coarse corrections (methods 17-28). */

@@ -433,6 +433,59 @@ def test_graph_replay_equals_launches(ctx, method):
         ctx.remove_case(cid)
 
 
+def class_uniform_system(og, rng, bct, bcv, n_other=3):
+    """Bands constant per block (a random factor per block, as 1/dx^2 is per level) with the outer
+    BC folded in: every block is uniform per stencil position class.  Then one cell's band changes
+    in n_other blocks, which makes those blocks non-uniform."""
+    nb = og.shape[0]
+    lb = rng.uniform(0.5, 2.0, nb)[:, None, None, None]
+    A = [og.zeros() for _ in range(7)]
+    for q in range(1, 7):
+        A[q][:] = lb
+    A[0][:] = -6.0 * lb - 0.05
+    b = rng.uniform(-1, 1, og.shape)
+    og.L.orc_matrix_bc(og.ref(), O.parr(A), O.ptr(b), O.cint6(bct), O.cdbl6(bcv))
+    other = rng.choice(nb, n_other, replace=False)
+    for blk in other:
+        i, j, k = (G + rng.integers(0, N, 3)).tolist()
+        A[2][blk, i, j, k] *= 1.1
+    return A, b, other
+
+
+@pytest.mark.parametrize("method,n_other", [(15, 3), (19, 3), (26, 3), (19, 0), (26, 0)])
+def test_stencil_classes_bit_identical(ctx, method, n_other):
+    """The compact-record methods read the bands and records of class-uniform blocks from the 27
+    class rows; with that on (default) and off the solve is bit-identical, on a system where every
+    block is class-uniform (the class-only kernels) and on one where a few are not (the mixed
+    kernels, which pick per block)."""
+    tree = trees()["three_level"]
+    params = ffvhc.case_params(bc=BC)
+    cid = ctx.add_case(tree, params)
+    try:
+        og = OracleGrid(tree)
+        rng = np.random.default_rng(77)
+        bct = [params.bct[ffvhc.BC_P][q] for q in range(6)]
+        bcv = [params.bcv[ffvhc.BC_P][q] for q in range(6)]
+        A, b, other = class_uniform_system(og, rng, bct, bcv, n_other)
+        fa, fb, fx = upload_system(ctx, cid, A, b, og.zeros())
+        nu, no = ctx.stencil_classes(fa)
+        assert (nu, no) == (og.shape[0] - len(other), len(other))
+        got = []
+        for classes in (True, False, True):
+            ctx.set_classes(classes)
+            ctx.upload(fx, cid, og.zeros())
+            cnt, res, _o, rc = ctx.solve_mixed(method, fx, fa, fb, ffvhc.BC_P, 1.4, 5000, 1e-10, 0.5, ncases=1)
+            assert rc == 0 and res[0] <= 1e-10
+            got.append((int(cnt[0]), ctx.download(fx, cid, out=og.zeros())))
+        for c_, x in got[1:]:
+            assert c_ == got[0][0] and np.array_equal(x, got[0][1])
+        for f in fa + [fb, fx]:
+            ctx.free(f)
+    finally:
+        ctx.set_classes(True)
+        ctx.remove_case(cid)
+
+
 def run_oracle_bicgstab(og, x, A, b, bct, bcv, omega, cmax, eps):
     cnt, res = ctypes.c_int(), ctypes.c_double()
     sc = [og.zeros() for _ in range(9)]

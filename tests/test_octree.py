@@ -1,6 +1,6 @@
 """Block octree (host code of the product): neighbour tables, 2:1 balance, ordering.
 
-BCMTools, whose octree the reference uses, is not vendored, so the tables are
+BCMTools, whose octree the reference uses, is not vendored in the reference, so the tables are
 pinned here by invariants and a brute-force geometric neighbour search instead of reference
 output: same-level neighbours share a full face, finer neighbours sit in the face quarter named
 by their subface (origins along (y,z) of X faces, (z,x) of Y faces, (x,y) of Z faces,
