@@ -252,8 +252,8 @@ MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64
 PROFILE_SCHEDULE = (26, 0.5, 20000, 1.6)
 
 # kernel names of the timed stages as rocprofv3 reports them
-PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0, %s>", "bi_q": "bi_matvec_kernel<%s, 0, %s, false, false, false>",
-              "bi_s": "bi_precond_kernel<%s, 1, %s>", "bi_t": "bi_matvec_kernel<%s, 1, %s, false, false, false>",
+PROF_NAMES = {"bi_p": "bi_precond_kernel<%s, 0, %s>", "bi_q": "bi_matvec_kernel<%s, 0, %s, false, false, 0>",
+              "bi_s": "bi_precond_kernel<%s, 1, %s>", "bi_t": "bi_matvec_kernel<%s, 1, %s, false, false, 0>",
               "bi_xr": "bi_xr_kernel<%s, false, false>", "rbgs": "rbgs_iter_kernel<%s, 0>",
               "cg_p": "cg_p_kernel<%s>", "cg_q": "cg_q_kernel<%s>", "cg_xr": "cg_xr_kernel<%s>"}
 
@@ -271,7 +271,7 @@ def mixed_prof_name(method, stage, uniform=False):
     mode = {"bi_p": 0, "bi_q": 0, "bi_s": 1, "bi_t": 1}.get(stage)
     hr = "true" if method == 10 else "false"
     tf = lambda v: "true" if v else "false"
-    reg = tf(uniform and method >= 11)
+    reg = 1 if uniform and method >= 11 else 0  # the class-only variant (all blocks uniform)
     # additive coarse correction (17-22): the matvec and XR add e to p_ / s_; multiplicative
     # (23-28): the sweep kernel stores p_ / s_ with e already added
     cc = 17 <= method <= 22
@@ -279,12 +279,12 @@ def mixed_prof_name(method, stage, uniform=False):
         return "bi_xr_kernel<double, %s, %s>" % (hr, tf(cc))
     if stage in ("bi_q", "bi_t"):
         if method >= 11:
-            return "bi_matvec_kernel<double, %d, float, false, %s, %s>" % (mode, tf(cc), reg)
-        return "bi_matvec_kernel<double, %d, %s, %s, false, false>" % (mode, "float" if method >= 7 else "double", hr)
+            return "bi_matvec_kernel<double, %d, float, false, %s, %d>" % (mode, tf(cc), reg)
+        return "bi_matvec_kernel<double, %d, %s, %s, false, 0>" % (mode, "float" if method >= 7 else "double", hr)
     if method == 7:
         return "bi_precond_kernel<double, %d, float>" % mode
     if method >= 11:
-        return "bi_precond_gsk_kernel<%d, %d, %s, %s>" % (mode, (method - 11) % 6 + 1, tf(method >= 23), reg)
+        return "bi_precond_gsk_kernel<%d, %d, %s, %d>" % (mode, (method - 11) % 6 + 1, tf(method >= 23), reg)
     tm, tz, sym = {5: ("float", "float", "false"), 6: ("double", "double", "false"),
                    8: ("float", "double", "false"), 9: ("float", "double", "true"),
                    10: ("float", "double", "false")}[method]

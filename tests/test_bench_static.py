@@ -50,22 +50,22 @@ def test_mixed_byte_model_and_names():
     # method 8: FP64 vectors, FP32 bands: 7 bands x 4 bytes in the GS and matvec kernels
     assert bench.kernel_bytes_mixed(8)["bi_q"] == 8 + 3 + 28 + 16
     assert bench.mixed_prof_name(8, "bi_p") == "bi_precond_gs_kernel<0, float, double, false>"
-    assert bench.mixed_prof_name(7, "bi_t") == "bi_matvec_kernel<double, 1, float, false, false, false>"
-    assert bench.prof_name("bi_q", 4) == "bi_matvec_kernel<float, 0, float, false, false, false>"
+    assert bench.mixed_prof_name(7, "bi_t") == "bi_matvec_kernel<double, 1, float, false, false, 0>"
+    assert bench.prof_name("bi_q", 4) == "bi_matvec_kernel<float, 0, float, false, false, 0>"
     assert bench.mixed_prof_name(10, "bi_xr") == "bi_xr_kernel<double, true, false>"
     assert bench.kernel_bytes_mixed(10)["bi_xr"] == 56
     assert abs(bench.kernel_bytes_mixed(12)["bi_p"] - (24 + 16 + 16 + 3)) < 1e-9
-    assert bench.mixed_prof_name(13, "bi_s") == "bi_precond_gsk_kernel<1, 3, false, false>"
-    assert bench.mixed_prof_name(21, "bi_p") == "bi_precond_gsk_kernel<0, 5, false, false>"
-    assert bench.mixed_prof_name(25, "bi_s") == "bi_precond_gsk_kernel<1, 3, true, false>"
-    assert bench.mixed_prof_name(25, "bi_s", True) == "bi_precond_gsk_kernel<1, 3, true, true>"
-    assert bench.mixed_prof_name(21, "bi_q") == "bi_matvec_kernel<double, 0, float, false, true, false>"
-    assert bench.mixed_prof_name(21, "bi_q", True) == "bi_matvec_kernel<double, 0, float, false, true, true>"
+    assert bench.mixed_prof_name(13, "bi_s") == "bi_precond_gsk_kernel<1, 3, false, 0>"
+    assert bench.mixed_prof_name(21, "bi_p") == "bi_precond_gsk_kernel<0, 5, false, 0>"
+    assert bench.mixed_prof_name(25, "bi_s") == "bi_precond_gsk_kernel<1, 3, true, 0>"
+    assert bench.mixed_prof_name(25, "bi_s", True) == "bi_precond_gsk_kernel<1, 3, true, 1>"
+    assert bench.mixed_prof_name(21, "bi_q") == "bi_matvec_kernel<double, 0, float, false, true, 0>"
+    assert bench.mixed_prof_name(21, "bi_q", True) == "bi_matvec_kernel<double, 0, float, false, true, 1>"
     # class-uniform blocks: no per-cell band or record is read
     assert abs(bench.kernel_bytes_mixed(26, True)["bi_p"] - (24 + 16 + 3)) < 1e-9
     assert abs(bench.kernel_bytes_mixed(26, True)["bi_q"] - (8 + 3 + 8 + 8)) < 1e-9
     # multiplicative methods store p_ / s_ with the coarse correction added: plain matvec and XR
-    assert bench.mixed_prof_name(26, "bi_t") == "bi_matvec_kernel<double, 1, float, false, false, false>"
+    assert bench.mixed_prof_name(26, "bi_t") == "bi_matvec_kernel<double, 1, float, false, false, 0>"
     assert bench.mixed_prof_name(26, "bi_xr") == "bi_xr_kernel<double, false, false>"
     assert bench.mixed_prof_name(19, "bi_xr") == "bi_xr_kernel<double, false, true>"
     assert len(bench.MIXED_METHODS) == 29
