@@ -78,9 +78,10 @@ struct BiOpts {
   bool hash_r0 = false;
   const uint4* rec = nullptr;
   int sweeps = 1;
-  // coarse correction (GSC only): FP32 inverse of A_c per case at offsets coff[case], the block
-  // sums of v in part[4 * nslots + slot], corrections ce[mode * nslots + slot]; mult: applied
-  // before the sweeps (multiplicative), from the block sums of r, q_, p in ce[2..4][slot]
+  // coarse correction (GSC only): FP32 inverse of A_c per case at offsets coff[case] (row pitch
+  // n rounded up to a multiple of 4), the block sums of v in part[4 * nslots + slot], corrections
+  // ce[mode * nslots + slot]; mult: applied before the sweeps (multiplicative), from the block
+  // sums of r, q_, p in ce[2..4][slot]
   const float* cinv = nullptr;
   const long long* coff = nullptr;
   double* ce = nullptr;
@@ -127,7 +128,8 @@ void launch_coarse_assemble(hipStream_t sm, const int* active, int nact, const D
 void launch_coarse_decouple(hipStream_t sm, double* M, int n);
 // T: scratch of 32 * n doubles
 void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* T);
-void launch_to_f32(hipStream_t sm, const double* x, float* y, size_t n);
+// FP32 copy of an n x n FP64 matrix with the row pitch np (>= n, pad zero)
+void launch_to_f32(hipStream_t sm, const double* x, float* y, int n, int np);
 void launch_coarse_apply(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                          const SolveState* st, const float* inv, const long long* off, const double* c, double* e);
 void launch_coarse_apply_mul(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,

@@ -215,19 +215,44 @@ void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* T) {
   }
 }
 
-__global__ void to_f32_kernel(const double* __restrict__ x, float* __restrict__ y, size_t n) {
+// FP32 copy of a case's FP64 inverse with the row pitch padded to np (a multiple of 4, pad zero),
+// so that every row starts 16-byte aligned for float4 loads
+__global__ void to_f32_kernel(const double* __restrict__ x, float* __restrict__ y, int n, int np) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n) y[e] = (float)x[e];
+  if (e >= (size_t)n * np) return;
+  const int a = (int)(e / np), b = (int)(e % np);
+  y[e] = b < n ? (float)x[(size_t)a * n + b] : 0.0f;
 }
 
-void launch_to_f32(hipStream_t sm, const double* x, float* y, size_t n) {
-  if (n) hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sm, x, y, n);
+void launch_to_f32(hipStream_t sm, const double* x, float* y, int n, int np) {
+  const size_t total = (size_t)n * np;
+  if (total) hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, sm, x, y, n, np);
 }
 
-// e[slot] = sum_b inv[a][b] c[b] for every block a of every selected case: one wavefront per row,
-// lanes strided over b, then a fixed shuffle tree (deterministic).  c[b] = part[slot_begin + b],
-// the block sums of the vector being preconditioned; inv is the case's FP32 inverse (row-major,
-// at offset off[case]).
+__device__ inline int coarse_pitch(int n) { return (n + 3) & ~3; }
+
+// e = A_c^-1 c for each active block's row: one wavefront per row, the row read as float4
+// (pitch coarse_pitch(n), offsets off[case]); c(b) gives the coarse right-hand side of block b
+// (0 past n); the sum order is fixed (lane-strided float4 groups, then the wave reduction), so a
+// case's result does not depend on what else is batched with it
+template <class Rhs>
+__device__ inline double coarse_row_dot(const float* __restrict__ row, int n, int lane, const Rhs& c) {
+  const float4* r4 = reinterpret_cast<const float4*>(row);
+  double s = 0.0;
+  const int nq = coarse_pitch(n) / 4;
+#pragma unroll 4
+  for (int q = lane; q < nq; q += 64) {
+    const float4 v = r4[q];
+    const int b = 4 * q;
+    s += (double)v.x * c(b);
+    s += (double)v.y * c(b + 1);
+    s += (double)v.z * c(b + 2);
+    s += (double)v.w * c(b + 3);
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
+
 __global__ __launch_bounds__(256) void coarse_apply_kernel(const int* __restrict__ active, int nact,
                                                            const DevBlock* __restrict__ blk,
                                                            const DevCase* __restrict__ cs,
@@ -242,23 +267,19 @@ __global__ __launch_bounds__(256) void coarse_apply_kernel(const int* __restrict
   if (st[cid].done) return;
   const DevCase& C = cs[cid];
   const int n = C.nblk, a = slot - C.slot_begin;
-  const float* row = inv + off[cid] + (size_t)a * n;
-  double s = 0.0;
+  const float* row = inv + off[cid] + (size_t)a * coarse_pitch(n);
   const double* cc = c + C.slot_begin;
-#pragma unroll 8
-  for (int b = lane; b < n; b += 64) s += (double)row[b] * cc[b];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  const double s = coarse_row_dot(row, n, lane, [&](int b) { return b < n ? cc[b] : 0.0; });
   if (lane == 0) e[slot] = s;
 }
 
-// Multiplicative variant: the restriction c = P^T v is formed from block sums the previous
-// kernels wrote (ce layout per slot: [2] r, [3] q_, [4] p), by the vector updates themselves:
+// Multiplicative two-level methods: e = A_c^-1 P^T v with P^T v formed from the block sums of r,
+// q_ and p the previous kernels kept (ce[2], ce[3], ce[4] per slot) and the case scalars:
 // p = r (first iteration) or r + beta (p - gamma q_), s = r - alpha q_.  A workgroup of CA_NT
 // threads handles CA_NT / 64 rows of one case; it forms c once in LDS (up to CA_MAXN blocks per
-// case, larger cases read it from the block sums per row).
-constexpr int CA_NT = 1024, CA_MAXN = 4096;
+// case) when all its rows belong to one case, else each wave forms c on the fly.
+constexpr int CA_NT = 256, CA_MAXN = 4096;
 
-// c_b of block b of a case from the block sums
 struct CoarseRhs {
   const double *br, *bq, *bp;
   bool first;
@@ -286,10 +307,11 @@ __global__ __launch_bounds__(CA_NT) void coarse_apply_mul_kernel(const int* __re
   // the active list holds each case's blocks contiguously: a workgroup whose first and last rows
   // share a case stages that case's c once in LDS
   const int c0 = blk[active[first_row]].case_id;
-  const bool uni = c0 == blk[active[last_row]].case_id && cs[c0].nblk <= CA_MAXN && !st[c0].done;
+  const int n0 = cs[c0].nblk;
+  const bool uni = c0 == blk[active[last_row]].case_id && coarse_pitch(n0) <= CA_MAXN && !st[c0].done;
   if (uni) {
     const CoarseRhs cb(ce, nslots, cs[c0], st[c0], mode);
-    for (int b = threadIdx.x; b < cs[c0].nblk; b += CA_NT) cl[b] = cb(b);
+    for (int b = threadIdx.x; b < coarse_pitch(n0); b += CA_NT) cl[b] = b < n0 ? cb(b) : 0.0;
   }
   __syncthreads();
   const int w = first_row + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -300,16 +322,14 @@ __global__ __launch_bounds__(CA_NT) void coarse_apply_mul_kernel(const int* __re
   if (S.done) return;
   const DevCase& C = cs[cid];
   const int n = C.nblk, a = slot - C.slot_begin;
-  const CoarseRhs cb(ce, nslots, C, S, mode);
-  const float* row = inv + off[cid] + (size_t)a * n;
-  double s = 0.0;
+  const float* row = inv + off[cid] + (size_t)a * coarse_pitch(n);
+  double s;
   if (uni) {
-#pragma unroll 8
-    for (int b = lane; b < n; b += 64) s += (double)row[b] * cl[b];
+    s = coarse_row_dot(row, n, lane, [&](int b) { return cl[b]; });
   } else {
-    for (int b = lane; b < n; b += 64) s += (double)row[b] * cb(b);
+    const CoarseRhs cb(ce, nslots, C, S, mode);
+    s = coarse_row_dot(row, n, lane, [&](int b) { return b < n ? cb(b) : 0.0; });
   }
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if (lane == 0) ce[mode * nslots + slot] = s;
 }
 

@@ -82,10 +82,12 @@ struct ffvhc_ctx {
   std::vector<std::pair<int, int>> scratch_ids;
   double* dpart = nullptr;  // 6 * cap per-block partials
   // coarse correction of the two-level preconditioners: dense A_c per case (inverted in place),
-  // its FP32 copy, per-case offsets, Gauss-Jordan panel scratch, corrections e_p / e_s per slot
+  // its FP32 copy (row pitch padded to a multiple of 4), per-case offsets of both (dcoff[0..MAX_CASES)
+  // and dcoff[MAX_CASES..)), Gauss-Jordan panel scratch, corrections and block sums per slot
   double* dcm = nullptr;
   size_t dcm_n = 0;
   float* dci = nullptr;
+  size_t dci_n = 0;
   long long* dcoff = nullptr;
   double* dcsnap = nullptr;
   size_t dcsnap_n = 0;

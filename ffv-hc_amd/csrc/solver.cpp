@@ -387,22 +387,27 @@ int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o, double* ac_out
   HIPCHK(hipMemcpyAsync(rows.data(), drows, rows.size() * sizeof(double), hipMemcpyDeviceToHost, c->sm));
   HIPCHK(hipStreamSynchronize(c->sm));
   const std::vector<int> ids = selected_cases(c);
-  std::vector<long long> off(c->cases.size(), 0);
-  size_t total = 0, maxn = 0;
+  // FP64 operators / inverses at off[case] (n x n); their FP32 copies at offp[case] with the row
+  // pitch padded to a multiple of 4 (float4 rows)
+  std::vector<long long> off(c->cases.size(), 0), offp(c->cases.size(), 0);
+  size_t total = 0, totalp = 0, maxn = 0;
   for (int id : ids) {
     off[id] = (long long)total;
+    offp[id] = (long long)totalp;
     const size_t n = (size_t)c->cases[id].nblk;
     total += n * n;
+    totalp += n * ((n + 3) & ~(size_t)3);
     maxn = std::max(maxn, n);
   }
-  if (total > c->dcm_n) {
+  if (total > c->dcm_n || totalp > c->dci_n) {
     (void)hipFree(c->dcm);
     (void)hipFree(c->dci);
     c->dcm = nullptr;
     c->dci = nullptr;
     HIPCHK(hipMalloc(&c->dcm, total * sizeof(double)));
-    HIPCHK(hipMalloc(&c->dci, total * sizeof(float)));
+    HIPCHK(hipMalloc(&c->dci, totalp * sizeof(float)));
     c->dcm_n = total;
+    c->dci_n = totalp;
   }
   if (32 * maxn > c->dcsnap_n) {  // panel rows of the blocked Gauss-Jordan
     (void)hipFree(c->dcsnap);
@@ -410,9 +415,11 @@ int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o, double* ac_out
     HIPCHK(hipMalloc(&c->dcsnap, 32 * maxn * sizeof(double)));
     c->dcsnap_n = 32 * maxn;
   }
-  if (!c->dcoff) HIPCHK(hipMalloc(&c->dcoff, MAX_CASES * sizeof(long long)));
+  if (!c->dcoff) HIPCHK(hipMalloc(&c->dcoff, 2 * MAX_CASES * sizeof(long long)));
   if (!c->dce) HIPCHK(hipMalloc(&c->dce, 5 * (size_t)c->cap * sizeof(double)));
   HIPCHK(hipMemcpyAsync(c->dcoff, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice, c->sm));
+  HIPCHK(hipMemcpyAsync(c->dcoff + MAX_CASES, offp.data(), offp.size() * sizeof(long long), hipMemcpyHostToDevice,
+                        c->sm));
   launch_coarse_assemble(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dcoff, drows, c->dcm, total);
   for (int id : ids) {
     // singular A_c (zero row sums throughout: no Dirichlet face) -> decouple the last unknown
@@ -440,7 +447,10 @@ int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o, double* ac_out
                           (size_t)c->cases[ids[0]].nblk * c->cases[ids[0]].nblk * sizeof(double),
                           hipMemcpyDeviceToHost, c->sm));
   for (int id : ids) gauss_jordan_inverse(c->sm, c->dcm + off[id], c->cases[id].nblk, c->dcsnap);
-  launch_to_f32(c->sm, c->dcm, c->dci, total);
+  for (int id : ids) {
+    const int n = c->cases[id].nblk;
+    launch_to_f32(c->sm, c->dcm + off[id], c->dci + offp[id], n, (n + 3) & ~3);
+  }
   HIPCHK(hipGetLastError());
   if (inv_out && !ids.empty())
     HIPCHK(hipMemcpyAsync(inv_out, c->dcm + off[ids[0]],
@@ -449,7 +459,7 @@ int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o, double* ac_out
   // the host vectors above must outlive the asynchronous copies
   HIPCHK(hipStreamSynchronize(c->sm));
   o->cinv = c->dci;
-  o->coff = c->dcoff;
+  o->coff = c->dcoff + MAX_CASES;
   o->ce = c->dce;
   return 0;
 }
