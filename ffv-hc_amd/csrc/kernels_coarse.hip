@@ -113,8 +113,9 @@ void launch_coarse_decouple(hipStream_t sm, double* M, int n) {
 // In-place blocked Gauss-Jordan inversion of a dense n x n matrix (row-major, no pivoting: the
 // coarse operators are diagonally dominant).  For each panel P of GJB pivots, with D = M_PP:
 //   M_PP <- D^-1,  M_Pj <- D^-1 M_Pj,  M_iP <- -M_iP D^-1,  M_ij <- M_ij - M_iP D^-1 M_Pj
-// (i, j outside P) -- the scalar Gauss-Jordan step applied to a block of pivots.  Six launches
-// per panel; every launch reads only values no other thread of it writes.
+// (i, j outside P) -- the scalar Gauss-Jordan step applied to a block of pivots.  Four launches
+// per panel (the trailing update also stores the new panel rows; the column step works in place
+// within a workgroup); no launch reads a value another workgroup of it writes.
 constexpr int GJB = 32;
 
 // D <- D^-1 in LDS by unblocked Gauss-Jordan; one workgroup of GJB * GJB threads
@@ -151,7 +152,8 @@ __global__ __launch_bounds__(256) void gjb_row_kernel(const double* __restrict__
   T[(size_t)r * n + j] = s;
 }
 
-// M_ij -= sum_c M_i,k0+c T[c][j] for i, j outside the panel; 16 x 16 tiles staged in LDS
+// M_ij -= sum_c M_i,k0+c T[c][j] for i, j outside the panel, and the panel rows M_Pj <- T[.][j]
+// (j outside the panel: nothing here reads them); 16 x 16 tiles staged in LDS
 __global__ __launch_bounds__(256) void gjb_update_kernel(double* __restrict__ M, int n, int k0, int kb,
                                                          const double* __restrict__ T) {
   __shared__ double a[16][GJB + 1], t[GJB][17];
@@ -164,43 +166,33 @@ __global__ __launch_bounds__(256) void gjb_update_kernel(double* __restrict__ M,
     t[cr][jc] = (jj < n && cr < kb) ? T[(size_t)cr * n + jj] : 0.0;
   }
   __syncthreads();
-  if (i >= n || j >= n || (i >= k0 && i < k0 + kb) || (j >= k0 && j < k0 + kb)) return;
+  if (i >= n || j >= n || (j >= k0 && j < k0 + kb)) return;
+  if (i >= k0 && i < k0 + kb) {
+    M[(size_t)i * n + j] = T[(size_t)(i - k0) * n + j];
+    return;
+  }
   double s = 0.0;
 #pragma unroll 8
   for (int c = 0; c < GJB; c++) s += a[ty][c] * t[c][tx];
   M[(size_t)i * n + j] -= s;
 }
 
-// T[i][c2] = -(M_iP D^-1)[i][c2] for the rows outside the panel (T reused after the panel rows
-// took their values); 8 rows x GJB columns per workgroup
-__global__ __launch_bounds__(256) void gjb_col_kernel(const double* __restrict__ M, int n, int k0, int kb,
-                                                      double* __restrict__ T) {
+// M_iP <- -M_iP D^-1 for the rows outside the panel, in place: the GJB threads of a row share a
+// workgroup (256 / GJB rows each), read the row's panel values, synchronise, then write them
+__global__ __launch_bounds__(256) void gjb_col_kernel(double* __restrict__ M, int n, int k0, int kb) {
   const int c2 = threadIdx.x % GJB, i = blockIdx.x * (256 / GJB) + threadIdx.x / GJB;
-  if (i >= n || c2 >= kb || (i >= k0 && i < k0 + kb)) return;
-  const double* a = M + (size_t)i * n + k0;
+  const bool act = i < n && c2 < kb && !(i >= k0 && i < k0 + kb);
   double s = 0.0;
-  for (int c = 0; c < kb; c++) s += a[c] * M[(size_t)(k0 + c) * n + k0 + c2];
-  T[(size_t)i * GJB + c2] = -s;
-}
-
-// M_iP <- T[i] for the rows outside the panel
-__global__ __launch_bounds__(256) void gjb_colcopy_kernel(double* __restrict__ M, int n, int k0, int kb,
-                                                          const double* __restrict__ T) {
-  const int c2 = threadIdx.x % GJB, i = blockIdx.x * (256 / GJB) + threadIdx.x / GJB;
-  if (i >= n || c2 >= kb || (i >= k0 && i < k0 + kb)) return;
-  M[(size_t)i * n + k0 + c2] = T[(size_t)i * GJB + c2];
-}
-
-// panel rows <- T outside the panel columns
-__global__ __launch_bounds__(256) void gjb_copy_kernel(double* __restrict__ M, int n, int k0, int kb,
-                                                       const double* __restrict__ T) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
-  if (j >= n || (j >= k0 && j < k0 + kb)) return;
-  M[(size_t)(k0 + r) * n + j] = T[(size_t)r * n + j];
+  if (act) {
+    const double* a = M + (size_t)i * n + k0;
+    for (int c = 0; c < kb; c++) s += a[c] * M[(size_t)(k0 + c) * n + k0 + c2];
+  }
+  __syncthreads();
+  if (act) M[(size_t)i * n + k0 + c2] = -s;
 }
 
 // T: scratch of GJB * n doubles.  Order per panel: D^-1; T = D^-1 M_P; the trailing update
-// (old M_iP, T); panel rows <- T; T <- -M_iP D^-1 (old M_iP); M_iP <- T.
+// (old M_iP, T) with panel rows <- T; M_iP <- -M_iP D^-1.
 void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* T) {
   for (int k0 = 0; k0 < n; k0 += GJB) {
     const int kb = n - k0 < GJB ? n - k0 : GJB;
@@ -209,9 +201,7 @@ void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* T) {
     hipLaunchKernelGGL(gjb_diag_kernel, dim3(1), dim3(GJB * GJB), 0, sm, M, n, k0, kb);
     hipLaunchKernelGGL(gjb_row_kernel, dim3(nt, kb), dim3(256), 0, sm, (const double*)M, n, k0, kb, T);
     hipLaunchKernelGGL(gjb_update_kernel, dim3(nt16, nt16), dim3(256), 0, sm, M, n, k0, kb, (const double*)T);
-    hipLaunchKernelGGL(gjb_copy_kernel, dim3(nt, kb), dim3(256), 0, sm, M, n, k0, kb, (const double*)T);
-    hipLaunchKernelGGL(gjb_col_kernel, dim3(nr), dim3(256), 0, sm, (const double*)M, n, k0, kb, T);
-    hipLaunchKernelGGL(gjb_colcopy_kernel, dim3(nr), dim3(256), 0, sm, M, n, k0, kb, (const double*)T);
+    hipLaunchKernelGGL(gjb_col_kernel, dim3(nr), dim3(256), 0, sm, M, n, k0, kb);
   }
 }
 

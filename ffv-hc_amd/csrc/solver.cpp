@@ -506,9 +506,18 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Co
     int rc = scratch_field(c, 4, R_A + q, &fa[q]);
     if (rc) return rc;
   }
-  for (int q = 0; q < 7; q++)
-    launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
-                   (const double*)c->fields[A[q]].d[0]);
+  // the compact-record methods (11-28) read no per-cell FP32 band or record when every active
+  // block is class-uniform: those arrays are then not built
+  bool percell = true;
+  if (method >= 11) {
+    const int rc = class_setup(c, *M64, o);
+    if (rc) return rc;
+    percell = !o->trec || o->nirr > 0;
+  }
+  if (percell)
+    for (int q = 0; q < 7; q++)
+      launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
+                     (const double*)c->fields[A[q]].d[0]);
   *M32 = coef<float>(c, fa);
   if (method == 5) {
     o->gs = BiOpts::GS32;
@@ -529,10 +538,8 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Co
     o->gs = BiOpts::GSC;
     o->rec = (const uint4*)c->fields[fr].d[0];
     o->sweeps = (method - 11) % 6 + 1;
-    launch_gs_pack(c->sm, c->dactive, c->nactive, *M64, (uint4*)c->fields[fr].d[0]);
+    if (percell) launch_gs_pack(c->sm, c->dactive, c->nactive, *M64, (uint4*)c->fields[fr].d[0]);
     o->mult = method >= 23;
-    const int rcc = class_setup(c, *M64, o);
-    if (rcc) return rcc;
     if (method >= 17) return coarse_setup(c, *M64, o);
   }
   return 0;
