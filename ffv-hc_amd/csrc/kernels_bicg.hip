@@ -886,25 +886,31 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
       // take their bands from the class rows.
       double* const bsy = (o.cinv && o.mult && MODE == 0) ? o.ce + 3 * nslots : nullptr;
       const int reg = !o.trec || o.nreg == 0 ? 0 : o.nirr == 0 ? 1 : 2;
-      auto go = [&](auto cc_tag, auto reg_tag) {
-        constexpr bool CC = decltype(cc_tag)::value;
+      auto go = [&](auto hr_tag, auto cc_tag, auto reg_tag) {
+        constexpr bool HR = decltype(hr_tag)::value, CC = decltype(cc_tag)::value;
         constexpr int REG = decltype(reg_tag)::value;
-        hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, false, CC, REG>), dim3(nact), dim3(BI_NT), 0, sm,
+        hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, HR, CC, REG>), dim3(nact), dim3(BI_NT), 0, sm,
                            active, blk, cs, st, *o.A32, Vd, part, nslots, CC ? (const double*)o.ce : nullptr, bsy,
                            o.tab, o.uni);
       };
-      auto regs = [&](auto cc_tag) {
+      auto regs = [&](auto hr_tag, auto cc_tag) {
         if (reg == 1)
-          go(cc_tag, std::integral_constant<int, 1>{});
+          go(hr_tag, cc_tag, std::integral_constant<int, 1>{});
         else if (reg == 2)
-          go(cc_tag, std::integral_constant<int, 2>{});
+          go(hr_tag, cc_tag, std::integral_constant<int, 2>{});
         else
-          go(cc_tag, std::integral_constant<int, 0>{});
+          go(hr_tag, cc_tag, std::integral_constant<int, 0>{});
       };
-      if (o.cinv && !o.mult)
-        regs(std::true_type{});
+      auto ccs = [&](auto hr_tag) {
+        if (o.cinv && !o.mult)
+          regs(hr_tag, std::true_type{});
+        else
+          regs(hr_tag, std::false_type{});
+      };
+      if (o.hash_r0)
+        ccs(std::true_type{});
       else
-        regs(std::false_type{});
+        ccs(std::false_type{});
     } else if (dbl && o.A32 && o.hash_r0)
       hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk,
                          cs, st, *o.A32, Vd, part, nslots);
@@ -946,18 +952,22 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_GAMMA, count);
   }
   if (mask & 16) {
-    if (o.cinv && o.mult)  // p_ / s_ already hold P e; block sum of r kept
-      hipLaunchKernelGGL((bi_xr_kernel<T, false>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
-                         nslots, count, nullptr, o.ce + 2 * nslots);
-    else if (o.cinv)
-      hipLaunchKernelGGL((bi_xr_kernel<T, false, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V,
-                         part, nslots, count, (const double*)o.ce, nullptr);
-    else if (o.hash_r0)
-      hipLaunchKernelGGL((bi_xr_kernel<T, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
-                         nslots, count);
+    auto xr = [&](auto hr_tag) {
+      constexpr bool HR = decltype(hr_tag)::value;
+      if (o.cinv && o.mult)  // p_ / s_ already hold P e; block sum of r kept
+        hipLaunchKernelGGL((bi_xr_kernel<T, HR>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
+                           nslots, count, nullptr, o.ce + 2 * nslots);
+      else if (o.cinv)
+        hipLaunchKernelGGL((bi_xr_kernel<T, HR, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V,
+                           part, nslots, count, (const double*)o.ce, nullptr);
+      else
+        hipLaunchKernelGGL((bi_xr_kernel<T, HR>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
+                           nslots, count);
+    };
+    if (o.hash_r0)
+      xr(std::true_type{});
     else
-      hipLaunchKernelGGL((bi_xr_kernel<T, false>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
-                         nslots, count);
+      xr(std::false_type{});
     if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_XR, count);
   }
 }

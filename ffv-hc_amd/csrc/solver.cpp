@@ -536,6 +536,7 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Co
     if (rc) return rc;
     o->A32 = M32;
     o->gs = BiOpts::GSC;
+    o->hash_r0 = c->shadow_hash;
     o->rec = (const uint4*)c->fields[fr].d[0];
     o->sweeps = (method - 11) % 6 + 1;
     if (percell) launch_gs_pack(c->sm, c->dactive, c->nactive, *M64, (uint4*)c->fields[fr].d[0]);
@@ -811,6 +812,10 @@ int ffvhc_ctx_set_option(ffvhc_ctx* c, int option, int value) {
   }
   if (option == FFVHC_OPTION_CLASSES) {
     c->classes = value != 0;
+    return 0;
+  }
+  if (option == FFVHC_OPTION_SHADOW) {
+    c->shadow_hash = value != 0;
     return 0;
   }
   return fail(-1, "unknown option");

@@ -283,6 +283,11 @@ class Context:
         """Replay the BiCGSTAB iteration as a captured HIP graph, or launch one by one (default)."""
         check(_L.ffvhc_ctx_set_option(self.h, 1, int(bool(on))), "ffvhc_ctx_set_option")
 
+    def set_shadow_hash(self, on):
+        """Seeded hash shadow residual for the compact-record solve methods (11-28) (default), or
+        r0 = r."""
+        check(_L.ffvhc_ctx_set_option(self.h, 3, int(bool(on))), "ffvhc_ctx_set_option")
+
     def set_classes(self, on):
         """Let the compact-record solve methods (11-28) read the bands of blocks that are uniform
         per stencil position class from 27 rows (default), or read every block per cell."""

@@ -104,6 +104,11 @@ long long ffvhc_ctx_device_bytes(const ffvhc_ctx* ctx);
  * bands and preconditioner records from 27 rows instead of per cell; 0 reads every block per
  * cell.  The results are identical. */
 #define FFVHC_OPTION_CLASSES 2
+/* FFVHC_OPTION_SHADOW: 1 (default) the compact-record methods (11-28) take the seeded hash vector
+ * of method 10 as shadow residual, computed where it is read (no r0 vector is stored or read);
+ * 0 the reference's r0 = r.  The iteration count changes; the accepted answer meets the same
+ * FP64 residual. */
+#define FFVHC_OPTION_SHADOW 3
 int ffvhc_ctx_set_option(ffvhc_ctx* ctx, int option, int value);
 
 /* Physical parameters and outer BCs of one flow case.  BC groups: 0 UX, 1 UY, 2 UZ, 3 P,

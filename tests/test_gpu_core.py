@@ -486,6 +486,39 @@ def test_stencil_classes_bit_identical(ctx, method, n_other):
         ctx.remove_case(cid)
 
 
+@pytest.mark.parametrize("method", [13, 19, 26])
+def test_shadow_residual_choices(ctx, method):
+    """The compact-record methods with the seeded hash shadow residual (default) and with the
+    reference's r0 = r both reach the FP64 residual and agree; each is deterministic."""
+    tree = trees()["two_level"]
+    params = ffvhc.case_params(bc=BC)
+    cid = ctx.add_case(tree, params)
+    try:
+        og = OracleGrid(tree)
+        rng = np.random.default_rng(5)
+        bct = [params.bct[ffvhc.BC_P][q] for q in range(6)]
+        bcv = [params.bcv[ffvhc.BC_P][q] for q in range(6)]
+        A, b, _ = class_uniform_system(og, rng, bct, bcv, 2)
+        fa, fb, fx = upload_system(ctx, cid, A, b, og.zeros())
+        got = {}
+        for hashed in (True, False, True, False):
+            ctx.set_shadow_hash(hashed)
+            ctx.upload(fx, cid, og.zeros())
+            cnt, res, _o, rc = ctx.solve_mixed(method, fx, fa, fb, ffvhc.BC_P, 1.4, 5000, 1e-10, 0.5, ncases=1)
+            assert rc == 0 and res[0] <= 1e-10
+            x = ctx.download(fx, cid, out=og.zeros())
+            if hashed in got:
+                assert got[hashed][0] == int(cnt[0]) and np.array_equal(got[hashed][1], x)
+            got[hashed] = (int(cnt[0]), x)
+        xa, xb = got[True][1][:, LIVE], got[False][1][:, LIVE]
+        assert np.max(np.abs(xa - xb)) <= 1e-6 * np.max(np.abs(xa))
+        for f in fa + [fb, fx]:
+            ctx.free(f)
+    finally:
+        ctx.set_shadow_hash(True)
+        ctx.remove_case(cid)
+
+
 def run_oracle_bicgstab(og, x, A, b, bct, bcv, omega, cmax, eps):
     cnt, res = ctypes.c_int(), ctypes.c_double()
     sc = [og.zeros() for _ in range(9)]
