@@ -269,7 +269,7 @@ def mixed_prof_name(method, stage, uniform=False):
     """rocprofv3 name of a stage's kernel in the FP64-recurrence methods 5-28; uniform: the
     variant of the class-uniform blocks (methods 11-28 read their bands from the class rows)."""
     mode = {"bi_p": 0, "bi_q": 0, "bi_s": 1, "bi_t": 1}.get(stage)
-    hr = "true" if method >= 10 else "false"  # 11-28: the hashed shadow residual by default
+    hr = "true" if method == 10 else "false"
     tf = lambda v: "true" if v else "false"
     reg = 1 if uniform and method >= 11 else 0  # the class-only variant (all blocks uniform)
     # additive coarse correction (17-22): the matvec and XR add e to p_ / s_; multiplicative
@@ -322,7 +322,7 @@ def kernel_bytes_mixed(method, uniform=False):
     if uniform and method >= 11:
         mv, mp = 0, 0
     nb = 1 if method == 7 else 7                    # (11-13: one 16-byte record per cell)
-    r0 = 0 if method >= 10 else 8                   # hashed shadow residual: not stored
+    r0 = 0 if method == 10 else 8                   # hashed shadow residual: not stored
     # 17-22 add 8 B per cell (the block sum needs no extra read; the corrections are per block)
     return {"bi_p": 24 + nb * mp + 16 + h,          # r, p, q in; p, p_ out; p_ ghosts
             "bi_q": 8 + h + 7 * mv + r0 + 8,        # p_ + ghosts, bands, r0 in; q_ out

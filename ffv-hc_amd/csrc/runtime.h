@@ -98,10 +98,10 @@ struct ffvhc_ctx {
   float* dctab = nullptr;
   void* dctrec = nullptr;
   bool classes = true;
-  // shadow residual of the compact-record methods: the seeded hash vector (default; on C2 it took
-  // 2-4 fewer iterations than r0 = r for methods 24-27 and saves the r0 reads) or r0 = r
-  // (FFVHC_OPTION_SHADOW 0)
-  bool shadow_hash = true;
+  // shadow residual of the compact-record methods: r0 = r (default, the reference's choice) or the
+  // seeded hash vector (FFVHC_OPTION_SHADOW 1: on C2 2-4 fewer iterations for methods 24-27 and no
+  // r0 reads, on the flow past a sphere 15% more iterations a step)
+  bool shadow_hash = false;
   // replay the BiCGSTAB iteration as a captured HIP graph (ffvhc_ctx_set_option; off by default:
   // measured 1-11% slower per iteration than direct launches on ROCm 7.2, 64-960 blocks)
   bool graphs = false;

@@ -284,8 +284,8 @@ class Context:
         check(_L.ffvhc_ctx_set_option(self.h, 1, int(bool(on))), "ffvhc_ctx_set_option")
 
     def set_shadow_hash(self, on):
-        """Seeded hash shadow residual for the compact-record solve methods (11-28) (default), or
-        r0 = r."""
+        """Seeded hash shadow residual for the compact-record solve methods (11-28), or r0 = r
+        (default)."""
         check(_L.ffvhc_ctx_set_option(self.h, 3, int(bool(on))), "ffvhc_ctx_set_option")
 
     def set_classes(self, on):

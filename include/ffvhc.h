@@ -104,10 +104,10 @@ long long ffvhc_ctx_device_bytes(const ffvhc_ctx* ctx);
  * bands and preconditioner records from 27 rows instead of per cell; 0 reads every block per
  * cell.  The results are identical. */
 #define FFVHC_OPTION_CLASSES 2
-/* FFVHC_OPTION_SHADOW: 1 (default) the compact-record methods (11-28) take the seeded hash vector
- * of method 10 as shadow residual, computed where it is read (no r0 vector is stored or read);
- * 0 the reference's r0 = r.  The iteration count changes; the accepted answer meets the same
- * FP64 residual. */
+/* FFVHC_OPTION_SHADOW: 0 (default) the compact-record methods (11-28) take the reference's shadow
+ * residual r0 = r; 1 the seeded hash vector of method 10, computed where it is read (no r0
+ * vector is stored or read).  The iteration count changes (on the C2 Poisson system 2-4 fewer,
+ * on the flow past a sphere more); the accepted answer meets the same FP64 residual. */
 #define FFVHC_OPTION_SHADOW 3
 int ffvhc_ctx_set_option(ffvhc_ctx* ctx, int option, int value);
 

@@ -44,7 +44,7 @@ def test_mixed_byte_model_and_names():
         by = bench.kernel_bytes_mixed(m)
         assert set(by) == {"bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"}
         # q_ and t_ differ by r0 (hashed in 10); p_ and s_ by the p read
-        assert by["bi_q"] == by["bi_t"] - (8 if m >= 10 else 0) and by["bi_p"] - by["bi_s"] == 8
+        assert by["bi_q"] == by["bi_t"] - (8 if m == 10 else 0) and by["bi_p"] - by["bi_s"] == 8
         for st in by:
             assert "<" in bench.mixed_prof_name(m, st)
     # method 8: FP64 vectors, FP32 bands: 7 bands x 4 bytes in the GS and matvec kernels
@@ -59,13 +59,13 @@ def test_mixed_byte_model_and_names():
     assert bench.mixed_prof_name(21, "bi_p") == "bi_precond_gsk_kernel<0, 5, false, 0>"
     assert bench.mixed_prof_name(25, "bi_s") == "bi_precond_gsk_kernel<1, 3, true, 0>"
     assert bench.mixed_prof_name(25, "bi_s", True) == "bi_precond_gsk_kernel<1, 3, true, 1>"
-    assert bench.mixed_prof_name(21, "bi_q") == "bi_matvec_kernel<double, 0, float, true, true, 0>"
-    assert bench.mixed_prof_name(21, "bi_q", True) == "bi_matvec_kernel<double, 0, float, true, true, 1>"
+    assert bench.mixed_prof_name(21, "bi_q") == "bi_matvec_kernel<double, 0, float, false, true, 0>"
+    assert bench.mixed_prof_name(21, "bi_q", True) == "bi_matvec_kernel<double, 0, float, false, true, 1>"
     # class-uniform blocks: no per-cell band or record is read
     assert abs(bench.kernel_bytes_mixed(26, True)["bi_p"] - (24 + 16 + 3)) < 1e-9
-    assert abs(bench.kernel_bytes_mixed(26, True)["bi_q"] - (8 + 3 + 8)) < 1e-9
+    assert abs(bench.kernel_bytes_mixed(26, True)["bi_q"] - (8 + 3 + 8 + 8)) < 1e-9
     # multiplicative methods store p_ / s_ with the coarse correction added: plain matvec and XR
-    assert bench.mixed_prof_name(26, "bi_t") == "bi_matvec_kernel<double, 1, float, true, false, 0>"
-    assert bench.mixed_prof_name(26, "bi_xr") == "bi_xr_kernel<double, true, false>"
-    assert bench.mixed_prof_name(19, "bi_xr") == "bi_xr_kernel<double, true, true>"
+    assert bench.mixed_prof_name(26, "bi_t") == "bi_matvec_kernel<double, 1, float, false, false, 0>"
+    assert bench.mixed_prof_name(26, "bi_xr") == "bi_xr_kernel<double, false, false>"
+    assert bench.mixed_prof_name(19, "bi_xr") == "bi_xr_kernel<double, false, true>"
     assert len(bench.MIXED_METHODS) == 29

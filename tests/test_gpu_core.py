@@ -488,8 +488,8 @@ def test_stencil_classes_bit_identical(ctx, method, n_other):
 
 @pytest.mark.parametrize("method", [13, 19, 26])
 def test_shadow_residual_choices(ctx, method):
-    """The compact-record methods with the seeded hash shadow residual (default) and with the
-    reference's r0 = r both reach the FP64 residual and agree; each is deterministic."""
+    """The compact-record methods with the reference's r0 = r (default) and with the seeded hash
+    shadow residual both reach the FP64 residual and agree; each is deterministic."""
     tree = trees()["two_level"]
     params = ffvhc.case_params(bc=BC)
     cid = ctx.add_case(tree, params)
@@ -515,7 +515,7 @@ def test_shadow_residual_choices(ctx, method):
         for f in fa + [fb, fx]:
             ctx.free(f)
     finally:
-        ctx.set_shadow_hash(True)
+        ctx.set_shadow_hash(False)
         ctx.remove_case(cid)
 
 
