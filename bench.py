@@ -414,6 +414,12 @@ def main():
                        for name, t in zip(("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"), ms)}
         kt["mixed_other_us"] = {"finalisations": round(msall[5] * 1e3, 2), "coarse_corrections": round(msall[6] * 1e3, 2)}
         log("method %d kernels: %s" % (sched[0], json.dumps(kt["mixed"])))
+    if args.profile:
+        # the other sweep counts the autotuner may pick (their sweep kernels appear in the trace
+        # and counter summaries too)
+        for m in (23, 24, 25, 27, 28):
+            ctx.fill(f64["x"], 0.0)
+            ctx.bicgstab_timing(f64["x"], f64["A"], f64["b"], PROFILE_SCHEDULE[3], 5, method=m)
 
     def solve64():
         ctx.fill(f64["x"], 0.0)
