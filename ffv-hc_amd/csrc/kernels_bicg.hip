@@ -706,19 +706,39 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
 
 // CC: p_ and s_ carry the coarse corrections ce[slot] / ce[nslots + slot] (two-level
 // preconditioners), added as the matvecs added them.
+// tpart: the T stage's block partials (t_.s, t_.t_); every workgroup adds its case's partials in
+// the fixed order of the per-case finalisation and takes gamma = (t_.s) / (t_.t_)
+// (FFVILS.h:905-915) itself, the case's first block also storing it in the case state, so no
+// finalisation launch sits between T and XR.
 template <class T, bool HR = false, bool CC = false>
 __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
                                                       const DevCase* __restrict__ cs, SolveState* __restrict__ st,
                                                       T* __restrict__ x, BiVecs<T> V, double* __restrict__ part,
                                                       size_t nslots, int count, const double* __restrict__ ce = nullptr,
-                                                      double* __restrict__ bsr = nullptr) {
-  __shared__ double red[BI_NT / 64];
+                                                      double* __restrict__ bsr = nullptr,
+                                                      const double* __restrict__ tpart = nullptr) {
+  __shared__ double red[2 * (BI_NT / 64)];
+  __shared__ double gshared;
   const int slot = active[blockIdx.x];
   const int c = blk[slot].case_id;
   if (st[c].done) return;
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
-  const T alpha = (T)st[c].alpha, gamma = (T)st[c].gamma, ng = (T)(-st[c].gamma);
+  double gd;
+  if (tpart) {
+    const DevCase& C = cs[c];
+    double t0, t1;
+    case_sum2<BI_NT>(tpart, tpart + nslots, C.slot_begin, C.nblk, red, t0, t1);
+    if (t == 0) {
+      gshared = t0 / t1;
+      if (slot == C.slot_begin) st[c].gamma = gshared;
+    }
+    __syncthreads();
+    gd = gshared;
+  } else {
+    gd = st[c].gamma;
+  }
+  const T alpha = (T)st[c].alpha, gamma = (T)gd, ng = (T)(-gd);
   const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
   constexpr int KB = Batch<T>::K;
   double rr = 0.0, rr1 = 0.0, rs = 0.0;
@@ -850,6 +870,7 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   if (!nact) return;
   const bool dbl = sizeof(T) == 8;
   BiVecs<double>& Vd = *reinterpret_cast<BiVecs<double>*>(&V);
+  double* const part0 = part;
   auto precond = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
     if (dbl && o.gs == BiOpts::GSC && o.cinv && o.mult)  // block sum of p kept for the next coarse rhs
@@ -878,8 +899,10 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
       hipLaunchKernelGGL((bi_precond_kernel<T, MODE>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V,
                          count);
   };
+  // the T stage keeps its partials in part[2..3] (XR reads them while writing part[0..1])
   auto matvec = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
+    double* const part = MODE == 0 ? part0 : part0 + 2 * nslots;
     if (dbl && o.A32 && o.gs == BiOpts::GSC) {
       // 11-28: the additive correction (17-22) adds e to the operand; the multiplicative one
       // (23-28) finds it in p_ / s_ already and keeps the block sum of q_.  Class-uniform blocks
@@ -947,22 +970,20 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   if (mask & 4) precond(M1{});
   if ((mask & 4) && !split && !pre) coarse(1);
   if (split && (mask & 128)) coarse(1);
-  if (mask & 8) {
-    matvec(M1{});
-    if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_GAMMA, count);
-  }
+  if (mask & 8) matvec(M1{});  // gamma is formed by XR from the T partials
   if (mask & 16) {
     auto xr = [&](auto hr_tag) {
       constexpr bool HR = decltype(hr_tag)::value;
+      const double* tpart = part + 2 * nslots;
       if (o.cinv && o.mult)  // p_ / s_ already hold P e; block sum of r kept
         hipLaunchKernelGGL((bi_xr_kernel<T, HR>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
-                           nslots, count, nullptr, o.ce + 2 * nslots);
+                           nslots, count, nullptr, o.ce + 2 * nslots, tpart);
       else if (o.cinv)
         hipLaunchKernelGGL((bi_xr_kernel<T, HR, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V,
-                           part, nslots, count, (const double*)o.ce, nullptr);
+                           part, nslots, count, (const double*)o.ce, nullptr, tpart);
       else
         hipLaunchKernelGGL((bi_xr_kernel<T, HR>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
-                           nslots, count);
+                           nslots, count, nullptr, nullptr, tpart);
     };
     if (o.hash_r0)
       xr(std::true_type{});

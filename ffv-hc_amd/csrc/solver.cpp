@@ -333,15 +333,15 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
                           (const T*)c->fields[b].d[0], (const T*)X.d[0], (const T*)X.h[0], V, c->dpart,
                           (size_t)c->cap, o.hash_r0, o.mult ? o.ce + 2 * c->cap : nullptr);
   fin(c, c->dst, FIN_BI_INIT, 0);
-  // stages in launch order: P, coarse(p_), Q, fin(alpha), S, coarse(s_), T, fin(gamma), XR,
-  // fin(convergence, beta); the coarse applies (two-level methods only) time as their own stage
-  // (the multiplicative coarse correction runs before P / S instead)
+  // stages in launch order: P, coarse(p_), Q, fin(alpha), S, coarse(s_), T, (gamma: formed inside
+  // XR, an empty stage), XR, fin(convergence, beta); the coarse applies (two-level methods only)
+  // time as their own stage (the multiplicative coarse correction runs before P / S instead)
   constexpr int NS = 10;
   static const int bit_add[NS] = {1, 64, 2, 0, 4, 128, 8, 0, 16, 0};
   static const int slot_add[NS] = {0, 6, 1, 5, 2, 6, 3, 5, 4, 5};
   static const int bit_mul[NS] = {64, 1, 2, 0, 128, 4, 8, 0, 16, 0};
   static const int slot_mul[NS] = {6, 0, 1, 5, 6, 2, 3, 5, 4, 5};
-  static const int fin_of[NS] = {-1, -1, -1, FIN_BI_ALPHA, -1, -1, -1, FIN_BI_GAMMA, -1, FIN_BI_XR};
+  static const int fin_of[NS] = {-1, -1, -1, FIN_BI_ALPHA, -1, -1, -1, -1, -1, FIN_BI_XR};
   const int* bit_of = o.mult ? bit_mul : bit_add;
   const int* slot_of = o.mult ? slot_mul : slot_add;
   std::vector<hipEvent_t> ev((NS + 1) * niter);
@@ -353,7 +353,7 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
       if (bit_of[k])
         launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase, c->dst,
                                 K, (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, bit_of[k] | 32 | 256, o);
-      else
+      else if (fin_of[k] >= 0)
         fin(c, c->dst, fin_of[k], n);
       HIPCHK(hipEventRecord(e[k + 1], c->sm));
     }
