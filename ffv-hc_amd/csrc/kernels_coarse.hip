@@ -113,18 +113,16 @@ void launch_coarse_decouple(hipStream_t sm, double* M, int n) {
 // In-place blocked Gauss-Jordan inversion of a dense n x n matrix (row-major, no pivoting: the
 // coarse operators are diagonally dominant).  For each panel P of GJB pivots, with D = M_PP:
 //   M_PP <- D^-1,  M_Pj <- D^-1 M_Pj,  M_iP <- -M_iP D^-1,  M_ij <- M_ij - M_iP D^-1 M_Pj
-// (i, j outside P) -- the scalar Gauss-Jordan step applied to a block of pivots.  Four launches
-// per panel (the trailing update also stores the new panel rows; the column step works in place
-// within a workgroup); no launch reads a value another workgroup of it writes.
+// (i, j outside P) -- the scalar Gauss-Jordan step applied to a block of pivots.  Three launches
+// per panel (the trailing update also stores the new panel rows and inverts the next panel's
+// diagonal block; the column step works in place within a workgroup); no launch reads a value
+// another workgroup of it writes.
 constexpr int GJB = 32;
 
-// D <- D^-1 in LDS by unblocked Gauss-Jordan; one workgroup of GJB * GJB threads
-__global__ __launch_bounds__(GJB * GJB) void gjb_diag_kernel(double* __restrict__ M, int n, int k0, int kb) {
-  __shared__ double d[GJB][GJB + 1];
-  const int r = threadIdx.x / GJB, c = threadIdx.x % GJB;
+// Unblocked Gauss-Jordan inversion of the leading kb x kb part of d in place; one thread per entry
+// (r, c) of a GJB * GJB workgroup
+__device__ inline void gj_inplace(double (*d)[GJB + 1], int r, int c, int kb) {
   const bool in = r < kb && c < kb;
-  if (in) d[r][c] = M[(size_t)(k0 + r) * n + k0 + c];
-  __syncthreads();
   for (int k = 0; k < kb; k++) {
     double v = 0.0;
     if (in) {
@@ -138,6 +136,16 @@ __global__ __launch_bounds__(GJB * GJB) void gjb_diag_kernel(double* __restrict_
     if (in) d[r][c] = v;
     __syncthreads();
   }
+}
+
+// D <- D^-1 for the first panel (later panels' diagonal blocks are inverted by the update kernel)
+__global__ __launch_bounds__(GJB * GJB) void gjb_diag_kernel(double* __restrict__ M, int n, int k0, int kb) {
+  __shared__ double d[GJB][GJB + 1];
+  const int r = threadIdx.x / GJB, c = threadIdx.x % GJB;
+  const bool in = r < kb && c < kb;
+  if (in) d[r][c] = M[(size_t)(k0 + r) * n + k0 + c];
+  __syncthreads();
+  gj_inplace(d, r, c, kb);
   if (in) M[(size_t)(k0 + r) * n + k0 + c] = d[r][c];
 }
 
@@ -153,30 +161,48 @@ __global__ __launch_bounds__(256) void gjb_row_kernel(const double* __restrict__
 }
 
 // M_ij -= sum_c M_i,k0+c T[c][j] for i, j outside the panel, and the panel rows M_Pj <- T[.][j]
-// (j outside the panel: nothing here reads them); 16 x 16 tiles staged in LDS
-__global__ __launch_bounds__(256) void gjb_update_kernel(double* __restrict__ M, int n, int k0, int kb,
-                                                         const double* __restrict__ T) {
-  __shared__ double a[16][GJB + 1], t[GJB][17];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int i = blockIdx.y * 16 + ty, j = blockIdx.x * 16 + tx;
-  for (int q = threadIdx.x; q < 16 * GJB; q += 256) {
-    const int rr = q / GJB, cc = q % GJB, ii = blockIdx.y * 16 + rr;
-    a[rr][cc] = (ii < n && cc < kb) ? M[(size_t)ii * n + k0 + cc] : 0.0;
-    const int cr = q / 16, jc = q % 16, jj = blockIdx.x * 16 + jc;
-    t[cr][jc] = (jj < n && cr < kb) ? T[(size_t)cr * n + jj] : 0.0;
+// (j outside the panel: nothing here reads them); GJB x GJB tiles, one entry per thread.  The
+// workgroup whose tile is the next panel's diagonal block (final once this update is done: the
+// column step touches only this panel's columns) inverts it in LDS right away, so the next panel
+// needs no diagonal launch of its own.
+__global__ __launch_bounds__(GJB * GJB) void gjb_update_kernel(double* __restrict__ M, int n, int k0, int kb,
+                                                               const double* __restrict__ T) {
+  __shared__ double a[GJB][GJB + 1], t[GJB][GJB + 1];
+  const int r = threadIdx.x / GJB, cc = threadIdx.x % GJB;
+  // workgroup 0 takes the next panel's diagonal tile (its inversion is the longest job, so it
+  // starts first) and that tile's workgroup takes tile (0, 0)
+  const int tn = (k0 + GJB) / GJB;
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (tn < (int)gridDim.x) {
+    if (bx == 0 && by == 0) bx = by = tn;
+    else if (bx == tn && by == tn) bx = by = 0;
   }
+  const int i0 = by * GJB, j0 = bx * GJB, i = i0 + r, j = j0 + cc;
+  a[r][cc] = (i < n && cc < kb) ? M[(size_t)i * n + k0 + cc] : 0.0;
+  t[r][cc] = (j < n && r < kb) ? T[(size_t)r * n + j] : 0.0;
   __syncthreads();
-  if (i >= n || j >= n || (j >= k0 && j < k0 + kb)) return;
-  if (i >= k0 && i < k0 + kb) {
-    M[(size_t)i * n + j] = T[(size_t)(i - k0) * n + j];
-    return;
-  }
-  double s = 0.0;
+  double v = 0.0;
+  if (i < n && j < n && !(j >= k0 && j < k0 + kb)) {
+    if (i >= k0 && i < k0 + kb) {
+      v = T[(size_t)(i - k0) * n + j];
+    } else {
+      double s = 0.0;
 #pragma unroll 8
-  for (int c = 0; c < GJB; c++) s += a[ty][c] * t[c][tx];
-  M[(size_t)i * n + j] -= s;
+      for (int c = 0; c < GJB; c++) s += a[r][c] * t[c][cc];
+      v = M[(size_t)i * n + j] - s;
+    }
+    M[(size_t)i * n + j] = v;
+  }
+  const int k1 = k0 + GJB;
+  if (i0 != k1 || j0 != k1) return;  // uniform over the workgroup
+  const int kb1 = n - k1 < GJB ? n - k1 : GJB;
+  const bool in = r < kb1 && cc < kb1;
+  __syncthreads();  // every read of a and t is done
+  a[r][cc] = in ? v : 0.0;
+  __syncthreads();
+  gj_inplace(a, r, cc, kb1);
+  if (in) M[(size_t)(k1 + r) * n + k1 + cc] = a[r][cc];
 }
-
 // M_iP <- -M_iP D^-1 for the rows outside the panel, in place: the GJB threads of a row share a
 // workgroup (256 / GJB rows each), read the row's panel values, synchronise, then write them
 __global__ __launch_bounds__(256) void gjb_col_kernel(double* __restrict__ M, int n, int k0, int kb) {
@@ -194,13 +220,14 @@ __global__ __launch_bounds__(256) void gjb_col_kernel(double* __restrict__ M, in
 // T: scratch of GJB * n doubles.  Order per panel: D^-1; T = D^-1 M_P; the trailing update
 // (old M_iP, T) with panel rows <- T; M_iP <- -M_iP D^-1.
 void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* T) {
+  if (n <= 0) return;
+  const unsigned nt = (unsigned)((n + 255) / 256), ntt = (unsigned)((n + GJB - 1) / GJB);
+  const unsigned nr = (unsigned)((n + 256 / GJB - 1) / (256 / GJB));
+  hipLaunchKernelGGL(gjb_diag_kernel, dim3(1), dim3(GJB * GJB), 0, sm, M, n, 0, n < GJB ? n : GJB);
   for (int k0 = 0; k0 < n; k0 += GJB) {
     const int kb = n - k0 < GJB ? n - k0 : GJB;
-    const unsigned nt = (unsigned)((n + 255) / 256), nt16 = (unsigned)((n + 15) / 16);
-    const unsigned nr = (unsigned)((n + 256 / GJB - 1) / (256 / GJB));
-    hipLaunchKernelGGL(gjb_diag_kernel, dim3(1), dim3(GJB * GJB), 0, sm, M, n, k0, kb);
     hipLaunchKernelGGL(gjb_row_kernel, dim3(nt, kb), dim3(256), 0, sm, (const double*)M, n, k0, kb, T);
-    hipLaunchKernelGGL(gjb_update_kernel, dim3(nt16, nt16), dim3(256), 0, sm, M, n, k0, kb, (const double*)T);
+    hipLaunchKernelGGL(gjb_update_kernel, dim3(ntt, ntt), dim3(GJB * GJB), 0, sm, M, n, k0, kb, (const double*)T);
     hipLaunchKernelGGL(gjb_col_kernel, dim3(nr), dim3(256), 0, sm, M, n, k0, kb);
   }
 }
