@@ -542,8 +542,10 @@ __global__ __launch_bounds__(GSK_NT) __attribute__((amdgpu_waves_per_eu(REG == 1
   // MUL: the stored direction is z + P e (e added to this block's values and to the ghost values
   // it pushes, which then come from z + e through every face kind)
   double* const out = (MODE == 0 ? V.pp : V.ss) + base;
+  unsigned oo = o0;  // opaque: the store offsets are formed here, not held across the sweeps
+  asm volatile("" : "+v"(oo));
 #pragma unroll
-  for (int k = 0; k < GSK_NK; k++) out[o0 + NF * k] = MUL ? z[o0 + NF * k] + eo : z[o0 + NF * k];
+  for (int k = 0; k < GSK_NK; k++) out[oo + NF * k] = MUL ? z[oo + NF * k] + eo : z[oo + NF * k];
   double* h = MODE == 0 ? V.hpp : V.hss;
   push_halo<double, GSK_NT>(
       B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return MUL ? z[zi(i, j, k)] + eo : z[zi(i, j, k)]; }, h,
