@@ -270,6 +270,38 @@ __device__ inline double coarse_row_dot(const float* __restrict__ row, int n, in
   return s;
 }
 
+// The same sum with the row's first CA_PRE float4 per lane already loaded (pre[i] = r4[lane + 64 i])
+constexpr int CA_PRE = 6;
+template <class Rhs>
+__device__ inline double coarse_row_dot_pre(const float* __restrict__ row, int n, int lane, const float4* pre,
+                                            const Rhs& c) {
+  const float4* r4 = reinterpret_cast<const float4*>(row);
+  double s = 0.0;
+  const int nq = coarse_pitch(n) / 4;
+#pragma unroll
+  for (int i = 0; i < CA_PRE; i++) {
+    const int q = lane + 64 * i;
+    if (q < nq) {
+      const float4 v = pre[i];
+      const int b = 4 * q;
+      s += (double)v.x * c(b);
+      s += (double)v.y * c(b + 1);
+      s += (double)v.z * c(b + 2);
+      s += (double)v.w * c(b + 3);
+    }
+  }
+#pragma unroll 4
+  for (int q = lane + 64 * CA_PRE; q < nq; q += 64) {
+    const float4 v = r4[q];
+    const int b = 4 * q;
+    s += (double)v.x * c(b);
+    s += (double)v.y * c(b + 1);
+    s += (double)v.z * c(b + 2);
+    s += (double)v.w * c(b + 3);
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
 __global__ __launch_bounds__(256) void coarse_apply_kernel(const int* __restrict__ active, int nact,
                                                            const DevBlock* __restrict__ blk,
                                                            const DevCase* __restrict__ cs,
@@ -326,23 +358,35 @@ __global__ __launch_bounds__(CA_NT) void coarse_apply_mul_kernel(const int* __re
   const int c0 = blk[active[first_row]].case_id;
   const int n0 = cs[c0].nblk;
   const bool uni = c0 == blk[active[last_row]].case_id && coarse_pitch(n0) <= CA_MAXN && !st[c0].done;
+  const int w = first_row + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  // this wave's row: its first float4s are loaded before c is staged, so the two memory round
+  // trips overlap
+  const int slot = active[w < nact ? w : first_row];  // a wave past the list reads a valid entry
+  const int cid = blk[slot].case_id;
+  const DevCase& C = cs[cid];
+  const int n = C.nblk, a = slot - C.slot_begin;
+  const float* row = inv + off[cid] + (size_t)a * coarse_pitch(n);
+  float4 pre[CA_PRE];
+  if (uni && w < nact) {
+    const float4* r4 = reinterpret_cast<const float4*>(row);
+    const int nq = coarse_pitch(n) / 4;
+#pragma unroll
+    for (int i = 0; i < CA_PRE; i++) {
+      const int q = lane + 64 * i;
+      pre[i] = q < nq ? r4[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
   if (uni) {
     const CoarseRhs cb(ce, nslots, cs[c0], st[c0], mode);
     for (int b = threadIdx.x; b < coarse_pitch(n0); b += CA_NT) cl[b] = b < n0 ? cb(b) : 0.0;
   }
   __syncthreads();
-  const int w = first_row + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (w >= nact) return;
-  const int slot = active[w];
-  const int cid = blk[slot].case_id;
   const SolveState& S = st[cid];
   if (S.done) return;
-  const DevCase& C = cs[cid];
-  const int n = C.nblk, a = slot - C.slot_begin;
-  const float* row = inv + off[cid] + (size_t)a * coarse_pitch(n);
   double s;
   if (uni) {
-    s = coarse_row_dot(row, n, lane, [&](int b) { return cl[b]; });
+    s = coarse_row_dot_pre(row, n, lane, pre, [&](int b) { return cl[b]; });
   } else {
     const CoarseRhs cb(ce, nslots, C, S, mode);
     s = coarse_row_dot(row, n, lane, [&](int b) { return b < n ? cb(b) : 0.0; });
