@@ -32,7 +32,8 @@ template <int NT>
 __device__ inline double case_sum(const double* __restrict__ part, int begin, int count, double* red) {
   // strided per-thread sums, then the fixed wave pattern and the waves in order (deterministic)
   double v = 0.0;
-  for (int q = threadIdx.x; q < count; q += NT) v += part[begin + q];
+#pragma unroll 4
+  for (int q = threadIdx.x; q < count; q += NT) v += part[begin + q];  // loads issued 4 at a time
   return block_sum<NT>(v, red);  // valid in thread 0
 }
 
@@ -41,6 +42,7 @@ template <int NT>
 __device__ inline void case_sum2(const double* __restrict__ p0, const double* __restrict__ p1, int begin, int count,
                                  double* red, double& s0, double& s1) {
   double v0 = 0.0, v1 = 0.0;
+#pragma unroll 4
   for (int q = threadIdx.x; q < count; q += NT) {
     v0 += p0[begin + q];
     v1 += p1[begin + q];

@@ -954,8 +954,9 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   // the block sums of r, p, q the previous kernels wrote
   auto coarse = [&](int mode) {
     if (!(dbl && o.gs == BiOpts::GSC && o.cinv)) return;
-    if (o.mult)
-      launch_coarse_apply_mul(sm, active, nact, blk, cs, st, o.cinv, o.coff, o.ce, nslots, mode);
+    if (o.mult)  // mode 1 forms alpha from the Q partials (no alpha finalisation)
+      launch_coarse_apply_mul(sm, active, nact, blk, cs, st, o.cinv, o.coff, o.ce, nslots, mode,
+                              mode == 1 ? part0 : nullptr);
     else
       launch_coarse_apply(sm, active, nact, blk, cs, st, o.cinv, o.coff, part + 4 * nslots, o.ce + mode * nslots);
   };
@@ -964,9 +965,10 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   if (mask & 1) precond(M0{});
   if ((mask & 1) && !split && !pre) coarse(0);
   if (split && (mask & 64)) coarse(0);
+  const bool alpha_in_coarse = dbl && o.gs == BiOpts::GSC && o.cinv && o.mult;
   if (mask & 2) {
     matvec(M0{});
-    if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_ALPHA, count);
+    if (!(mask & 32) && !alpha_in_coarse) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_ALPHA, count);
   }
   if ((mask & 4) && !split && pre) coarse(1);
   if (mask & 4) precond(M1{});

@@ -334,21 +334,28 @@ struct CoarseRhs {
   bool first;
   double beta, ng, na;
   int mode;
-  __device__ CoarseRhs(const double* ce, size_t nslots, const DevCase& C, const SolveState& S, int m)
+  // mode 1 takes alpha from the caller (formed in the kernel), mode 0 reads its scalars from S
+  __device__ CoarseRhs(const double* ce, size_t nslots, const DevCase& C, const SolveState& S, int m, double alpha)
       : br(ce + 2 * nslots + C.slot_begin), bq(ce + 3 * nslots + C.slot_begin), bp(ce + 4 * nslots + C.slot_begin),
-        first(S.iter == 0 || S.restart), beta(S.beta), ng(-S.gamma), na(-S.alpha), mode(m) {}
+        first(m == 0 && (S.iter == 0 || S.restart)), beta(m == 0 ? S.beta : 0.0), ng(m == 0 ? -S.gamma : 0.0),
+        na(-alpha), mode(m) {}
   __device__ double operator()(int b) const {
     return mode == 0 ? (first ? br[b] : br[b] + beta * (bp[b] + ng * bq[b])) : na * bq[b] + br[b];
   }
 };
 
+// mode 1: apart holds the Q stage's block partials (q_.r0); every wave forms its case's
+// alpha = rho / (q_.r0) (FFVILS.h:868-872) itself with a fixed-order wave sum (the same value in
+// every wave of the case, whatever else the launch holds) and the wave of the case's first block
+// stores it, so no alpha finalisation launch is needed.
 __global__ __launch_bounds__(CA_NT) void coarse_apply_mul_kernel(const int* __restrict__ active, int nact,
                                                                  const DevBlock* __restrict__ blk,
                                                                  const DevCase* __restrict__ cs,
-                                                                 const SolveState* __restrict__ st,
+                                                                 SolveState* __restrict__ st,
                                                                  const float* __restrict__ inv,
                                                                  const long long* __restrict__ off,
-                                                                 double* __restrict__ ce, size_t nslots, int mode) {
+                                                                 double* __restrict__ ce, size_t nslots, int mode,
+                                                                 const double* __restrict__ apart) {
   __shared__ double cl[CA_MAXN];
   constexpr int R = CA_NT / 64;  // rows per workgroup
   const int first_row = blockIdx.x * R;
@@ -376,30 +383,45 @@ __global__ __launch_bounds__(CA_NT) void coarse_apply_mul_kernel(const int* __re
       pre[i] = q < nq ? r4[q] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
+  // mode 1: alpha of the case this wave stages for (c0 in a one-case workgroup) or computes
+  const int acid = uni ? c0 : cid;
+  double alpha = 0.0;
+  if (mode == 1 && (uni || w < nact) && !st[acid].done) {
+    const DevCase& A = cs[acid];
+    double v = 0.0;
+#pragma unroll 8
+    for (int q = lane; q < A.nblk; q += 64) v += apart[A.slot_begin + q];  // loads issued 8 at a time
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    alpha = st[acid].rr1 / v;
+  }
   if (uni) {
-    const CoarseRhs cb(ce, nslots, cs[c0], st[c0], mode);
+    const CoarseRhs cb(ce, nslots, cs[c0], st[c0], mode, alpha);
     for (int b = threadIdx.x; b < coarse_pitch(n0); b += CA_NT) cl[b] = b < n0 ? cb(b) : 0.0;
   }
   __syncthreads();
   if (w >= nact) return;
-  const SolveState& S = st[cid];
+  SolveState& S = st[cid];
   if (S.done) return;
+  if (mode == 1 && a == 0 && lane == 0) {  // the case's first block stores alpha (FIN_BI_ALPHA)
+    S.alpha = alpha;
+    S.restart = 0;
+  }
   double s;
   if (uni) {
     s = coarse_row_dot_pre(row, n, lane, pre, [&](int b) { return cl[b]; });
   } else {
-    const CoarseRhs cb(ce, nslots, C, S, mode);
+    const CoarseRhs cb(ce, nslots, C, S, mode, alpha);
     s = coarse_row_dot(row, n, lane, [&](int b) { return b < n ? cb(b) : 0.0; });
   }
   if (lane == 0) ce[mode * nslots + slot] = s;
 }
 
 void launch_coarse_apply_mul(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
-                             const SolveState* st, const float* inv, const long long* off, double* ce, size_t nslots,
-                             int mode) {
+                             SolveState* st, const float* inv, const long long* off, double* ce, size_t nslots,
+                             int mode, const double* apart) {
   if (nact)
     hipLaunchKernelGGL(coarse_apply_mul_kernel, dim3((nact + CA_NT / 64 - 1) / (CA_NT / 64)), dim3(CA_NT), 0, sm,
-                       active, nact, blk, cs, st, inv, off, ce, nslots, mode);
+                       active, nact, blk, cs, st, inv, off, ce, nslots, mode, apart);
 }
 
 void launch_coarse_apply(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,

@@ -335,13 +335,17 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
   fin(c, c->dst, FIN_BI_INIT, 0);
   // stages in launch order: P, coarse(p_), Q, fin(alpha), S, coarse(s_), T, (gamma: formed inside
   // XR, an empty stage), XR, fin(convergence, beta); the coarse applies (two-level methods only)
-  // time as their own stage (the multiplicative coarse correction runs before P / S instead)
+  // time as their own stage (the multiplicative coarse correction runs before P / S instead, and
+  // its s_ application forms alpha, so those methods have no alpha finalisation stage)
   constexpr int NS = 10;
   static const int bit_add[NS] = {1, 64, 2, 0, 4, 128, 8, 0, 16, 0};
   static const int slot_add[NS] = {0, 6, 1, 5, 2, 6, 3, 5, 4, 5};
   static const int bit_mul[NS] = {64, 1, 2, 0, 128, 4, 8, 0, 16, 0};
   static const int slot_mul[NS] = {6, 0, 1, 5, 6, 2, 3, 5, 4, 5};
-  static const int fin_of[NS] = {-1, -1, -1, FIN_BI_ALPHA, -1, -1, -1, -1, -1, FIN_BI_XR};
+  static const int fin_add[NS] = {-1, -1, -1, FIN_BI_ALPHA, -1, -1, -1, -1, -1, FIN_BI_XR};
+  static const int fin_mul[NS] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, FIN_BI_XR};  // alpha: coarse(s_)
+  const bool mul_alpha = o.mult && o.gs == BiOpts::GSC && o.cinv && sizeof(T) == 8;
+  const int* fin_of = mul_alpha ? fin_mul : fin_add;
   const int* bit_of = o.mult ? bit_mul : bit_add;
   const int* slot_of = o.mult ? slot_mul : slot_add;
   std::vector<hipEvent_t> ev((NS + 1) * niter);
