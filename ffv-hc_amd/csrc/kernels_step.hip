@@ -175,7 +175,11 @@ __global__ __launch_bounds__(ST_NT) void predict_kernel(const int* __restrict__ 
   const int t = threadIdx.x, i = t & 15, j = t >> 4;
   const double dx = B.dx, dt = C.dt, rhof = C.rho, mu = C.mu, Uc = 0.0;
   const bool step0 = C.step == 0;
-  for (int dir = 0; dir < 3; dir++) {
+  {
+    // one workgroup per block and component (the components are independent: each reads its own
+    // u, the face velocities v and p, and writes its own uc, ud and u), which triples the
+    // workgroups of small cases
+    const int dir = blockIdx.y;
     const double* __restrict__ u = F.u[dir];
     const double* __restrict__ hu = F.hu[dir];
     double unew[NB];
@@ -680,7 +684,7 @@ __global__ __launch_bounds__(ST_NT) void stats_sum_kernel(const int* __restrict_
 // ------------------------------------------------------------------ launchers
 void launch_predict(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                     const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc) {
-  if (nact) hipLaunchKernelGGL(predict_kernel, dim3(nact), dim3(ST_NT), 0, sm, active, blk, cs, F, cut, flag, capnc);
+  if (nact) hipLaunchKernelGGL(predict_kernel, dim3(nact, 3), dim3(ST_NT), 0, sm, active, blk, cs, F, cut, flag, capnc);
 }
 void launch_abd_u(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                   const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc, int dir, double alpha) {
