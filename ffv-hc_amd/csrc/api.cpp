@@ -258,6 +258,7 @@ void ffvhc_ctx_destroy(ffvhc_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->sm);
+  if (c->sc) (void)hipStreamSynchronize(c->sc);
   for (auto& f : c->fields)
     for (int b = 0; b < 2; b++) {
       if (f.d[b]) (void)hipFree(f.d[b]);
@@ -282,6 +283,11 @@ void ffvhc_ctx_destroy(ffvhc_ctx* c) {
   (void)hipFree(c->dflag);
   (void)hipFree(c->dcid);
   (void)hipStreamDestroy(c->sm);
+  if (c->sc) {
+    (void)hipEventDestroy(c->ev_cs);
+    (void)hipEventDestroy(c->ev_cd);
+    (void)hipStreamDestroy(c->sc);
+  }
   delete c;
 }
 
@@ -312,6 +318,7 @@ int ffvhc_case_add(ffvhc_ctx* c, const ffvhc_tree* t, const ffvhc_case_params* p
   if (!c || !t || !p || !case_id) return fail(-1, "null argument");
   if (t->tree->spec().n != NB) return fail(-1, "device blocks are 16^3 cells");
   if ((cut || cid || pid) && g < 1) return fail(-1, "geometry needs g >= 1");
+  c->geom_serial++;
   const auto& B = t->tree->blocks();
   const int64_t nb = (int64_t)B.size();
   // first-fit contiguous slot range
@@ -425,6 +432,7 @@ int ffvhc_case_remove(ffvhc_ctx* c, int id) {
   r = CaseRec();
   c->sel.erase(std::remove(c->sel.begin(), c->sel.end(), id), c->sel.end());
   c->dirty = true;
+  c->geom_serial++;
   return refresh_active(c);
 }
 
@@ -438,6 +446,7 @@ int ffvhc_case_set_bc(ffvhc_ctx* c, int id, int group, int face, int type, doubl
   if (group < 0 || group >= NUM_USER_BC || face < 0 || face > 5) return fail(-1, "bad group/face");
   c->cases[id].params.bct[group][face] = type;
   c->cases[id].params.bcv[group][face] = value;
+  c->geom_serial++;
   return upload_case_meta(c, id);
 }
 

@@ -10,6 +10,7 @@
 
 #include "../../include/ffvhc.h"
 #include "device.h"
+#include "kernels.h"
 #include "octree.h"
 
 struct ffvhc_tree {
@@ -92,6 +93,35 @@ struct ffvhc_ctx {
   double* dcsnap = nullptr;
   size_t dcsnap_n = 0;
   double* dce = nullptr;
+  // host copies the setups read back or upload asynchronously; they live here so that they
+  // outlive the copies (a setup returns without waiting for its last kernels)
+  std::vector<double> hcrows;   // 25 coarse row entries per slot
+  std::vector<long long> hcoff;  // the offsets uploaded to dcoff
+  std::vector<int> hcuni;        // class-uniform flag per slot
+  // the coarse inversion runs on stream sc (default priority) beside the solver's first kernels
+  // on sm: ev_cs orders it after the assembly on sm, ev_cd marks its end; coarse_pending says
+  // that sm has not yet been made to wait for ev_cd
+  hipStream_t sc = nullptr;
+  hipEvent_t ev_cs = nullptr, ev_cd = nullptr;
+  bool coarse_pending = false;
+  // setup_serial counts the writes of the solver setup buffers (FP32 band copies, preconditioner
+  // records, class tables, coarse inverse); geom_serial counts the changes of what a time step's
+  // pressure matrix is made from (cases added or removed, boundary conditions).  The time step
+  // keeps the setup of its pressure solve (ffvhc_solve_mixed methods 11-28) while neither has
+  // moved and the method, matrix fields, active blocks and options are the same
+  // (FFVHC_OPTION_SETUP_REUSE).
+  bool reuse_setup = true;
+  bool step_solve = false;  // set by the time step around its pressure solve
+  uint64_t setup_serial = 0, geom_serial = 0;
+  struct PressureSetup {
+    bool valid = false;
+    int method = -1;
+    int A[7] = {};
+    uint64_t setup_serial = 0, geom_serial = 0;
+    bool classes = false, shadow = false;
+    std::vector<int> active;
+    ffvhc::BiOpts o;
+  } psetup;
   // stencil classes of the compact-record methods (FFVHC_OPTION_CLASSES): per slot the
   // class-uniform flag and the 27 class rows as FP32 bands and as 16-byte preconditioner records
   int* dcuni = nullptr;

@@ -19,6 +19,20 @@ constexpr int kBatchMax = 256;
 // scratch roles
 enum { R_R = 1, R_P = 2, R_Q = 3, R_E = 4, R_RHS = 5, R_A = 10, R_BI = 20, R_REC = 30 };
 
+// Orders c->sm after a coarse inversion still running on c->sc (a no-op when none is).
+int coarse_join(ffvhc_ctx* c) {
+  if (!c->coarse_pending) return 0;
+  c->coarse_pending = false;
+  HIPCHK(hipStreamWaitEvent(c->sm, c->ev_cd, 0));
+  return 0;
+}
+
+// Joins the coarse inversion when the entry point that started it returns, on every path.
+struct CoarseJoin {
+  ffvhc_ctx* c;
+  ~CoarseJoin() { (void)coarse_join(c); }
+};
+
 template <class T>
 Coef7<T> coef(ffvhc_ctx* c, const int* A) {
   Coef7<T> k;
@@ -161,6 +175,9 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
                           (const T*)X.h[0], V, c->dpart, (size_t)c->cap, opts.hash_r0,
                           opts.mult ? opts.ce + 2 * c->cap : nullptr);
   fin(c, st, FIN_BI_INIT, 0);
+  // the iterations read the coarse operator of the two-level preconditioners
+  int rcj = coarse_join(c);
+  if (rcj) return rcj;
   // One iteration's launches (count 0: the iteration number lives in the case state) captured
   // once per solve and replayed: one graph launch per iteration instead of up to ten kernel launches.
   hipGraphExec_t exec = nullptr;
@@ -376,24 +393,50 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
 }
 
 // Coarse correction of the two-level methods: A_c = P^T A P per selected case (rows from the
-// device, assembled densely here), inverted on the device (Gauss-Jordan, FP64), kept as FP32.
-// A case whose A_c has zero row sums throughout (no Dirichlet face: A_c singular) has its last
-// coarse unknown decoupled, which keeps the coarse correction a fixed linear map.
-int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o, double* ac_out = nullptr,
-                 double* inv_out = nullptr) {
+// device, assembled densely on the device), inverted on the device (Gauss-Jordan, FP64), kept as
+// FP32.  A case whose A_c has zero row sums throughout (no Dirichlet face: A_c singular) has its
+// last coarse unknown decoupled, which keeps the coarse correction a fixed linear map.
+// First part: the coarse rows of every active block and their readback into c->hcrows, complete
+// after the next synchronisation of c->sm.
+int coarse_rows_launch(ffvhc_ctx* c, const Coef7<double>& K) {
+  c->setup_serial++;
   int fr;
   int rc = scratch_field(c, 8, R_REC, &fr);
   if (rc) return rc;
   double* drows = (double*)c->fields[fr].d[0];  // 25 doubles per slot (cap * NC doubles available)
   launch_coarse_rows(c->sm, c->dactive, c->nactive, c->dblk, K, drows);
   HIPCHK(hipGetLastError());
-  std::vector<double> rows((size_t)c->cap * 25);
-  HIPCHK(hipMemcpyAsync(rows.data(), drows, rows.size() * sizeof(double), hipMemcpyDeviceToHost, c->sm));
-  HIPCHK(hipStreamSynchronize(c->sm));
+  c->hcrows.resize((size_t)c->cap * 25);
+  HIPCHK(hipMemcpyAsync(c->hcrows.data(), drows, c->hcrows.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        c->sm));
+  return 0;
+}
+
+// Second part, after that synchronisation: assembly and decoupling are queued on c->sm, the
+// inversion and the FP32 copy on c->sc after them, and the call returns without waiting (the
+// solver's first kernels, which do not read the coarse operator, run beside the inversion);
+// coarse_join makes c->sm wait for it before anything reads it.  With ac_out / inv_out (copies
+// of A_c / its inverse for the caller) everything runs on c->sm and the call waits.
+int coarse_finish(ffvhc_ctx* c, BiOpts* o, double* ac_out, double* inv_out) {
+  int fr;
+  int rc = scratch_field(c, 8, R_REC, &fr);
+  if (rc) return rc;
+  rc = coarse_join(c);  // a previous inversion still writing dcm / dci
+  if (rc) return rc;
+  const bool side = !ac_out && !inv_out;
+  if (side && !c->sc) {
+    HIPCHK(hipStreamCreateWithFlags(&c->sc, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_cs, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_cd, hipEventDisableTiming));
+  }
+  const double* drows = (const double*)c->fields[fr].d[0];
+  const std::vector<double>& rows = c->hcrows;
   const std::vector<int> ids = selected_cases(c);
   // FP64 operators / inverses at off[case] (n x n); their FP32 copies at offp[case] with the row
   // pitch padded to a multiple of 4 (float4 rows)
-  std::vector<long long> off(c->cases.size(), 0), offp(c->cases.size(), 0);
+  c->hcoff.assign(2 * (size_t)MAX_CASES, 0);
+  long long* off = c->hcoff.data();
+  long long* offp = off + MAX_CASES;
   size_t total = 0, totalp = 0, maxn = 0;
   for (int id : ids) {
     off[id] = (long long)total;
@@ -421,9 +464,7 @@ int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o, double* ac_out
   }
   if (!c->dcoff) HIPCHK(hipMalloc(&c->dcoff, 2 * MAX_CASES * sizeof(long long)));
   if (!c->dce) HIPCHK(hipMalloc(&c->dce, 5 * (size_t)c->cap * sizeof(double)));
-  HIPCHK(hipMemcpyAsync(c->dcoff, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice, c->sm));
-  HIPCHK(hipMemcpyAsync(c->dcoff + MAX_CASES, offp.data(), offp.size() * sizeof(long long), hipMemcpyHostToDevice,
-                        c->sm));
+  HIPCHK(hipMemcpyAsync(c->dcoff, off, 2 * (size_t)MAX_CASES * sizeof(long long), hipMemcpyHostToDevice, c->sm));
   launch_coarse_assemble(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dcoff, drows, c->dcm, total);
   for (int id : ids) {
     // singular A_c (zero row sums throughout: no Dirichlet face) -> decouple the last unknown
@@ -450,30 +491,51 @@ int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o, double* ac_out
     HIPCHK(hipMemcpyAsync(ac_out, c->dcm + off[ids[0]],
                           (size_t)c->cases[ids[0]].nblk * c->cases[ids[0]].nblk * sizeof(double),
                           hipMemcpyDeviceToHost, c->sm));
-  for (int id : ids) gauss_jordan_inverse(c->sm, c->dcm + off[id], c->cases[id].nblk, c->dcsnap);
+  hipStream_t gs = c->sm;
+  if (side) {
+    HIPCHK(hipEventRecord(c->ev_cs, c->sm));
+    HIPCHK(hipStreamWaitEvent(c->sc, c->ev_cs, 0));
+    gs = c->sc;
+  }
+  for (int id : ids) gauss_jordan_inverse(gs, c->dcm + off[id], c->cases[id].nblk, c->dcsnap);
   for (int id : ids) {
     const int n = c->cases[id].nblk;
-    launch_to_f32(c->sm, c->dcm + off[id], c->dci + offp[id], n, (n + 3) & ~3);
+    launch_to_f32(gs, c->dcm + off[id], c->dci + offp[id], n, (n + 3) & ~3);
   }
   HIPCHK(hipGetLastError());
+  if (side) {
+    HIPCHK(hipEventRecord(c->ev_cd, c->sc));
+    c->coarse_pending = true;
+  }
   if (inv_out && !ids.empty())
     HIPCHK(hipMemcpyAsync(inv_out, c->dcm + off[ids[0]],
                           (size_t)c->cases[ids[0]].nblk * c->cases[ids[0]].nblk * sizeof(double),
                           hipMemcpyDeviceToHost, c->sm));
-  // the host vectors above must outlive the asynchronous copies
-  HIPCHK(hipStreamSynchronize(c->sm));
+  if (ac_out || inv_out) HIPCHK(hipStreamSynchronize(c->sm));  // the caller reads them on return
   o->cinv = c->dci;
   o->coff = c->dcoff + MAX_CASES;
   o->ce = c->dce;
   return 0;
 }
 
+int coarse_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o, double* ac_out = nullptr,
+                 double* inv_out = nullptr) {
+  int rc = coarse_rows_launch(c, K);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->sm));
+  return coarse_finish(c, o, ac_out, inv_out);
+}
+
 // Stencil classes of the active blocks for the compact-record methods: the 27 class rows and the
 // class-uniform flag of every block (device) and the counts of uniform and other blocks, which
 // pick the kernel variant.  FFVHC_OPTION_CLASSES 0 leaves o without them (every block then reads
-// the per-cell arrays; the results are the same bit for bit).
-int class_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o) {
-  if (!c->classes || !c->nactive) return 0;
+// the per-cell arrays; the results are the same bit for bit).  class_launch queues the class
+// kernel and the readback of the flags into c->hcuni; class_finish, after the next
+// synchronisation of c->sm, counts them.
+bool class_on(ffvhc_ctx* c) { return c->classes && c->nactive; }
+
+int class_launch(ffvhc_ctx* c, const Coef7<double>& K) {
+  c->setup_serial++;
   const size_t cap = (size_t)c->cap;
   if (!c->dcuni) {
     HIPCHK(hipMalloc(&c->dcuni, cap * sizeof(int)));
@@ -482,22 +544,34 @@ int class_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o) {
   }
   launch_stencil_classes(c->sm, c->dactive, c->nactive, K, c->dcuni, c->dctab, (uint4*)c->dctrec);
   HIPCHK(hipGetLastError());
-  std::vector<int> uni(cap);
-  HIPCHK(hipMemcpyAsync(uni.data(), c->dcuni, cap * sizeof(int), hipMemcpyDeviceToHost, c->sm));
-  HIPCHK(hipStreamSynchronize(c->sm));
+  c->hcuni.resize(cap);
+  HIPCHK(hipMemcpyAsync(c->hcuni.data(), c->dcuni, cap * sizeof(int), hipMemcpyDeviceToHost, c->sm));
+  return 0;
+}
+
+void class_finish(ffvhc_ctx* c, BiOpts* o) {
   int nreg = 0;
-  for (int s : c->hactive) nreg += uni[s] != 0;
+  for (int s : c->hactive) nreg += c->hcuni[s] != 0;
   o->tab = c->dctab;
   o->trec = (const uint4*)c->dctrec;
   o->uni = c->dcuni;
   o->nreg = nreg;
   o->nirr = c->nactive - nreg;
+}
+
+int class_setup(ffvhc_ctx* c, const Coef7<double>& K, BiOpts* o) {
+  if (!class_on(c)) return 0;
+  int rc = class_launch(c, K);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->sm));
+  class_finish(c, o);
   return 0;
 }
 
 // Storage options of the FP64-recurrence BiCGSTAB methods 5-9: an FP32 copy of the FP64 matrix A
 // (scratch fields, converted here) in *M32 and/or the FP64 matrix in *M64, referenced by *o.
 int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Coef7<double>* M64, BiOpts* o) {
+  c->setup_serial++;
   *o = BiOpts();
   *M64 = coef<double>(c, A);
   if (method == 6) {
@@ -514,8 +588,14 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Co
   // block is class-uniform: those arrays are then not built
   bool percell = true;
   if (method >= 11) {
-    const int rc = class_setup(c, *M64, o);
+    // the class flags and (methods 17-28) the coarse rows come back to the host in one
+    // synchronisation
+    const bool cl = class_on(c);
+    int rc = cl ? class_launch(c, *M64) : 0;
+    if (!rc && method >= 17) rc = coarse_rows_launch(c, *M64);
     if (rc) return rc;
+    if (cl || method >= 17) HIPCHK(hipStreamSynchronize(c->sm));
+    if (cl) class_finish(c, o);
     percell = !o->trec || o->nirr > 0;
   }
   if (percell)
@@ -545,9 +625,37 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Co
     o->sweeps = (method - 11) % 6 + 1;
     if (percell) launch_gs_pack(c->sm, c->dactive, c->nactive, *M64, (uint4*)c->fields[fr].d[0]);
     o->mult = method >= 23;
-    if (method >= 17) return coarse_setup(c, *M64, o);
+    if (method >= 17) return coarse_finish(c, o, nullptr, nullptr);
   }
   return 0;
+}
+
+// The kept setup of the time step's pressure solve: valid while no setup buffer was written since
+// (setup_serial), the matrix inputs are unchanged (geom_serial) and method, matrix fields, active
+// blocks and options match.
+bool psetup_match(ffvhc_ctx* c, int method, const int* A) {
+  const auto& s = c->psetup;
+  if (!c->reuse_setup || !s.valid || s.method != method || s.setup_serial != c->setup_serial ||
+      s.geom_serial != c->geom_serial || s.classes != c->classes || s.shadow != c->shadow_hash ||
+      s.active != c->hactive)
+    return false;
+  for (int q = 0; q < 7; q++)
+    if (s.A[q] != A[q]) return false;
+  return true;
+}
+
+void psetup_store(ffvhc_ctx* c, int method, const int* A, const BiOpts& o) {
+  auto& s = c->psetup;
+  s.valid = c->reuse_setup;
+  s.method = method;
+  for (int q = 0; q < 7; q++) s.A[q] = A[q];
+  s.setup_serial = c->setup_serial;
+  s.geom_serial = c->geom_serial;
+  s.classes = c->classes;
+  s.shadow = c->shadow_hash;
+  s.active = c->hactive;
+  s.o = o;
+  s.o.A32 = nullptr;  // points into the caller's frame; set again on reuse
 }
 
 int prologue(ffvhc_ctx* c, int x, const int* A, int b, int bcg, int* prec) {
@@ -640,6 +748,7 @@ int bicgstab_reliable(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double 
   BiVecs<float> V;
   rc = bicg_vectors<float>(c, &V);
   if (rc) return rc;
+  c->setup_serial++;
   for (int q = 0; q < 7; q++)
     launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
                    (const double*)c->fields[A[q]].d[0]);
@@ -803,7 +912,9 @@ int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* c, int method, int x, const int* A, i
   Coef7<float> M32;
   Coef7<double> M64;
   BiOpts o;
+  CoarseJoin join{c};
   rc = fp64_bicg_opts(c, method, A, &M32, &M64, &o);
+  if (!rc) rc = coarse_join(c);
   if (rc) return rc;
   return bicg_timing_run<double>(c, x, A, b, omega, niter, ms6, o);
 }
@@ -820,6 +931,11 @@ int ffvhc_ctx_set_option(ffvhc_ctx* c, int option, int value) {
   }
   if (option == FFVHC_OPTION_SHADOW) {
     c->shadow_hash = value != 0;
+    return 0;
+  }
+  if (option == FFVHC_OPTION_SETUP_REUSE) {
+    c->reuse_setup = value != 0;
+    c->psetup.valid = false;
     return 0;
   }
   return fail(-1, "unknown option");
@@ -889,6 +1005,7 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   int rc = prologue(c, x, A, b, bcg, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
+  CoarseJoin join{c};
   if (method < 0 || method > 28)
     return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB, 3 Jacobi-BiCGSTAB with reliable updates, "
                     "4 the same with a seeded shadow residual, 5 FP64 BiCGSTAB with an FP32 red-black GS "
@@ -918,8 +1035,22 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
     Coef7<float> M32;
     Coef7<double> K64;
     BiOpts o;
-    rc = fp64_bicg_opts(c, method, A, &M32, &K64, &o);
-    if (rc) return rc;
+    if (c->step_solve && method >= 11 && psetup_match(c, method, A)) {
+      // the time step's pressure matrix is the one this setup was made from
+      K64 = coef<double>(c, A);
+      int fa[7];
+      for (int q = 0; q < 7; q++) {
+        rc = scratch_field(c, 4, R_A + q, &fa[q]);
+        if (rc) return rc;
+      }
+      M32 = coef<float>(c, fa);
+      o = c->psetup.o;
+      o.A32 = &M32;
+    } else {
+      rc = fp64_bicg_opts(c, method, A, &M32, &K64, &o);
+      if (rc) return rc;
+      if (c->step_solve && method >= 11) psetup_store(c, method, A, o);
+    }
     // The Krylov recurrence converges for the FP32-stored matrix.  Its answer is accepted only on the
     // FP64 residual ||b - A x|| of the FP64 matrix; a case above eps (a matrix whose coefficients are
     // not all FP32-representable) restarts from that residual, so the reported residual is always
@@ -995,6 +1126,7 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
     if (rc) return rc;
   }
   // FP32 copy of the matrix
+  c->setup_serial++;
   for (int q = 0; q < 7; q++)
     launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
                    (const double*)c->fields[A[q]].d[0]);
@@ -1293,8 +1425,10 @@ int ffvhc_step(ffvhc_ctx* c, ffvhc_step_fields* F, const ffvhc_step_params* P, f
   int st = 0;
   if (P->mixed) {
     const int method = P->mixed_method >= 0 ? P->mixed_method : P->solver == 0 ? 0 : (P->solver == 2 ? 1 : 2);
+    c->step_solve = true;  // A is this step's matrix, made from the geometry and the BC types
     st = ffvhc_solve_mixed(c, method, F->p, F->A, F->b, BC_P, P->omega, P->countMax, P->eps, P->inner_eps,
                            cnt.data(), res.data(), outer.data());
+    c->step_solve = false;
   } else if (P->solver == 0) {
     st = ffvhc_rbgs(c, F->p, F->A, F->b, BC_P, P->omega, P->countMax, P->eps, cnt.data(), res.data());
   } else if (P->solver == 1) {

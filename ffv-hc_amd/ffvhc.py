@@ -288,6 +288,11 @@ class Context:
         (default)."""
         check(_L.ffvhc_ctx_set_option(self.h, 3, int(bool(on))), "ffvhc_ctx_set_option")
 
+    def set_setup_reuse(self, on):
+        """Let ffvhc_step keep its mixed-precision pressure setup from step to step while the
+        matrix cannot have changed (default), or make it every step."""
+        check(_L.ffvhc_ctx_set_option(self.h, 4, int(bool(on))), "ffvhc_ctx_set_option")
+
     def set_classes(self, on):
         """Let the compact-record solve methods (11-28) read the bands of blocks that are uniform
         per stencil position class from 27 rows (default), or read every block per cell."""

@@ -109,6 +109,14 @@ long long ffvhc_ctx_device_bytes(const ffvhc_ctx* ctx);
  * vector is stored or read).  The iteration count changes (on the C2 Poisson system 2-4 fewer,
  * on the flow past a sphere more); the accepted answer meets the same FP64 residual. */
 #define FFVHC_OPTION_SHADOW 3
+/* FFVHC_OPTION_SETUP_REUSE: 1 (default) lets ffvhc_step keep the setup of its mixed-precision
+ * pressure solve (methods 11-28: FP32 band copies, preconditioner records, stencil class tables,
+ * coarse inverse) from one step to the next while the pressure matrix cannot have changed: the
+ * step rebuilds A from the geometry, the density and the BC types, and the setup is made again
+ * after a case is added or removed, a BC is set, the active blocks, the matrix fields, the
+ * method or an option change, or any other solve writes the setup buffers.  0 makes the setup
+ * every step.  The results are identical. */
+#define FFVHC_OPTION_SETUP_REUSE 4
 int ffvhc_ctx_set_option(ffvhc_ctx* ctx, int option, int value);
 
 /* Physical parameters and outer BCs of one flow case.  BC groups: 0 UX, 1 UY, 2 UZ, 3 P,

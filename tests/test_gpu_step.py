@@ -328,6 +328,61 @@ def test_step_with_fast_poisson_matches_fp64(ctx, method):
         assert np.max(np.abs(a[inner] - b[inner])) <= 1e-6 * scale
 
 
+def test_step_setup_reuse_bit_identical(ctx):
+    """The time step keeps the setup of its mixed-precision pressure solve from step to step
+    (FFVHC_OPTION_SETUP_REUSE): three steps of a flow past a sphere, then the case replaced by
+    one with a smaller sphere in the same block slots, three more steps (the kept setup must not
+    be used for the other geometry), and a standalone solve between two steps (which writes the
+    setup buffers): every field is the same bit for bit as with the setup made every step."""
+    from serve import uniform_flow_init
+    tree = ffvhc.Tree(root=(4, 2, 2), root_length=2.0)
+    bc = {ffvhc.BC_UX: [(0, 1.0)] + [(1, 0.0)] * 5, ffvhc.BC_UY: [(0, 0.0)] + [(1, 0.0)] * 5,
+          ffvhc.BC_UZ: [(0, 0.0)] + [(1, 0.0)] * 5, ffvhc.BC_P: [(1, 0.0), (0, 0.0)] + [(1, 0.0)] * 4}
+    params = ffvhc.case_params(mu=0.01, dt=0.01, bc=bc)
+    kw = dict(solver=3, omega=1.6, count_max=5000, eps=1e-10, mixed=True, mixed_method=26)
+    shp = (tree.nblk, 20, 20, 20)
+    runs = []
+    for reuse in (False, True):
+        ctx.set_setup_reuse(reuse)
+        F = ctx.step_fields()
+        out = []
+        try:
+            for radius in (0.5, 0.4):
+                c, cid, pid, counts = tree.geometry_sphere(center=(2.0, 2.0, 2.0), radius=radius, cut_id=1)
+                case = ctx.add_case(tree, params, geometry=(c, cid, pid))
+                try:
+                    ctx.select([case])
+                    uniform_flow_init(0, u0=1.0, noise=0.0)(ctx, case, F)
+                    for k in range(3):
+                        info, _ms, rc = ctx.step(F, ncases=1, **kw)
+                        assert rc == 0 and info[0].residualP <= 1e-10
+                        if k == 1 and radius == 0.4:
+                            # a solve of another matrix (2 A) writes the setup buffers in between
+                            A2 = [ctx.field(8) for _ in range(7)]
+                            x = ctx.field(8)
+                            try:
+                                for f, g in zip(F.A, A2):
+                                    ctx.upload(g, case, 2.0 * ctx.download(f, case, out=np.zeros(shp)))
+                                ctx.fill(x, 0.0)
+                                *_, st = ctx.solve_mixed(26, x, A2, F.b, ffvhc.BC_P, 1.6, 50, 1e-6, 0.5)
+                                assert st == 0
+                            finally:
+                                for f in A2 + [x]:
+                                    ctx.free(f)
+                    out += [ctx.download(f, case, out=np.zeros(shp)) for f in (F.ux, F.uy, F.uz, F.p)]
+                finally:
+                    ctx.select(None)
+                    ctx.remove_case(case)
+        finally:
+            ctx.set_setup_reuse(True)
+            for f in [F.ux, F.uy, F.uz, F.p, F.b, F.lapp] + list(F.uc) + list(F.ucp) + list(F.ud) + list(F.udp) + \
+                    list(F.v) + list(F.A):
+                ctx.free(f)
+        runs.append(out)
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+
+
 def test_checkpoint_round_trip(ctx, tmp_path):
     """Dump2 of a field, then Load2 into another: interior and first ghost layer identical; the
     file has the reference's layout (header + padded blocks)."""
