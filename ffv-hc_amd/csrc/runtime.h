@@ -112,6 +112,11 @@ struct ffvhc_ctx {
   // (FFVHC_OPTION_SETUP_REUSE).
   bool reuse_setup = true;
   bool step_solve = false;  // set by the time step around its pressure solve
+  // first launch batch of the BiCGSTAB rounds of ffvhc_solve_mixed: the iterations round r of
+  // the previous solve with method bicg_hint_method needed (bicg_round: the round running, or -1)
+  int bicg_round = -1;
+  int bicg_hint_method = -1;
+  int bicg_hint[4] = {0, 0, 0, 0};
   uint64_t setup_serial = 0, geom_serial = 0;
   struct PressureSetup {
     bool valid = false;

@@ -192,7 +192,12 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
     (void)hipGraphDestroy(graph);
     if (ei != hipSuccess) return fail(-2, std::string("graph instantiate: ") + hipGetErrorString(ei));
   }
-  int n = 1, batch = kBatchMin;
+  // a round of a repeated mixed solve starts with one batch of the iterations that round needed
+  // last time (plus one), then continues from the smallest batch; the device skips the launches
+  // of finished cases either way, so the batches change only the launches and host waits
+  const int hround = c->bicg_round >= 0 ? std::min(c->bicg_round, 3) : -1;
+  bool hinted = hround >= 0 && c->bicg_hint[hround] > 0;
+  int n = 1, batch = hinted ? std::min(c->bicg_hint[hround] + 1, kBatchMax) : kBatchMin;
   bool done = false;
   while (!done) {
     int last = std::min(n + batch - 1, countMax);
@@ -215,10 +220,16 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
       return rc;
     }
     if (n > countMax) done = true;
-    batch = std::min(batch * 2, kBatchMax);
+    batch = hinted ? kBatchMin : std::min(batch * 2, kBatchMax);
+    hinted = false;
   }
   if (exec) HIPCHK(hipGraphExecDestroy(exec));
   if (launched) *launched = n - 1;
+  if (hround >= 0) {
+    int m = 0;
+    for (int id : ids) m = std::max(m, hst[id].count);
+    c->bicg_hint[hround] = m;
+  }
   launch_ghost_pull<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, (const T*)X.d[0], (T*)X.h[0], bcg, 1,
                        c->multilevel);
   HIPCHK(hipGetLastError());
@@ -1084,9 +1095,15 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
         if (!c->hst2[id].done && its[id] < countMax) return true;
       return false;
     };
+    if (c->bicg_hint_method != method) {
+      c->bicg_hint_method = method;
+      for (int& h : c->bicg_hint) h = 0;
+    }
     while (more()) {
       // each case's budget: countMax less its own iterations so far (dst2 used)
+      c->bicg_round = o5;
       rc = bicgstab_core<double>(c, x, K64, b64, bcg, omega, countMax, eps, c->dst, c->hst, c->dst2, nullptr, o);
+      c->bicg_round = -1;
       if (rc) return rc;
       launch_add_used(c->sm, c->dst2, c->dst, (int)c->cases.size());
       bool inner_done;
