@@ -4,7 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -112,11 +114,12 @@ struct ffvhc_ctx {
   // (FFVHC_OPTION_SETUP_REUSE).
   bool reuse_setup = true;
   bool step_solve = false;  // set by the time step around its pressure solve
-  // first launch batch of the BiCGSTAB rounds of ffvhc_solve_mixed: the iterations round r of
-  // the previous solve with method bicg_hint_method needed (bicg_round: the round running, or -1)
+  // first launch batch of a BiCGSTAB round: the iterations round r of the previous solve of the
+  // same kind needed (key: the ffvhc_solve_mixed method, or -100 / -200 less the BC group for
+  // ffvhc_bicgstab in FP64 / FP32); bicg_round is the round running, or -1 (no hint)
   int bicg_round = -1;
-  int bicg_hint_method = -1;
-  int bicg_hint[4] = {0, 0, 0, 0};
+  int bicg_hint_key = 0;
+  std::map<int, std::array<int, 4>> bicg_hints;
   uint64_t setup_serial = 0, geom_serial = 0;
   struct PressureSetup {
     bool valid = false;

@@ -196,8 +196,9 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
   // last time (plus one), then continues from the smallest batch; the device skips the launches
   // of finished cases either way, so the batches change only the launches and host waits
   const int hround = c->bicg_round >= 0 ? std::min(c->bicg_round, 3) : -1;
-  bool hinted = hround >= 0 && c->bicg_hint[hround] > 0;
-  int n = 1, batch = hinted ? std::min(c->bicg_hint[hround] + 1, kBatchMax) : kBatchMin;
+  int* hint = hround >= 0 ? &c->bicg_hints[c->bicg_hint_key][hround] : nullptr;  // zero when new
+  bool hinted = hint && *hint > 0;
+  int n = 1, batch = hinted ? std::min(*hint + 1, kBatchMax) : kBatchMin;
   bool done = false;
   while (!done) {
     int last = std::min(n + batch - 1, countMax);
@@ -225,10 +226,10 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
   }
   if (exec) HIPCHK(hipGraphExecDestroy(exec));
   if (launched) *launched = n - 1;
-  if (hround >= 0) {
+  if (hint) {
     int m = 0;
     for (int id : ids) m = std::max(m, hst[id].count);
-    c->bicg_hint[hround] = m;
+    *hint = m;
   }
   launch_ghost_pull<T>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, (const T*)X.d[0], (T*)X.h[0], bcg, 1,
                        c->multilevel);
@@ -996,12 +997,15 @@ int ffvhc_bicgstab(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double ome
       if (count) count[i] = 1;
     return 0;
   }
+  c->bicg_round = 0;
+  c->bicg_hint_key = (prec == 8 ? -100 : -200) - bcg;  // per BC group: the P and U solves of a step differ
   if (prec == 8)
     rc = bicgstab_core<double>(c, x, coef<double>(c, A), (const double*)c->fields[b].d[0], bcg, omega,
                                countMax, eps, c->dst, c->hst, nullptr, nullptr);
   else
     rc = bicgstab_core<float>(c, x, coef<float>(c, A), (const float*)c->fields[b].d[0], bcg, omega,
                               countMax, eps, c->dst, c->hst, nullptr, nullptr);
+  c->bicg_round = -1;
   if (rc) return rc;
   rc = sync_state(c, c->dst, c->hst);
   if (rc) return rc;
@@ -1095,13 +1099,10 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
         if (!c->hst2[id].done && its[id] < countMax) return true;
       return false;
     };
-    if (c->bicg_hint_method != method) {
-      c->bicg_hint_method = method;
-      for (int& h : c->bicg_hint) h = 0;
-    }
     while (more()) {
       // each case's budget: countMax less its own iterations so far (dst2 used)
       c->bicg_round = o5;
+      c->bicg_hint_key = method;
       rc = bicgstab_core<double>(c, x, K64, b64, bcg, omega, countMax, eps, c->dst, c->hst, c->dst2, nullptr, o);
       c->bicg_round = -1;
       if (rc) return rc;
