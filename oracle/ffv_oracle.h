@@ -185,7 +185,7 @@ typedef struct {
   double bcv_u[3][6];
   int bct_p[6];
   double bcv_p[6];
-  int solver; /* 0 RBGS, 1 RBGS_WithFixedAverage, 2 CG (Jacobi PCG) */
+  int solver; /* 0 RBGS, 1 RBGS_WithFixedAverage, 2 CG (Jacobi PCG), 3 Jacobi-BiCGSTAB */
   double omega;
   int countMax;
   double eps;
@@ -210,6 +210,11 @@ typedef struct {
 
 void orc_update(const orc_grid* gr, orc_fields* F, const orc_params* P, int step, orc_step_info* info);
 void orc_set_threads(int n);
+/* Planted mistakes for the pin tests (0 = none): 1 BiCGSTAB beta sign, 2 BiCGSTAB alpha/gamma
+ * swapped in the x update, 3 Jacobi driver drops each sweep's result, 4 RBGS_WithFixedAverage
+ * without the mean shift, 5 UpdateU before the pressure solve, 6 bcut_remove_p skipped,
+ * 7 CG beta inverted, 8 ghost fill with C2F before F2C. */
+void orc_plant(int id);
 
 #ifdef __cplusplus
 }

@@ -21,6 +21,12 @@
 #define NPAD(gr) ((size_t)((gr)->n + 2 * (gr)->g) * ((gr)->n + 2 * (gr)->g) * ((gr)->n + 2 * (gr)->g))
 #define BLK(f, gr, b) ((f) + (size_t)(b) * NPAD(gr))
 
+/* Planted mistakes (test infrastructure): the pin tests switch one on to show that their check
+ * fails when the oracle is wrong.  0 = none (the only value outside those tests). */
+static int plant_id = 0;
+void orc_plant(int id) { plant_id = id; }
+#define PLANTED(id) (plant_id == (id))
+
 void orc_set_threads(int n) {
 #ifdef _OPENMP
   if (n > 0) omp_set_num_threads(n);
@@ -82,9 +88,22 @@ static void u1_c2f(const orc_grid* gr, real* dst, const real* src, int face, int
  * The VC update is done in three passes, F2C -> same level -> C2F (the order made explicit by
  * SeparateXYZ_SeparateLevelDiff, LocalScalar3D.h:252-262); Scalar3DUpdater5's C2F reads the
  * coarse ghost that F2C filled (Scalar3DUpdater5.h:785-786), so F2C must come first. */
+static void ghost_c2f(const orc_grid* gr, real* f, int updater) {
+  const int sz[3] = {gr->n, gr->n, gr->n};
+#pragma omp parallel for schedule(static)
+  for (int b = 0; b < gr->nblk; b++)
+    for (int face = 0; face < 6; face++)
+      if (gr->ftype[b * 6 + face] == 2) {
+        int src = gr->nbr[(b * 6 + face) * 4], s = gr->sub[b * 6 + face];
+        if (updater == 5) orc_u5_c2f(BLK(f, gr, b), BLK(f, gr, src), face, s, sz, gr->g);
+        else u1_c2f(gr, BLK(f, gr, b), BLK(f, gr, src), face, s);
+      }
+}
+
 void orc_ghost_fill(const orc_grid* gr, real* f, int updater, const int* bctype, const real* bcval) {
   const int nb = gr->nblk;
   const int sz[3] = {gr->n, gr->n, gr->n};
+  if (PLANTED(8)) ghost_c2f(gr, f, updater);
 #pragma omp parallel for schedule(static)
   for (int b = 0; b < nb; b++)
     for (int face = 0; face < 6; face++)
@@ -98,14 +117,7 @@ void orc_ghost_fill(const orc_grid* gr, real* f, int updater, const int* bctype,
   for (int b = 0; b < nb; b++)
     for (int face = 0; face < 6; face++)
       if (gr->ftype[b * 6 + face] == 1) u1_same(gr, BLK(f, gr, b), BLK(f, gr, gr->nbr[(b * 6 + face) * 4]), face);
-#pragma omp parallel for schedule(static)
-  for (int b = 0; b < nb; b++)
-    for (int face = 0; face < 6; face++)
-      if (gr->ftype[b * 6 + face] == 2) {
-        int src = gr->nbr[(b * 6 + face) * 4], s = gr->sub[b * 6 + face];
-        if (updater == 5) orc_u5_c2f(BLK(f, gr, b), BLK(f, gr, src), face, s, sz, gr->g);
-        else u1_c2f(gr, BLK(f, gr, b), BLK(f, gr, src), face, s);
-      }
+  if (!PLANTED(8)) ghost_c2f(gr, f, updater);
   /* BlockScalar3D.h:174-181 in face order X_M X_P Y_M Y_P Z_M Z_P; Neumann value * dx (74-76) */
   if (!bctype) return;
 #pragma omp parallel for schedule(static)
@@ -222,6 +234,7 @@ int orc_rbgs_fixed_average(const orc_grid* gr, real* x, real* const* A, const re
     rbgs_sweep(gr, x, A, b, omega, bctype, bcval);
     real sum_x1 = orc_grid_sum(gr, x);
     real a = -sum_x1 / (double)Ncells;
+    if (PLANTED(4)) a = 0.0;
     for (int k = 0; k < gr->nblk; k++) orc_adda(BLK(x, gr, k), a, sz, gr->g);
     orc_ghost_fill(gr, x, 1, bctype, bcval);
     real rr = orc_grid_r2(gr, A, x, b);
@@ -247,8 +260,10 @@ static void jacobi_sweep(const orc_grid* gr, real** x, real* const* A, const rea
     orc_jacobi_smoother(BLK(x0s, gr, k), BLK(xs, gr, k), BLK(A[0], gr, k), BLK(A[1], gr, k),
                         BLK(A[2], gr, k), BLK(A[3], gr, k), BLK(A[4], gr, k), BLK(A[5], gr, k),
                         BLK(A[6], gr, k), BLK(b, gr, k), omega, sz, gr->g);
-  *x = x0s;
-  *x0 = xs;
+  if (!PLANTED(3)) { /* planted: no swap, the sweep's result is dropped */
+    *x = x0s;
+    *x0 = xs;
+  }
   orc_ghost_fill(gr, *x, 1, bctype, bcval);
 }
 
@@ -300,7 +315,7 @@ int orc_cg(const orc_grid* gr, real* x, real* const* A, const real* b, real* r, 
     for (int it = 0; it < countPre; it++) jacobi_sweep(gr, &zz, A, r, &xx0, omega, bt_null, bv_null);
     rr1 = orc_grid_dot(gr, r, zz);
     if (fabs(rr1) < FLT_MIN) { *residual = rr1; c = 0; break; }
-    real beta = rr1 / rr0;
+    real beta = PLANTED(7) ? rr0 / rr1 : rr1 / rr0;
     if (c == 1) {
 #pragma omp parallel for schedule(static)
       for (int k = 0; k < nb; k++) orc_copy(BLK(p, gr, k), BLK(zz, gr, k), sz, gr->g);
@@ -360,6 +375,7 @@ int orc_bicgstab(const orc_grid* gr, real* x, real* const* A, const real* b, rea
       for (int k = 0; k < nb; k++) orc_copy(BLK(p, gr, k), BLK(r, gr, k), sz, gr->g);
     } else {
       real beta = rr1 / rr0 * alpha / gamma;
+      if (PLANTED(1)) beta = -beta;
 #pragma omp parallel for schedule(static)
       for (int k = 0; k < nb; k++) {
         orc_axpy(BLK(p, gr, k), BLK(q_, gr, k), -gamma, sz, gr->g);
@@ -399,7 +415,8 @@ int orc_bicgstab(const orc_grid* gr, real* x, real* const* A, const real* b, rea
     gamma = t_s / t_t_;
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < nb; k++) {
-      orc_axpbypz(BLK(x, gr, k), BLK(p_, gr, k), BLK(s_, gr, k), alpha, gamma, sz, gr->g);
+      if (PLANTED(2)) orc_axpbypz(BLK(x, gr, k), BLK(p_, gr, k), BLK(s_, gr, k), gamma, alpha, sz, gr->g);
+      else orc_axpbypz(BLK(x, gr, k), BLK(p_, gr, k), BLK(s_, gr, k), alpha, gamma, sz, gr->g);
       orc_axpyz(BLK(r, gr, k), BLK(t_, gr, k), BLK(s, gr, k), -gamma, sz, gr->g);
     }
     real rr = orc_grid_dot(gr, r, r);
@@ -422,6 +439,22 @@ int orc_bicgstab(const orc_grid* gr, real* x, real* const* A, const real* b, rea
                       F->cid[2] + (size_t)k * NPAD(gr), F->cid[3] + (size_t)k * NPAD(gr),           \
                       F->cid[4] + (size_t)k * NPAD(gr), F->cid[5] + (size_t)k * NPAD(gr)};          \
   const int* pb = F->pid + (size_t)k * NPAD(gr);
+
+/* UpdateU (Solver.cpp:5690-5771): bcut_corr_u per block, then the velocity ghosts. */
+static void corr_stage(const orc_grid* gr, orc_fields* F, const orc_params* P, real* const* u,
+                       real (*bcv_u)[6]) {
+  const int sz[3] = {gr->n, gr->n, gr->n};
+  const int nb = gr->nblk, g = gr->g;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nb; k++) {
+    GEOM(F, gr, k)
+    real* vb[6];
+    for (int q = 0; q < 6; q++) vb[q] = BLK(F->v[q], gr, k);
+    orc_bcut_corr_u(BLK(F->ux, gr, k), BLK(F->uy, gr, k), BLK(F->uz, gr, k), vb, BLK(F->lapp, gr, k),
+                    BLK(F->p, gr, k), cb, ib, pb, P->rho, gr->dx[k], P->dt, sz, g);
+  }
+  for (int d = 0; d < 3; d++) orc_ghost_fill(gr, u[d], 5, P->bct_u[d], bcv_u[d]);
+}
 
 /* Solver::Update (Solver.cpp:3344-3416) with UpdateUXe/UYe/UZe (3535-3744, explicit U),
  * UpdateP (5294-5688) and UpdateU (5690-5771). */
@@ -505,7 +538,7 @@ void orc_update(const orc_grid* gr, orc_fields* F, const orc_params* P, int step
     orc_bcut_add_g(ux, uy, uz, BLK(F->t0, gr, k), P->gx, P->gy, P->gz, P->betag, P->tr, dx, dt, sz, g);
     orc_bcut_add_f(ux, uy, uz, BLK(F->f[0], gr, k), BLK(F->f[1], gr, k), BLK(F->f[2], gr, k), pb, rhof,
                    dx, dt, sz, g);
-    orc_bcut_remove_p(ux, uy, uz, BLK(F->p, gr, k), cb, ib, pb, rhof, dx, dt, sz, g);
+    if (!PLANTED(6)) orc_bcut_remove_p(ux, uy, uz, BLK(F->p, gr, k), cb, ib, pb, rhof, dx, dt, sz, g);
   }
   for (int d = 0; d < 3; d++) orc_ghost_fill(gr, u[d], 5, P->bct_u[d], bcv_u[d]);
 #pragma omp parallel for schedule(static)
@@ -527,6 +560,7 @@ void orc_update(const orc_grid* gr, orc_fields* F, const orc_params* P, int step
     }
   }
   orc_matrix_bc(gr, F->A, F->b, P->bct_p, P->bcv_p);
+  if (PLANTED(5)) corr_stage(gr, F, P, u, bcv_u);
   int st = 0;
   if (P->solver == 0)
     st = orc_rbgs(gr, F->p, F->A, F->b, P->bct_p, P->bcv_p, P->omega, P->countMax, P->eps, &info->countP,
@@ -534,22 +568,17 @@ void orc_update(const orc_grid* gr, orc_fields* F, const orc_params* P, int step
   else if (P->solver == 1)
     st = orc_rbgs_fixed_average(gr, F->p, F->A, F->b, P->bct_p, P->bcv_p, P->ncells, P->omega, P->countMax,
                                 P->eps, &info->countP, &info->residualP);
-  else
+  else if (P->solver == 2)
     st = orc_cg(gr, F->p, F->A, F->b, F->cgs[0], F->cgs[1], F->cgs[2], F->cgs[3], F->cgs[4], P->bct_p,
                 P->bcv_p, P->omega, P->countPre, P->countMax, P->eps, &info->countP, &info->residualP);
+  else /* Jacobi-BiCGSTAB (SolvingMethod/P = BiCGSTAB, Solver.cpp:5582-5677) */
+    st = orc_bicgstab(gr, F->p, F->A, F->b, F->bis, P->bct_p, P->bcv_p, P->omega, P->countPre, P->countMax,
+                      P->eps, &info->countP, &info->residualP);
   orc_ghost_fill(gr, F->p, 1, P->bct_p, P->bcv_p);
   tt[5] = now_s();
 
   /* UpdateU */
-#pragma omp parallel for schedule(static)
-  for (int k = 0; k < nb; k++) {
-    GEOM(F, gr, k)
-    real* vb[6];
-    for (int q = 0; q < 6; q++) vb[q] = BLK(F->v[q], gr, k);
-    orc_bcut_corr_u(BLK(F->ux, gr, k), BLK(F->uy, gr, k), BLK(F->uz, gr, k), vb, BLK(F->lapp, gr, k),
-                    BLK(F->p, gr, k), cb, ib, pb, rhof, gr->dx[k], dt, sz, g);
-  }
-  for (int d = 0; d < 3; d++) orc_ghost_fill(gr, u[d], 5, P->bct_u[d], bcv_u[d]);
+  if (!PLANTED(5)) corr_stage(gr, F, P, u, bcv_u);
   orc_ghost_fill(gr, F->p, 1, P->bct_p, P->bcv_p);
   orc_ghost_fill(gr, F->lapp, 1, bt_null, bv_null);
   tt[6] = now_s();
