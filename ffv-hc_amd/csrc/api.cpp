@@ -40,6 +40,9 @@ std::vector<int> selected_cases(ffvhc_ctx* ctx) {
 
 int refresh_active(ffvhc_ctx* ctx) {
   if (!ctx->dirty) return 0;
+  // the launch-batch hints are iteration counts of the previous mix of cases: a new active set
+  // starts without them (the batches change launches and host waits only, never results)
+  ctx->bicg_hints.clear();
   ctx->hactive.clear();
   ctx->hcaseact.clear();
   ctx->multilevel = false;

@@ -114,6 +114,10 @@ struct ffvhc_ctx {
   // (FFVHC_OPTION_SETUP_REUSE).
   bool reuse_setup = true;
   bool step_solve = false;  // set by the time step around its pressure solve
+  // the step's reference-pressure point (bcut_set_reference_value rewrites the rows of the cell
+  // holding it and of its neighbours, so the matrix depends on it)
+  bool step_ref_active = false;
+  double step_ref[3] = {0.0, 0.0, 0.0};
   // first launch batch of a BiCGSTAB round: the iterations round r of the previous solve of the
   // same kind needed (key: the ffvhc_solve_mixed method, or -100 / -200 less the BC group for
   // ffvhc_bicgstab in FP64 / FP32); bicg_round is the round running, or -1 (no hint)
@@ -127,6 +131,8 @@ struct ffvhc_ctx {
     int A[7] = {};
     uint64_t setup_serial = 0, geom_serial = 0;
     bool classes = false, shadow = false;
+    bool ref_active = false;
+    double ref[3] = {0.0, 0.0, 0.0};
     std::vector<int> active;
     ffvhc::BiOpts o;
   } psetup;

@@ -644,13 +644,16 @@ int fp64_bicg_opts(ffvhc_ctx* c, int method, const int* A, Coef7<float>* M32, Co
 
 // The kept setup of the time step's pressure solve: valid while no setup buffer was written since
 // (setup_serial), the matrix inputs are unchanged (geom_serial) and method, matrix fields, active
-// blocks and options match.
+// blocks, options and the reference-pressure point match.
 bool psetup_match(ffvhc_ctx* c, int method, const int* A) {
   const auto& s = c->psetup;
   if (!c->reuse_setup || !s.valid || s.method != method || s.setup_serial != c->setup_serial ||
       s.geom_serial != c->geom_serial || s.classes != c->classes || s.shadow != c->shadow_hash ||
-      s.active != c->hactive)
+      s.active != c->hactive || s.ref_active != c->step_ref_active)
     return false;
+  if (s.ref_active)
+    for (int a = 0; a < 3; a++)
+      if (s.ref[a] != c->step_ref[a]) return false;
   for (int q = 0; q < 7; q++)
     if (s.A[q] != A[q]) return false;
   return true;
@@ -665,6 +668,8 @@ void psetup_store(ffvhc_ctx* c, int method, const int* A, const BiOpts& o) {
   s.geom_serial = c->geom_serial;
   s.classes = c->classes;
   s.shadow = c->shadow_hash;
+  s.ref_active = c->step_ref_active;
+  for (int a = 0; a < 3; a++) s.ref[a] = c->step_ref[a];
   s.active = c->hactive;
   s.o = o;
   s.o.A32 = nullptr;  // points into the caller's frame; set again on reuse
@@ -1444,6 +1449,8 @@ int ffvhc_step(ffvhc_ctx* c, ffvhc_step_fields* F, const ffvhc_step_params* P, f
   if (P->mixed) {
     const int method = P->mixed_method >= 0 ? P->mixed_method : P->solver == 0 ? 0 : (P->solver == 2 ? 1 : 2);
     c->step_solve = true;  // A is this step's matrix, made from the geometry and the BC types
+    c->step_ref_active = P->ref_active != 0;
+    for (int a = 0; a < 3; a++) c->step_ref[a] = P->ref_active ? P->ref[a] : 0.0;
     st = ffvhc_solve_mixed(c, method, F->p, F->A, F->b, BC_P, P->omega, P->countMax, P->eps, P->inner_eps,
                            cnt.data(), res.data(), outer.data());
     c->step_solve = false;

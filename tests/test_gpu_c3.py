@@ -1,0 +1,95 @@
+"""The three-level flow past a sphere at full size (roots 16x8x8 of edge D, sphere + wake refinement,
+4,048 blocks = 16.6 M cells, Re 300, dt at CFL 0.3 on the finest cells), checked through properties
+that hold at any size:
+
+* a uniform flow without the sphere stays uniform bit for bit through the mixed-precision step
+  (every stage of the fractional step preserves it exactly: zero convection and diffusion terms,
+  zero divergence, so the pressure solve returns at once and the correction is zero);
+* with the sphere, every step's pressure solve reaches the FP64 relative residual 1e-10 it
+  reports (recomputed by ffvhc_residual on the step's own A, b and p), for the reference's FP64
+  Jacobi-BiCGSTAB and for the mixed-precision two-level solver;
+* the two paths' fields agree to the solver tolerance.
+The same configuration at reduced size (516 blocks) is compared with the oracle bit for bit in
+test_gpu_step.py."""
+import numpy as np
+import pytest
+
+import ffvhc
+
+pytestmark = pytest.mark.gpu
+
+RE = 300.0
+MIXED = dict(solver=3, omega=1.6, count_max=20000, eps=1e-10, mixed=True, mixed_method=26)
+FP64 = dict(solver=3, omega=1.0, count_max=20000, eps=1e-10)
+
+
+@pytest.fixture(scope="module")
+def c3():
+    import flows
+    tree, geo, counts = flows.three_level_sphere()
+    assert tree.nblk == 4048 and tree.max_level == 2
+    ctx = ffvhc.Context(max_blocks=tree.nblk)
+    yield ctx, tree, geo, counts
+    ctx.close()
+
+
+def _fields(ctx):
+    F = ctx.step_fields()
+    return F
+
+
+def _free(ctx, F):
+    for f in [F.ux, F.uy, F.uz, F.p, F.b, F.lapp] + list(F.uc) + list(F.ucp) + list(F.ud) + list(F.udp) + \
+            list(F.v) + list(F.A):
+        ctx.free(f)
+
+
+def test_c3_fullsize_uniform_flow_is_kept_exactly(c3):
+    import flows
+    from serve import uniform_flow_init
+    ctx, tree, _geo, _ = c3
+    params = flows.flow_params(RE, flows.cfl_dt(tree, 0.3))
+    case = ctx.add_case(tree, params)
+    F = _fields(ctx)
+    try:
+        ctx.select([case])
+        uniform_flow_init(0, u0=1.0, noise=0.0)(ctx, case, F)
+        for _ in range(2):
+            info, _ms, rc = ctx.step(F, ncases=1, **MIXED)
+            assert rc == 0 and info[0].residualP == 0.0
+        for f, want in ((F.ux, 1.0), (F.uy, 0.0), (F.uz, 0.0), (F.p, 0.0)):
+            st = ctx.stats(f, ncases=1)[0]
+            assert st[1] == want and st[2] == want, st
+    finally:
+        ctx.select(None)
+        _free(ctx, F)
+        ctx.remove_case(case)
+
+
+def test_c3_fullsize_steps_reach_residual_and_agree(c3):
+    import flows
+    from serve import uniform_flow_init
+    ctx, tree, geo, counts = c3
+    params = flows.flow_params(RE, flows.cfl_dt(tree, 0.3))
+    shp = (tree.nblk, 20, 20, 20)
+    inner = (slice(None),) + (slice(2, 18),) * 3
+    out = []
+    for kw in (FP64, MIXED):
+        case = ctx.add_case(tree, params, geometry=geo)
+        F = _fields(ctx)
+        try:
+            ctx.select([case])
+            uniform_flow_init(0, u0=1.0, noise=0.0)(ctx, case, F)
+            for _ in range(2):
+                info, _ms, rc = ctx.step(F, ncases=1, **kw)
+                assert rc == 0 and info[0].residualP <= 1e-10, (kw, info[0].countP, info[0].residualP)
+                true = ctx.residual(F.p, F.A, F.b, ncases=1)[0]
+                assert true <= 1e-10 * 1.0001, (kw, true)
+            out.append([ctx.download(f, case, out=np.zeros(shp))[inner] for f in (F.ux, F.uy, F.uz, F.p)])
+        finally:
+            ctx.select(None)
+            _free(ctx, F)
+            ctx.remove_case(case)
+    for a, b in zip(out[0], out[1]):
+        scale = np.max(np.abs(a))
+        assert np.max(np.abs(a - b)) <= 1e-6 * scale

@@ -315,7 +315,7 @@ def true_residual(og, A, x, b):
     return np.sqrt(rr / bb)
 
 
-@pytest.mark.parametrize("method", [0, 1, 3, 5, 6, 7, 8, 9, 10, 11, 12, 13, 16, 17, 19, 21, 23, 25, 27])
+@pytest.mark.parametrize("method", [0, 1, 3, 5, 6, 7, 8, 9, 10, 11, 12, 13, 16, 17, 19, 21, 23, 25, 26, 27])
 def test_mixed_precision_reaches_fp64_residual(ctx, method):
     tree = trees()["two_level"]
     params = ffvhc.case_params(bc=BC)
@@ -430,6 +430,40 @@ def test_graph_replay_equals_launches(ctx, method):
             ctx.free(f)
     finally:
         ctx.set_graphs(False)
+        ctx.remove_case(cid)
+
+
+@pytest.mark.parametrize("method", [None, 19, 26])
+def test_launch_batch_hint_changes_nothing(ctx, method):
+    """The first solve after a selection change launches growing batches (no hint); a repeated solve
+    starts with one batch of the previous count (hinted).  Counts and every bit of x are equal."""
+    tree = trees()["two_level"]
+    params = ffvhc.case_params(bc=BC)
+    cid = ctx.add_case(tree, params)
+    try:
+        og = OracleGrid(tree)
+        rng = np.random.default_rng(43)
+        bct = [params.bct[ffvhc.BC_P][q] for q in range(6)]
+        bcv = [params.bcv[ffvhc.BC_P][q] for q in range(6)]
+        A, b = make_system(og, rng, bct, bcv)
+        fa, fb, fx = upload_system(ctx, cid, A, b, og.zeros())
+        ctx.select(None)
+        ctx.select([cid])  # a new active set starts without launch hints
+        got = []
+        for _ in range(3):
+            ctx.upload(fx, cid, og.zeros())
+            if method is None:
+                cnt, res, rc = ctx.bicgstab(fx, fa, fb, ffvhc.BC_P, 1.0, 1, 5000, 1e-10, ncases=1)
+            else:
+                cnt, res, _o, rc = ctx.solve_mixed(method, fx, fa, fb, ffvhc.BC_P, 1.4, 5000, 1e-10, 0.5, ncases=1)
+            assert rc == 0 and res[0] <= 1e-10
+            got.append((int(cnt[0]), ctx.download(fx, cid, out=og.zeros())))
+        for c_, x in got[1:]:
+            assert c_ == got[0][0] and np.array_equal(x, got[0][1])
+        for f in fa + [fb, fx]:
+            ctx.free(f)
+    finally:
+        ctx.select(None)
         ctx.remove_case(cid)
 
 

@@ -74,16 +74,28 @@ def ctx():
     c.close()
 
 
-def run_case(ctx, name, solver, omega, count_max, eps, tol, seed=5, nsteps=2, implicit=None):
-    tree = trees()[name]
+def run_case(ctx, name, solver, omega, count_max, eps, tol, seed=5, nsteps=2, implicit=None, tree=None,
+             bc=None, params_kw=None, ref=None, forcing=False, geo=None):
+    """n steps of the device and of the oracle from the same seeded state; every field compared.
+    tree / bc / params_kw override the synthetic case; ref = (x, y, z, p) turns on
+    bcut_set_reference_value; forcing uploads a seeded temperature deviation and body force (their
+    terms act through gravity/betag and f); geo = (c, cid, pid) replaces the seeded cut cells."""
+    tree = tree if tree is not None else trees()[name]
     og = OracleGrid(tree)
     rng = np.random.default_rng(seed)
-    geo = geometry(og, rng)
-    params = ffvhc.case_params(rho=1.0, mu=0.01, dt=0.002, bc=BC)
+    geo = geo if geo is not None else geometry(og, rng)
+    params = ffvhc.case_params(**dict(dict(rho=1.0, mu=0.01, dt=0.002, bc=bc or BC), **(params_kw or {})))
     cid = ctx.add_case(tree, params, geometry=(geo[0], geo[1], geo[2]))
-    F = ctx.step_fields()
+    F = ctx.step_fields(with_t=forcing, with_f=forcing)
     try:
         a = initial_arrays(og, rng)
+        if forcing:
+            a["t"][:] = rng.standard_normal(og.shape)
+            for d in range(3):
+                a["f"][d][:] = rng.standard_normal(og.shape)
+            ctx.upload(F.t, cid, a["t"])
+            for d in range(3):
+                ctx.upload(F.f[d], cid, a["f"][d])
         # the oracle starts from ghost-filled fields, as Solver's initial fills leave them
         bl = lambda grp: ([params.bct[grp][f] for f in range(6)], [params.bcv[grp][f] for f in range(6)])
         for d, key in enumerate(("ux", "uy", "uz")):
@@ -91,11 +103,16 @@ def run_case(ctx, name, solver, omega, count_max, eps, tol, seed=5, nsteps=2, im
         og.ghost_fill(a["p"], 1, *bl(ffvhc.BC_P))
         upload_all(ctx, cid, F, a)
         orc = OracleStep(og, params, geo, a, solver, omega, count_max, eps, implicit)
+        if ref is not None:
+            orc.P.ref_active = 1
+            orc.P.ref_x, orc.P.ref_y, orc.P.ref_z, orc.P.ref_p = ref
         ctx.select([cid])
         kw = {}
         if implicit is not None:
             kw = dict(implicit_u=True, alpha_u=implicit[0], omega_u=implicit[1], count_max_u=implicit[3],
                       eps_u=implicit[2])
+        if ref is not None:
+            kw["ref"] = ref
         for n in range(nsteps):
             info_o = orc.step(n)
             info_d, ms, rc = ctx.step(F, solver=solver, omega=omega, count_max=count_max, eps=eps, ncases=1, **kw)
@@ -106,8 +123,9 @@ def run_case(ctx, name, solver, omega, count_max, eps, tol, seed=5, nsteps=2, im
             compare(ctx, cid, F, a, tol, "%s step %d" % (name, n))
     finally:
         ctx.select(None)
+        extra = ([F.t] + list(F.f)) if forcing else []
         for fld in ([F.ux, F.uy, F.uz, F.p, F.b, F.lapp] + list(F.uc) + list(F.ucp) + list(F.ud) + list(F.udp) +
-                    list(F.v) + list(F.A)):
+                    list(F.v) + list(F.A) + extra):
             ctx.free(fld)
         ctx.remove_case(cid)
 
@@ -127,6 +145,77 @@ def test_step_implicit_u_matches(ctx, name):
     """UpdateUX/UY/UZ: calc_abd_u (bit-exact per cell) + BiCGSTAB per component (reduction order
     differs) then the RBGS pressure solve."""
     run_case(ctx, name, solver=0, omega=1.2, count_max=20, eps=1e-30, tol=1e-9, implicit=(0.5, 1.0, 1e-30, 8))
+
+
+@pytest.mark.parametrize("name", ["flat", "two_level"])
+def test_step_reference_pressure_bit_exact(ctx, name):
+    """bcut_set_reference_value (bcut.f90:5626-5710, with its Ae(i+1) index) pins p at a point:
+    the rows of the cell holding it and of its six neighbours are rewritten after calc_ab_p."""
+    ref = (0.53, 0.41, 0.37, 0.25) if name == "flat" else (0.97, 1.06, 0.84, -0.1)
+    run_case(ctx, name, solver=0, omega=1.2, count_max=25, eps=1e-30, tol=0.0, ref=ref)
+
+
+@pytest.mark.parametrize("name", ["flat", "two_level"])
+def test_step_gravity_buoyancy_body_force_bit_exact(ctx, name):
+    """Nonzero gravity in the predictor and bcut_add_g (with the buoyancy term betag (T - Tr) on
+    z), and a body force through bcut_add_f (bcut.f90:3793-3877)."""
+    run_case(ctx, name, solver=0, omega=1.2, count_max=25, eps=1e-30, tol=0.0, forcing=True,
+             params_kw=dict(gravity=(0.3, -0.7, -9.8), betag=0.02, tr=0.4))
+
+
+def test_step_periodic_outer_bc_bit_exact(ctx):
+    """Periodic outer faces (type 9, bcx.f90:585-658): a tree one block thick in y and z, every
+    field periodic there; the plus-side call writes both ghost layers, the minus side is a no-op."""
+    tree = ffvhc.Tree(root=(3, 1, 1), root_length=1.0)
+    per = (9, 0.0)
+    bc = {g: faces[:2] + [per] * 4 for g, faces in BC.items()}
+    run_case(ctx, "periodic", solver=0, omega=1.2, count_max=25, eps=1e-30, tol=0.0, tree=tree, bc=bc)
+
+
+def test_step_three_level_bit_exact(ctx):
+    """A time step on a three-level tree (level jumps of one at every interface)."""
+    tree = ffvhc.Tree(root=(2, 2, 2), root_length=1.0,
+                      regions=[dict(kind="ball", a=(0.9, 1.1, 1.05), r=0.2, level=2)])
+    assert tree.max_level == 2
+    run_case(ctx, "three_level", solver=0, omega=1.2, count_max=25, eps=1e-30, tol=0.0, tree=tree)
+
+
+def test_step_bicgstab_pressure_matches(ctx):
+    """SolvingMethod/P = BiCGSTAB (the reference example's choice): fixed count, reduction order
+    differs from the oracle."""
+    run_case(ctx, "two_level", solver=3, omega=1.0, count_max=12, eps=1e-30, tol=1e-10)
+
+
+def c1_case(radius=0.5):
+    """The flow past a sphere of the single-level configuration: 8x4x4 roots of 16^3 cells, root
+    length 2 (h = 1/8), sphere of diameter 1 at (4, 4, 4), cut cells from the analytic sphere."""
+    tree = ffvhc.Tree(root=(8, 4, 4), root_length=2.0)
+    c, cid, pid, counts = tree.geometry_sphere(center=(4.0, 4.0, 4.0), radius=radius, cut_id=1)
+    return tree, (c, cid, pid), counts
+
+
+def test_step_c1_sphere_geometry_bit_exact(ctx):
+    """Two steps of the reference's single-level flow past a sphere (Re 100, dt 0.01, RBGS omega
+    1.2, fixed sweep count) with the sphere's cut cells: every field equals the oracle's."""
+    tree, geo, counts = c1_case()
+    assert counts[1] > 0
+    bc = {ffvhc.BC_UX: [(0, 1.0)] + [(1, 0.0)] * 5, ffvhc.BC_UY: [(0, 0.0)] + [(1, 0.0)] * 5,
+          ffvhc.BC_UZ: [(0, 0.0)] + [(1, 0.0)] * 5, ffvhc.BC_P: [(1, 0.0), (0, 0.0)] + [(1, 0.0)] * 4}
+    run_case(ctx, "c1", solver=0, omega=1.2, count_max=12, eps=1e-30, tol=0.0, tree=tree, bc=bc, geo=geo,
+             params_kw=dict(mu=0.01, dt=0.01))
+
+
+def test_step_three_level_sphere_wake_bit_exact(ctx):
+    """The three-level sphere + wake configuration (SphereDivider3-style refinement, Re 300, dt at
+    CFL 0.3 on the finest cells) at reduced size: roots 6x3x3, sphere of diameter 0.3, 516 blocks
+    on levels 0-2 with the sphere's cut cells; two steps equal the oracle's bit for bit."""
+    import flows
+    tree, geo, counts = flows.three_level_sphere(root=(6, 3, 3), root_length=1.0, d=0.3)
+    assert tree.max_level == 2 and counts[1] > 0
+    lv = np.bincount(tree.table()["level"], minlength=3)
+    assert np.all(lv > 0), lv
+    run_case(ctx, "c3_reduced", solver=0, omega=1.2, count_max=10, eps=1e-30, tol=0.0, tree=tree,
+             bc=flows.FLOW_BC, geo=geo, params_kw=dict(mu=1.0 * 0.3 / 300.0, dt=flows.cfl_dt(tree, 0.3)))
 
 
 @pytest.mark.parametrize("name", ["flat", "two_level"])
@@ -301,6 +390,56 @@ def test_step_setup_reuse_bit_identical(ctx):
                     list(F.v) + list(F.A):
                 ctx.free(f)
         runs.append(out)
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("method", [19, 26])
+def test_step_setup_reuse_follows_reference_point_and_selection(ctx, method):
+    """The kept pressure setup must be made again when the reference-pressure point is switched
+    on, moved or switched off (bcut_set_reference_value rewrites matrix rows), and stays valid
+    across a selection that changes and comes back: two resident flow cases, a schedule of steps
+    with ref off / at one point / at another / off, the second case selected in and out; every
+    field bit-identical with the setup kept (FFVHC_OPTION_SETUP_REUSE 1) and made every step (0),
+    for an additive (19) and a multiplicative (26) two-level method."""
+    from serve import uniform_flow_init
+    tree = ffvhc.Tree(root=(4, 2, 2), root_length=2.0)
+    bc = {ffvhc.BC_UX: [(0, 1.0)] + [(1, 0.0)] * 5, ffvhc.BC_UY: [(0, 0.0)] + [(1, 0.0)] * 5,
+          ffvhc.BC_UZ: [(0, 0.0)] + [(1, 0.0)] * 5, ffvhc.BC_P: [(1, 0.0), (0, 0.0)] + [(1, 0.0)] * 4}
+    params = ffvhc.case_params(mu=0.01, dt=0.01, bc=bc)
+    kw = dict(solver=3, omega=1.6, count_max=5000, eps=1e-10, mixed=True, mixed_method=method)
+    schedule = [(None, 1), (None, 2), ((1.1, 2.3, 1.7, 0.0), 1), ((1.1, 2.3, 1.7, 0.0), 2),
+                ((3.2, 1.4, 2.6, 0.1), 2), ((3.2, 1.4, 2.6, 0.1), 1), (None, 1)]
+    shp = (tree.nblk, 20, 20, 20)
+    runs = []
+    for reuse in (False, True):
+        ctx.set_setup_reuse(reuse)
+        cases, Fs = [], []
+        try:
+            for radius in (0.5, 0.4):
+                c, cid, pid, _ = tree.geometry_sphere(center=(2.0, 2.0, 2.0), radius=radius, cut_id=1)
+                cases.append(ctx.add_case(tree, params, geometry=(c, cid, pid)))
+                Fs.append(ctx.step_fields())
+                ctx.select([cases[-1]])
+                uniform_flow_init(0, u0=1.0, noise=0.0)(ctx, cases[-1], Fs[-1])
+            out = []
+            for ref, nsel in schedule:
+                for k in range(nsel):
+                    ctx.select([cases[k]])
+                    info, _ms, rc = ctx.step(Fs[k], ncases=1, ref=ref, **kw)
+                    assert rc == 0 and info[0].residualP <= 1e-10
+                for k in range(2):
+                    out += [ctx.download(f, cases[k], out=np.zeros(shp)) for f in (Fs[k].ux, Fs[k].p)]
+            runs.append(out)
+        finally:
+            ctx.set_setup_reuse(True)
+            ctx.select(None)
+            for F in Fs:
+                for f in [F.ux, F.uy, F.uz, F.p, F.b, F.lapp] + list(F.uc) + list(F.ucp) + list(F.ud) + \
+                        list(F.udp) + list(F.v) + list(F.A):
+                    ctx.free(f)
+            for cid in cases:
+                ctx.remove_case(cid)
     for a, b in zip(runs[0], runs[1]):
         assert np.array_equal(a, b)
 
