@@ -39,13 +39,18 @@ def lib():
     return _lib
 
 
-def ref(kind="d8"):
-    """Reference Fortran kernels: kind 'd8' (all-double literals) or 'q16' (as configured)."""
+def ref_path(kind="d8"):
+    """Path of the reference Fortran kernels of `kind`, or None when oracle/_ref is not built.
+    Checking it maps nothing into the process."""
     name = {"d8": "libffvref.so", "q16": "libffvref_q.so"}[kind]
     path = os.path.join(HERE, "_ref", name)
-    if not os.path.exists(path):
-        return None
-    return ctypes.CDLL(path)
+    return path if os.path.exists(path) else None
+
+
+def ref(kind="d8"):
+    """Reference Fortran kernels: kind 'd8' (all-double literals) or 'q16' (as configured)."""
+    path = ref_path(kind)
+    return None if path is None else ctypes.CDLL(path)
 
 
 def ptr(a):

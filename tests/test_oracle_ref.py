@@ -37,7 +37,7 @@ def test_oracle_matches_reference_fixture(golden, name):
             assert np.max(np.abs(val.ravel() - q)) <= 1e-13 * scale
 
 
-@pytest.mark.skipif(O.ref("d8") is None, reason="oracle/_ref not built")
+@pytest.mark.skipif(O.ref_path("d8") is None, reason="oracle/_ref not built")
 @pytest.mark.parametrize("seed", [11, 12])
 @pytest.mark.parametrize("name", K.CASES)
 def test_oracle_matches_reference_live(name, seed):
