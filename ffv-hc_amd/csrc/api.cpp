@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 
 #include "kernels.h"
@@ -46,11 +47,20 @@ int refresh_active(ffvhc_ctx* ctx) {
   ctx->hactive.clear();
   ctx->hcaseact.clear();
   ctx->multilevel = false;
-  for (int c : selected_cases(ctx)) {
+  // the selected cases in selection order (as selected_cases lists them), without building the
+  // list: the active lists have their full capacity reserved, so this allocates nothing
+  auto add = [ctx](int c) {
     const CaseRec& r = ctx->cases[c];
     for (int s = 0; s < r.nblk; s++) ctx->hactive.push_back(r.slot_begin + s);
     if (r.nblk > 0) ctx->hcaseact.push_back(c);
     if (r.max_level > 0) ctx->multilevel = true;
+  };
+  if (!ctx->sel.empty()) {
+    for (int c : ctx->sel)
+      if (c >= 0 && c < (int)ctx->cases.size() && ctx->cases[c].used) add(c);
+  } else {
+    for (int c = 0; c < (int)ctx->cases.size(); c++)
+      if (ctx->cases[c].used) add(c);
   }
   ctx->nactive = (int)ctx->hactive.size();
   ctx->ncaseact = (int)ctx->hcaseact.size();
@@ -222,6 +232,33 @@ void ffvhc_tree_free(ffvhc_tree* t) {
   delete t;
 }
 
+// ------------------------------------------------------------------ allocation-free scratch
+// Device memory of the allocation-free entry point (ffvhc_force_p_noalloc): the per-block partial
+// triples of up to FFVHC_NOALLOC_SLOTS slots, then the per-case triples of up to MAX_CASES cases
+// (3.2 MB).  Every device the library runs on holds its own copy until the process ends.  In host
+// code `scratch` is a host-side stand-in, never a device address: the address on a device comes
+// from hipGetSymbolAddress with that device current, looked up once per device by the first
+// context created on it and kept in g_scratch_dev.
+constexpr size_t SCRATCH_DOUBLES = 3 * (size_t)FFVHC_NOALLOC_SLOTS + 3 * (size_t)MAX_CASES;
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static double* g_scratch_dev[SCRATCH_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// the current device's scratch address (the caller has made `device` current)
+static int scratch_address(int device, double** out) {
+  if (device >= SCRATCH_MAX_DEVICES) return fail(-1, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(-2, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_dev[device] = (double*)p;
+  }
+  *out = g_scratch_dev[device];
+  return 0;
+}
+
 // ---------------------------------------------------------------------------------- context
 
 int ffvhc_ctx_create(int device, long long max_blocks, ffvhc_ctx** out) {
@@ -251,6 +288,11 @@ int ffvhc_ctx_create(int device, long long max_blocks, ffvhc_ctx** out) {
   HIPCHK(hipMalloc(&c->dcid, sizeof(uint8_t) * 6 * NC * max_blocks));
   HIPCHK(hipMemset(c->dst, 0, sizeof(SolveState) * MAX_CASES));
   HIPCHK(hipMemset(c->dst2, 0, sizeof(SolveState) * MAX_CASES));
+  int rc = scratch_address(device, &c->dscratch);
+  if (rc) return rc;
+  // the active lists never grow past these, so refreshing them allocates no host memory
+  c->hactive.reserve((size_t)max_blocks);
+  c->hcaseact.reserve(MAX_CASES);
   c->bytes = (long long)(sizeof(DevBlock) + sizeof(int) + 4 * sizeof(double) + (6 * 4 + 4 + 6) * NC) * max_blocks;
   c->free_slots.push_back({0, max_blocks});
   *out = c.release();

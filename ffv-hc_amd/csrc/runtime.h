@@ -84,6 +84,9 @@ struct ffvhc_ctx {
   // internally allocated scratch fields, keyed by precision * 100 + role
   std::vector<std::pair<int, int>> scratch_ids;
   double* dpart = nullptr;  // 6 * cap per-block partials
+  // this device's copy of the allocation-free scratch array (its address from
+  // hipGetSymbolAddress at context creation)
+  double* dscratch = nullptr;
   // coarse correction of the two-level preconditioners: dense A_c per case (inverted in place),
   // its FP32 copy (row pitch padded to a multiple of 4), per-case offsets of both (dcoff[0..MAX_CASES)
   // and dcoff[MAX_CASES..)), Gauss-Jordan panel scratch, corrections and block sums per slot

@@ -1270,6 +1270,47 @@ int ffvhc_force_p(ffvhc_ctx* c, int p, int cut_id, double* f) {
   });
 }
 
+int ffvhc_force_p_noalloc(ffvhc_ctx* c, int p, int cut_id, double* f) {
+  if (!c || !f) return fail(-1, "null argument");
+  if (!field_ok(c, p, 8)) return fail(-1, "p must be an FP64 field");
+  if (cut_id < 1 || cut_id > 255) return fail(-1, "cut_id must be in 1..255");
+  if (c->cap > FFVHC_NOALLOC_SLOTS)
+    return fail(-1, "ffvhc_force_p_noalloc: the context has " + std::to_string(c->cap) +
+                        " block slots, the scratch holds " + std::to_string(FFVHC_NOALLOC_SLOTS));
+  if (!c->dscratch) return fail(-2, "no scratch address for this device");
+  int rc = refresh_active(c);
+  if (rc) return rc;
+  // scratch: block partials [3][FFVHC_NOALLOC_SLOTS] (force_p_kernel writes every active slot
+  // before force_sum_kernel reads a case's slots, all of which are active), then the case triples
+  // [MAX_CASES][3] (written for every active case); nothing needs clearing
+  double* part = c->dscratch;
+  double* out = c->dscratch + 3 * (size_t)FFVHC_NOALLOC_SLOTS;
+  launch_force_p(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase,
+                 (const double*)c->fields[p].d[0], (const double*)c->fields[p].h[0], c->dcid, c->dflag,
+                 (size_t)c->cap * NC, cut_id, part, (size_t)FFVHC_NOALLOC_SLOTS, out);
+  HIPCHK(hipGetLastError());
+  double h[3 * MAX_CASES];
+  if (c->ncaseact)
+    HIPCHK(hipMemcpyAsync(h, out, sizeof(double) * 3 * c->ncaseact, hipMemcpyDeviceToHost, c->sm));
+  HIPCHK(hipStreamSynchronize(c->sm));
+  // f in selection order, zeros for a selected case without blocks (as ffvhc_force_p)
+  size_t i = 0, q = 0;
+  auto put = [&](int id) {
+    const bool has = c->cases[id].nblk > 0;
+    for (int d = 0; d < 3; d++) f[3 * i + d] = has ? h[3 * q + d] : 0.0;
+    q += has;
+    i++;
+  };
+  if (!c->sel.empty()) {
+    for (int id : c->sel)
+      if (id >= 0 && id < (int)c->cases.size() && c->cases[id].used) put(id);
+  } else {
+    for (int id = 0; id < (int)c->cases.size(); id++)
+      if (c->cases[id].used) put(id);
+  }
+  return 0;
+}
+
 int ffvhc_force_v(ffvhc_ctx* c, int ux, int uy, int uz, int cut_id, double* f) {
   if (!c || !f) return fail(-1, "null argument");
   if (!field_ok(c, ux, 8) || !field_ok(c, uy, 8) || !field_ok(c, uz, 8))

@@ -105,6 +105,7 @@ _sig = {
     "ffvhc_step": (c_int, [_vp, _P(StepFields), _P(StepParams), _vp, _vp]),
     "ffvhc_case_set_step": (c_int, [_vp, c_int, c_int, c_int]),
     "ffvhc_force_p": (c_int, [_vp, c_int, c_int, _vp]),
+    "ffvhc_force_p_noalloc": (c_int, [_vp, c_int, c_int, _vp]),
     "ffvhc_field_stats": (c_int, [_vp, c_int, _vp]),
     "ffvhc_cfl_check": (c_int, [_vp, c_int, _vp]),
     "ffvhc_field_dump": (c_int, [_vp, c_int, c_int, c_int, ctypes.c_char_p]),
@@ -460,6 +461,12 @@ class Context:
         """Pressure force (fx, fy, fz) on the faces with boundary id cut_id, per selected case."""
         f = np.zeros((ncases or self._nsel(), 3))
         check(_L.ffvhc_force_p(self.h, p, cut_id, _ptr(f)), "ffvhc_force_p")
+        return f
+
+    def force_p_noalloc(self, p, cut_id, ncases=None):
+        """force_p through the allocation-free entry point (fixed per-device scratch)."""
+        f = np.zeros((ncases or self._nsel(), 3))
+        check(_L.ffvhc_force_p_noalloc(self.h, p, cut_id, _ptr(f)), "ffvhc_force_p_noalloc")
         return f
 
     def coarse_operator(self, A, case_id):

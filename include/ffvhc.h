@@ -308,6 +308,15 @@ int ffvhc_force_p(ffvhc_ctx* ctx, int p, int cut_id, double* f);
  * by Solver::PrintForceCID with the case's mu): f[3*i .. 3*i+2] for the i-th selected case.  The
  * ghost layers of ux, uy, uz must be current. */
 int ffvhc_force_v(ffvhc_ctx* ctx, int ux, int uy, int uz, int cut_id, double* f);
+/* No counterpart in the reference.  ffvhc_force_p for callers that cannot allocate device memory
+ * (HIP graph capture, tight serving loops that read the drag every step, harnesses that forbid
+ * hipMalloc): the same kernels and the same result, with the block partials and the per-case
+ * triples in a fixed per-device scratch array instead of a per-call allocation.  It rejects a
+ * context of more than FFVHC_NOALLOC_SLOTS block slots (status -1).  Every call on one GPU shares
+ * that GPU's scratch, so calls that overlap on one GPU (two host threads, two contexts on the same
+ * device) race; calls in sequence are independent (every value read is written by the same call). */
+#define FFVHC_NOALLOC_SLOTS 131072
+int ffvhc_force_p_noalloc(ffvhc_ctx* ctx, int p, int cut_id, double* f);
 
 /* LocalScalar3D::CalcStats (LocalScalar3D.cpp:9-80): out[5*i ..] = {sum, max, min, |max|, |min|}
  * over the interior cells of the i-th selected case. */
