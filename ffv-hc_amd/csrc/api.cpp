@@ -6,6 +6,8 @@
 #include <mutex>
 #include <stdexcept>
 
+#include <rocblas/rocblas.h>
+
 #include "kernels.h"
 #include "runtime.h"
 
@@ -327,6 +329,7 @@ void ffvhc_ctx_destroy(ffvhc_ctx* c) {
   (void)hipFree(c->dcut);
   (void)hipFree(c->dflag);
   (void)hipFree(c->dcid);
+  if (c->blas) (void)rocblas_destroy_handle((rocblas_handle)c->blas);
   (void)hipStreamDestroy(c->sm);
   if (c->sc) {
     (void)hipEventDestroy(c->ev_cs);
@@ -346,11 +349,11 @@ long long ffvhc_ctx_device_bytes(const ffvhc_ctx* c) { return c ? c->bytes : -1;
 
 int ffvhc_ctx_select(ffvhc_ctx* c, const int* ids, int n) {
   if (!c) return fail(-1, "null context");
-  c->sel.clear();
-  for (int i = 0; i < n; i++) {
+  for (int i = 0; i < n; i++)
     if (ids[i] < 0 || ids[i] >= (int)c->cases.size() || !c->cases[ids[i]].used) return fail(-4, "no such case");
-    c->sel.push_back(ids[i]);
-  }
+  // the same selection again keeps the active lists (and with them the launch-batch hints)
+  if (!c->dirty && (int)c->sel.size() == n && std::equal(c->sel.begin(), c->sel.end(), ids)) return 0;
+  c->sel.assign(ids, ids + n);
   c->dirty = true;
   return refresh_active(c);
 }
@@ -624,9 +627,12 @@ int ffvhc_dump_header(const char* path, int* head6) {
   return ok ? 0 : fail(-1, std::string("short header: ") + path);
 }
 
-/* LocalScalar3D<real>::Load2 (LocalScalar3D.cpp:266-342), same-resolution branch: the file must
- * hold nE = 1 and the case's block count of 16^3 blocks; the interior and the first ghost layer
- * are uploaded (the caller fills the ghosts afterwards, as Load2 does). */
+/* LocalScalar3D<real>::Load2 (LocalScalar3D.cpp:266-342): the file must hold nE = 1 and the
+ * case's block count.  Same resolution (16^3 blocks): the interior and the first ghost layer are
+ * uploaded.  Half resolution (8^3 blocks, LocalScalar3D.cpp:313-331): each fine cell takes the
+ * value of the coarse cell that holds it, x(i, j, k) = xc((i+1)/2, (j+1)/2, (k+1)/2) in 1-based
+ * indices (sf3d_copy_x2, bsf3d.f90:5-33); the ghosts are left for the caller's fill, as Load2
+ * leaves them to ImposeBoundaryCondition. */
 int ffvhc_field_load(ffvhc_ctx* c, int f, int id, const char* path) {
   if (!c || !path) return fail(-1, "bad arguments");
   if (!field_ok(c, f, 8)) return fail(-1, "Load2 reads FP64 fields");
@@ -639,16 +645,28 @@ int ffvhc_field_load(ffvhc_ctx* c, int f, int id, const char* path) {
     return fail(-1, "short header");
   }
   const int nb = c->cases[id].nblk, g = head[3];
-  if (head[0] != NB || head[1] != NB || head[2] != NB || head[4] != 1 || head[5] != nb || g < 1 || g > 8) {
+  const bool same = head[0] == NB && head[1] == NB && head[2] == NB;
+  const bool half = head[0] == NB / 2 && head[1] == NB / 2 && head[2] == NB / 2;
+  if (!(same || half) || head[4] != 1 || head[5] != nb || g < 1 || g > 8) {
     fclose(fp);
-    return fail(-1, "file does not match the case (block size, nE, block count)");
+    return fail(-1, "file does not match the case (block size 16^3 or 8^3, nE, block count)");
   }
-  const size_t npad = (size_t)(NB + 2 * g) * (NB + 2 * g) * (NB + 2 * g);
-  std::vector<double> data((size_t)nb * npad);
+  const int nf = same ? NB : NB / 2, npf = nf + 2 * g, np = NB + 2 * g;
+  const size_t nfile = (size_t)npf * npf * npf, npad = (size_t)np * np * np;
+  std::vector<double> data((size_t)nb * nfile);
   const bool ok = fread(data.data(), sizeof(double), data.size(), fp) == data.size();
   fclose(fp);
   if (!ok) return fail(-1, "short file");
-  return ffvhc_field_upload(c, f, id, g, data.data());
+  if (same) return ffvhc_field_upload(c, f, id, g, data.data());
+  std::vector<double> fine((size_t)nb * npad, 0.0);
+  for (int b = 0; b < nb; b++)
+    for (int k = 0; k < NB; k++)
+      for (int j = 0; j < NB; j++)
+        for (int i = 0; i < NB; i++)
+          fine[(size_t)b * npad + (size_t)(i + g) + (size_t)np * ((size_t)(j + g) + (size_t)np * (size_t)(k + g))] =
+              data[(size_t)b * nfile + (size_t)(i / 2 + g) +
+                   (size_t)npf * ((size_t)(j / 2 + g) + (size_t)npf * (size_t)(k / 2 + g))];
+  return ffvhc_field_upload(c, f, id, g, fine.data());
 }
 
 int ffvhc_ghost_fill(ffvhc_ctx* c, int f, int bcg, int updater) {

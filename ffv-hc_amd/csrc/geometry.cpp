@@ -17,9 +17,11 @@
 // level interface.
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <thread>
 #include <vector>
 
 #include "../../include/ffvhc.h"
@@ -30,6 +32,26 @@ namespace {
 
 using ffvhc::Block;
 using ffvhc::Octree;
+
+// Runs f(b) for b in [0, nb) on up to 16 host threads.  Every pass below that runs this way
+// writes only the padded arrays of its own block b (reading other blocks' interiors at most), so
+// the blocks are independent and the result does not depend on the thread count.
+template <class F>
+void for_blocks(int nb, F f) {
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int nt = (int)std::min<unsigned>(hw, (unsigned)std::max(nb, 1));
+  if (nt <= 1) {
+    for (int b = 0; b < nb; b++) f(b);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; t++)
+    pool.emplace_back([&]() {
+      for (int b = next.fetch_add(1); b < nb; b = next.fetch_add(1)) f(b);
+    });
+  for (auto& th : pool) th.join();
+}
 
 struct Arrays {
   int n, g, np, npad, nb;
@@ -53,7 +75,7 @@ void face_ijk(int f, int a, int b, int l, int n, int& i, int& j, int& k) {
 
 void exchange(Arrays& A, const std::vector<Block>& blk) {
   const int n = A.n, g = A.g;
-  for (int b = 0; b < A.nb; b++)
+  for_blocks(A.nb, [&](int b) {
     for (int f = 0; f < 6; f++) {
       const Block& B = blk[b];
       if (B.ftype[f] != ffvhc::FACE_SAME && B.ftype[f] != ffvhc::FACE_OUTER) continue;
@@ -76,6 +98,7 @@ void exchange(Arrays& A, const std::vector<Block>& blk) {
             }
           }
     }
+  });
 }
 
 }  // namespace
@@ -101,9 +124,17 @@ extern "C" int ffvhc_geometry_sphere(const ffvhc_tree* tree, const ffvhc_sphere_
   }
   const int n = A.n;
   const double R2 = S->radius * S->radius;
-  // analytic cuts for every padded cell
-  for (int b = 0; b < A.nb; b++) {
+  // analytic cuts for every padded cell (a block whose padded box stays farther than the radius
+  // from the centre has none)
+  for_blocks(A.nb, [&](int b) {
     const Block& B = blk[b];
+    double d2 = 0.0;
+    for (int a = 0; a < 3; a++) {
+      const double lo = B.org[a] - g * B.dx, hi = B.org[a] + (n + g) * B.dx;
+      const double e = S->center[a] < lo ? lo - S->center[a] : S->center[a] > hi ? S->center[a] - hi : 0.0;
+      d2 += e * e;
+    }
+    if (d2 > R2) return;
     for (int k = -g; k < n + g; k++)
       for (int j = -g; j < n + g; j++)
         for (int i = -g; i < n + g; i++) {
@@ -129,22 +160,23 @@ extern "C" int ffvhc_geometry_sphere(const ffvhc_tree* tree, const ffvhc_sphere_
             }
           }
         }
-  }
+  });
   exchange(A, blk);
   // bstl_cutoff (ShapeApproximation/Method "cut", Cutoff > 0)
   if (S->cutoff > 0.0) {
-    for (int b = 0; b < A.nb; b++)
+    for_blocks(A.nb, [&](int b) {
       for (int k = 0; k < n; k++)
         for (int j = 0; j < n; j++)
           for (int i = 0; i < n; i++) {
             const size_t m = A.at(b, i, j, k);
             for (int d = 0; d < 6; d++) A.c[d][m] = std::max(S->cutoff, A.c[d][m]);
           }
+    });
     exchange(A, blk);
   }
   // bstl_detect_zerocut
-  long long nzero = 0;
-  for (int b = 0; b < A.nb; b++)
+  std::atomic<long long> nzero{0};
+  for_blocks(A.nb, [&](int b) {
     for (int k = 0; k < n; k++)
       for (int j = 0; j < n; j++)
         for (int i = 0; i < n; i++) {
@@ -168,12 +200,14 @@ extern "C" int ffvhc_geometry_sphere(const ffvhc_tree* tree, const ffvhc_sphere_
             nzero++;
           }
         }
+  });
   exchange(A, blk);
   // bstl_fill_holes_v1 / v2: close an open face whose four side faces are cut (v1), or either
   // pair of them (v2); one sweep in (k, j, i) order per pass
   auto fill_pass = [&](bool v2, bool close) {
-    long long cnt = 0;
-    for (int b = 0; b < A.nb; b++)
+    std::atomic<long long> total{0};
+    for_blocks(A.nb, [&](int b) {
+      long long cnt = 0;
       for (int k = 0; k < n; k++)
         for (int j = 0; j < n; j++)
           for (int i = 0; i < n; i++) {
@@ -210,8 +244,10 @@ extern "C" int ffvhc_geometry_sphere(const ffvhc_tree* tree, const ffvhc_sphere_
                                         C(5, s) || C(2, p) || C(2, t), C(5, nn) || C(3, p) || C(3, t)))
               closef(5, p, 4, t);
           }
+      total += cnt;
+    });
     exchange(A, blk);
-    return cnt;
+    return total.load();
   };
   const long long nholes1 = fill_pass(false, S->hole_filling[0] != 0);
   long long nholes2 = 0;
@@ -304,9 +340,10 @@ extern "C" int ffvhc_geometry_sphere(const ffvhc_tree* tree, const ffvhc_sphere_
       }
     }
   }
-  long long nf = 0, ns = 0;
+  std::atomic<long long> nf{0}, ns{0};
   const size_t np = (size_t)A.npad;
-  for (int b = 0; b < A.nb; b++)
+  for_blocks(A.nb, [&](int b) {
+    long long f1 = 0, s1 = 0;
     for (int k = -g; k < n + g; k++)
       for (int j = -g; j < n + g; j++)
         for (int i = -g; i < n + g; i++) {
@@ -315,8 +352,8 @@ extern "C" int ffvhc_geometry_sphere(const ffvhc_tree* tree, const ffvhc_sphere_
           int v = 0;
           if (inside) {
             v = fluid[cell(b, i, j, k)] ? 1 : 0;
-            if (v) nf++;
-            else ns++;
+            if (v) f1++;
+            else s1++;
           }
           pid[m] = v;
           for (int d = 0; d < 6; d++) {
@@ -324,6 +361,9 @@ extern "C" int ffvhc_geometry_sphere(const ffvhc_tree* tree, const ffvhc_sphere_
             if (cid) cid[(size_t)d * A.nb * np + m] = A.id[d][m];
           }
         }
+    nf += f1;
+    ns += s1;
+  });
   if (counts) {
     counts[0] = nf;
     counts[1] = ns;

@@ -128,6 +128,10 @@ void launch_coarse_assemble(hipStream_t sm, const int* active, int nact, const D
 void launch_coarse_decouple(hipStream_t sm, double* M, int n);
 // T: scratch of 32 * n doubles
 void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* T);
+void launch_gjw_get_diag(hipStream_t sm, const double* M, int n, int k0, int kb, double* D);
+void launch_gjw_get_cols(hipStream_t sm, const double* M, int n, int k0, int kb, double* C);
+void launch_gjw_zero_tcols(hipStream_t sm, double* T, int n, int k0, int kb);
+void launch_gjw_put_rows(hipStream_t sm, double* M, int n, int k0, int kb, const double* T, const double* D);
 // FP32 copy of an n x n FP64 matrix with the row pitch np (>= n, pad zero)
 void launch_to_f32(hipStream_t sm, const double* x, float* y, int n, int np);
 void launch_coarse_apply(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,

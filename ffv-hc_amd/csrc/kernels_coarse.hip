@@ -217,6 +217,55 @@ __global__ __launch_bounds__(256) void gjb_col_kernel(double* __restrict__ M, in
   if (act) M[(size_t)i * n + k0 + c2] = -s;
 }
 
+// Helpers of the wide-panel Gauss-Jordan (gauss_jordan_inverse_gemm in solver.cpp), whose three
+// products per panel run as FP64 GEMMs.  All matrices row-major; the panel K = [k0, k0 + kb).
+// D (kb x kb, pitch kb) <- M_KK
+__global__ __launch_bounds__(256) void gjw_get_diag_kernel(const double* __restrict__ M, int n, int k0, int kb,
+                                                           double* __restrict__ D) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= kb * kb) return;
+  const int r = e / kb, c = e % kb;
+  D[e] = M[(size_t)(k0 + r) * n + k0 + c];
+}
+// C (n x kb, pitch kb) <- M_iK for the rows i outside K, 0 for the rows in K
+__global__ __launch_bounds__(256) void gjw_get_cols_kernel(const double* __restrict__ M, int n, int k0, int kb,
+                                                           double* __restrict__ C) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)n * kb) return;
+  const int i = (int)(e / kb), c = (int)(e % kb);
+  C[e] = (i >= k0 && i < k0 + kb) ? 0.0 : M[(size_t)i * n + k0 + c];
+}
+// T (kb x n, pitch n): the columns in K set to 0 (T = D^-1 M_K. before this)
+__global__ __launch_bounds__(256) void gjw_zero_tcols_kernel(double* __restrict__ T, int n, int k0, int kb) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= kb * kb) return;
+  T[(size_t)(e / kb) * n + k0 + e % kb] = 0.0;
+}
+// M_K. <- T outside K and D^-1 (in D) inside K
+__global__ __launch_bounds__(256) void gjw_put_rows_kernel(double* __restrict__ M, int n, int k0, int kb,
+                                                           const double* __restrict__ T,
+                                                           const double* __restrict__ D) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)n * kb) return;
+  const int r = (int)(e / n), j = (int)(e % n);
+  M[(size_t)(k0 + r) * n + j] = (j >= k0 && j < k0 + kb) ? D[(size_t)r * kb + j - k0] : T[e];
+}
+
+void launch_gjw_get_diag(hipStream_t sm, const double* M, int n, int k0, int kb, double* D) {
+  hipLaunchKernelGGL(gjw_get_diag_kernel, dim3((unsigned)((kb * kb + 255) / 256)), dim3(256), 0, sm, M, n, k0, kb, D);
+}
+void launch_gjw_get_cols(hipStream_t sm, const double* M, int n, int k0, int kb, double* C) {
+  hipLaunchKernelGGL(gjw_get_cols_kernel, dim3((unsigned)(((size_t)n * kb + 255) / 256)), dim3(256), 0, sm, M, n, k0,
+                     kb, C);
+}
+void launch_gjw_zero_tcols(hipStream_t sm, double* T, int n, int k0, int kb) {
+  hipLaunchKernelGGL(gjw_zero_tcols_kernel, dim3((unsigned)((kb * kb + 255) / 256)), dim3(256), 0, sm, T, n, k0, kb);
+}
+void launch_gjw_put_rows(hipStream_t sm, double* M, int n, int k0, int kb, const double* T, const double* D) {
+  hipLaunchKernelGGL(gjw_put_rows_kernel, dim3((unsigned)(((size_t)n * kb + 255) / 256)), dim3(256), 0, sm, M, n, k0,
+                     kb, T, D);
+}
+
 // T: scratch of GJB * n doubles.  Order per panel: D^-1; T = D^-1 M_P; the trailing update
 // (old M_iP, T) with panel rows <- T; M_iP <- -M_iP D^-1.
 void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* T) {

@@ -98,6 +98,10 @@ struct ffvhc_ctx {
   double* dcsnap = nullptr;
   size_t dcsnap_n = 0;
   double* dce = nullptr;
+  // the coarse inverse of n >= coarse_gemm_min blocks runs its panel products as FP64 GEMMs
+  // (rocBLAS handle `blas`, made on first use); smaller ones use the 32-wide VALU Gauss-Jordan
+  void* blas = nullptr;
+  int coarse_gemm_min = 2048;
   // host copies the setups read back or upload asynchronously; they live here so that they
   // outlive the copies (a setup returns without waiting for its last kernels)
   std::vector<double> hcrows;   // 25 coarse row entries per slot

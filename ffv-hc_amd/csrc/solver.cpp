@@ -4,6 +4,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <rocblas/rocblas.h>
+
 #include "kernels.h"
 #include "runtime.h"
 
@@ -404,6 +406,71 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
   return 0;
 }
 
+// Panel width of the GEMM Gauss-Jordan: each panel's trailing update is a rank-GJW FP64 GEMM
+constexpr int GJW = 256;
+
+// doubles of scratch the Gauss-Jordan of an n x n matrix needs: the 32-wide form GJB * n; the wide
+// form D^-1 (GJW^2), the 32-wide scratch of D's own inversion, the old panel columns C (n x GJW)
+// and the new panel rows T (GJW x n)
+size_t gj_scratch(ffvhc_ctx* c, size_t n) {
+  if ((int)n < c->coarse_gemm_min) return 32 * n;
+  return (size_t)GJW * GJW + 32 * (size_t)GJW + 2 * (size_t)GJW * n;
+}
+
+#define BLASCHK(expr)                                                                     \
+  do {                                                                                    \
+    rocblas_status s_ = (expr);                                                           \
+    if (s_ != rocblas_status_success)                                                     \
+      return fail(-2, std::string(#expr) + ": " + rocblas_status_to_string(s_));          \
+  } while (0)
+
+// In-place inverse of the row-major n x n matrix M (no pivoting: A_c is an M-matrix-like operator
+// with a dominant diagonal) by Gauss-Jordan over panels K of GJW columns.  Per panel, with D = M_KK
+// and I, J the indices outside K:
+//   D^-1 (D copied out and inverted by the 32-wide Gauss-Jordan), C = M_.K with the rows of K zeroed,
+//   T = D^-1 M_K.  with the columns of K zeroed   (GEMM, GJW x GJW x n)
+//   M_IJ -= C_I T_J                               (GEMM, n x n x GJW; the zero rows / columns leave
+//                                                  the rows and columns of K alone)
+//   M_.K = -C D^-1                                (GEMM, n x GJW x GJW; rows of K come out 0)
+//   M_K. = T, M_KK = D^-1
+// rocBLAS is column-major: a row-major matrix is its transpose there, so every product is issued
+// with its operands swapped.
+int gauss_jordan_inverse_gemm(ffvhc_ctx* c, hipStream_t sm, double* M, int n, double* work) {
+  if (n <= 0) return 0;
+  if (!c->blas) {
+    rocblas_handle h = nullptr;
+    BLASCHK(rocblas_create_handle(&h));
+    c->blas = h;
+  }
+  rocblas_handle h = (rocblas_handle)c->blas;
+  BLASCHK(rocblas_set_stream(h, sm));
+  BLASCHK(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
+  double* D = work;
+  double* Ts = D + (size_t)GJW * GJW;
+  double* C = Ts + 32 * (size_t)GJW;
+  double* T = C + (size_t)GJW * n;
+  const double one = 1.0, zero = 0.0, mone = -1.0;
+  for (int k0 = 0; k0 < n; k0 += GJW) {
+    const int kb = std::min(GJW, n - k0);
+    launch_gjw_get_diag(sm, M, n, k0, kb, D);
+    gauss_jordan_inverse(sm, D, kb, Ts);
+    launch_gjw_get_cols(sm, M, n, k0, kb, C);
+    // T (row-major kb x n) = D^-1 M_K.  <=>  T^T (n x kb, ld n) = M_K.^T (n x kb, ld n) D^-T
+    BLASCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, kb, kb, &one, M + (size_t)k0 * n, n,
+                          D, kb, &zero, T, n));
+    launch_gjw_zero_tcols(sm, T, n, k0, kb);
+    // M -= C T  <=>  M^T (n x n, ld n) -= T^T (n x kb, ld n) C^T (kb x n, ld kb)
+    BLASCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, kb, &mone, T, n, C, kb, &one, M,
+                          n));
+    // M_.K = -C D^-1  <=>  M_.K^T (kb x n, ld n) = -D^-T (kb x kb, ld kb) C^T (kb x n, ld kb)
+    BLASCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, kb, n, kb, &mone, D, kb, C, kb, &zero,
+                          M + k0, n));
+    launch_gjw_put_rows(sm, M, n, k0, kb, T, D);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 // Coarse correction of the two-level methods: A_c = P^T A P per selected case (rows from the
 // device, assembled densely on the device), inverted on the device (Gauss-Jordan, FP64), kept as
 // FP32.  A case whose A_c has zero row sums throughout (no Dirichlet face: A_c singular) has its
@@ -468,11 +535,11 @@ int coarse_finish(ffvhc_ctx* c, BiOpts* o, double* ac_out, double* inv_out) {
     c->dcm_n = total;
     c->dci_n = totalp;
   }
-  if (32 * maxn > c->dcsnap_n) {  // panel rows of the blocked Gauss-Jordan
+  if (gj_scratch(c, maxn) > c->dcsnap_n) {  // panel scratch of the blocked Gauss-Jordan
     (void)hipFree(c->dcsnap);
     c->dcsnap = nullptr;
-    HIPCHK(hipMalloc(&c->dcsnap, 32 * maxn * sizeof(double)));
-    c->dcsnap_n = 32 * maxn;
+    HIPCHK(hipMalloc(&c->dcsnap, gj_scratch(c, maxn) * sizeof(double)));
+    c->dcsnap_n = gj_scratch(c, maxn);
   }
   if (!c->dcoff) HIPCHK(hipMalloc(&c->dcoff, 2 * MAX_CASES * sizeof(long long)));
   if (!c->dce) HIPCHK(hipMalloc(&c->dce, 5 * (size_t)c->cap * sizeof(double)));
@@ -509,7 +576,15 @@ int coarse_finish(ffvhc_ctx* c, BiOpts* o, double* ac_out, double* inv_out) {
     HIPCHK(hipStreamWaitEvent(c->sc, c->ev_cs, 0));
     gs = c->sc;
   }
-  for (int id : ids) gauss_jordan_inverse(gs, c->dcm + off[id], c->cases[id].nblk, c->dcsnap);
+  for (int id : ids) {
+    const int n = c->cases[id].nblk;
+    if (n >= c->coarse_gemm_min) {
+      rc = gauss_jordan_inverse_gemm(c, gs, c->dcm + off[id], n, c->dcsnap);
+      if (rc) return rc;
+    } else {
+      gauss_jordan_inverse(gs, c->dcm + off[id], n, c->dcsnap);
+    }
+  }
   for (int id : ids) {
     const int n = c->cases[id].nblk;
     launch_to_f32(gs, c->dcm + off[id], c->dci + offp[id], n, (n + 3) & ~3);
@@ -952,6 +1027,12 @@ int ffvhc_ctx_set_option(ffvhc_ctx* c, int option, int value) {
   }
   if (option == FFVHC_OPTION_SETUP_REUSE) {
     c->reuse_setup = value != 0;
+    c->psetup.valid = false;
+    return 0;
+  }
+  if (option == FFVHC_OPTION_COARSE_GEMM) {
+    if (value < 1) return fail(-1, "FFVHC_OPTION_COARSE_GEMM takes a block count >= 1");
+    c->coarse_gemm_min = value;
     c->psetup.valid = false;
     return 0;
   }

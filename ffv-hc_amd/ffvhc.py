@@ -294,6 +294,10 @@ class Context:
         matrix cannot have changed (default), or make it every step."""
         check(_L.ffvhc_ctx_set_option(self.h, 4, int(bool(on))), "ffvhc_ctx_set_option")
 
+    def set_coarse_gemm_min(self, nblk):
+        """Smallest case (blocks) whose coarse operator is inverted with GEMM panels (default 2048)."""
+        check(_L.ffvhc_ctx_set_option(self.h, 5, int(nblk)), "ffvhc_ctx_set_option")
+
     def set_classes(self, on):
         """Let the compact-record solve methods (11-28) read the bands of blocks that are uniform
         per stencil position class from 27 rows (default), or read every block per cell."""

@@ -29,6 +29,22 @@ def single_level_sphere(radius=0.5):
     return tree, (c, cid, pid), counts
 
 
+def sphere_regions(center, d, length, levels):
+    """SphereDivider3-style regions for a tree of `levels` levels (1-3): three_level_regions, with
+    the finest level capped at levels - 1 (two levels: the sphere surface, the ball and the wake box
+    at level 1)."""
+    if levels <= 1:
+        return []
+    top = levels - 1
+    out = []
+    for r in three_level_regions(center, d, length):
+        if r["level"] <= top:
+            out.append(r)
+        elif r["kind"] == "shell":
+            out.append(dict(r, level=top))
+    return out
+
+
 def three_level_regions(center, d=1.0, length=16.0):
     """SphereDivider3-style refinement: the sphere surface at level 2, a ball of radius 2d at
     level 1, the wake box x in [cx, cx + 12d], |y - cy|, |z - cz| <= 2d at level 1 and the near
@@ -58,3 +74,26 @@ def three_level_sphere(root=(16, 8, 8), root_length=1.0, d=1.0, geometry=True):
 def cfl_dt(tree, cfl, u0=1.0):
     """Time step for a CFL number on the finest cells: cfl * h_min / u0."""
     return cfl * float(tree.table()["dx"].min()) / u0
+
+
+def served_case(target_blocks, levels, d=1.0):
+    """A flow past a sphere of diameter d with about target_blocks blocks on `levels` levels, for
+    the served stream: roots (2k, k, k) with the sphere at (X / 4, Y / 2, Z / 2); one level: root
+    edge 2d (h = d / 8, the reference's single-level case); more levels: root edge d, 2d or 4d with
+    the sphere + wake refinement of sphere_regions.  (k, root edge) is the pair whose block count is
+    closest to the target (in log scale).  Returns (tree, center)."""
+    import math
+    best = None
+    for rl in ((2.0 * d,) if levels <= 1 else (d, 2.0 * d, 4.0 * d)):
+        for k in range(2, 64):
+            root = (2 * k, k, k)
+            center = (0.25 * root[0] * rl, 0.5 * root[1] * rl, 0.5 * root[2] * rl)
+            tree = ffvhc.Tree(root=root, root_length=rl, regions=sphere_regions(center, d, root[0] * rl, levels))
+            if tree.max_level + 1 != levels:
+                continue
+            err = abs(math.log(tree.nblk / float(target_blocks)))
+            if best is None or err < best[0]:
+                best = (err, tree, center)
+            if tree.nblk > 2 * target_blocks:
+                break
+    return best[1], best[2]

@@ -117,6 +117,12 @@ long long ffvhc_ctx_device_bytes(const ffvhc_ctx* ctx);
  * method or an option change, or any other solve writes the setup buffers.  0 makes the setup
  * every step.  The results are identical. */
 #define FFVHC_OPTION_SETUP_REUSE 4
+/* FFVHC_OPTION_COARSE_GEMM: the smallest case (in blocks) whose coarse operator (two-level methods
+ * 17-28) is inverted with 256-wide panels whose products run as FP64 GEMMs (matrix cores); smaller
+ * ones use 32-wide panels on the vector units.  Default 2048 (measured: at 1,128 blocks the
+ * vector form is as fast, at 4,048 the GEMM form takes 10 ms instead of 22, at 8,512 41 instead of
+ * 182).  Both are Gauss-Jordan without pivoting; their inverses differ in rounding only. */
+#define FFVHC_OPTION_COARSE_GEMM 5
 int ffvhc_ctx_set_option(ffvhc_ctx* ctx, int option, int value);
 
 /* Physical parameters and outer BCs of one flow case.  BC groups: 0 UX, 1 UY, 2 UZ, 3 P,

@@ -69,3 +69,13 @@ def test_mixed_byte_model_and_names():
     assert bench.mixed_prof_name(26, "bi_xr") == "bi_xr_kernel<double, false, false>"
     assert bench.mixed_prof_name(19, "bi_xr") == "bi_xr_kernel<double, false, true>"
     assert len(bench.MIXED_METHODS) == 29
+
+
+def test_step_iteration_bytes():
+    sys.path[:0] = [ROOT, os.path.join(ROOT, "ffv-hc_amd")]
+    import bench
+    # FP64 Jacobi-BiCGSTAB: the five stages only (51 + 83 + 43 + 83 + 64 bytes per cell)
+    assert bench.step_iteration_bytes(None, 1.0) == 324.0
+    u, o = bench.step_iteration_bytes(26, 1.0), bench.step_iteration_bytes(26, 0.0)
+    assert abs(u - (43 + 27 + 35 + 27 + 64)) < 1e-9 and o > u
+    assert abs(bench.step_iteration_bytes(26, 0.5) - 0.5 * (u + o)) < 1e-9

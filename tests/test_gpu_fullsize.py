@@ -68,3 +68,24 @@ def test_fullsize_solve_is_homogeneous(c2, method):
         ctx.upload(f64["b"], cid, b)
     assert c1 == c2_
     assert np.array_equal(x2, 2.0 * x1)
+
+
+def test_fullsize_coarse_inverse_gemm_and_vector_paths(c2):
+    """The coarse operator of the 1,128-block system inverted both ways: 256-wide panels with FP64
+    GEMM products (five panels, the last one partial) and 32-wide panels on the vector units.  Both
+    invert A_c (A_c^-1 A_c = I to 1e-9) and agree with each other to rounding."""
+    ctx, cid, f64, tree, b = c2
+    n = tree.nblk
+    out = {}
+    try:
+        for gemm_min in (1, 1 << 30):
+            ctx.set_coarse_gemm_min(gemm_min)
+            ac, ainv = ctx.coarse_operator(f64["A"], cid)
+            assert ac.shape == (n, n)
+            err = np.max(np.abs(ainv @ ac - np.eye(n)))
+            assert err <= 1e-9, (gemm_min, err)
+            out[gemm_min] = ainv
+    finally:
+        ctx.set_coarse_gemm_min(2048)
+    scale = np.max(np.abs(out[1]))
+    assert np.max(np.abs(out[1] - out[1 << 30])) <= 1e-10 * scale

@@ -469,6 +469,42 @@ def test_checkpoint_round_trip(ctx, tmp_path):
         ctx.remove_case(cid)
 
 
+def test_load2_half_resolution(ctx, tmp_path):
+    """Load2 of a file written for blocks of half the cell count (8^3, LocalScalar3D.cpp:313-331):
+    every fine cell takes its parent coarse cell, x(i, j, k) = xc((i+1)/2, (j+1)/2, (k+1)/2) with
+    1-based indices and integer division (sf3d_copy_x2, bsf3d.f90:5-33)."""
+    tree = trees()["two_level"]
+    og = OracleGrid(tree)
+    cid = ctx.add_case(tree, ffvhc.case_params(bc=BC))
+    f = ctx.field(8)
+    try:
+        nc, g = N // 2, G
+        xc = np.random.default_rng(5).standard_normal((og.nb,) + (nc + 2 * g,) * 3)
+        path = tmp_path / "pP-00000-0000000003.bin"
+        with open(path, "wb") as fh:
+            np.array([nc, nc, nc, g, 1, og.nb], np.int32).tofile(fh)
+            xc.tofile(fh)
+        ctx.load(f, cid, path)
+        got = ctx.download(f, cid, out=np.zeros(og.shape))
+        want = np.empty((og.nb, N, N, N))
+        for k in range(1, N + 1):          # the reference's loop, 1-based, padded offset g - 1
+            for j in range(1, N + 1):
+                for i in range(1, N + 1):
+                    want[:, k - 1, j - 1, i - 1] = xc[:, (k + 1) // 2 + g - 1, (j + 1) // 2 + g - 1,
+                                                      (i + 1) // 2 + g - 1]
+        assert np.array_equal(got[:, G:G + N, G:G + N, G:G + N], want)
+        # a file whose blocks are neither 16^3 nor 8^3 is refused
+        bad = tmp_path / "bad.bin"
+        with open(bad, "wb") as fh:
+            np.array([4, 4, 4, g, 1, og.nb], np.int32).tofile(fh)
+            np.zeros((og.nb, 8, 8, 8)).tofile(fh)
+        with pytest.raises(Exception):
+            ctx.load(f, cid, bad)
+    finally:
+        ctx.free(f)
+        ctx.remove_case(cid)
+
+
 def test_field_stats_and_cfl_guard(ctx):
     """CalcStats per case against numpy on the same interior; the CFL guard of PrintStats."""
     tree = trees()["two_level"]
