@@ -227,9 +227,11 @@ def cpu_time_step_baseline(steps=4, threads=16):
 
 
 def serve_leg(ncases, seed, budget_s, method=None, omega=1.0):
-    """Served stream: flow cases of uneven block counts arrive as a Poisson process and run a
-    number of time steps each under continuous batching; p50/p99 time-to-solution.  The pressure
-    solve is the reference's FP64 Jacobi-BiCGSTAB, or ffvhc_solve_mixed `method` with `omega`."""
+    """Served stream of small cases without a body: flow cases of uneven block counts arrive as a
+    Poisson process of 50 per second (faster than they finish: the GPU, not the arrivals, sets
+    cases/hour) and run a number of time steps each under continuous batching; p50/p99
+    time-to-solution.  The pressure solve is the reference's FP64 Jacobi-BiCGSTAB, or
+    ffvhc_solve_mixed `method` with `omega`."""
     import ffvhc
     from serve import Request, Server, uniform_flow_init
     rng = np.random.default_rng(seed)
@@ -242,7 +244,7 @@ def serve_leg(ncases, seed, budget_s, method=None, omega=1.0):
         if rng.random() < 0.5:  # refined ball: a 2-level case
             regions = [dict(kind="ball", a=(nx / 2, ny / 2, ny / 2), r=float(rng.uniform(0.3, 0.8)), level=1)]
         tree = ffvhc.Tree(root=(nx, ny, ny), root_length=1.0, regions=regions)
-        t += float(rng.exponential(0.15))
+        t += float(rng.exponential(0.02))
         reqs.append(Request(name="case%d" % i, arrival=t, tree=tree,
                             params=ffvhc.case_params(mu=float(1.0 / rng.uniform(50, 300)), dt=0.002, bc=bc),
                             steps=int(rng.integers(5, 21)), init=uniform_flow_init(i)))
@@ -353,6 +355,91 @@ def serve_c5_leg(ncases, seed, budget_s, method=None, omega=1.0, hbm_gib=96.0, p
                unfinished_blocks=[int(r.tree.nblk) for r in srv.active + srv.waiting])
     ctx.close()
     return rep
+
+
+def c4_leg(ncases=64, steps=3, seed=0):
+    """Batched sweep of independent cases seeded by index: sphere D uniform in [0.5, 1.5], Re
+    uniform in [50, 300], 1-3 levels, 200-2,000 blocks (log-uniform target) of 16^3 cells, implicit
+    U, dt at CFL 0.3 on each case's finest cells, pressure to 1e-8.  Every case is resident in one
+    context and each stage of the time step is one launch over all cases' blocks.  The autotuner
+    times two batched steps of each candidate pressure schedule (the multiplicative two-level
+    solver with 3, 4 or 5 SOR sweeps) and the batch then runs `steps` timed steps with the fastest."""
+    import ffvhc
+    import flows
+    from serve import uniform_flow_init
+    trees, params = [], []
+    for i in range(ncases):
+        rng = np.random.default_rng(1000 + i)
+        d = float(rng.uniform(0.5, 1.5))
+        re = float(rng.uniform(50.0, 300.0))
+        levels = int(rng.integers(1, 4))
+        target = float(np.exp(rng.uniform(np.log(200.0), np.log(2000.0))))
+        tree, center = flows.served_case(target, levels, d=d)
+        trees.append((tree, center, d))
+        params.append(flows.flow_params(re, flows.cfl_dt(tree, 0.3), d=d))
+    nblk = sum(t[0].nblk for t in trees)
+    cells = nblk * N ** 3
+    log("c4: %d cases, %d blocks, %d cells; building geometry" % (ncases, nblk, cells))
+    ctx = ffvhc.Context(max_blocks=nblk)
+    F = ctx.step_fields()
+    ids = []
+    t0 = time.perf_counter()
+    for (tree, center, d), prm in zip(trees, params):
+        geo = tree.geometry_sphere(center=center, radius=0.5 * d, cut_id=1)[:3]
+        cid = ctx.add_case(tree, prm, geometry=geo)
+        del geo
+        ctx.select([cid])
+        uniform_flow_init(cid, noise=0.0)(ctx, cid, F)
+        ids.append(cid)
+        if len(ids) % 8 == 0:
+            log("c4: %d cases resident (%.1f s)" % (len(ids), time.perf_counter() - t0))
+    setup_s = time.perf_counter() - t0
+    log("c4: geometry and upload %.1f s" % setup_s)
+    ctx.select(ids)
+    ukw = dict(implicit_u=True, alpha_u=1.0, omega_u=1.0, count_max_u=1000, eps_u=1e-8)
+    cands = [(25, 1.6), (26, 1.6), (27, 1.6)]
+    tuned = []
+    for m, om in cands:
+        kw = dict(solver=3, omega=om, count_max=20000, eps=1e-8, mixed=True, mixed_method=m, **ukw)
+        ctx.step(F, ncases=ncases, **kw)  # setup of this method (the step keeps it while A is unchanged)
+        ctx.sync()
+        t1 = time.perf_counter()
+        for _ in range(2):
+            info, _ms, rc = ctx.step(F, ncases=ncases, **kw)
+            if rc:
+                raise RuntimeError("c4 step failed with status %d" % rc)
+        ctx.sync()
+        tuned.append((m, om, (time.perf_counter() - t1) / 2))
+        log("c4 candidate method %d: %.3f s per batched step" % (m, tuned[-1][2]))
+    m, om, _ = min(tuned, key=lambda r: r[2])
+    kw = dict(solver=3, omega=om, count_max=20000, eps=1e-8, mixed=True, mixed_method=m, **ukw)
+    ctx.step(F, ncases=ncases, **kw)
+    ctx.sync()
+    t1 = time.perf_counter()
+    its, res = [], []
+    for _ in range(steps):
+        info, _ms, rc = ctx.step(F, ncases=ncases, **kw)
+        if rc:
+            raise RuntimeError("c4 step failed with status %d" % rc)
+        its.append([int(q.countP) for q in info])
+        res.append(max(float(q.residualP) for q in info))
+    ctx.sync()
+    dt = (time.perf_counter() - t1) / steps
+    umax = float(np.max(ctx.stats(F.ux, ncases=ncases)[:, 3]))
+    ctx.close()
+    lv = np.bincount([t[0].max_level for t in trees], minlength=3)
+    return {"workload": "batched sweep of %d cases seeded by index: sphere D U[0.5, 1.5], Re U[50, 300], 1-3 levels, "
+                        "200-2,000 blocks of 16^3 (log-uniform), implicit U, CFL 0.3, pressure to 1e-8; one launch "
+                        "per stage over all cases" % ncases,
+            "cases": ncases, "blocks": nblk, "cells": cells, "cases_per_level_count": [int(v) for v in lv],
+            "blocks_min_max": [min(t[0].nblk for t in trees), max(t[0].nblk for t in trees)],
+            "mcell_steps_s": round(cells / dt / 1e6, 3), "ms_per_batched_step": round(dt * 1e3, 2),
+            "pressure_iterations_mean": round(float(np.mean(its)), 1),
+            "pressure_iterations_max": int(np.max(its)), "pressure_residual_max": max(res), "ux_abs_max": umax,
+            "schedule": "solve_mixed method %d (multiplicative two-level, %d SOR sweeps), omega %.1f"
+                        % (m, m - 22, om),
+            "autotune": [{"method": c[0], "omega": c[1], "s_per_step": round(c[2], 4)} for c in tuned],
+            "setup_s": round(setup_s, 1)}
 
 
 def time_step_leg(case, steps, warmup, method, omega, fp64_steps=None):
@@ -568,17 +655,24 @@ def main():
     ap.add_argument("--time-step-case", default="c1", choices=("c1", "c3"),
                     help="flow of --time-step-only: c1 single level, c3 three levels")
     ap.add_argument("--c3-steps", type=int, default=5, help="timed mixed steps of the three-level leg (0: skip)")
-    ap.add_argument("--c5-cases", type=int, default=0,
+    ap.add_argument("--c5-cases", type=int, default=8,
                     help="cases of the drag-stopped served stream (block counts log-uniform in [100, 8000]; 0: skip)")
-    ap.add_argument("--c5-seconds", type=float, default=120.0, help="wall budget of the served stream")
+    ap.add_argument("--c5-seconds", type=float, default=100.0, help="wall budget of the served stream")
+    ap.add_argument("--c5-max-steps", type=int, default=2000, help="most steps a served case runs")
     ap.add_argument("--c5-only", action="store_true", help="only the drag-stopped served stream")
+    ap.add_argument("--c4-only", action="store_true", help="only the batched sweep of 64 cases")
+    ap.add_argument("--c4-cases", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this benchmark runs on one GPU only; multi-GPU runs are not implemented (the "
               "block octree would shard by contiguous Hilbert-ordered block ranges per GPU)", file=sys.stderr)
         sys.exit(2)
+    if args.c4_only:
+        print(json.dumps(c4_leg(args.c4_cases)))
+        return
     if args.c5_only:
-        rep = serve_c5_leg(max(args.c5_cases, 1), 11, args.c5_seconds, method=26, omega=1.6)
+        rep = serve_c5_leg(max(args.c5_cases, 1), 11, args.c5_seconds, method=26, omega=1.6,
+                           max_steps=args.c5_max_steps)
         print(json.dumps(rep))
         return
     if args.time_step_only:
@@ -767,13 +861,19 @@ def main():
                      "bytes_per_cell": dbytes},
         "kernels": kt,
     }
+    if args.c5_cases > 0:
+        log("served stream of %d flows past a sphere, drag-stopped" % args.c5_cases)
+        out["serving"] = serve_c5_leg(args.c5_cases, 11, args.c5_seconds, method=sched[0], omega=sched[3],
+                                      max_steps=args.c5_max_steps)
+        log("serving: " + json.dumps({k: v for k, v in out["serving"].items() if k != "cases_detail"}))
     if args.serve_cases > 0:
-        log("served stream of %d cases" % args.serve_cases)
-        out["serving"] = serve_leg(args.serve_cases, 7, 60.0, method=sched[0], omega=sched[3])
-        log("serving: " + json.dumps({k: v for k, v in out["serving"].items() if k != "blocks_per_case"}))
-        out["serving_fp64"] = serve_leg(args.serve_cases, 7, 60.0)
-        log("serving (FP64 reference solver): " +
-            json.dumps({k: v for k, v in out["serving_fp64"].items() if k != "blocks_per_case"}))
+        log("served stream of %d small cases" % args.serve_cases)
+        out["serving_small"] = serve_leg(args.serve_cases, 7, 60.0, method=sched[0], omega=sched[3])
+        log("serving (small cases): " +
+            json.dumps({k: v for k, v in out["serving_small"].items() if k != "blocks_per_case"}))
+        out["serving_small_fp64"] = serve_leg(args.serve_cases, 7, 60.0)
+        log("serving (small cases, FP64 reference solver): " +
+            json.dumps({k: v for k, v in out["serving_small_fp64"].items() if k != "blocks_per_case"}))
     if args.step_steps > 0:
         log("time steps of the flow past a sphere")
         out["time_step"] = time_step_leg("c1", args.step_steps, 2, sched[0], sched[3])

@@ -1,5 +1,6 @@
 // C ABI implementation: octree, context, cases, fields, ghost fill.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -466,6 +467,104 @@ int ffvhc_case_add(ffvhc_ctx* c, const ffvhc_tree* t, const ffvhc_case_params* p
   return refresh_active(c);
 }
 
+int ffvhc_case_fill_phase(ffvhc_ctx* c, int id, const double* seed, int cut_id, long long* counts) {
+  if (!c || !seed) return fail(-1, "null argument");
+  if (id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
+  if (cut_id < 1 || cut_id > 255) return fail(-1, "cut_id must be in 1..255");
+  const CaseRec& r = c->cases[id];
+  const int nb = r.nblk, b0 = r.slot_begin;
+  // the seed cell: the first block (in block order) whose cells hold the point, as the host
+  // geometry's search starts
+  int sb = -1, si = 0, sj = 0, sk = 0;
+  for (int b = 0; b < nb && sb < 0; b++) {
+    const DevBlock& B = c->hblk[b0 + b];
+    int q[3];
+    bool in = true;
+    for (int a = 0; a < 3; a++) {
+      q[a] = (int)std::floor((seed[a] - B.org[a]) / B.dx);
+      if (q[a] < 0 || q[a] >= NB) in = false;
+    }
+    if (in) {
+      sb = b;
+      si = q[0];
+      sj = q[1];
+      sk = q[2];
+    }
+  }
+  if (sb < 0) return fail(-1, "the seed point lies outside the case's blocks");
+  std::vector<int> slots(nb);
+  for (int b = 0; b < nb; b++) slots[b] = b0 + b;
+  std::vector<uint16_t> hlab((size_t)nb * NF, 0);
+  hlab[(size_t)sb * NF + si + NB * sj] = (uint16_t)(1u << sk);
+  constexpr int BATCH = 8;
+  int* dslots = nullptr;
+  int* dflags = nullptr;  // changed[BATCH], then nfluid[cap], irregular[cap]
+  uint16_t* dlab = nullptr;
+  HIPCHK(hipMallocAsync((void**)&dslots, sizeof(int) * nb, c->sm));
+  HIPCHK(hipMallocAsync((void**)&dflags, sizeof(int) * (BATCH + 2 * (size_t)c->cap), c->sm));
+  HIPCHK(hipMallocAsync((void**)&dlab, sizeof(uint16_t) * NF * (size_t)c->cap, c->sm));
+  HIPCHK(hipMemcpyAsync(dslots, slots.data(), sizeof(int) * nb, hipMemcpyHostToDevice, c->sm));
+  HIPCHK(hipMemcpyAsync(dlab + (size_t)b0 * NF, hlab.data(), sizeof(uint16_t) * NF * nb, hipMemcpyHostToDevice,
+                        c->sm));
+  const size_t capnc = (size_t)c->cap * NC;
+  int hchg[BATCH];
+  bool done = false;
+  for (int pass = 0; !done; pass += BATCH) {
+    if (pass > 64 * nb * NB) return fail(-2, "phase flood fill did not settle");
+    HIPCHK(hipMemsetAsync(dflags, 0, sizeof(int) * BATCH, c->sm));
+    for (int q = 0; q < BATCH; q++)
+      launch_phase_pass(c->sm, dslots, nb, c->dblk, c->dcid, capnc, cut_id, dlab, dflags + q);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hchg, dflags, sizeof(hchg), hipMemcpyDeviceToHost, c->sm));
+    HIPCHK(hipStreamSynchronize(c->sm));
+    done = hchg[BATCH - 1] == 0;  // a pass that changed nothing: every later one is the same
+  }
+  int* dnf = dflags + BATCH;
+  int* dirr = dnf + c->cap;
+  launch_phase_store(c->sm, dslots, nb, dlab, c->dflag, dnf, dirr);
+  HIPCHK(hipGetLastError());
+  std::vector<int> nf(nb), irr(nb);
+  HIPCHK(hipMemcpyAsync(nf.data(), dnf + b0, sizeof(int) * nb, hipMemcpyDeviceToHost, c->sm));
+  HIPCHK(hipMemcpyAsync(irr.data(), dirr + b0, sizeof(int) * nb, hipMemcpyDeviceToHost, c->sm));
+  HIPCHK(hipFreeAsync(dslots, c->sm));
+  HIPCHK(hipFreeAsync(dflags, c->sm));
+  HIPCHK(hipFreeAsync(dlab, c->sm));
+  HIPCHK(hipStreamSynchronize(c->sm));
+  long long fluid = 0;
+  for (int b = 0; b < nb; b++) {
+    fluid += nf[b];
+    c->hblk[b0 + b].flags = irr[b];
+  }
+  HIPCHK(hipMemcpyAsync(c->dblk + b0, c->hblk.data() + b0, sizeof(DevBlock) * nb, hipMemcpyHostToDevice, c->sm));
+  HIPCHK(hipStreamSynchronize(c->sm));
+  c->geom_serial++;  // the pressure matrix depends on the phase
+  if (counts) {
+    counts[0] = fluid;
+    counts[1] = (long long)nb * NC - fluid;
+  }
+  return 0;
+}
+
+int ffvhc_case_download_phase(ffvhc_ctx* c, int id, int g, int* pid) {
+  if (!c || !pid || g < 0) return fail(-1, "bad arguments");
+  if (id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
+  const CaseRec& r = c->cases[id];
+  std::vector<uint32_t> fl((size_t)r.nblk * NC);
+  HIPCHK(hipMemcpyAsync(fl.data(), c->dflag + (size_t)r.slot_begin * NC, sizeof(uint32_t) * fl.size(),
+                        hipMemcpyDeviceToHost, c->sm));
+  HIPCHK(hipStreamSynchronize(c->sm));
+  const int np = NB + 2 * g;
+  const size_t npad = (size_t)np * np * np;
+  std::fill(pid, pid + (size_t)r.nblk * npad, 0);
+  for (int b = 0; b < r.nblk; b++)
+    for (int k = 0; k < NB; k++)
+      for (int j = 0; j < NB; j++)
+        for (int i = 0; i < NB; i++)
+          pid[(size_t)b * npad + (size_t)(i + g) + (size_t)np * ((size_t)(j + g) + (size_t)np * (size_t)(k + g))] =
+              (int)((fl[(size_t)b * NC + i + NB * (j + NB * k)] >> 6) & 1u);
+  return 0;
+}
+
 int ffvhc_case_remove(ffvhc_ctx* c, int id) {
   if (!c || id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
   CaseRec& r = c->cases[id];
@@ -600,14 +699,19 @@ int ffvhc_field_download(ffvhc_ctx* c, int f, int id, int g, void* padded) {
 
 /* LocalScalar3D<real>::Dump2 (LocalScalar3D.cpp:215-264) file layout: six ints {nx, ny, nz, vc,
  * nE = 1, nB} then, per block in block order, the padded (n+2vc)^3 FP64 array. */
-int ffvhc_field_dump(ffvhc_ctx* c, int f, int id, int g, const char* path) {
-  if (!c || !path || g < 1) return fail(-1, "bad arguments");
+int ffvhc_field_dump(ffvhc_ctx* c, int f, int id, int g, int bcg, int updater, const char* path) {
+  if (!c || !path || g < 1 || g > 2) return fail(-1, "bad arguments (ghost width 1 or 2)");
+  if (bcg < 0 || bcg >= NUM_BC || (updater != 1 && updater != 5)) return fail(-1, "bad bc group / updater");
   if (!field_ok(c, f, 8)) return fail(-1, "Dump2 writes FP64 fields");
   if (id < 0 || id >= (int)c->cases.size() || !c->cases[id].used) return fail(-4, "no such case");
   const int nb = c->cases[id].nblk;
   const size_t npad = (size_t)(NB + 2 * g) * (NB + 2 * g) * (NB + 2 * g);
   std::vector<double> data((size_t)nb * npad, 0.0);
   int rc = ffvhc_field_download(c, f, id, g, data.data());
+  if (rc) return rc;
+  // Dump2 fills the ghosts first (ImposeBoundaryCondition): every ghost layer, rebuilt on the
+  // host from the interiors
+  rc = host_impose_bc(c, id, data.data(), g, bcg, updater);
   if (rc) return rc;
   FILE* fp = fopen(path, "wb");
   if (!fp) return fail(-1, std::string("cannot open ") + path);

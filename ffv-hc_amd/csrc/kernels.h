@@ -128,6 +128,10 @@ void launch_coarse_assemble(hipStream_t sm, const int* active, int nact, const D
 void launch_coarse_decouple(hipStream_t sm, double* M, int n);
 // T: scratch of 32 * n doubles
 void gauss_jordan_inverse(hipStream_t sm, double* M, int n, double* T);
+void launch_phase_pass(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const uint8_t* cid8,
+                       size_t capnc, int target, uint16_t* lab, int* changed);
+void launch_phase_store(hipStream_t sm, const int* active, int nact, const uint16_t* lab, uint32_t* flag,
+                        int* nfluid, int* irregular);
 void launch_gjw_get_diag(hipStream_t sm, const double* M, int n, int k0, int kb, double* D);
 void launch_gjw_get_cols(hipStream_t sm, const double* M, int n, int k0, int kb, double* C);
 void launch_gjw_zero_tcols(hipStream_t sm, double* T, int n, int k0, int kb);

@@ -170,4 +170,5 @@ int alloc_field(ffvhc_ctx* ctx, int prec, int* field);
 int scratch_field(ffvhc_ctx* ctx, int prec, int role, int* field);
 bool field_ok(ffvhc_ctx* ctx, int f, int prec);
 int upload_case_meta(ffvhc_ctx* c, int id);
+int host_impose_bc(ffvhc_ctx* c, int id, double* padded, int g, int bcg, int updater);
 }  // namespace ffvhc

@@ -104,11 +104,13 @@ _sig = {
     "ffvhc_jacobi": (c_int, [_vp, c_int, _vp, c_int, c_int, c_double, c_int, c_double, _vp, _vp]),
     "ffvhc_step": (c_int, [_vp, _P(StepFields), _P(StepParams), _vp, _vp]),
     "ffvhc_case_set_step": (c_int, [_vp, c_int, c_int, c_int]),
+    "ffvhc_case_fill_phase": (c_int, [_vp, c_int, _vp, c_int, _vp]),
+    "ffvhc_case_download_phase": (c_int, [_vp, c_int, c_int, _vp]),
     "ffvhc_force_p": (c_int, [_vp, c_int, c_int, _vp]),
     "ffvhc_force_p_noalloc": (c_int, [_vp, c_int, c_int, _vp]),
     "ffvhc_field_stats": (c_int, [_vp, c_int, _vp]),
     "ffvhc_cfl_check": (c_int, [_vp, c_int, _vp]),
-    "ffvhc_field_dump": (c_int, [_vp, c_int, c_int, c_int, ctypes.c_char_p]),
+    "ffvhc_field_dump": (c_int, [_vp, c_int, c_int, c_int, c_int, c_int, ctypes.c_char_p]),
     "ffvhc_field_load": (c_int, [_vp, c_int, c_int, ctypes.c_char_p]),
     "ffvhc_dump_header": (c_int, [ctypes.c_char_p, _vp]),
     "ffvhc_case_step": (c_int, [_vp, c_int]),
@@ -279,6 +281,21 @@ class Context:
     def remove_case(self, case_id):
         check(_L.ffvhc_case_remove(self.h, int(case_id)), "ffvhc_case_remove")
         self.cases.pop(case_id, None)
+
+    def fill_phase(self, case_id, seed, cut_id=1):
+        """Device flood fill of the case's phase from the cell holding `seed`; (fluid, solid) counts."""
+        sd = np.asarray(seed, np.float64)
+        counts = np.zeros(2, np.int64)
+        check(_L.ffvhc_case_fill_phase(self.h, int(case_id), _ptr(sd), int(cut_id), _ptr(counts)),
+              "ffvhc_case_fill_phase")
+        return counts
+
+    def download_phase(self, case_id, g=2):
+        """The case's phase (1 fluid, 0 solid) as padded int blocks."""
+        nb = self.cases[case_id].nblk
+        out = np.zeros((nb, 16 + 2 * g, 16 + 2 * g, 16 + 2 * g), np.int32)
+        check(_L.ffvhc_case_download_phase(self.h, int(case_id), int(g), _ptr(out)), "ffvhc_case_download_phase")
+        return out
 
     def set_graphs(self, on):
         """Replay the BiCGSTAB iteration as a captured HIP graph, or launch one by one (default)."""
@@ -508,9 +525,10 @@ class Context:
             check(rc, "ffvhc_cfl_check")
         return out, rc
 
-    def dump(self, f, case_id, path, g=2):
-        """Write a field of a case in the reference's Dump2 layout."""
-        check(_L.ffvhc_field_dump(self.h, f, case_id, g, str(path).encode()), "ffvhc_field_dump")
+    def dump(self, f, case_id, path, g=2, group=BC_P, updater=1):
+        """Write a field of a case in the reference's Dump2 layout, ghosts filled first with the BC
+        group and updater (1 for p, 5 for the velocity components)."""
+        check(_L.ffvhc_field_dump(self.h, f, case_id, g, group, updater, str(path).encode()), "ffvhc_field_dump")
 
     def load(self, f, case_id, path):
         """Read a Dump2 file into a field of a case (ghosts: call ghost_fill afterwards)."""

@@ -144,6 +144,14 @@ int ffvhc_case_remove(ffvhc_ctx* ctx, int case_id);
 int ffvhc_case_num_blocks(const ffvhc_ctx* ctx, int case_id);
 /* LocalScalar3D::ResetBoundaryConditionValue (LocalScalar3D.h:325-335) */
 int ffvhc_case_set_bc(ffvhc_ctx* ctx, int case_id, int group, int face, int type, double value);
+/* Phase flood fill on the device (FillCellFlag / InitPhase, Solver.cpp:1182-1287, 1404-1537): the
+ * case's cells reachable from the cell holding `seed` through faces whose cut id is not cut_id
+ * become fluid, all others solid (a move is blocked when the entered cell's face toward the cell
+ * it comes from carries cut_id), from the cut ids the case was added with.  counts[0..1] = fluid,
+ * solid cells (may be null).  Status -1 when the seed lies outside the case's blocks. */
+int ffvhc_case_fill_phase(ffvhc_ctx* ctx, int case_id, const double* seed, int cut_id, long long* counts);
+/* The case's phase (1 fluid, 0 solid) as padded int blocks of ghost width g (ghosts 0). */
+int ffvhc_case_download_phase(ffvhc_ctx* ctx, int case_id, int g, int* pid);
 /* Restrict subsequent calls to these cases (NULL / 0 = every resident case). */
 int ffvhc_ctx_select(ffvhc_ctx* ctx, const int* case_ids, int ncases);
 
@@ -155,11 +163,15 @@ int ffvhc_field_free(ffvhc_ctx* ctx, int field);
 int ffvhc_field_upload(ffvhc_ctx* ctx, int field, int case_id, int g, const void* padded);
 int ffvhc_field_download(ffvhc_ctx* ctx, int field, int case_id, int g, void* padded);
 /* Checkpoint in the reference's Dump2 layout (LocalScalar3D.cpp:215-264): six ints {nx, ny, nz,
- * vc, nE = 1, nB} then each block's padded (n+2vc)^3 FP64 array; g = vc.  The device keeps one
- * ghost layer: layers beyond the first are written as 0 (Load2 refills every ghost anyway).
- * The file name is the caller's (the reference: <path>/<prefix><label>-<rank:5>-<step:10>.bin). */
-int ffvhc_field_dump(ffvhc_ctx* ctx, int field, int case_id, int g, const char* path);
-/* Load2 (LocalScalar3D.cpp:266-342), same-resolution branch; then call ffvhc_ghost_fill. */
+ * vc, nE = 1, nB} then each block's padded (n+2vc)^3 FP64 array; g = vc (1 or 2).  As Dump2, the
+ * ghost layers are filled first (ImposeBoundaryCondition with BC group bc_group and updater 1 or
+ * 5: fine-to-coarse, same level, coarse-to-fine, then the outer BC over the whole padded face);
+ * cells none of these writes (block edges and corners away from outer faces) are written as 0,
+ * where the reference writes whatever they last held.  The file name is the caller's (the
+ * reference: <path>/<prefix><label>-<rank:5>-<step:10>.bin). */
+int ffvhc_field_dump(ffvhc_ctx* ctx, int field, int case_id, int g, int bc_group, int updater, const char* path);
+/* Load2 (LocalScalar3D.cpp:266-342): the same-resolution and the half-resolution (8^3 blocks,
+ * sf3d_copy_x2) branches; then call ffvhc_ghost_fill. */
 int ffvhc_field_load(ffvhc_ctx* ctx, int field, int case_id, const char* path);
 int ffvhc_dump_header(const char* path, int* head6);
 /* FFVILS::Fill (FFVILS.h:425-445): every cell and ghost = value */
@@ -241,6 +253,9 @@ int ffvhc_residual(ffvhc_ctx* ctx, int x, const int* A, int b, double* residual)
  * plus a coarse correction P A_c^-1 P^T (one value per block, A_c = P^T A P inverted densely),
  * 23-28 the same correction applied first and the sweeps on the remaining residual
  * (multiplicative two-level).
+ * Methods 2, 3 and 4 are experimental, kept for comparison only: on every workload measured here
+ * 2 is 4x slower than the FP64 Jacobi-BiCGSTAB and 3 / 4 stall near a relative residual of 1e-4
+ * (the FP32 recurrence loses the bi-orthogonality coefficient); the autotuner never picks them.
  * For 5 and 7-28 the answer is accepted on the FP64 residual of the FP64 matrix; a case above
  * eps restarts from it (inner_eps unused; omega is the GS / SOR relaxation factor).
  * inner_eps: relative reduction per correction; countMax caps each inner solve (a capped inner

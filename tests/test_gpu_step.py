@@ -444,9 +444,41 @@ def test_step_setup_reuse_follows_reference_point_and_selection(ctx, method):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("updater,group", [(1, ffvhc.BC_P), (5, ffvhc.BC_UX)])
+def test_dump2_ghost_layers_match_oracle(ctx, tmp_path, updater, group):
+    """Dump2 writes what ImposeBoundaryCondition leaves in both ghost layers: the file's padded
+    blocks equal the oracle's ghost fill (vc = 2; same level, F2C, C2F, outer D/N BCs over the whole
+    padded face) applied to the same interiors, on a three-level tree, bit for bit.  Cells the
+    update never writes (block edges and corners away from outer faces) are 0 in both."""
+    tree = trees()["two_level"] if updater == 1 else ffvhc.Tree(
+        root=(2, 2, 2), root_length=1.0, regions=[dict(kind="ball", a=(0.9, 1.1, 1.05), r=0.2, level=2)])
+    og = OracleGrid(tree)
+    # Dirichlet and Neumann faces with nonzero values
+    params = ffvhc.case_params(bc={group: [(0, 1.0), (1, 0.25), (0, -0.5), (1, 0.0), (1, 0.1), (0, 0.0)]})
+    cid = ctx.add_case(tree, params)
+    f1 = ctx.field(8)
+    try:
+        x = np.zeros(og.shape)
+        x[(slice(None),) + INTERIOR] = np.random.default_rng(13).standard_normal((og.nb, N, N, N))
+        ctx.upload(f1, cid, x)
+        path = tmp_path / "pX-00000-0000000001.bin"
+        ctx.dump(f1, cid, path, group=group, updater=updater)
+        raw = np.fromfile(path, dtype=np.float64, offset=24).reshape(og.shape)
+        want = x.copy()
+        og.ghost_fill(want, updater, [params.bct[group][f] for f in range(6)],
+                      [params.bcv[group][f] for f in range(6)])
+        assert np.array_equal(raw, want)
+        # the second layer is really written (not left at 0) on the outer faces and between blocks
+        assert np.count_nonzero(raw[:, 0, G:G + N, G:G + N]) > 0 and np.count_nonzero(raw[:, G:G + N, G:G + N, 0]) > 0
+    finally:
+        ctx.free(f1)
+        ctx.remove_case(cid)
+
+
 def test_checkpoint_round_trip(ctx, tmp_path):
-    """Dump2 of a field, then Load2 into another: interior and first ghost layer identical; the
-    file has the reference's layout (header + padded blocks)."""
+    """Dump2 of a field, then Load2 into another: the interior is identical, and after the ghost
+    fill of both (Load2 ends with ImposeBoundaryCondition) so is the first ghost layer; the file has
+    the reference's layout (header + padded blocks)."""
     tree = trees()["two_level"]
     og = OracleGrid(tree)
     cid = ctx.add_case(tree, ffvhc.case_params(bc=BC))
@@ -459,10 +491,15 @@ def test_checkpoint_round_trip(ctx, tmp_path):
         head = ffvhc.dump_header(path)
         assert list(head) == [N, N, N, G, 1, og.nb]
         raw = np.fromfile(path, dtype=np.float64, offset=24).reshape(og.shape)
-        assert np.array_equal(raw[:, LIVE], x[:, LIVE])
+        inner = (slice(None),) + INTERIOR
+        assert np.array_equal(raw[inner], x[inner])
         ctx.load(f2, cid, path)
+        ctx.ghost_fill(f1, ffvhc.BC_P, 1)
+        ctx.ghost_fill(f2, ffvhc.BC_P, 1)
         got = ctx.download(f2, cid, out=np.zeros(og.shape))
-        assert np.array_equal(got[:, LIVE], x[:, LIVE])
+        ref = ctx.download(f1, cid, out=np.zeros(og.shape))
+        assert np.array_equal(got[:, LIVE], ref[:, LIVE])
+        assert np.array_equal(raw[:, LIVE], ref[:, LIVE])  # the file's first ghost layer is the fill's
     finally:
         ctx.free(f1)
         ctx.free(f2)
