@@ -10,11 +10,11 @@
 // 1404-1537): a cell joins the fluid when a neighbour is fluid and its own face toward that
 // neighbour is not a phase boundary.
 //
-// Ghost updates: same-level neighbours copy the facing interior layers and outer faces take the
-// Neumann-0 copy of the first interior layer (boundaryTypeNULL).  At a coarse/fine face the ghost
-// cells keep their analytic cut values at the block's own resolution (the reference interpolates
-// the cut arrays there with its Updater1 instead); this matters only where the surface crosses a
-// level interface.
+// Ghost updates (the reference's ImposeBoundaryCondition of the cut arrays): fine-to-coarse means
+// and coarse-to-fine trilinear values at level interfaces (cut distances as reals, cut ids
+// truncated to int as the Scalar3DUpdater1 template does), same-level neighbours copy the facing
+// interior layers, outer faces take the Neumann-0 copy of the first interior layer
+// (boundaryTypeNULL).
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/ffvhc.h"
+#include "ghost.h"
 #include "octree.h"
 #include "runtime.h"
 
@@ -73,8 +74,112 @@ void face_ijk(int f, int a, int b, int l, int n, int& i, int& j, int& k) {
   }
 }
 
+// Coarse/fine ghost updates of the cut arrays, as ImposeBoundaryCondition does them: the cut
+// distances are LocalScalar3D<real> (Scalar3DUpdater1 with sup_copy_from_neighbor_f2c / _c2f,
+// sup.f90:54-179), the cut ids LocalScalar3D<int> (the Scalar3DUpdater1 template,
+// Scalar3DUpdater1.h:164-252 and 503-700: the same means and trilinear values computed in
+// double and truncated to int on assignment).  f2c: a coarse block's ghost layers from its finer
+// neighbours; c2f: a fine block's ghost layers from its coarser neighbour (interior values only).
+void face_cell_l(int f, int l, int a, int b, int n, int& i, int& j, int& k) {
+  // normal layer l counted from face f into the block (negative: ghost layers), position (a, b)
+  face_ijk(f, a, b, 0, n, i, j, k);
+  int* q = (f >> 1) == 0 ? &i : (f >> 1) == 1 ? &j : &k;
+  *q = (f & 1) ? n - 1 - l : l;
+}
+
+void exchange_f2c(Arrays& A, const std::vector<Block>& blk) {
+  const int n = A.n, g = A.g;
+  for_blocks(A.nb, [&](int b) {
+    const Block& B = blk[b];
+    for (int f = 0; f < 6; f++) {
+      if (B.ftype[f] != ffvhc::FACE_FINER) continue;
+      for (int s = 0; s < 4; s++) {
+        const int fb = B.nbr[f][s];
+        int oa, ob;
+        ffvhc::sub_origins_ab(f, s, oa, ob);
+        for (int l = 0; l < g; l++)
+          for (int bb = 0; bb < n / 2; bb++)
+            for (int aa = 0; aa < n / 2; aa++) {
+              int i, j, k, fi, fj, fk, gi, gj, gk;
+              face_cell_l(f, -1 - l, aa + (n / 2) * oa, bb + (n / 2) * ob, n, i, j, k);
+              face_cell_l(f ^ 1, 2 * l, 2 * aa, 2 * bb, n, fi, fj, fk);
+              face_cell_l(f ^ 1, 2 * l + 1, 2 * aa + 1, 2 * bb + 1, n, gi, gj, gk);
+              const int x = std::min(fi, gi), y = std::min(fj, gj), z = std::min(fk, gk);
+              const size_t dst = A.at(b, i, j, k);
+              auto at = [&](int di, int dj, int dk) { return A.at(fb, x + di, y + dj, z + dk); };
+              const size_t q[8] = {at(0, 0, 0), at(1, 0, 0), at(0, 1, 0), at(0, 0, 1),
+                                   at(0, 1, 1), at(1, 0, 1), at(1, 1, 0), at(1, 1, 1)};
+              for (int d = 0; d < 6; d++) {
+                double sc = 0.0;
+                int si = 0;
+                for (int m = 0; m < 8; m++) {  // sup order: nnn pnn npn nnp npp pnp ppn ppp
+                  sc += A.c[d][q[m]];
+                  si += A.id[d][q[m]];
+                }
+                A.c[d][dst] = 0.125 * sc;
+                A.id[d][dst] = (int)(0.125 * (double)si);
+              }
+            }
+      }
+    }
+  });
+}
+
+// linear-interpolation index of the Scalar3DUpdater1 template on a 0-based doubled index
+void lin_index(int i, int n, int& I, double& r) {
+  I = std::min(std::max(i / 2 - 1 + i % 2, 0), n - 2);
+  r = -0.25 + 0.5 * i - (double)I;
+}
+
+void exchange_c2f(Arrays& A, const std::vector<Block>& blk) {
+  const int n = A.n, g = A.g;
+  for_blocks(A.nb, [&](int b) {
+    const Block& B = blk[b];
+    for (int f = 0; f < 6; f++) {
+      if (B.ftype[f] != ffvhc::FACE_COARSER) continue;
+      const int cb = B.nbr[f][0];
+      int oa, ob, ax_a, ax_b;
+      ffvhc::sub_origins_ab(f, B.sub[f], oa, ob);
+      ffvhc::face_axes(f, ax_a, ax_b);
+      const int nrm = f >> 1;
+      for (int l = 0; l < g; l++)
+        for (int bb = 0; bb < n; bb++)
+          for (int aa = 0; aa < n; aa++) {
+            int i, j, k;
+            face_cell_l(f, -1 - l, aa, bb, n, i, j, k);
+            // 0-based doubled index of the ghost cell in the coarse block's frame
+            int dd[3];
+            dd[nrm] = (f & 1) ? l : 2 * n - 1 - l;
+            dd[ax_a] = aa + n * oa;
+            dd[ax_b] = bb + n * ob;
+            int I[3];
+            double w[3];
+            for (int q = 0; q < 3; q++) lin_index(dd[q], n, I[q], w[q]);
+            const double r = w[0], sw = w[1], t = w[2];
+            const size_t dst = A.at(b, i, j, k);
+            auto at = [&](int di, int dj, int dk) { return A.at(cb, I[0] + di, I[1] + dj, I[2] + dk); };
+            const size_t c000 = at(0, 0, 0), c100 = at(1, 0, 0), c010 = at(0, 1, 0), c110 = at(1, 1, 0);
+            const size_t c001 = at(0, 0, 1), c101 = at(1, 0, 1), c011 = at(0, 1, 1), c111 = at(1, 1, 1);
+            for (int d = 0; d < 6; d++) {
+              auto tri = [&](auto V) {
+                return (1.0 - t) * ((1.0 - sw) * ((1.0 - r) * V(c000) + r * V(c100)) +
+                                    sw * ((1.0 - r) * V(c010) + r * V(c110))) +
+                       t * ((1.0 - sw) * ((1.0 - r) * V(c001) + r * V(c101)) +
+                            sw * ((1.0 - r) * V(c011) + r * V(c111)));
+              };
+              const std::vector<double>& cv = A.c[d];
+              const std::vector<int>& iv = A.id[d];
+              A.c[d][dst] = tri([&](size_t m) { return cv[m]; });
+              A.id[d][dst] = (int)tri([&](size_t m) { return (double)iv[m]; });
+            }
+          }
+    }
+  });
+}
+
 void exchange(Arrays& A, const std::vector<Block>& blk) {
   const int n = A.n, g = A.g;
+  exchange_f2c(A, blk);
   for_blocks(A.nb, [&](int b) {
     for (int f = 0; f < 6; f++) {
       const Block& B = blk[b];
@@ -99,6 +204,7 @@ void exchange(Arrays& A, const std::vector<Block>& blk) {
           }
     }
   });
+  exchange_c2f(A, blk);
 }
 
 }  // namespace

@@ -434,9 +434,11 @@ __global__ __launch_bounds__(GSK_NT) __attribute__((amdgpu_waves_per_eu(REG == 1
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
       const unsigned o = o0 + NF * (k0 + kk);
-      rv[kk] = rb[o];
-      if (MODE == 1 || !first) qv[kk] = qb[o];
-      if (MODE == 0 && !first) pv[kk] = vb[o];
+      // streamed once through this kernel: non-temporal loads and stores leave the caches to what
+      // the next kernels read (measured: this kernel 3% faster, the matvecs after it 3-7%)
+      rv[kk] = __builtin_nontemporal_load(rb + o);
+      if (MODE == 1 || !first) qv[kk] = __builtin_nontemporal_load(qb + o);
+      if (MODE == 0 && !first) pv[kk] = __builtin_nontemporal_load(vb + o);
       if (REG == 0 || (REG == 2 && !cls)) rc[kk] = recb[o];
     }
 #pragma unroll
@@ -452,7 +454,7 @@ __global__ __launch_bounds__(GSK_NT) __attribute__((amdgpu_waves_per_eu(REG == 1
         v = first ? rv[kk] : rv[kk] + beta * (pv[kk] + ng * qv[kk]);
       else
         v = na * qv[kk] + rv[kk];
-      vb[o0 + NF * k] = v;
+      __builtin_nontemporal_store(v, vb + o0 + NF * k);
       vsum += v;
       double uk = v * (double)__uint_as_float(rc[kk].x);
       const int ka = kb + k;
@@ -545,7 +547,8 @@ __global__ __launch_bounds__(GSK_NT) __attribute__((amdgpu_waves_per_eu(REG == 1
   unsigned oo = o0;  // opaque: the store offsets are formed here, not held across the sweeps
   asm volatile("" : "+v"(oo));
 #pragma unroll
-  for (int k = 0; k < GSK_NK; k++) out[oo + NF * k] = MUL ? z[oo + NF * k] + eo : z[oo + NF * k];
+  for (int k = 0; k < GSK_NK; k++)
+    __builtin_nontemporal_store(MUL ? z[oo + NF * k] + eo : z[oo + NF * k], out + oo + NF * k);
   double* h = MODE == 0 ? V.hpp : V.hss;
   push_halo<double, GSK_NT>(
       B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return MUL ? z[zi(i, j, k)] + eo : z[zi(i, j, k)]; }, h,

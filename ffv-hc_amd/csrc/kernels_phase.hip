@@ -3,15 +3,22 @@
 // boundaries; a move into a cell is blocked when the entered cell's own face toward the cell it
 // comes from carries the boundary id.
 //
+// The reference repeats block sweeps in which a cell takes the seed's id when a neighbour (a ghost
+// across a block face) holds it and the cell's own face toward that neighbour is not a boundary,
+// refreshing the CellFlag ghosts by Scalar3DUpdater6 between sweeps: a coarse ghost takes the id
+// when any of the 2 x 2 fine cells along the interface holds it (interpolateF2C_X_M ..
+// Scalar3DUpdater6.h:286-376), a fine ghost when the coarse cell holding it does
+// (interpolateC2F, 378-387).
+//
 // Labels: one 16-bit mask per (slot, column c = i + 16 j), bit k = cell (i, j, k) reached.  A pass
 // runs one 256-thread workgroup per block (thread = column): it pulls the labels of the
-// neighbouring blocks' facing cells (as the previous passes left them) by the same transitions
-// the host search makes across block faces -- same level: the facing cell; from a coarser
-// neighbour: the coarse cell whose face footprint holds the fine cell; from finer neighbours:
-// the 2 x 2 fine cells under the coarse cell -- then closes the block locally in LDS (x and y
-// through the neighbouring columns, k through the column's own bits) and stores the labels.  Labels
-// only ever go from 0 to 1 and every move is the host search's, so repeating passes until one
-// changes nothing gives exactly the set the host breadth-first search reaches.
+// neighbouring blocks' facing cells (as the previous passes left them) by those same rules --
+// same level: the facing cell; from a coarser neighbour: the coarse cell whose face footprint
+// holds the fine cell; from finer neighbours: the 2 x 2 fine cells under the coarse cell -- then
+// closes the block locally in LDS (x and y through the neighbouring columns, k through the
+// column's own bits) and stores the labels.  Labels only ever go from 0 to 1, so repeating passes
+// until one changes nothing gives the same fixed point as the reference's sweeps (and as the host
+// geometry's breadth-first search, which moves by the same rules).
 #include "kcommon.h"
 #include "kernels.h"
 

@@ -112,3 +112,55 @@ def test_bstl_postprocessing_matches_oracle(radius, center):
     assert np.array_equal(cid[:, 0][:, live], oid[:, live])
     assert np.array_equal(c[:, 0][:, live], oc[:, live])
     assert counts[2] == n1 and counts[3] == n2
+
+
+def test_cut_array_ghosts_follow_the_level_updaters():
+    """The cut arrays' ghost cells after the front end are what the reference's ImposeBoundaryCondition
+    leaves there, on a tree whose sphere surface crosses a level interface: cut distances as the
+    oracle's Updater1 fill of the interior (F2C means, same level, C2F trilinear, Neumann 0 outside),
+    cut ids the same values in double truncated to int (the Scalar3DUpdater1 template on
+    LocalScalar3D<int>, Scalar3DUpdater1.h:164-252, 503-700).  Face ghosts of both layers; edges and
+    corners are not updated by the reference either."""
+    tree = ffvhc.Tree(root=(2, 2, 2), root_length=1.0, regions=[dict(kind="ball", a=(0.6, 0.6, 0.6), r=0.35,
+                                                                      level=1)])
+    assert tree.max_level == 1
+    c, cid, pid, counts = tree.geometry_sphere((0.8, 0.85, 0.75), 0.47, cutoff=0.0, hole_filling=(0, 0, 0))
+    tb = tree.table()
+    tb["g"] = G
+    tab = dict(tb)
+    tab.update(ftype=tb["ftype"].ravel(), nbr=tb["nbr"].ravel(), sub=tb["sub"].ravel(), org=tb["org"].ravel())
+    grid = O.make_grid(tab)
+    L = O.lib()
+    shp = (tree.nblk, NP, NP, NP)
+    idx = np.arange(NP) - G
+    inside = (idx >= 0) & (idx < N)
+    nout = (~inside)[:, None, None].astype(int) + (~inside)[None, :, None] + (~inside)[None, None, :]
+    face_ghost = nout == 1
+    interior_m = nout == 0
+    crossing = 0
+    for f in range(6):
+        cf = c[f].reshape(shp)
+        want = np.zeros(shp)
+        want[:, interior_m] = cf[:, interior_m]
+        L.orc_ghost_fill(ctypes_ref(grid), O.ptr(want), 1, O.cint6([1] * 6), O.cdbl6([0.0] * 6))
+        assert np.array_equal(cf[:, face_ghost], want[:, face_ghost]), f
+        idf = cid[f].reshape(shp)
+        wi = np.zeros(shp)
+        wi[:, interior_m] = idf[:, interior_m]
+        L.orc_ghost_fill(ctypes_ref(grid), O.ptr(wi), 1, O.cint6([1] * 6), O.cdbl6([0.0] * 6))
+        assert np.array_equal(idf[:, face_ghost], np.trunc(wi[:, face_ghost]).astype(np.int32)), f
+        # the surface reaches ghosts at level interfaces: interpolated (not float32) cut values there
+        for b in range(tree.nblk):
+            for ff in range(6):
+                if tb["ftype"][b, ff] in (2, 3):
+                    ax = 2 - (ff >> 1)
+                    sl = [slice(G, G + N)] * 3
+                    sl[ax] = slice(G + N, NP) if ff & 1 else slice(0, G)
+                    v = cf[(b,) + tuple(sl)]
+                    crossing += int(np.count_nonzero((v != 1.0) & (v != v.astype(np.float32))))
+    assert crossing > 0
+
+
+def ctypes_ref(grid):
+    import ctypes
+    return ctypes.byref(grid)
