@@ -753,21 +753,23 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
-      pv[kk] = V.pp[g];
-      sv_[kk] = V.ss[g];
+      // the last reads of p_, s_, t_ and s in the iteration, and x's only one: non-temporal
+      // (measured 3% faster; r0, read every iteration, stays cached)
+      pv[kk] = __builtin_nontemporal_load(V.pp + g);
+      sv_[kk] = __builtin_nontemporal_load(V.ss + g);
       if constexpr (CC) {
         pv[kk] += (T)ce[slot];
         sv_[kk] += (T)ce[nslots + slot];
       }
-      xv[kk] = x[g];
-      tv[kk] = V.t[g];
-      sv[kk] = V.s[g];
+      xv[kk] = __builtin_nontemporal_load(x + g);
+      tv[kk] = __builtin_nontemporal_load(V.t + g);
+      sv[kk] = __builtin_nontemporal_load(V.s + g);
       r0v[kk] = HR ? (T)shadow_value(local, (uint32_t)(g - base)) : V.r0[g];
     }
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
       const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
-      x[g] = alpha * pv[kk] + gamma * sv_[kk] + xv[kk];  // AXPBYPZ(x, p_, s_, alpha, gamma)
+      __builtin_nontemporal_store(alpha * pv[kk] + gamma * sv_[kk] + xv[kk], x + g);  // AXPBYPZ(x, p_, s_, alpha, gamma)
       const T r = ng * tv[kk] + sv[kk];                  // AXPYZ(r, t_, s, -gamma)
       V.r[g] = r;
       rs += (double)r;
