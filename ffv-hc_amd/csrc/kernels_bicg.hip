@@ -729,21 +729,23 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
   if (st[c].done) return;
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
-  double gd;
-  if (tpart) {
-    const DevCase& C = cs[c];
-    double t0, t1;
-    case_sum2<BI_NT>(tpart, tpart + nslots, C.slot_begin, C.nblk, red, t0, t1);
-    if (t == 0) {
-      gshared = t0 / t1;
-      if (slot == C.slot_begin) st[c].gamma = gshared;
+  // gamma: formed after the first batch of loads is issued (the partials' round trips then
+  // overlap the block's own loads)
+  auto form_gamma = [&]() -> double {
+    if (tpart) {
+      const DevCase& C = cs[c];
+      double t0, t1;
+      case_sum2<BI_NT>(tpart, tpart + nslots, C.slot_begin, C.nblk, red, t0, t1);
+      if (t == 0) {
+        gshared = t0 / t1;
+        if (slot == C.slot_begin) st[c].gamma = gshared;
+      }
+      __syncthreads();
+      return gshared;
     }
-    __syncthreads();
-    gd = gshared;
-  } else {
-    gd = st[c].gamma;
-  }
-  const T alpha = (T)st[c].alpha, gamma = (T)gd, ng = (T)(-gd);
+    return st[c].gamma;
+  };
+  T alpha = 0, gamma = 0, ng = 0;
   const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
   constexpr int KB = Batch<T>::K;
   double rr = 0.0, rr1 = 0.0, rs = 0.0;
@@ -765,6 +767,12 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
       tv[kk] = __builtin_nontemporal_load(V.t + g);
       sv[kk] = __builtin_nontemporal_load(V.s + g);
       r0v[kk] = HR ? (T)shadow_value(local, (uint32_t)(g - base)) : V.r0[g];
+    }
+    if (k0 == 0) {
+      const double gd = form_gamma();
+      alpha = (T)st[c].alpha;
+      gamma = (T)gd;
+      ng = (T)(-gd);
     }
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
