@@ -121,6 +121,93 @@ __device__ inline void converged(SolveState& s, double rr, int count) {
   }
 }
 
+// case_sum2<256> formed by one wavefront: lane l stands for the threads l, l + 64, l + 128 and
+// l + 192 of the 256-thread form (the same strided sums in the same order, the same butterfly per
+// wave, the four waves added in order), so the result has the same bits; valid in every lane.
+__device__ inline void case_sum2_wave(const double* __restrict__ p0, const double* __restrict__ p1, int begin,
+                                      int count, double& s0, double& s1) {
+  const int lane = threadIdx.x & 63;
+  double v0[4] = {0.0, 0.0, 0.0, 0.0}, v1[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int q0 = 0; q0 < count; q0 += 256) {  // loads issued four rounds at a time
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+      const int q = q0 + lane + 64 * w;
+      if (q < count) {
+        v0[w] += p0[begin + q];
+        v1[w] += p1[begin + q];
+      }
+    }
+  }
+  s0 = 0.0;
+  s1 = 0.0;
+#pragma unroll
+  for (int w = 0; w < 4; w++) {
+    for (int off = 32; off > 0; off >>= 1) {
+      v0[w] += __shfl_xor(v0[w], off, 64);
+      v1[w] += __shfl_xor(v1[w], off, 64);
+    }
+    s0 += v0[w];
+    s1 += v1[w];
+  }
+}
+
+// The end of a BiCGSTAB iteration (FFVILS.h:925-965): IsConverged on t0 = r.r, then the next
+// rho t1 = r.r0 and beta = (t1 / rho_n) (alpha / gamma), rho_n being this iteration's rho.  The
+// evaluation reads only fields of S that no kernel writes while it runs, so any workgroup can form
+// it; fin_bi_xr_store writes it as the per-case finalisation does.
+struct FinXr {
+  double rr, residual, rr1, beta;
+  int count, done, status;
+};
+
+__device__ inline FinXr fin_bi_xr_eval(const SolveState& S, double t0, double t1, int n, double rho_n) {
+  FinXr f;
+  f.rr = t0;
+  const double bb = S.bb;
+  f.residual = (fabs(bb) > FLT_MIN) ? sqrt(fabs(t0) / fabs(bb)) : sqrt(fabs(t0));
+  f.count = n;
+  f.done = 0;
+  f.status = 0;
+  if (f.residual <= S.eps) {
+    f.done = 1;
+  } else if (isnan(f.residual) || isinf(f.residual)) {
+    f.done = 1;
+    f.status = 1;
+  } else if (n >= S.countMax) {
+    f.done = 1;
+    f.count = S.countMax + 1;
+  }
+  f.rr1 = 0.0;
+  f.beta = 0.0;
+  if (!f.done) {
+    if (fabs(t1) < FLT_MIN) {  // next iteration's r.r0 test (FFVILS.h:829-834)
+      f.residual = t1;
+      f.count = 0;
+      f.done = 1;
+    } else {
+      f.rr1 = t1;
+      f.beta = t1 / rho_n * S.alpha / S.gamma;
+    }
+  }
+  return f;
+}
+
+__device__ inline void fin_bi_xr_store(SolveState& S, const FinXr& f, int n, double rho_n) {
+  S.iter = n;
+  S.rr = f.rr;
+  S.residual = f.residual;
+  S.count = f.count;
+  if (f.status) S.status = 1;
+  S.rr0 = rho_n;  // this iteration's rho becomes the previous one (also kept for a reliable update)
+  if (f.done) {
+    S.done = 1;
+  } else {
+    S.rr1 = f.rr1;
+    S.beta = f.beta;
+  }
+}
+
 
 // Neighbour slot a face position (a, b) of face f takes its ghost value from.
 __device__ inline int ghost_source(const DevBlock& B, int slot, int f, int a, int b) {

@@ -140,10 +140,11 @@ void launch_gjw_put_rows(hipStream_t sm, double* M, int n, int k0, int kb, const
 void launch_to_f32(hipStream_t sm, const double* x, float* y, int n, int np);
 void launch_coarse_apply(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                          const SolveState* st, const float* inv, const long long* off, const double* c, double* e);
-// mode 1 also forms alpha from the Q partials apart (and stores it in the case state)
+// mode 1 also forms alpha from the Q partials apart (and stores it in the case state); mode 0
+// with xpart also forms and stores the finalisation of iteration count - 1 from its XR partials
 void launch_coarse_apply_mul(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                              SolveState* st, const float* inv, const long long* off, double* ce, size_t nslots,
-                             int mode, const double* apart);
+                             int mode, const double* apart, int count = 0, const double* xpart = nullptr);
 
 void launch_predict(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                     const StepFields& F, const float* cut, const uint32_t* flag, size_t capnc);

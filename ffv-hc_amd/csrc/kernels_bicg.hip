@@ -714,7 +714,9 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
 // tpart: the T stage's block partials (t_.s, t_.t_); every workgroup adds its case's partials in
 // the fixed order of the per-case finalisation and takes gamma = (t_.s) / (t_.t_)
 // (FFVILS.h:905-915) itself, the case's first block also storing it in the case state, so no
-// finalisation launch sits between T and XR.
+// finalisation launch sits between T and XR.  The case's first block also copies this
+// iteration's rho (rr1) to rr0, as the iteration's finalisation does, so that a finalisation formed
+// inside the next kernel can read it there while rr1 is rewritten.
 template <class T, bool HR = false, bool CC = false>
 __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ active, const DevBlock* __restrict__ blk,
                                                       const DevCase* __restrict__ cs, SolveState* __restrict__ st,
@@ -729,6 +731,7 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
   if (st[c].done) return;
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
+  if (t == 0 && slot == cs[c].slot_begin) st[c].rr0 = st[c].rr1;
   // gamma: formed after the first batch of loads is issued (the partials' round trips then
   // overlap the block's own loads)
   auto form_gamma = [&]() -> double {
@@ -882,8 +885,12 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
                           double* part, size_t nslots, int count, int mask, const BiOpts& o) {
   // mask bits 0-4 select P, Q, S, T, XR; bit 5 leaves out the per-case finalisations; bit 8
   // splits the coarse corrections of the two-level methods into stages of their own (bits 6, 7).
+  // Multiplicative two-level methods (count > 0): bit 9 has this iteration's first coarse
+  // application form the previous iteration's finalisation from the XR partials still in part[0..1]
+  // (whose launch bit 10 left out).
   if (!nact) return;
   const bool dbl = sizeof(T) == 8;
+  if (!(dbl && o.gs == BiOpts::GSC && o.cinv && o.mult && count > 0) || (mask & 256)) mask &= ~(512 | 1024);
   BiVecs<double>& Vd = *reinterpret_cast<BiVecs<double>*>(&V);
   double* const part0 = part;
   auto precond = [&](auto mode_tag) {
@@ -969,7 +976,7 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
     if (!(dbl && o.gs == BiOpts::GSC && o.cinv)) return;
     if (o.mult)  // mode 1 forms alpha from the Q partials (no alpha finalisation)
       launch_coarse_apply_mul(sm, active, nact, blk, cs, st, o.cinv, o.coff, o.ce, nslots, mode,
-                              mode == 1 ? part0 : nullptr);
+                              mode == 1 ? part0 : nullptr, count, mode == 0 && (mask & 512) ? part0 : nullptr);
     else
       launch_coarse_apply(sm, active, nact, blk, cs, st, o.cinv, o.coff, part + 4 * nslots, o.ce + mode * nslots);
   };
@@ -1006,7 +1013,7 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
       xr(std::true_type{});
     else
       xr(std::false_type{});
-    if (!(mask & 32)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_XR, count);
+    if (!(mask & 32) && !(mask & 1024)) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_XR, count);
   }
 }
 

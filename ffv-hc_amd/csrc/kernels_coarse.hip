@@ -383,11 +383,14 @@ struct CoarseRhs {
   bool first;
   double beta, ng, na;
   int mode;
-  // mode 1 takes alpha from the caller (formed in the kernel), mode 0 reads its scalars from S
-  __device__ CoarseRhs(const double* ce, size_t nslots, const DevCase& C, const SolveState& S, int m, double alpha)
+  // mode 1 takes alpha from the caller (formed in the kernel); mode 0 reads its scalars from S,
+  // or (fused, the previous iteration's finalisation formed in the kernel) takes beta from the
+  // caller and tells the first iteration by count, S.iter and S.beta being rewritten meanwhile
+  __device__ CoarseRhs(const double* ce, size_t nslots, const DevCase& C, const SolveState& S, int m, double alpha,
+                       bool fused = false, double fbeta = 0.0, int count = 0)
       : br(ce + 2 * nslots + C.slot_begin), bq(ce + 3 * nslots + C.slot_begin), bp(ce + 4 * nslots + C.slot_begin),
-        first(m == 0 && (S.iter == 0 || S.restart)), beta(m == 0 ? S.beta : 0.0), ng(m == 0 ? -S.gamma : 0.0),
-        na(-alpha), mode(m) {}
+        first(m == 0 && ((fused ? count == 1 : S.iter == 0) || S.restart)),
+        beta(m == 0 ? (fused ? fbeta : S.beta) : 0.0), ng(m == 0 ? -S.gamma : 0.0), na(-alpha), mode(m) {}
   __device__ double operator()(int b) const {
     return mode == 0 ? (first ? br[b] : br[b] + beta * (bp[b] + ng * bq[b])) : na * bq[b] + br[b];
   }
@@ -397,6 +400,12 @@ struct CoarseRhs {
 // alpha = rho / (q_.r0) (FFVILS.h:868-872) itself with a fixed-order wave sum (the same value in
 // every wave of the case, whatever else the launch holds) and the wave of the case's first block
 // stores it, so no alpha finalisation launch is needed.
+// mode 0 with xpart (the XR partials of iteration count - 1, r.r and r.r0): every wave forms that
+// iteration's finalisation (IsConverged, the next rho and beta) itself, with the per-case
+// finalisation's sums in its order (the same bits), from fields of the case state that nothing
+// rewrites during this launch (rho_n in rr0, put there by XR); the wave of the case's first block
+// stores it as the finalisation would, so no finalisation launch sits between XR and this one.  A
+// case that converged there takes no correction.
 __global__ __launch_bounds__(CA_NT) void coarse_apply_mul_kernel(const int* __restrict__ active, int nact,
                                                                  const DevBlock* __restrict__ blk,
                                                                  const DevCase* __restrict__ cs,
@@ -404,8 +413,10 @@ __global__ __launch_bounds__(CA_NT) void coarse_apply_mul_kernel(const int* __re
                                                                  const float* __restrict__ inv,
                                                                  const long long* __restrict__ off,
                                                                  double* __restrict__ ce, size_t nslots, int mode,
-                                                                 const double* __restrict__ apart) {
+                                                                 const double* __restrict__ apart, int count,
+                                                                 const double* __restrict__ xpart) {
   __shared__ double cl[CA_MAXN];
+  __shared__ double fred[2 * (CA_NT / 64)], fsum[2];
   constexpr int R = CA_NT / 64;  // rows per workgroup
   const int first_row = blockIdx.x * R;
   const int last_row = min(first_row + R - 1, nact - 1);
@@ -413,7 +424,8 @@ __global__ __launch_bounds__(CA_NT) void coarse_apply_mul_kernel(const int* __re
   // share a case stages that case's c once in LDS
   const int c0 = blk[active[first_row]].case_id;
   const int n0 = cs[c0].nblk;
-  const bool uni = c0 == blk[active[last_row]].case_id && coarse_pitch(n0) <= CA_MAXN && !st[c0].done;
+  const bool same = c0 == blk[active[last_row]].case_id && coarse_pitch(n0) <= CA_MAXN;  // uniform per workgroup
+  const bool uni = same && !st[c0].done;
   const int w = first_row + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   // this wave's row: its first float4s are loaded before c is staged, so the two memory round
   // trips overlap
@@ -443,12 +455,40 @@ __global__ __launch_bounds__(CA_NT) void coarse_apply_mul_kernel(const int* __re
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     alpha = st[acid].rr1 / v;
   }
-  if (uni) {
-    const CoarseRhs cb(ce, nslots, cs[c0], st[c0], mode, alpha);
+  // mode 0 with xpart: the previous iteration's finalisation.  A one-case workgroup forms the sums
+  // with all its threads as the finalisation does (its barriers taken by every thread, whatever the
+  // case's state); a workgroup that straddles cases forms them per wave.
+  const bool fused = mode == 0 && xpart;
+  double fbeta = 0.0;
+  bool fdone = false;
+  if (fused) {
+    double t0 = 0.0, t1 = 0.0;
+    if (same) {
+      case_sum2<CA_NT>(xpart, xpart + nslots, cs[c0].slot_begin, n0, fred, t0, t1);
+      if (threadIdx.x == 0) {
+        fsum[0] = t0;
+        fsum[1] = t1;
+      }
+      __syncthreads();
+      t0 = fsum[0];
+      t1 = fsum[1];
+    } else if (w < nact) {
+      case_sum2_wave(xpart, xpart + nslots, cs[acid].slot_begin, cs[acid].nblk, t0, t1);
+    }
+    if ((same || w < nact) && !st[acid].done) {
+      const double rho_n = st[acid].rr0;
+      const FinXr f = fin_bi_xr_eval(st[acid], t0, t1, count - 1, rho_n);
+      fbeta = f.beta;
+      fdone = f.done != 0;
+      if (w < nact && slot == cs[acid].slot_begin && lane == 0) fin_bi_xr_store(st[acid], f, count - 1, rho_n);
+    }
+  }
+  if (uni && !fdone) {
+    const CoarseRhs cb(ce, nslots, cs[c0], st[c0], mode, alpha, fused, fbeta, count);
     for (int b = threadIdx.x; b < coarse_pitch(n0); b += CA_NT) cl[b] = b < n0 ? cb(b) : 0.0;
   }
   __syncthreads();
-  if (w >= nact) return;
+  if (w >= nact || fdone) return;
   SolveState& S = st[cid];
   if (S.done) return;
   if (mode == 1 && a == 0 && lane == 0) {  // the case's first block stores alpha (FIN_BI_ALPHA)
@@ -459,7 +499,7 @@ __global__ __launch_bounds__(CA_NT) void coarse_apply_mul_kernel(const int* __re
   if (uni) {
     s = coarse_row_dot_pre(row, n, lane, pre, [&](int b) { return cl[b]; });
   } else {
-    const CoarseRhs cb(ce, nslots, C, S, mode, alpha);
+    const CoarseRhs cb(ce, nslots, C, S, mode, alpha, fused, fbeta, count);
     s = coarse_row_dot(row, n, lane, [&](int b) { return b < n ? cb(b) : 0.0; });
   }
   if (lane == 0) ce[mode * nslots + slot] = s;
@@ -467,10 +507,10 @@ __global__ __launch_bounds__(CA_NT) void coarse_apply_mul_kernel(const int* __re
 
 void launch_coarse_apply_mul(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
                              SolveState* st, const float* inv, const long long* off, double* ce, size_t nslots,
-                             int mode, const double* apart) {
+                             int mode, const double* apart, int count, const double* xpart) {
   if (nact)
     hipLaunchKernelGGL(coarse_apply_mul_kernel, dim3((nact + CA_NT / 64 - 1) / (CA_NT / 64)), dim3(CA_NT), 0, sm,
-                       active, nact, blk, cs, st, inv, off, ce, nslots, mode, apart);
+                       active, nact, blk, cs, st, inv, off, ce, nslots, mode, apart, count, xpart);
 }
 
 void launch_coarse_apply(hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,

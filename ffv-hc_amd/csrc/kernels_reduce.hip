@@ -59,19 +59,8 @@ __global__ __launch_bounds__(FIN_NT) void finalize_kernel(const int* __restrict_
       break;
     case FIN_BI_XR: {  // FFVILS.h:925-965: t0 = r.r (IsConverged), t1 = next r.r0
       const int n = arg > 0 ? arg : S.iter + 1;  // arg 0: count kept on the device
-      S.iter = n;
-      converged(S, t0, n);
-      S.rr0 = S.rr1;  // this iteration's rho becomes the previous one (also kept for a reliable update)
-      if (!S.done) {
-        if (fabs(t1) < FLT_MIN) {  // next iteration's r.r0 test (FFVILS.h:829-834)
-          S.residual = t1;
-          S.count = 0;
-          S.done = 1;
-        } else {
-          S.rr1 = t1;
-          S.beta = t1 / S.rr0 * S.alpha / S.gamma;
-        }
-      }
+      const double rho_n = S.rr1;
+      fin_bi_xr_store(S, fin_bi_xr_eval(S, t0, t1, n, rho_n), n, rho_n);
       break;
     }
     case FIN_CG_INIT:  // FFVILS.h:679-722: t0 = b.b, t1 = r.z

@@ -120,6 +120,8 @@ struct ffvhc_ctx {
   // moved and the method, matrix fields, active blocks and options are the same
   // (FFVHC_OPTION_SETUP_REUSE).
   bool reuse_setup = true;
+  // the multiplicative methods form an iteration's finalisation inside the next coarse application
+  bool fuse_fin = true;
   bool step_solve = false;  // set by the time step around its pressure solve
   // the step's reference-pressure point (bcut_set_reference_value rewrites the rows of the cell
   // holding it and of its neighbours, so the matrix depends on it)

@@ -202,8 +202,13 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
   bool hinted = hint && *hint > 0;
   int n = 1, batch = hinted ? std::min(*hint + 1, kBatchMax) : kBatchMin;
   bool done = false;
+  // multiplicative two-level methods: inside a batch, each iteration's finalisation runs in the
+  // next iteration's first coarse application; the batch's last iteration has its own launch, so
+  // the state the host reads after the batch is complete
+  const bool fuse = c->fuse_fin && !exec && sizeof(T) == 8 && opts.gs == BiOpts::GSC && opts.cinv && opts.mult;
   while (!done) {
     int last = std::min(n + batch - 1, countMax);
+    const int first = n;
     for (; n <= last; n++) {
       if (exec) {
         const hipError_t e = hipGraphLaunch(exec, c->sm);
@@ -212,8 +217,9 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
           return fail(-2, std::string("graph launch: ") + hipGetErrorString(e));
         }
       } else {
+        const int mask = 31 | (fuse && n > first ? 512 : 0) | (fuse && n < last ? 1024 : 0);
         launch_bicgstab_iter<T>(c->sm, c->dactive, c->nactive, c->dcaseact, c->ncaseact, c->dblk, c->dcase, st, K,
-                                (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, 31, opts);
+                                (T*)X.d[0], V, c->dpart, (size_t)c->cap, n, mask, opts);
       }
     }
     HIPCHK(hipGetLastError());
@@ -1028,6 +1034,10 @@ int ffvhc_ctx_set_option(ffvhc_ctx* c, int option, int value) {
   if (option == FFVHC_OPTION_SETUP_REUSE) {
     c->reuse_setup = value != 0;
     c->psetup.valid = false;
+    return 0;
+  }
+  if (option == FFVHC_OPTION_FUSE_FIN) {
+    c->fuse_fin = value != 0;
     return 0;
   }
   if (option == FFVHC_OPTION_COARSE_GEMM) {

@@ -315,6 +315,12 @@ class Context:
         """Smallest case (blocks) whose coarse operator is inverted with GEMM panels (default 2048)."""
         check(_L.ffvhc_ctx_set_option(self.h, 5, int(nblk)), "ffvhc_ctx_set_option")
 
+    def set_fuse_fin(self, on):
+        """Let the multiplicative two-level methods (23-28) form each iteration's end (convergence
+        test, next rho, beta) in the next iteration's first coarse application (default), or give
+        it a launch of its own every iteration."""
+        check(_L.ffvhc_ctx_set_option(self.h, 6, int(bool(on))), "ffvhc_ctx_set_option")
+
     def set_classes(self, on):
         """Let the compact-record solve methods (11-28) read the bands of blocks that are uniform
         per stencil position class from 27 rows (default), or read every block per cell."""

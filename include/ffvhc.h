@@ -123,6 +123,11 @@ long long ffvhc_ctx_device_bytes(const ffvhc_ctx* ctx);
  * vector form is as fast, at 4,048 the GEMM form takes 10 ms instead of 22, at 8,512 41 instead of
  * 182).  Both are Gauss-Jordan without pivoting; their inverses differ in rounding only. */
 #define FFVHC_OPTION_COARSE_GEMM 5
+/* FFVHC_OPTION_FUSE_FIN: 1 (default) lets the multiplicative two-level methods (23-28) form the
+ * end of each BiCGSTAB iteration (convergence test, next rho and beta) inside the next iteration's
+ * first coarse application instead of a launch of its own, except after the last iteration of a
+ * launch batch; 0 launches it every iteration.  The results are identical. */
+#define FFVHC_OPTION_FUSE_FIN 6
 int ffvhc_ctx_set_option(ffvhc_ctx* ctx, int option, int value);
 
 /* Physical parameters and outer BCs of one flow case.  BC groups: 0 UX, 1 UY, 2 UZ, 3 P,

@@ -467,6 +467,51 @@ def test_launch_batch_hint_changes_nothing(ctx, method):
         ctx.remove_case(cid)
 
 
+@pytest.mark.parametrize("method", [23, 26, 28])
+def test_fused_finalisation_bit_identical(ctx, method):
+    """The multiplicative two-level methods form each iteration's end (convergence test, next rho,
+    beta) inside the next iteration's first coarse application, except after a batch's last
+    iteration.  With that on (default) and off, two cases solved in one launch (converging at
+    different iterations, so one finishes inside a batch while the other goes on) give the same
+    counts, residuals and every bit of x; so does a solve whose inner cap ends every round."""
+    trs = (trees()["two_level"], trees()["flat"])
+    params = ffvhc.case_params(bc=BC)
+    bct = [params.bct[ffvhc.BC_P][q] for q in range(6)]
+    bcv = [params.bcv[ffvhc.BC_P][q] for q in range(6)]
+    ids = [ctx.add_case(tr, params) for tr in trs]
+    fa = [ctx.field(8) for _ in range(7)]
+    fb, fx = ctx.field(8), ctx.field(8)
+    try:
+        ogs = [OracleGrid(tr) for tr in trs]
+        for k in range(2):
+            A, b = make_system(ogs[k], np.random.default_rng(60 + k), bct, bcv)
+            for q in range(7):
+                ctx.upload(fa[q], ids[k], A[q])
+            ctx.upload(fb, ids[k], b)
+        for cap, eps in ((5000, 1e-10), (6, 1e-10)):
+            got = []
+            for fuse in (True, False, True):
+                ctx.set_fuse_fin(fuse)
+                for k in range(2):
+                    ctx.upload(fx, ids[k], ogs[k].zeros())
+                cnt, res, outer, rc = ctx.solve_mixed(method, fx, fa, fb, ffvhc.BC_P, 1.4, cap, eps, 0.5, ncases=2)
+                xs = [ctx.download(fx, ids[k], out=ogs[k].zeros()) for k in range(2)]
+                got.append((list(cnt), list(res), list(outer), rc, xs))
+            if cap == 5000:
+                assert got[0][3] == 0 and max(got[0][1]) <= 1e-10
+                assert got[0][0][0] != got[0][0][1]  # the cases stop at different iterations
+            for g in got[1:]:
+                assert g[0] == got[0][0] and g[1] == got[0][1] and g[2] == got[0][2] and g[3] == got[0][3]
+                for k in range(2):
+                    assert np.array_equal(g[4][k], got[0][4][k])
+    finally:
+        ctx.set_fuse_fin(True)
+        for f in fa + [fb, fx]:
+            ctx.free(f)
+        for i in ids:
+            ctx.remove_case(i)
+
+
 def class_uniform_system(og, rng, bct, bcv, n_other=3):
     """Bands constant per block (a random factor per block, as 1/dx^2 is per level) with the outer
     BC folded in: every block is uniform per stencil position class.  Then one cell's band changes
