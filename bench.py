@@ -11,8 +11,8 @@ A step is one Poisson solve from x = 0 to relative residual ||b-Ax||/||b|| <= 1e
 the residual always computed in FP64 with the FP64 matrix.  The FP64 reference path is the reference's
 solver, Jacobi right-preconditioned BiCGSTAB (FFVILS.h:793-974; the coarse/fine interpolation makes the
 octree operator non-symmetric, where Jacobi-PCG stagnates).  value = cells * steps / seconds of the
-mixed-precision schedule the autotuner picks (timed on another seeded right-hand side of the same
-matrix) among FP32-inner defect-correction solvers and FP64-recurrence BiCGSTAB variants on an
+mixed-precision schedule the autotuner picks (timed on three other seeded right-hand sides of the
+same matrix) among FP32-inner defect-correction solvers and FP64-recurrence BiCGSTAB variants on an
 FP32-stored matrix with block red-black GS / SOR preconditioners; the FP64 path is timed beside it.
 
 Prints one JSON line.  --gpus N > 1 exits: every box here has one GPU.
@@ -718,13 +718,17 @@ def main():
         sched = (1, args.inner_eps, 500, 1.0)
         tuned = []
     else:
-        # tuned on another seeded right-hand side of the same matrix, not on the timed one
-        log("autotuning the mixed-precision schedule (right-hand side seed 1)")
-        b_tune = ctx.field(8)
-        ctx.upload(b_tune, cid, seeded_rhs(tree, 1))
-        # best of two runs per candidate: the second sees the launch batches a repeated solve uses
+        # tuned on three other seeded right-hand sides of the same matrix, not on the timed ones
+        # (on one, schedules within a few percent of each other swap places from seed to seed)
+        log("autotuning the mixed-precision schedule (right-hand side seeds 1-3)")
+        b_tune = [ctx.field(8) for _ in range(3)]
+        for k, bf in enumerate(b_tune):
+            ctx.upload(bf, cid, seeded_rhs(tree, 1 + k))
+        # best of two runs per candidate and right-hand side: the second sees the launch batches a
+        # repeated solve uses
         sched, tuned = autotune.tune_mixed(ctx, f64["x"], f64["A"], b_tune, P, args.eps, repeats=2)
-        ctx.free(b_tune)
+        for bf in b_tune:
+            ctx.free(bf)
         log("schedule %s; candidates %s" % (sched, [(c, round(t, 4), it, o) for c, t, it, o, r in tuned]))
 
     if sched[0] >= 5:  # stage times of the tuned method's own kernels
@@ -851,7 +855,7 @@ def main():
                   "schedule": {"method": MIXED_METHODS[sched[0]], "inner_eps_or_update_factor": sched[1],
                                "inner_cap": sched[2], "omega": sched[3]},
                   "autotune": [{"method": MIXED_METHODS[c[0]], "factor": c[1], "cap": c[2], "omega": c[3],
-                                "seconds": (None if t == float("inf") else round(t, 4)), "inner": it,
+                                "seconds_3_rhs": (None if t == float("inf") else round(t, 4)), "inner_max": it,
                                 "corrections": o} for c, t, it, o, r in tuned]},
         "speedup_mixed_vs_fp64": round(mix["mcell_steps_s"] / d64["mcell_steps_s"], 3),
         # where the speedup comes from: fewer iterations (the preconditioner: algorithm) and a cheaper

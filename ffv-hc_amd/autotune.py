@@ -31,28 +31,36 @@ DEFAULT_CANDIDATES = (
 
 def tune_mixed(ctx, x, A, b, group, eps, candidates=DEFAULT_CANDIDATES, repeats=1, ncases=1, x0=None):
     """Time every candidate mixed-precision schedule on the resident system; returns
-    (best candidate, list of (candidate, seconds, inner iterations, corrections, residual))."""
+    (best candidate, list of (candidate, seconds, inner iterations, corrections, residual)).
+    b: one right-hand side field or a list of them; a candidate's time is the sum over the list
+    of its best of `repeats` solves (iteration counts move by a few from one right-hand side to
+    the next, enough to reorder schedules that tie on one), its iterations the largest count."""
     results = []
+    rhs = list(b) if isinstance(b, (list, tuple)) else [b]
     for cand in candidates:
         method, ieps, cap, omega = cand
-        best_t = None
-        for _ in range(repeats):
-            if x0 is None:
-                ctx.fill(x, 0.0)
-            else:
-                x0()
-            ctx.sync()
-            t0 = time.perf_counter()
-            try:
-                cnt, res, outer, rc = ctx.solve_mixed(method, x, A, b, group, omega, cap, eps, ieps, ncases=ncases)
-            except RuntimeError:  # a schedule the solver rejects is skipped
-                cnt, res, outer, rc = np.zeros(1, int), np.ones(1), np.zeros(1, int), 1
-            ctx.sync()
-            dt = time.perf_counter() - t0
-            best_t = dt if best_t is None else min(best_t, dt)
-        ok = rc == 0 and bool(np.all(res <= eps))
-        results.append((cand, best_t if ok else float("inf"), int(np.max(cnt)), int(np.max(outer)),
-                        float(np.max(res))))
+        total, ok, its, outs, resmax = 0.0, True, 0, 0, 0.0
+        for bf in rhs:
+            best_t = None
+            for _ in range(repeats):
+                if x0 is None:
+                    ctx.fill(x, 0.0)
+                else:
+                    x0()
+                ctx.sync()
+                t0 = time.perf_counter()
+                try:
+                    cnt, res, outer, rc = ctx.solve_mixed(method, x, A, bf, group, omega, cap, eps, ieps,
+                                                          ncases=ncases)
+                except RuntimeError:  # a schedule the solver rejects is skipped
+                    cnt, res, outer, rc = np.zeros(1, int), np.ones(1), np.zeros(1, int), 1
+                ctx.sync()
+                dt = time.perf_counter() - t0
+                best_t = dt if best_t is None else min(best_t, dt)
+            ok = ok and rc == 0 and bool(np.all(res <= eps))
+            total += best_t
+            its, outs, resmax = max(its, int(np.max(cnt))), max(outs, int(np.max(outer))), max(resmax, float(np.max(res)))
+        results.append((cand, total if ok else float("inf"), its, outs, resmax))
     best = min(results, key=lambda r: r[1])
     if not np.isfinite(best[1]):
         raise RuntimeError("no mixed-precision schedule reached the tolerance")

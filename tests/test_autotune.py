@@ -42,3 +42,26 @@ def test_no_candidate_converges(monkeypatch):
     monkeypatch.setattr(autotune.time, "perf_counter", lambda: ctx.now)
     with pytest.raises(RuntimeError):
         autotune.tune_mixed(ctx, 0, [0] * 7, 0, 3, 1e-10, candidates=((3, 0.1, 10, 1.0),))
+
+
+class FakeCtxPerRhs(FakeCtx):
+    def solve_mixed(self, method, x, A, b, group, omega, cap, eps, ieps, ncases=1):
+        sec, ok = self.behaviour[(method, b)]
+        self.now += sec
+        return np.array([40 + b]), np.array([eps * 0.5]), np.array([1]), 0
+
+
+def test_several_right_hand_sides_are_summed(monkeypatch):
+    """A schedule that wins on one right-hand side but loses on the others is not picked: the
+    time is the sum over the right-hand sides, the iterations the largest count."""
+    beh = {(26, 1): (0.010, True), (26, 2): (0.010, True), (26, 3): (0.010, True),
+           (27, 1): (0.009, True), (27, 2): (0.012, True), (27, 3): (0.011, True)}
+    ctx = FakeCtxPerRhs(beh)
+    monkeypatch.setattr(autotune.time, "perf_counter", lambda: ctx.now)
+    cands = ((27, 0.5, 20000, 1.6), (26, 0.5, 20000, 1.6))
+    best, res = autotune.tune_mixed(ctx, 0, [0] * 7, [1, 2, 3], 3, 1e-10, candidates=cands, repeats=2)
+    assert best == (26, 0.5, 20000, 1.6)
+    assert res[0][1] == pytest.approx(0.032) and res[1][1] == pytest.approx(0.030)
+    assert res[0][2] == 43
+    best1, _ = autotune.tune_mixed(ctx, 0, [0] * 7, 1, 3, 1e-10, candidates=cands)
+    assert best1 == (27, 0.5, 20000, 1.6)
