@@ -18,6 +18,12 @@ namespace {
 constexpr int kBatchMin = 8;
 constexpr int kBatchMax = 256;
 
+// BiCGSTAB batches without a hint grow 8, 8, 16, 16, 32, ... (doubling every second batch): the
+// Krylov solves need tens to hundreds of iterations, and doubling every batch launched up to as many
+// iterations after convergence as before it (8 + 16 + 32 for 28 needed) where an extra host check
+// costs about one launched iteration
+int next_bicg_batch(int batch, int nbatches) { return nbatches % 2 == 0 ? std::min(batch * 2, kBatchMax) : batch; }
+
 // scratch roles
 enum { R_R = 1, R_P = 2, R_Q = 3, R_E = 4, R_RHS = 5, R_A = 10, R_BI = 20, R_REC = 30 };
 
@@ -200,7 +206,7 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
   const int hround = c->bicg_round >= 0 ? std::min(c->bicg_round, 3) : -1;
   int* hint = hround >= 0 ? &c->bicg_hints[c->bicg_hint_key][hround] : nullptr;  // zero when new
   bool hinted = hint && *hint > 0;
-  int n = 1, batch = hinted ? std::min(*hint + 1, kBatchMax) : kBatchMin;
+  int n = 1, batch = hinted ? std::min(*hint + 1, kBatchMax) : kBatchMin, nbatches = 0;
   bool done = false;
   // multiplicative two-level methods: inside a batch, each iteration's finalisation runs in the
   // next iteration's first coarse application; the batch's last iteration has its own launch, so
@@ -229,7 +235,8 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
       return rc;
     }
     if (n > countMax) done = true;
-    batch = hinted ? kBatchMin : std::min(batch * 2, kBatchMax);
+    batch = hinted ? kBatchMin : next_bicg_batch(batch, ++nbatches);
+    if (hinted) nbatches = 0;  // after the hinted batch: 8, 8, 16, 16, ...
     hinted = false;
   }
   if (exec) HIPCHK(hipGraphExecDestroy(exec));
