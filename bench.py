@@ -529,6 +529,56 @@ def time_step_leg(case, steps, warmup, method, omega, fp64_steps=None):
     return out
 
 
+def c3_run_leg(steps=500, method=26, omega=1.6, every=25):
+    """C3 as specified: the three-level flow past a sphere (4,048 blocks, 16.6 M cells, Re 300, dt at
+    CFL 0.3 on the finest cells, implicit U, pressure to 1e-10) run for `steps` steps from the
+    uniform flow with solve_mixed `method`; every `every` steps the drag coefficient (pressure and
+    viscous forces on the sphere's cut faces, PrintForceCID's sum, over 0.5 rho U^2 pi D^2 / 4),
+    max |u_x| and the step's pressure iterations are recorded, and a progress line is logged."""
+    import ffvhc
+    import flows
+    from serve import uniform_flow_init
+    tree, geo, counts = flows.three_level_sphere()
+    params = flows.flow_params(300.0, flows.cfl_dt(tree, 0.3))
+    kw = dict(solver=3, omega=omega, count_max=20000, eps=1e-10, mixed=True, mixed_method=method,
+              implicit_u=True, alpha_u=1.0, omega_u=1.0, count_max_u=1000, eps_u=1e-8)
+    ctx = ffvhc.Context(max_blocks=tree.nblk)
+    cid = ctx.add_case(tree, params, geometry=geo)
+    F = ctx.step_fields()
+    ctx.select([cid])
+    uniform_flow_init(0, u0=1.0, noise=0.0)(ctx, cid, F)
+    area = np.pi / 4.0
+    hist, its, uits = [], [], []
+    ctx.sync()
+    t0 = time.perf_counter()
+    for n in range(1, steps + 1):
+        info, _ms, rc = ctx.step(F, ncases=1, **kw)
+        if rc:
+            raise RuntimeError("C3 step %d failed with status %d" % (n, rc))
+        its.append(int(info[0].countP))
+        uits.append(sum(int(v) for v in info[0].countU))
+        if n % every == 0 or n == steps:
+            fp = ctx.force_p(F.p, 1, ncases=1)[0]
+            fv = ctx.force_v(F.ux, F.uy, F.uz, 1, ncases=1)[0]
+            cd = float(fp[0] + fv[0]) / (0.5 * params.rho * area)
+            st = ctx.stats(F.ux, ncases=1)[0]
+            hist.append({"step": n, "t": round(n * params.dt, 4), "cd": round(cd, 5), "ux_max": round(float(st[1]), 5),
+                         "poisson_iterations": its[-1], "s": round(time.perf_counter() - t0, 2)})
+            log("C3 step %d: Cd %.4f, max u_x %.4f, %d pressure iterations, %.1f s" %
+                (n, cd, st[1], its[-1], time.perf_counter() - t0))
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    ctx.close()
+    return {"workload": "C3: flow past a sphere, three levels (sphere and wake refinement), %d blocks of 16^3 "
+                        "(%d cells), Re 300, dt %.5f (CFL 0.3 on the finest cells), implicit U, pressure to "
+                        "1e-10, %d steps from the uniform flow" % (tree.nblk, tree.nblk * N ** 3, params.dt, steps),
+            "solver": "solve_mixed method %d, omega %.1f" % (method, omega), "steps": steps,
+            "seconds": round(dt, 2), "ms_per_step": round(dt / steps * 1e3, 3),
+            "mcell_steps_s": round(tree.nblk * N ** 3 * steps / dt / 1e6, 3),
+            "pressure_iterations_mean": round(float(np.mean(its)), 2), "pressure_iterations_max": int(max(its)),
+            "velocity_iterations_mean": round(float(np.mean(uits)), 2), "history": hist}
+
+
 MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64 defect correction",
                  "FP32 Jacobi-BiCGSTAB in FP64 defect correction",
                  "FP32 Jacobi-BiCGSTAB with FP64 reliable updates", "the same with a seeded shadow residual",
@@ -662,6 +712,8 @@ def main():
     ap.add_argument("--c5-only", action="store_true", help="only the drag-stopped served stream")
     ap.add_argument("--c4-only", action="store_true", help="only the batched sweep of 64 cases")
     ap.add_argument("--c4-cases", type=int, default=64)
+    ap.add_argument("--c3-run", type=int, default=0,
+                    help="only C3 as specified: this many steps of the three-level flow (mixed solver), drag history")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this benchmark runs on one GPU only; multi-GPU runs are not implemented (the "
@@ -669,6 +721,9 @@ def main():
         sys.exit(2)
     if args.c4_only:
         print(json.dumps(c4_leg(args.c4_cases)))
+        return
+    if args.c3_run > 0:
+        print(json.dumps(c3_run_leg(args.c3_run)))
         return
     if args.c5_only:
         rep = serve_c5_leg(max(args.c5_cases, 1), 11, args.c5_seconds, method=26, omega=1.6,
