@@ -529,24 +529,42 @@ def time_step_leg(case, steps, warmup, method, omega, fp64_steps=None):
     return out
 
 
-def c3_run_leg(steps=500, method=26, omega=1.6, every=25):
-    """C3 as specified: the three-level flow past a sphere (4,048 blocks, 16.6 M cells, Re 300, dt at
-    CFL 0.3 on the finest cells, implicit U, pressure to 1e-10) run for `steps` steps from the
-    uniform flow with solve_mixed `method`; every `every` steps the drag coefficient (pressure and
-    viscous forces on the sphere's cut faces, PrintForceCID's sum, over 0.5 rho U^2 pi D^2 / 4),
-    max |u_x| and the step's pressure iterations are recorded, and a progress line is logged."""
+def flow_run_leg(case, steps, method=26, omega=1.6, every=25, fp64=False):
+    """C1 or C3 as specified, run for `steps` steps with solve_mixed `method`.  C1: the single-level
+    flow past a sphere (128 blocks, 524,288 cells, Re 100, dt 0.01, explicit U, pressure to 1e-8,
+    u = (1, 0, 0) plus seeded N(0, 0.01) noise, seed 1).  C3: the three-level flow (4,048 blocks, 16.6 M
+    cells, Re 300, dt at CFL 0.3 on the finest cells, implicit U, pressure to 1e-10) from the
+    uniform flow.  Every `every` steps the drag coefficient (pressure and viscous forces on the
+    sphere's cut faces, PrintForceCID's sum, over 0.5 rho U^2 pi D^2 / 4), max u_x and the step's
+    pressure iterations are recorded, and a progress line is logged.  fp64: the reference's FP64
+    Jacobi-BiCGSTAB pressure solve instead."""
     import ffvhc
     import flows
     from serve import uniform_flow_init
-    tree, geo, counts = flows.three_level_sphere()
-    params = flows.flow_params(300.0, flows.cfl_dt(tree, 0.3))
-    kw = dict(solver=3, omega=omega, count_max=20000, eps=1e-10, mixed=True, mixed_method=method,
-              implicit_u=True, alpha_u=1.0, omega_u=1.0, count_max_u=1000, eps_u=1e-8)
+    if case == "c1":
+        tree, geo, counts = flows.single_level_sphere()
+        params = flows.flow_params(100.0, 0.01)
+        kw = dict(solver=3, omega=omega, count_max=20000, eps=1e-8, mixed=True, mixed_method=method)
+        init = uniform_flow_init(1, u0=1.0, noise=0.01)
+        desc = ("C1: flow past a sphere, single level, %d blocks of 16^3 (%d cells), Re 100, dt 0.01, explicit U, "
+                "pressure to 1e-8" % (tree.nblk, tree.nblk * N ** 3))
+    else:
+        tree, geo, counts = flows.three_level_sphere()
+        params = flows.flow_params(300.0, flows.cfl_dt(tree, 0.3))
+        kw = dict(solver=3, omega=omega, count_max=20000, eps=1e-10, mixed=True, mixed_method=method,
+                  implicit_u=True, alpha_u=1.0, omega_u=1.0, count_max_u=1000, eps_u=1e-8)
+        init = uniform_flow_init(0, u0=1.0, noise=0.0)
+        desc = ("C3: flow past a sphere, three levels (sphere and wake refinement), %d blocks of 16^3 (%d cells), "
+                "Re 300, dt %.5f (CFL 0.3 on the finest cells), implicit U, pressure to 1e-10"
+                % (tree.nblk, tree.nblk * N ** 3, params.dt))
+    if fp64:
+        kw.update(omega=1.0, mixed=False)
+        kw.pop("mixed_method")
     ctx = ffvhc.Context(max_blocks=tree.nblk)
     cid = ctx.add_case(tree, params, geometry=geo)
     F = ctx.step_fields()
     ctx.select([cid])
-    uniform_flow_init(0, u0=1.0, noise=0.0)(ctx, cid, F)
+    init(ctx, cid, F)
     area = np.pi / 4.0
     hist, its, uits = [], [], []
     ctx.sync()
@@ -554,9 +572,9 @@ def c3_run_leg(steps=500, method=26, omega=1.6, every=25):
     for n in range(1, steps + 1):
         info, _ms, rc = ctx.step(F, ncases=1, **kw)
         if rc:
-            raise RuntimeError("C3 step %d failed with status %d" % (n, rc))
+            raise RuntimeError("%s step %d failed with status %d" % (case.upper(), n, rc))
         its.append(int(info[0].countP))
-        uits.append(sum(int(v) for v in info[0].countU))
+        uits.append(sum(int(v) for v in info[0].countU) if case != "c1" else 0)
         if n % every == 0 or n == steps:
             fp = ctx.force_p(F.p, 1, ncases=1)[0]
             fv = ctx.force_v(F.ux, F.uy, F.uz, 1, ncases=1)[0]
@@ -564,15 +582,14 @@ def c3_run_leg(steps=500, method=26, omega=1.6, every=25):
             st = ctx.stats(F.ux, ncases=1)[0]
             hist.append({"step": n, "t": round(n * params.dt, 4), "cd": round(cd, 5), "ux_max": round(float(st[1]), 5),
                          "poisson_iterations": its[-1], "s": round(time.perf_counter() - t0, 2)})
-            log("C3 step %d: Cd %.4f, max u_x %.4f, %d pressure iterations, %.1f s" %
-                (n, cd, st[1], its[-1], time.perf_counter() - t0))
+            log("%s step %d: Cd %.4f, max u_x %.4f, %d pressure iterations, %.1f s" %
+                (case.upper(), n, cd, st[1], its[-1], time.perf_counter() - t0))
     ctx.sync()
     dt = time.perf_counter() - t0
     ctx.close()
-    return {"workload": "C3: flow past a sphere, three levels (sphere and wake refinement), %d blocks of 16^3 "
-                        "(%d cells), Re 300, dt %.5f (CFL 0.3 on the finest cells), implicit U, pressure to "
-                        "1e-10, %d steps from the uniform flow" % (tree.nblk, tree.nblk * N ** 3, params.dt, steps),
-            "solver": "solve_mixed method %d, omega %.1f" % (method, omega), "steps": steps,
+    return {"workload": "%s, %d steps" % (desc, steps),
+            "solver": ("FP64 Jacobi-BiCGSTAB" if fp64 else "solve_mixed method %d, omega %.1f" % (method, omega)),
+            "steps": steps,
             "seconds": round(dt, 2), "ms_per_step": round(dt / steps * 1e3, 3),
             "mcell_steps_s": round(tree.nblk * N ** 3 * steps / dt / 1e6, 3),
             "pressure_iterations_mean": round(float(np.mean(its)), 2), "pressure_iterations_max": int(max(its)),
@@ -714,6 +731,9 @@ def main():
     ap.add_argument("--c4-cases", type=int, default=64)
     ap.add_argument("--c3-run", type=int, default=0,
                     help="only C3 as specified: this many steps of the three-level flow (mixed solver), drag history")
+    ap.add_argument("--c1-run", type=int, default=0,
+                    help="only C1 as specified: this many steps of the single-level flow (mixed solver), drag history")
+    ap.add_argument("--run-fp64", action="store_true", help="--c1-run / --c3-run with the FP64 pressure solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this benchmark runs on one GPU only; multi-GPU runs are not implemented (the "
@@ -723,7 +743,10 @@ def main():
         print(json.dumps(c4_leg(args.c4_cases)))
         return
     if args.c3_run > 0:
-        print(json.dumps(c3_run_leg(args.c3_run)))
+        print(json.dumps(flow_run_leg("c3", args.c3_run, fp64=args.run_fp64)))
+        return
+    if args.c1_run > 0:
+        print(json.dumps(flow_run_leg("c1", args.c1_run, fp64=args.run_fp64)))
         return
     if args.c5_only:
         rep = serve_c5_leg(max(args.c5_cases, 1), 11, args.c5_seconds, method=26, omega=1.6,
