@@ -89,3 +89,30 @@ def test_fullsize_coarse_inverse_gemm_and_vector_paths(c2):
         ctx.set_coarse_gemm_min(2048)
     scale = np.max(np.abs(out[1]))
     assert np.max(np.abs(out[1] - out[1 << 30])) <= 1e-10 * scale
+
+
+def test_large_case_fused_finalisation_bit_identical():
+    """A case of 4,352 blocks (flat 17 x 16 x 16 roots), above the 4,096 blocks whose coarse
+    right-hand side the coarse application stages in LDS: every wave then forms the coarse
+    right-hand side and the previous iteration's end itself (17 strided rounds of the per-wave
+    sum).  Method 26 with the finalisation fused and in its own launch: the same count and every
+    bit of x, and the FP64 residual recomputed from the answer meets the tolerance."""
+    tree = ffvhc.Tree(root=(17, 16, 16), root_length=1.0)
+    assert tree.nblk == 4352
+    A, b, bct = bench.poisson_system(tree)
+    ctx, cid, f64, f32 = bench.setup(tree, A, b, bct)
+    del A
+    try:
+        shp = (tree.nblk, 20, 20, 20)
+        got = []
+        for fuse in (True, False):
+            ctx.set_fuse_fin(fuse)
+            cnt, res = solve(ctx, f64, 26)
+            true = ctx.residual(f64["x"], f64["A"], f64["b"], ncases=1)[0]
+            assert res <= 1e-10 and true <= 1e-10 * 1.0001, (fuse, cnt, res, true)
+            got.append((cnt, res, ctx.download(f64["x"], cid, out=np.zeros(shp))))
+        assert got[0][0] == got[1][0] and got[0][1] == got[1][1]
+        assert np.array_equal(got[0][2], got[1][2])
+    finally:
+        ctx.set_fuse_fin(True)
+        ctx.close()
