@@ -170,10 +170,36 @@ def cpu_time_step_baseline(steps=4, threads=16):
     import ctypes
     import flows
     from oracle import oracle as O
+    threads = min(threads, os.cpu_count() or 1)
+    st = oracle_c1(1e-8, threads)
+    L, grid, F, P, nb = st["L"], st["grid"], st["F"], st["P"], st["nb"]
+    times, ils = [], []
+    for n in range(steps):
+        info = O.StepInfo()
+        L.orc_update(ctypes.byref(grid), ctypes.byref(F), ctypes.byref(P), n, ctypes.byref(info))
+        times.append([float(info.times[q]) for q in range(7)])
+        ils.append("%010d %04d %.16f %04d %.16f %04d %.16f %04d %.16f %04d %.16f" % (
+            n, 0, 0.0, 0, 0.0, 0, 0.0, info.countP, info.residualP, 0, 0.0))
+    cells = nb * N ** 3
+    t26 = sum(sum(t[2:7]) for t in times)
+    return {"value": cells * steps / t26 / 1e6, "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "sample": "%d time steps of the single-level flow past a sphere (%d cells, Re 100, dt 0.01, explicit U, "
+                      "pressure by Jacobi-BiCGSTAB to 1e-8) on %d threads, timed as Solver::times[2..6]"
+                      % (steps, cells, threads),
+            "times_per_step": [{"UX": t[2], "UY": t[3], "UZ": t[4], "P": t[5], "U": t[6]} for t in times],
+            "data_ils": ils, "host": host_info()}
+
+
+def oracle_c1(eps, threads):
+    """The oracle's state for the single-level flow past a sphere (C1 geometry, Re 100, dt 0.01,
+    explicit U) from the uniform flow u = (1, 0, 0): grid, fields, parameters (pressure by
+    Jacobi-BiCGSTAB to eps) and the host arrays behind them."""
+    import ctypes
+    import flows
+    from oracle import oracle as O
     tree, geo, counts = flows.single_level_sphere()
     grid, tab = oracle_grid(tree)
     L = O.lib()
-    threads = min(threads, os.cpu_count() or 1)
     L.orc_set_threads(threads)
     nb = tree.nblk
     shp = (nb, N + 2 * G, N + 2 * G, N + 2 * G)
@@ -208,22 +234,66 @@ def cpu_time_step_baseline(steps=4, threads=16):
             P.bct_u[d][f], P.bcv_u[d][f] = params.bct[d][f], params.bcv[d][f]
     for f in range(6):
         P.bct_p[f], P.bcv_p[f] = params.bct[3][f], params.bcv[3][f]
-    P.solver, P.omega, P.countMax, P.eps, P.countPre, P.ncells = 3, 1.0, 20000, 1e-8, 1, nb * N ** 3
-    times, ils = [], []
+    P.solver, P.omega, P.countMax, P.eps, P.countPre, P.ncells = 3, 1.0, 20000, eps, 1, nb * N ** 3
+    return {"L": L, "grid": grid, "F": F, "P": P, "nb": nb, "tree": tree, "geo": geo, "params": params,
+            "arrays": a, "lists": lists, "keep": (c, cid, pid, tab)}
+
+
+def field_error_leg(steps=3, eps=1e-10, method=26, omega=1.6, threads=16):
+    """Error against the FP64 oracle, per field: `steps` time steps of the single-level flow past a
+    sphere (C1 geometry from the uniform flow, explicit U, pressure to eps) on the oracle (the C
+    restatement of Solver::Update, FP64, Jacobi-BiCGSTAB), on the device's FP64 path (the same
+    solver) and on the device's mixed-precision path (solve_mixed `method`); for each field the
+    largest |device - oracle| over the interior cells, absolute and relative to the field's largest
+    |value|."""
+    import ctypes
+    import ffvhc
+    from oracle import oracle as O
+    from serve import uniform_flow_init
+    threads = min(threads, os.cpu_count() or 1)
+    st = oracle_c1(eps, threads)
+    L, grid, F0, P0, nb = st["L"], st["grid"], st["F"], st["P"], st["nb"]
+    its_o = []
     for n in range(steps):
         info = O.StepInfo()
-        L.orc_update(ctypes.byref(grid), ctypes.byref(F), ctypes.byref(P), n, ctypes.byref(info))
-        times.append([float(info.times[q]) for q in range(7)])
-        ils.append("%010d %04d %.16f %04d %.16f %04d %.16f %04d %.16f %04d %.16f" % (
-            n, 0, 0.0, 0, 0.0, 0, 0.0, info.countP, info.residualP, 0, 0.0))
-    cells = nb * N ** 3
-    t26 = sum(sum(t[2:7]) for t in times)
-    return {"value": cells * steps / t26 / 1e6, "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
-            "sample": "%d time steps of the single-level flow past a sphere (%d cells, Re 100, dt 0.01, explicit U, "
-                      "pressure by Jacobi-BiCGSTAB to 1e-8) on %d threads, timed as Solver::times[2..6]"
-                      % (steps, cells, threads),
-            "times_per_step": [{"UX": t[2], "UY": t[3], "UZ": t[4], "P": t[5], "U": t[6]} for t in times],
-            "data_ils": ils, "host": host_info()}
+        L.orc_update(ctypes.byref(grid), ctypes.byref(F0), ctypes.byref(P0), n, ctypes.byref(info))
+        its_o.append(int(info.countP))
+    a, lists = st["arrays"], st["lists"]
+    ref = {"ux": a["ux"], "uy": a["uy"], "uz": a["uz"], "p": a["p"], "lapp": a["lapp"]}
+    for q in range(6):
+        ref["v%d" % q] = lists["v"][q]
+    shp = (nb, N + 2 * G, N + 2 * G, N + 2 * G)
+    inner = (slice(None),) + (slice(G, G + N),) * 3
+    out = {"workload": "%d time steps of the single-level flow past a sphere (C1 geometry, uniform flow start, "
+                       "explicit U, pressure to %g), oracle on %d host threads" % (steps, eps, threads),
+           "oracle_pressure_iterations": its_o, "fields": {}}
+    tree, geo, params = st["tree"], st["geo"], st["params"]
+    for tag, kw in (("fp64", dict(solver=3, omega=1.0, count_max=20000, eps=eps)),
+                    ("mixed", dict(solver=3, omega=omega, count_max=20000, eps=eps, mixed=True,
+                                   mixed_method=method))):
+        ctx = ffvhc.Context(max_blocks=tree.nblk)
+        cid = ctx.add_case(tree, params, geometry=geo)
+        F = ctx.step_fields()
+        ctx.select([cid])
+        uniform_flow_init(0, u0=1.0, noise=0.0)(ctx, cid, F)
+        its = []
+        for _ in range(steps):
+            info, _ms, rc = ctx.step(F, ncases=1, **kw)
+            if rc:
+                raise RuntimeError("time step failed with status %d" % rc)
+            its.append(int(info[0].countP))
+        got = {"ux": F.ux, "uy": F.uy, "uz": F.uz, "p": F.p, "lapp": F.lapp}
+        for q in range(6):
+            got["v%d" % q] = F.v[q]
+        for k, f in got.items():
+            d = ctx.download(f, cid, out=np.zeros(shp))[inner]
+            r = ref[k][inner]
+            err = float(np.max(np.abs(d - r)))
+            scale = float(np.max(np.abs(r)))
+            out["fields"].setdefault(k, {})[tag] = {"max_abs": err, "rel": err / scale if scale > 0 else err}
+        out[tag + "_pressure_iterations"] = its
+        ctx.close()
+    return out
 
 
 def serve_leg(ncases, seed, budget_s, method=None, omega=1.0):
@@ -734,6 +804,9 @@ def main():
     ap.add_argument("--c1-run", type=int, default=0,
                     help="only C1 as specified: this many steps of the single-level flow (mixed solver), drag history")
     ap.add_argument("--run-fp64", action="store_true", help="--c1-run / --c3-run with the FP64 pressure solve")
+    ap.add_argument("--field-errors", type=int, default=0,
+                    help="only the per-field error of the FP64 and mixed device paths against the oracle over this "
+                         "many C1 steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this benchmark runs on one GPU only; multi-GPU runs are not implemented (the "
@@ -744,6 +817,9 @@ def main():
         return
     if args.c3_run > 0:
         print(json.dumps(flow_run_leg("c3", args.c3_run, fp64=args.run_fp64)))
+        return
+    if args.field_errors > 0:
+        print(json.dumps(field_error_leg(args.field_errors)))
         return
     if args.c1_run > 0:
         print(json.dumps(flow_run_leg("c1", args.c1_run, fp64=args.run_fp64)))

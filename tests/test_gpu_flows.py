@@ -25,3 +25,18 @@ def test_c1_drag_mixed_equals_fp64():
         assert 1.0 < b["cd"] < 3.0 and 1.0 < b["ux_max"] < 1.5
     # the mixed solve needs far fewer pressure iterations per step
     assert mixed["pressure_iterations_mean"] * 5 < fp64["pressure_iterations_mean"]
+
+
+def test_c1_fields_against_oracle():
+    """Two steps of the C1 flow from the uniform flow, pressure to 1e-10: the device's FP64 path and
+    its mixed-precision path against the oracle's FP64 Solver::Update (C restatement, Jacobi-
+    BiCGSTAB), field by field.  Velocities and face velocities agree to 1e-10 of their size; the
+    pressure, which the solves fix only to their residual tolerance, to 1e-7, and its Laplacian lapp
+    (which carries that error through two differences) to 1e-8."""
+    d = bench.field_error_leg(steps=2)
+    assert set(d["fields"]) == {"ux", "uy", "uz", "p", "lapp"} | {"v%d" % q for q in range(6)}
+    for k, v in d["fields"].items():
+        bound = {"p": 1e-7, "lapp": 1e-8}.get(k, 1e-10)
+        for tag in ("fp64", "mixed"):
+            assert v[tag]["rel"] <= bound, (k, tag, v[tag])
+    assert max(d["mixed_pressure_iterations"]) * 5 < min(d["fp64_pressure_iterations"])
