@@ -79,3 +79,16 @@ def test_step_iteration_bytes():
     u, o = bench.step_iteration_bytes(26, 1.0), bench.step_iteration_bytes(26, 0.0)
     assert abs(u - (43 + 27 + 35 + 27 + 64)) < 1e-9 and o > u
     assert abs(bench.step_iteration_bytes(26, 0.5) - 0.5 * (u + o)) < 1e-9
+
+
+def test_cpu_time_step_baseline_runs_one_step():
+    """The bench's CPU baseline of the time step (the oracle's Solver::Update on the C1 flow) runs
+    without a GPU: one step on 4 threads, its pressure solve reaching 1e-8 in the layout of
+    data-ils.txt, with the stage times of Solver::times[] filled."""
+    sys.path.insert(0, ROOT)
+    import bench
+    r = bench.cpu_time_step_baseline(steps=1, threads=4)
+    assert r["value"] > 0 and r["cores"] == 4 and r["kind"] == "port"
+    line = r["data_ils"][0].split()
+    assert line[0] == "0000000000" and int(line[7]) > 100 and float(line[8]) <= 1e-8
+    assert all(v > 0 for v in r["times_per_step"][0].values())
