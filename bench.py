@@ -682,7 +682,10 @@ MIXED_METHODS = ["FP32 RBGS in FP64 defect correction", "FP32 Jacobi-PCG in FP64
                      % (n, "" if n == 1 else "s") for n in range(1, 7)] + [
                      "FP64 BiCGSTAB on an FP32-stored matrix, coarse (one value per block) correction, then "
                      "%d block red-black SOR sweep%s on the remaining residual (multiplicative two-level)"
-                     % (n, "" if n == 1 else "s") for n in range(1, 7)]
+                     % (n, "" if n == 1 else "s") for n in range(1, 7)] + [
+                     "FP32 BiCGSTAB on an FP32-stored matrix, coarse (one value per block) correction, then 4 block "
+                     "red-black SOR sweeps on the remaining residual (method 26 on FP32 vectors), in FP64 defect "
+                     "correction"]
 
 # schedule of `bench.py --profile` (the rocprofv3 runs): FP32-stored matrix, coarse correction then
 # 4 SOR sweeps, omega 1.6
@@ -703,25 +706,28 @@ def prof_name(stage, w):
 
 
 def mixed_prof_name(method, stage, uniform=False):
-    """rocprofv3 name of a stage's kernel in the FP64-recurrence methods 5-28; uniform: the
-    variant of the class-uniform blocks (methods 11-28 read their bands from the class rows)."""
+    """rocprofv3 name of a stage's kernel in the FP64-recurrence methods 5-28 and in method 29's
+    FP32 inner solve; uniform: the variant of the class-uniform blocks (methods 11-29 read their
+    bands from the class rows)."""
     mode = {"bi_p": 0, "bi_q": 0, "bi_s": 1, "bi_t": 1}.get(stage)
     hr = "true" if method == 10 else "false"
     tf = lambda v: "true" if v else "false"
     reg = 1 if uniform and method >= 11 else 0  # the class-only variant (all blocks uniform)
+    vt = "float" if method == 29 else "double"  # vector type
+    sweeps = 4 if method == 29 else (method - 11) % 6 + 1
     # additive coarse correction (17-22): the matvec and XR add e to p_ / s_; multiplicative
-    # (23-28): the sweep kernel stores p_ / s_ with e already added
+    # (23-29): the sweep kernel stores p_ / s_ with e already added
     cc = 17 <= method <= 22
     if stage == "bi_xr":
-        return "bi_xr_kernel<double, %s, %s>" % (hr, tf(cc))
+        return "bi_xr_kernel<%s, %s, %s>" % (vt, hr, tf(cc))
     if stage in ("bi_q", "bi_t"):
         if method >= 11:
-            return "bi_matvec_kernel<double, %d, float, %s, %s, %d>" % (mode, hr, tf(cc), reg)
+            return "bi_matvec_kernel<%s, %d, float, %s, %s, %d>" % (vt, mode, hr, tf(cc), reg)
         return "bi_matvec_kernel<double, %d, %s, %s, false, 0>" % (mode, "float" if method >= 7 else "double", hr)
     if method == 7:
         return "bi_precond_kernel<double, %d, float>" % mode
     if method >= 11:
-        return "bi_precond_gsk_kernel<%d, %d, %s, %d>" % (mode, (method - 11) % 6 + 1, tf(method >= 23), reg)
+        return "bi_precond_gsk_kernel<%s, %d, %d, %s, %d>" % (vt, mode, sweeps, tf(method >= 23), reg)
     tm, tz, sym = {5: ("float", "float", "false"), 6: ("double", "double", "false"),
                    8: ("float", "double", "false"), 9: ("float", "double", "true"),
                    10: ("float", "double", "false")}[method]
@@ -753,19 +759,20 @@ def kernel_bytes_mixed(method, uniform=False):
     FP32 (ma = 4 bytes) or FP64 where the method stores them so; the GS preconditioner reads all
     seven bands, the Jacobi one the diagonal only.  uniform: every block is class-uniform, so the
     compact-record methods (11-28) read no per-cell band or record (27 rows per block)."""
-    h = 0.375 * 8                                   # FP64 ghost layer of p_ / s_
+    w = 4 if method == 29 else 8                    # vectors: FP32 in method 29's inner solve
+    h = 0.375 * w                                   # ghost layer of p_ / s_
     mv = 4 if method >= 7 else 8                    # matvec bands
     mp = {5: 4, 6: 8, 7: 4, 8: 4, 9: 4, 10: 4}.get(method, 16 / 7)  # preconditioner bands
     if uniform and method >= 11:
         mv, mp = 0, 0
     nb = 1 if method == 7 else 7                    # (11-13: one 16-byte record per cell)
-    r0 = 0 if method == 10 else 8                   # hashed shadow residual: not stored
+    r0 = 0 if method == 10 else w                   # hashed shadow residual: not stored
     # 17-22 add 8 B per cell (the block sum needs no extra read; the corrections are per block)
-    return {"bi_p": 24 + nb * mp + 16 + h,          # r, p, q in; p, p_ out; p_ ghosts
-            "bi_q": 8 + h + 7 * mv + r0 + 8,        # p_ + ghosts, bands, r0 in; q_ out
-            "bi_s": 16 + nb * mp + 16 + h,          # r, q_ in; s, s_ out; s_ ghosts
-            "bi_t": 8 + h + 7 * mv + 8 + 8,         # s_ + ghosts, bands, s in; t_ out
-            "bi_xr": 56 + r0}                       # x, p_, s_, t_, s, r0 in; x, r out
+    return {"bi_p": 3 * w + nb * mp + 2 * w + h,    # r, p, q in; p, p_ out; p_ ghosts
+            "bi_q": w + h + 7 * mv + r0 + w,        # p_ + ghosts, bands, r0 in; q_ out
+            "bi_s": 2 * w + nb * mp + 2 * w + h,    # r, q_ in; s, s_ out; s_ ghosts
+            "bi_t": w + h + 7 * mv + w + w,         # s_ + ghosts, bands, s in; t_ out
+            "bi_xr": 7 * w + r0}                    # x, p_, s_, t_, s, r0 in; x, r out
 
 
 def log(msg):
@@ -988,7 +995,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f32 inner / f64 defect correction" if sched[0] <= 4 else
+        "dtype": ("f32 inner / f64 defect correction" if sched[0] <= 4 or sched[0] == 29 else
                   "f64 vectors and arithmetic / f32 matrix storage" if sched[0] >= 7 else
                   "f64 / f32 preconditioner" if sched[0] == 5 else "f64"),
         "data": "synthetic (seeded uniform[-1,1] RHS, mean removed)",

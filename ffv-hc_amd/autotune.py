@@ -26,6 +26,10 @@ DEFAULT_CANDIDATES = (
     (26, 0.5, 20000, 1.6),
     (27, 0.5, 20000, 1.6),
     (28, 0.5, 20000, 1.6),
+    # method 26's preconditioner on FP32 vectors inside FP64 defect correction: inner tolerance,
+    # inner cap per correction
+    (29, 1e-4, 200, 1.6),
+    (29, 1e-5, 200, 1.6),
 )
 
 

@@ -351,12 +351,12 @@ constexpr int GSK_NK = NB * 256 / GSK_NT;  // cells of a column per thread
 // thread's inner cells and one for its end cell (k = 0 or NB - 1) instead of one per cell, which
 // leaves registers for three workgroups per CU.  REG 2 (mixed launch): each block takes its
 // records from the class rows when uni[slot] is set, else per cell.
-template <int MODE, int NS, bool MUL, int REG>
+template <class T, int MODE, int NS, bool MUL, int REG>
 __global__ __launch_bounds__(GSK_NT) __attribute__((amdgpu_waves_per_eu(REG == 1 ? 6 : 1))) void bi_precond_gsk_kernel(const int* __restrict__ active,
                                                                const DevBlock* __restrict__ blk,
                                                                const DevCase* __restrict__ cs,
                                                                const SolveState* __restrict__ st,
-                                                               const uint4* __restrict__ rec, BiVecs<double> V,
+                                                               const uint4* __restrict__ rec, BiVecs<T> V,
                                                                int count, double* __restrict__ vs_out,
                                                                const double* __restrict__ ce,
                                                                const uint4* __restrict__ trec,
@@ -376,9 +376,9 @@ __global__ __launch_bounds__(GSK_NT) __attribute__((amdgpu_waves_per_eu(REG == 1
   // the block's vectors from uniform base pointers, indexed by 32-bit cell offsets (one offset
   // register serves every vector)
   const unsigned o0 = tx + NB * ty + NF * kb;
-  const double* const rb = V.r + base;
-  const double* const qb = V.q + base;
-  double* const vb = (MODE == 0 ? V.p : V.s) + base;  // MODE 0 reads the old p here too
+  const T* const rb = V.r + base;
+  const T* const qb = V.q + base;
+  T* const vb = (MODE == 0 ? V.p : V.s) + base;  // MODE 0 reads the old p here too
   const uint4* const recb = rec + base;
   const double omega = st[c].omega;
   const double ng = -st[c].gamma, beta = st[c].beta, na = -st[c].alpha;
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(GSK_NT) __attribute__((amdgpu_waves_per_eu(REG == 1
   }
 #pragma unroll
   for (int k0 = 0; k0 < GSK_NK; k0 += 4) {
-    double rv[4], pv[4], qv[4];
+    T rv[4], pv[4], qv[4];
     uint4 rc[4];
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
@@ -449,12 +449,15 @@ __global__ __launch_bounds__(GSK_NT) __attribute__((amdgpu_waves_per_eu(REG == 1
         else if (k == GSK_NK - 1) rc[kk] = lower ? rin : rend;
         else rc[kk] = rin;
       }
+      // FP64 vectors: the recurrence's own values; FP32 vectors (an inner solve of a defect
+      // correction): formed in FP64 and rounded once when stored
       double v;
       if (MODE == 0)
-        v = first ? rv[kk] : rv[kk] + beta * (pv[kk] + ng * qv[kk]);
+        v = first ? (double)rv[kk] : (double)rv[kk] + beta * ((double)pv[kk] + ng * (double)qv[kk]);
       else
-        v = na * qv[kk] + rv[kk];
-      __builtin_nontemporal_store(v, vb + o0 + NF * k);
+        v = na * (double)qv[kk] + (double)rv[kk];
+      if constexpr (sizeof(T) == 4) v = (double)(T)v;  // the value the recurrence keeps
+      __builtin_nontemporal_store((T)v, vb + o0 + NF * k);
       vsum += v;
       double uk = v * (double)__uint_as_float(rc[kk].x);
       const int ka = kb + k;
@@ -543,27 +546,27 @@ __global__ __launch_bounds__(GSK_NT) __attribute__((amdgpu_waves_per_eu(REG == 1
   }
   // MUL: the stored direction is z + P e (e added to this block's values and to the ghost values
   // it pushes, which then come from z + e through every face kind)
-  double* const out = (MODE == 0 ? V.pp : V.ss) + base;
+  T* const out = (MODE == 0 ? V.pp : V.ss) + base;
   unsigned oo = o0;  // opaque: the store offsets are formed here, not held across the sweeps
   asm volatile("" : "+v"(oo));
 #pragma unroll
   for (int k = 0; k < GSK_NK; k++)
-    __builtin_nontemporal_store(MUL ? z[oo + NF * k] + eo : z[oo + NF * k], out + oo + NF * k);
-  double* h = MODE == 0 ? V.hpp : V.hss;
-  push_halo<double, GSK_NT>(
+    __builtin_nontemporal_store((T)(MUL ? z[oo + NF * k] + eo : z[oo + NF * k]), out + oo + NF * k);
+  T* h = MODE == 0 ? V.hpp : V.hss;
+  push_halo<T, GSK_NT>(
       B, cs[c], BC_NULL, slot, [&](int i, int j, int k) { return MUL ? z[zi(i, j, k)] + eo : z[zi(i, j, k)]; }, h,
       h);
 }
 
 // Launches the sweep kernel; with the class rows (o.trec) the class-uniform variant when every
 // active block is class-uniform, the mixed one when some are, else the per-cell one.
-template <int MODE, bool MUL>
+template <class T, int MODE, bool MUL>
 void launch_gsk(const BiOpts& o, hipStream_t sm, const int* active, int nact, const DevBlock* blk, const DevCase* cs,
-                const SolveState* st, BiVecs<double> V, int count, double* vs_out, const double* ce) {
+                const SolveState* st, BiVecs<T> V, int count, double* vs_out, const double* ce) {
   const dim3 g(nact), b(GSK_NT);
   const int reg = !o.trec || o.nreg == 0 ? 0 : o.nirr == 0 ? 1 : 2;
 #define GSK1(NS, R)                                                                                                 \
-  hipLaunchKernelGGL((bi_precond_gsk_kernel<MODE, NS, MUL, R>), g, b, 0, sm, active, blk, cs, st, o.rec, V, count, \
+  hipLaunchKernelGGL((bi_precond_gsk_kernel<T, MODE, NS, MUL, R>), g, b, 0, sm, active, blk, cs, st, o.rec, V, count, \
                      vs_out, ce, o.trec, o.uni)
 #define GSK(NS)            \
   do {                     \
@@ -890,18 +893,21 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   // (whose launch bit 10 left out).
   if (!nact) return;
   const bool dbl = sizeof(T) == 8;
-  if (!(dbl && o.gs == BiOpts::GSC && o.cinv && o.mult && count > 0) || (mask & 256)) mask &= ~(512 | 1024);
+  // FP32 vectors (the inner solve of a defect correction) take the compact-record path only with
+  // the multiplicative coarse correction (methods 23-28's preconditioner)
+  const bool gsc_mult = o.gs == BiOpts::GSC && o.cinv && o.mult;
+  if (!(gsc_mult && count > 0) || (mask & 256)) mask &= ~(512 | 1024);
   BiVecs<double>& Vd = *reinterpret_cast<BiVecs<double>*>(&V);
   double* const part0 = part;
   auto precond = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
-    if (dbl && o.gs == BiOpts::GSC && o.cinv && o.mult)  // block sum of p kept for the next coarse rhs
-      launch_gsk<MODE, true>(o, sm, active, nact, blk, cs, st, Vd, count, MODE == 0 ? o.ce + 4 * nslots : nullptr,
-                             o.ce + MODE * nslots);
+    if (gsc_mult)  // block sum of p kept for the next coarse rhs
+      launch_gsk<T, MODE, true>(o, sm, active, nact, blk, cs, st, V, count, MODE == 0 ? o.ce + 4 * nslots : nullptr,
+                                o.ce + MODE * nslots);
     else if (dbl && o.gs == BiOpts::GSC && o.cinv)
-      launch_gsk<MODE, false>(o, sm, active, nact, blk, cs, st, Vd, count, part + 4 * nslots, nullptr);
+      launch_gsk<double, MODE, false>(o, sm, active, nact, blk, cs, st, Vd, count, part + 4 * nslots, nullptr);
     else if (dbl && o.gs == BiOpts::GSC)
-      launch_gsk<MODE, false>(o, sm, active, nact, blk, cs, st, Vd, count, nullptr, nullptr);
+      launch_gsk<double, MODE, false>(o, sm, active, nact, blk, cs, st, Vd, count, nullptr, nullptr);
     else if (dbl && o.gs == BiOpts::GS32)
       hipLaunchKernelGGL((bi_precond_gs_kernel<MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st,
                          *o.M32, Vd, count);
@@ -925,7 +931,7 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   auto matvec = [&](auto mode_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
     double* const part = MODE == 0 ? part0 : part0 + 2 * nslots;
-    if (dbl && o.A32 && o.gs == BiOpts::GSC) {
+    if ((dbl || gsc_mult) && o.A32 && o.gs == BiOpts::GSC) {
       // 11-28: the additive correction (17-22) adds e to the operand; the multiplicative one
       // (23-28) finds it in p_ / s_ already and keeps the block sum of q_.  Class-uniform blocks
       // take their bands from the class rows.
@@ -934,8 +940,8 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
       auto go = [&](auto hr_tag, auto cc_tag, auto reg_tag) {
         constexpr bool HR = decltype(hr_tag)::value, CC = decltype(cc_tag)::value;
         constexpr int REG = decltype(reg_tag)::value;
-        hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, HR, CC, REG>), dim3(nact), dim3(BI_NT), 0, sm,
-                           active, blk, cs, st, *o.A32, Vd, part, nslots, CC ? (const double*)o.ce : nullptr, bsy,
+        hipLaunchKernelGGL((bi_matvec_kernel<T, MODE, float, HR, CC, REG>), dim3(nact), dim3(BI_NT), 0, sm,
+                           active, blk, cs, st, *o.A32, V, part, nslots, CC ? (const double*)o.ce : nullptr, bsy,
                            o.tab, o.uni);
       };
       auto regs = [&](auto hr_tag, auto cc_tag) {
@@ -947,10 +953,13 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
           go(hr_tag, cc_tag, std::integral_constant<int, 0>{});
       };
       auto ccs = [&](auto hr_tag) {
-        if (o.cinv && !o.mult)
-          regs(hr_tag, std::true_type{});
-        else
-          regs(hr_tag, std::false_type{});
+        if constexpr (sizeof(T) == 8) {
+          if (o.cinv && !o.mult) {
+            regs(hr_tag, std::true_type{});
+            return;
+          }
+        }
+        regs(hr_tag, std::false_type{});
       };
       if (o.hash_r0)
         ccs(std::true_type{});
@@ -973,7 +982,7 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   // additive: after P / S from the block sums that kernel wrote; multiplicative: before P / S from
   // the block sums of r, p, q the previous kernels wrote
   auto coarse = [&](int mode) {
-    if (!(dbl && o.gs == BiOpts::GSC && o.cinv)) return;
+    if (!(dbl && o.gs == BiOpts::GSC && o.cinv) && !gsc_mult) return;
     if (o.mult)  // mode 1 forms alpha from the Q partials (no alpha finalisation)
       launch_coarse_apply_mul(sm, active, nact, blk, cs, st, o.cinv, o.coff, o.ce, nslots, mode,
                               mode == 1 ? part0 : nullptr, count, mode == 0 && (mask & 512) ? part0 : nullptr);
@@ -985,7 +994,7 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   if (mask & 1) precond(M0{});
   if ((mask & 1) && !split && !pre) coarse(0);
   if (split && (mask & 64)) coarse(0);
-  const bool alpha_in_coarse = dbl && o.gs == BiOpts::GSC && o.cinv && o.mult;
+  const bool alpha_in_coarse = gsc_mult;
   if (mask & 2) {
     matvec(M0{});
     if (!(mask & 32) && !alpha_in_coarse) launch_finalize(sm, caseact, ncase, cs, st, part, nslots, FIN_BI_ALPHA, count);

@@ -211,7 +211,7 @@ int bicgstab_core(ffvhc_ctx* c, int xf, Coef7<T> K, const T* bv, int bcg, double
   // multiplicative two-level methods: inside a batch, each iteration's finalisation runs in the
   // next iteration's first coarse application; the batch's last iteration has its own launch, so
   // the state the host reads after the batch is complete
-  const bool fuse = c->fuse_fin && !exec && sizeof(T) == 8 && opts.gs == BiOpts::GSC && opts.cinv && opts.mult;
+  const bool fuse = c->fuse_fin && !exec && opts.gs == BiOpts::GSC && opts.cinv && opts.mult;
   while (!done) {
     int last = std::min(n + batch - 1, countMax);
     const int first = n;
@@ -388,7 +388,7 @@ int bicg_timing_run(ffvhc_ctx* c, int x, const int* A, int b, double omega, int 
   static const int slot_mul[NS] = {6, 0, 1, 5, 6, 2, 3, 5, 4, 5};
   static const int fin_add[NS] = {-1, -1, -1, FIN_BI_ALPHA, -1, -1, -1, -1, -1, FIN_BI_XR};
   static const int fin_mul[NS] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, FIN_BI_XR};  // alpha: coarse(s_)
-  const bool mul_alpha = o.mult && o.gs == BiOpts::GSC && o.cinv && sizeof(T) == 8;
+  const bool mul_alpha = o.mult && o.gs == BiOpts::GSC && o.cinv;
   const int* fin_of = mul_alpha ? fin_mul : fin_add;
   const int* bit_of = o.mult ? bit_mul : bit_add;
   const int* slot_of = o.mult ? slot_mul : slot_add;
@@ -937,6 +937,95 @@ int bicgstab_reliable(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double 
 
 }  // namespace
 
+namespace {
+
+// Method 29: FP64 defect correction (iterative refinement) around an FP32 inner solve, the inner solve
+// being method 26's BiCGSTAB (FP32 matrix, coarse correction then 4 block SOR sweeps) on FP32
+// vectors: each round forms the FP64 residual r = b - A x of the FP64 matrix (and its FP32 copy),
+// solves A e = r in FP32 to the relative tolerance inner_eps (at most countMax iterations), and adds
+// e to x in FP64; the answer is accepted on the FP64 residual, as for every other method.  The
+// inner solve moves half the bytes per iteration of method 26; FP32 limits each round to about
+// 1e-6, so eps = 1e-10 takes two or three rounds.
+int two_level_defect_correction(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countMax,
+                                double eps, double inner_eps, int* count, double* residual, int* outer) {
+  if (!(inner_eps > 0.0 && inner_eps < 1.0)) return fail(-1, "method 29: inner tolerance must be in (0, 1)");
+  Coef7<float> M32;
+  Coef7<double> K64;
+  BiOpts o;
+  int rc;
+  int fa[7];
+  for (int q = 0; q < 7; q++) {
+    rc = scratch_field(c, 4, R_A + q, &fa[q]);
+    if (rc) return rc;
+  }
+  if (c->step_solve && psetup_match(c, 29, A)) {  // the time step's matrix this setup was made from
+    K64 = coef<double>(c, A);
+    M32 = coef<float>(c, fa);
+    o = c->psetup.o;
+    o.A32 = &M32;
+  } else {
+    rc = fp64_bicg_opts(c, 26, A, &M32, &K64, &o);
+    if (rc) return rc;
+    // the inner solve's first residual reads the FP32 bands of every block (class-uniform ones
+    // included, which the sweeps and matvecs take from their class rows)
+    for (int q = 0; q < 7; q++)
+      launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
+                     (const double*)c->fields[A[q]].d[0]);
+    if (c->step_solve) psetup_store(c, 29, A, o);
+  }
+  int fr, fe;
+  rc = scratch_field(c, 4, R_RHS, &fr);
+  if (!rc) rc = scratch_field(c, 4, R_E, &fe);
+  if (rc) return rc;
+  const std::vector<int> ids = selected_cases(c);
+  const FieldRec X = c->fields[x], E = c->fields[fe], Rf = c->fields[fr];
+  std::vector<int> inner_total(c->cases.size(), 0), outers(c->cases.size(), -1);
+  const int max_outer = 64;
+  launch_solve_init(c->sm, c->dst2, (int)c->cases.size(), omega, eps, max_outer);
+  for (int k = 0;; k++) {
+    launch_defect(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, c->dst2, K64, (const double*)c->fields[b].d[0],
+                  (const double*)X.d[0], (const double*)X.h[0], (float*)Rf.d[0], c->dpart, (size_t)c->cap, k);
+    fin(c, c->dst2, FIN_DEFECT, k);
+    HIPCHK(hipGetLastError());
+    bool done = false;
+    rc = read_state(c, c->dst2, c->hst2, ids, &done);
+    if (rc) return rc;
+    for (int id : ids)
+      if (c->hst2[id].done && outers[id] < 0) outers[id] = k;
+    if (done || k >= max_outer) break;
+    launch_fill<float>(c->sm, c->dactive, c->nactive, (float*)E.d[0], (float*)E.h[0], 0.0f);
+    c->bicg_round = k;
+    c->bicg_hint_key = 29;
+    rc = bicgstab_core<float>(c, fe, M32, (const float*)Rf.d[0], BC_HOM, omega, countMax, inner_eps, c->dst, c->hst,
+                              c->dst2, nullptr, o);
+    c->bicg_round = -1;
+    if (rc) return rc;
+    rc = sync_state(c, c->dst, c->hst);
+    if (rc) return rc;
+    for (int id : ids)
+      if (!c->hst2[id].done) inner_total[id] += std::max(c->hst[id].count, 0);
+    launch_add_correction(c->sm, c->dactive, c->nactive, c->dblk, c->dst2, c->dst, (double*)X.d[0],
+                          (const float*)E.d[0]);
+    launch_ghost_pull<double>(c->sm, c->dactive, c->nactive, c->dblk, c->dcase, (const double*)X.d[0],
+                              (double*)X.h[0], bcg, 1, c->multilevel);
+    HIPCHK(hipGetLastError());
+  }
+  rc = sync_state(c, c->dst2, c->hst2);
+  if (rc) return rc;
+  int status = 0;
+  for (size_t i = 0; i < ids.size(); i++) {
+    const SolveState& s = c->hst2[ids[i]];
+    if (count) count[i] = inner_total[ids[i]];
+    if (residual) residual[i] = s.residual;
+    if (outer) outer[i] = outers[ids[i]] < 0 ? max_outer : outers[ids[i]];
+    if (s.status) status = 1;
+  }
+  if (status) set_error("residual is NaN or Inf");
+  return status;
+}
+
+}  // namespace
+
 extern "C" {
 
 int ffvhc_rbgs(ffvhc_ctx* c, int x, const int* A, int b, int bcg, double omega, int countMax, double eps, int* count,
@@ -1012,15 +1101,30 @@ int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* c, int method, int x, const int* A, i
   int rc = prologue(c, x, A, b, BC_NULL, &prec);
   if (rc) return rc;
   if (prec != 8) return fail(-1, "methods 5-9 take FP64 x, A, b");
-  if (method < 5 || method > 28) return fail(-1, "method must be 5-28");
+  if (method < 5 || method > 29) return fail(-1, "method must be 5-29");
   if (niter < 1 || !ms6) return fail(-1, "niter >= 1 and ms[7] required");
   Coef7<float> M32;
   Coef7<double> M64;
   BiOpts o;
   CoarseJoin join{c};
-  rc = fp64_bicg_opts(c, method, A, &M32, &M64, &o);
+  rc = fp64_bicg_opts(c, method == 29 ? 26 : method, A, &M32, &M64, &o);
   if (!rc) rc = coarse_join(c);
   if (rc) return rc;
+  if (method == 29) {  // the inner solve of method 29: FP32 copies of A, b and x = 0
+    int fa[7], fr, fe;
+    for (int q = 0; q < 7; q++) {
+      rc = scratch_field(c, 4, R_A + q, &fa[q]);
+      if (rc) return rc;
+      launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fa[q]].d[0],
+                     (const double*)c->fields[A[q]].d[0]);
+    }
+    rc = scratch_field(c, 4, R_RHS, &fr);
+    if (!rc) rc = scratch_field(c, 4, R_E, &fe);
+    if (rc) return rc;
+    launch_convert(c->sm, c->dactive, c->nactive, (float*)c->fields[fr].d[0], (const double*)c->fields[b].d[0]);
+    launch_fill<float>(c->sm, c->dactive, c->nactive, (float*)c->fields[fe].d[0], (float*)c->fields[fe].h[0], 0.0f);
+    return bicg_timing_run<float>(c, fe, fa, fr, omega, niter, ms6, o);
+  }
   return bicg_timing_run<double>(c, x, A, b, omega, niter, ms6, o);
 }
 
@@ -1124,7 +1228,7 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
   if (rc) return rc;
   if (prec != 8) return fail(-1, "mixed precision takes FP64 x, A, b");
   CoarseJoin join{c};
-  if (method < 0 || method > 28)
+  if (method < 0 || method > 29)
     return fail(-1, "method: 0 RBGS, 1 Jacobi-PCG, 2 Jacobi-BiCGSTAB, 3 Jacobi-BiCGSTAB with reliable updates, "
                     "4 the same with a seeded shadow residual, 5 FP64 BiCGSTAB with an FP32 red-black GS "
                     "preconditioner, 6 the same all in FP64 (reference point), 7 FP64 Jacobi-BiCGSTAB with an FP32 "
@@ -1132,7 +1236,8 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
                     "symmetric red-black-red sweep, 10 as 8 with the seeded hash vector as shadow residual, "
                     "11-16 as 8 with 1-6 red-black SOR sweeps on a compact FP32/FP16 copy of the matrix, "
                     "17-22 as 11-16 plus a coarse (one value per block) correction, 23-28 the same correction "
-                    "applied before the sweeps (multiplicative)");
+                    "applied before the sweeps (multiplicative), 29 method 26's BiCGSTAB on FP32 vectors inside "
+                    "FP64 defect correction");
   if (method == 6) {
     Coef7<float> M32;
     Coef7<double> M64;
@@ -1149,6 +1254,8 @@ int ffvhc_solve_mixed(ffvhc_ctx* c, int method, int x, const int* A, int b, int 
       if (outer) outer[i] = 0;
     return report(c, c->hst, count, residual);
   }
+  if (method == 29) return two_level_defect_correction(c, x, A, b, bcg, omega, countMax, eps, inner_eps, count,
+                                                       residual, outer);
   if (method == 5 || method >= 7) {  // FP64 recurrence, dots and residual; FP32 matrix
     Coef7<float> M32;
     Coef7<double> K64;

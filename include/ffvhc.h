@@ -228,7 +228,8 @@ int ffvhc_coarse_operator(ffvhc_ctx* ctx, const int* A, int case_id, double* ac,
 int ffvhc_stencil_classes(ffvhc_ctx* ctx, const int* A, int* n_uniform, int* n_other);
 /* The same stage timing for the FP64-recurrence methods 5-28 of ffvhc_solve_mixed (FP64 x, A, b;
  * omega = GS relaxation factor): ms[0..4] = P, Q, S, T, XR kernels, ms[5] = finalisations,
- * ms[6] = coarse corrections (methods 17-28). */
+ * ms[6] = coarse corrections (methods 17-29; 29: the inner solve's FP32 kernels on FP32 copies of
+ * A and b from x = 0). */
 int ffvhc_bicgstab_timing_mixed(ffvhc_ctx* ctx, int method, int x, const int* A, int b, double omega, int niter,
                                 double* ms);
 /* niter PCG iterations (no convergence test) with HIP events between the launches; ms[0..2] =
@@ -258,6 +259,9 @@ int ffvhc_residual(ffvhc_ctx* ctx, int x, const int* A, int b, double* residual)
  * plus a coarse correction P A_c^-1 P^T (one value per block, A_c = P^T A P inverted densely),
  * 23-28 the same correction applied first and the sweeps on the remaining residual
  * (multiplicative two-level).
+ * Method 29 is the defect-correction form with that preconditioner: each correction solves
+ * A e = r (r the FP64 residual, rounded to FP32) by method 26's BiCGSTAB on FP32 vectors to the
+ * relative tolerance inner_eps (at most countMax iterations), then x += e in FP64.
  * Methods 2, 3 and 4 are experimental, kept for comparison only: on every workload measured here
  * 2 is 4x slower than the FP64 Jacobi-BiCGSTAB and 3 / 4 stall near a relative residual of 1e-4
  * (the FP32 recurrence loses the bi-orthogonality coefficient); the autotuner never picks them.

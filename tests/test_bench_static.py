@@ -55,10 +55,16 @@ def test_mixed_byte_model_and_names():
     assert bench.mixed_prof_name(10, "bi_xr") == "bi_xr_kernel<double, true, false>"
     assert bench.kernel_bytes_mixed(10)["bi_xr"] == 56
     assert abs(bench.kernel_bytes_mixed(12)["bi_p"] - (24 + 16 + 16 + 3)) < 1e-9
-    assert bench.mixed_prof_name(13, "bi_s") == "bi_precond_gsk_kernel<1, 3, false, 0>"
-    assert bench.mixed_prof_name(21, "bi_p") == "bi_precond_gsk_kernel<0, 5, false, 0>"
-    assert bench.mixed_prof_name(25, "bi_s") == "bi_precond_gsk_kernel<1, 3, true, 0>"
-    assert bench.mixed_prof_name(25, "bi_s", True) == "bi_precond_gsk_kernel<1, 3, true, 1>"
+    assert bench.mixed_prof_name(13, "bi_s") == "bi_precond_gsk_kernel<double, 1, 3, false, 0>"
+    assert bench.mixed_prof_name(21, "bi_p") == "bi_precond_gsk_kernel<double, 0, 5, false, 0>"
+    assert bench.mixed_prof_name(25, "bi_s") == "bi_precond_gsk_kernel<double, 1, 3, true, 0>"
+    assert bench.mixed_prof_name(25, "bi_s", True) == "bi_precond_gsk_kernel<double, 1, 3, true, 1>"
+    # method 29: method 26's kernels on FP32 vectors
+    assert bench.mixed_prof_name(29, "bi_p", True) == "bi_precond_gsk_kernel<float, 0, 4, true, 1>"
+    assert bench.mixed_prof_name(29, "bi_t", True) == "bi_matvec_kernel<float, 1, float, false, false, 1>"
+    assert bench.mixed_prof_name(29, "bi_xr") == "bi_xr_kernel<float, false, false>"
+    b26, b29 = bench.kernel_bytes_mixed(26, True), bench.kernel_bytes_mixed(29, True)
+    assert all(b29[k] == b26[k] / 2 for k in b26)  # no per-cell matrix: only the vectors, at half width
     assert bench.mixed_prof_name(21, "bi_q") == "bi_matvec_kernel<double, 0, float, false, true, 0>"
     assert bench.mixed_prof_name(21, "bi_q", True) == "bi_matvec_kernel<double, 0, float, false, true, 1>"
     # class-uniform blocks: no per-cell band or record is read
@@ -68,7 +74,7 @@ def test_mixed_byte_model_and_names():
     assert bench.mixed_prof_name(26, "bi_t") == "bi_matvec_kernel<double, 1, float, false, false, 0>"
     assert bench.mixed_prof_name(26, "bi_xr") == "bi_xr_kernel<double, false, false>"
     assert bench.mixed_prof_name(19, "bi_xr") == "bi_xr_kernel<double, false, true>"
-    assert len(bench.MIXED_METHODS) == 29
+    assert len(bench.MIXED_METHODS) == 30
 
 
 def test_step_iteration_bytes():

@@ -31,6 +31,9 @@ def solve(ctx, f64, method):
     ctx.fill(f64["x"], 0.0)
     if method is None:
         cnt, res, rc = ctx.bicgstab(f64["x"], f64["A"], f64["b"], ffvhc.BC_P, 1.0, 1, 20000, 1e-10, ncases=1)
+    elif method == 29:  # FP32 inner solves to 1e-5 per correction, at most 200 iterations each
+        cnt, res, _o, rc = ctx.solve_mixed(method, f64["x"], f64["A"], f64["b"], ffvhc.BC_P, 1.6, 200, 1e-10,
+                                           1e-5, ncases=1)
     else:
         cnt, res, _o, rc = ctx.solve_mixed(method, f64["x"], f64["A"], f64["b"], ffvhc.BC_P, 1.6, 20000, 1e-10,
                                            0.5, ncases=1)
@@ -43,17 +46,17 @@ def test_fullsize_solvers_reach_and_agree(c2):
     shp = (tree.nblk, 20, 20, 20)
     inner = (slice(None),) + (slice(2, 18),) * 3
     sols = {}
-    for method in (None, 15, 19, 25, 26):
+    for method in (None, 15, 19, 25, 26, 29):
         cnt, res = solve(ctx, f64, method)
         true = ctx.residual(f64["x"], f64["A"], f64["b"], ncases=1)[0]
         assert res <= 1e-10 and true <= 1e-10 * 1.0001, (method, cnt, res, true)
         sols[method] = ctx.download(f64["x"], cid, out=np.zeros(shp))[inner]
     scale = np.max(np.abs(sols[None]))
-    for m in (15, 19, 25, 26):
+    for m in (15, 19, 25, 26, 29):
         assert np.max(np.abs(sols[m] - sols[None])) <= 1e-7 * scale
 
 
-@pytest.mark.parametrize("method", [None, 19, 25, 26])
+@pytest.mark.parametrize("method", [None, 19, 25, 26, 29])
 def test_fullsize_solve_is_homogeneous(c2, method):
     ctx, cid, f64, tree, b = c2
     shp = (tree.nblk, 20, 20, 20)
