@@ -612,17 +612,27 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
   static_assert(NC * sizeof(T) >= BI_NT / 64 * sizeof(double), "reduction scratch fits");
   static_assert(BI_NT == NF, "one thread per (i, j) column and per face position");
   const int slot = active[blockIdx.x];
-  const int c = blk[slot].case_id;
-  if (st[c].done) return;
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
   const T* __restrict__ w = MODE == 0 ? V.r0 : V.s;
   T* __restrict__ y = MODE == 0 ? V.q : V.t;
-  const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
   const int o0 = tx + NB * ty;
   constexpr int KB = Batch<T>::K;
   double d0 = 0.0, d1 = 0.0, ys = 0.0;
-  T zm, zp;  // ghosts below k = 0 / above k = NB - 1
+  // The operand's values and ghosts (and the class rows) depend on the slot alone: they are loaded
+  // before the case state is read, so the chain of dependent reads (block -> case -> state)
+  // overlaps the loads instead of preceding them (a finished case's workgroup then discards them;
+  // every address is inside its own block).  The empty asm with a memory clobber keeps the
+  // compiler from sinking the loads past the test; it waits for nothing.
+  const T* __restrict__ xo = (MODE == 0 ? V.pp : V.ss) + base;
+  const T* __restrict__ h = (MODE == 0 ? V.hpp : V.hss) + (size_t)slot * NH;
+  T v[NB], gv[4];
+#pragma unroll
+  for (int k = 0; k < NB; k++) v[k] = xo[o0 + NF * k];
+#pragma unroll
+  for (int f = 0; f < 4; f++) gv[f] = h[f * NF + t];
+  T zm = h[halo_index(4, tx, ty)];  // ghost below k = 0
+  T zp = h[halo_index(5, tx, ty)];  // ghost above k = NB - 1
   // REG: the bands of the column's classes (k = 0, inner, NB - 1)
   const bool cls = REG == 1 || (REG == 2 && uni[slot] != 0);  // uniform over the workgroup
   TA al[7], ai[7], ah[7];
@@ -636,9 +646,12 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
       ah[q] = tr[18 * 8 + q];
     }
   }
-#pragma unroll
-  for (int k0 = 0; k0 < NB; k0 += KB) {
-    T a[KB][7], wv[KB];
+  asm volatile("" ::: "memory");
+  const int c = blk[slot].case_id;
+  if (st[c].done) return;
+  const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
+  T a[KB][7], wv[KB];
+  auto load_batch = [&](int k0) {
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
       const int k = k0 + kk;
@@ -652,33 +665,27 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
       }
       wv[kk] = (MODE == 0 && HR) ? (T)shadow_value(local, (uint32_t)(g - base)) : w[g];
     }
-    if (k0 == 0) {
-      const T* __restrict__ xo = (MODE == 0 ? V.pp : V.ss) + base;
-      const T* __restrict__ h = (MODE == 0 ? V.hpp : V.hss) + (size_t)slot * NH;
-      T v[NB], gv[4];
+  };
+  load_batch(0);  // in flight while the operand is staged
+  if constexpr (CC) {  // each thread corrects the values it stages
+    const DevBlock& Bk = blk[slot];
+    const double* e = ce + MODE * nslots;
+    const T eo = (T)e[slot];
 #pragma unroll
-      for (int k = 0; k < NB; k++) v[k] = xo[o0 + NF * k];
+    for (int k = 0; k < NB; k++) v[k] += eo;
 #pragma unroll
-      for (int f = 0; f < 4; f++) gv[f] = h[f * NF + t];
-      zm = h[halo_index(4, tx, ty)];
-      zp = h[halo_index(5, tx, ty)];
-      if constexpr (CC) {  // each thread corrects the values it stages
-        const DevBlock& Bk = blk[slot];
-        const double* e = ce + MODE * nslots;
-        const T eo = (T)e[slot];
+    for (int f = 0; f < 4; f++) gv[f] += (T)e[ghost_source(Bk, slot, f, t & 15, t >> 4)];
+    zm += (T)e[ghost_source(Bk, slot, 4, tx, ty)];
+    zp += (T)e[ghost_source(Bk, slot, 5, tx, ty)];
+  }
 #pragma unroll
-        for (int k = 0; k < NB; k++) v[k] += eo;
+  for (int k = 0; k < NB; k++) s[o0 + NF * k] = v[k];
 #pragma unroll
-        for (int f = 0; f < 4; f++) gv[f] += (T)e[ghost_source(Bk, slot, f, t & 15, t >> 4)];
-        zm += (T)e[ghost_source(Bk, slot, 4, tx, ty)];
-        zp += (T)e[ghost_source(Bk, slot, 5, tx, ty)];
-      }
+  for (int f = 0; f < 4; f++) s[NC + f * NF + t] = gv[f];
+  __syncthreads();
 #pragma unroll
-      for (int k = 0; k < NB; k++) s[o0 + NF * k] = v[k];
-#pragma unroll
-      for (int f = 0; f < 4; f++) s[NC + f * NF + t] = gv[f];
-      __syncthreads();
-    }
+  for (int k0 = 0; k0 < NB; k0 += KB) {
+    if (k0 > 0) load_batch(k0);
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
       const int k = k0 + kk;
@@ -692,12 +699,12 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
       const T vzm = k > 0 ? s[li - NF] : zm;
       const T vzp = k < NB - 1 ? s[li + NF] : zp;
       // calc_ax order (bils.f90:195-202)
-      const T v = a[kk][1] * vxm + a[kk][2] * vxp + a[kk][3] * vym + a[kk][4] * vyp + a[kk][5] * vzm +
+      const T r = a[kk][1] * vxm + a[kk][2] * vxp + a[kk][3] * vym + a[kk][4] * vyp + a[kk][5] * vzm +
                   a[kk][6] * vzp + a[kk][0] * s[li];
-      y[g] = v;
-      ys += (double)v;
-      d0 += (double)v * (double)wv[kk];
-      if (MODE == 1) d1 += (double)v * (double)v;
+      y[g] = r;
+      ys += (double)r;
+      d0 += (double)r * (double)wv[kk];
+      if (MODE == 1) d1 += (double)r * (double)r;
     }
   }
   double s0 = block_sum<BI_NT>(d0, red);
@@ -730,11 +737,33 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
   __shared__ double red[2 * (BI_NT / 64)];
   __shared__ double gshared;
   const int slot = active[blockIdx.x];
-  const int c = blk[slot].case_id;
-  if (st[c].done) return;
   const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
   const size_t base = (size_t)slot * NC;
-  if (t == 0 && slot == cs[c].slot_begin) st[c].rr0 = st[c].rr1;
+  constexpr int KB = Batch<T>::K;
+  T pv[KB], sv_[KB], xv[KB], tv[KB], sv[KB], r0v[KB];
+  auto load_batch = [&](int k0) {
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
+      // the last reads of p_, s_, t_ and s in the iteration, and x's only one: non-temporal
+      // (measured 3% faster; r0, read every iteration, stays cached)
+      pv[kk] = __builtin_nontemporal_load(V.pp + g);
+      sv_[kk] = __builtin_nontemporal_load(V.ss + g);
+      xv[kk] = __builtin_nontemporal_load(x + g);
+      tv[kk] = __builtin_nontemporal_load(V.t + g);
+      sv[kk] = __builtin_nontemporal_load(V.s + g);
+      if constexpr (!HR) r0v[kk] = V.r0[g];
+    }
+  };
+  // The first batch depends on the slot alone: it is issued before the case state is read, so the
+  // chain of dependent reads (block -> case -> state) overlaps the loads instead of preceding them
+  // (a finished case's workgroup then discards them; every address is inside its own block).  The
+  // empty asm with a memory clobber keeps the compiler from sinking the loads past the test
+  // below; it waits for nothing.
+  load_batch(0);
+  asm volatile("" ::: "memory");
+  const int c = blk[slot].case_id;
+  if (st[c].done) return;
   // gamma: formed after the first batch of loads is issued (the partials' round trips then
   // overlap the block's own loads)
   auto form_gamma = [&]() -> double {
@@ -752,29 +781,22 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
     return st[c].gamma;
   };
   T alpha = 0, gamma = 0, ng = 0;
-  const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
-  constexpr int KB = Batch<T>::K;
+  const int sb = cs[c].slot_begin;
+  const uint32_t local = (uint32_t)(slot - sb);
   double rr = 0.0, rr1 = 0.0, rs = 0.0;
 #pragma unroll
   for (int k0 = 0; k0 < NB; k0 += KB) {
-    T pv[KB], sv_[KB], xv[KB], tv[KB], sv[KB], r0v[KB];
+    if (k0 > 0) load_batch(k0);
 #pragma unroll
     for (int kk = 0; kk < KB; kk++) {
-      const size_t g = base + tx + NB * (ty + NB * (k0 + kk));
-      // the last reads of p_, s_, t_ and s in the iteration, and x's only one: non-temporal
-      // (measured 3% faster; r0, read every iteration, stays cached)
-      pv[kk] = __builtin_nontemporal_load(V.pp + g);
-      sv_[kk] = __builtin_nontemporal_load(V.ss + g);
       if constexpr (CC) {
         pv[kk] += (T)ce[slot];
         sv_[kk] += (T)ce[nslots + slot];
       }
-      xv[kk] = __builtin_nontemporal_load(x + g);
-      tv[kk] = __builtin_nontemporal_load(V.t + g);
-      sv[kk] = __builtin_nontemporal_load(V.s + g);
-      r0v[kk] = HR ? (T)shadow_value(local, (uint32_t)(g - base)) : V.r0[g];
+      if constexpr (HR) r0v[kk] = (T)shadow_value(local, (uint32_t)(tx + NB * (ty + NB * (k0 + kk))));
     }
     if (k0 == 0) {
+      if (t == 0 && slot == sb) st[c].rr0 = st[c].rr1;
       const double gd = form_gamma();
       alpha = (T)st[c].alpha;
       gamma = (T)gd;
