@@ -909,6 +909,7 @@ def main():
         ms = msall[:5]
         by = kernel_bytes_mixed(sched[0], uniform)
         kt["mixed"] = {name: {"us": round(t * 1e3, 2), "GBs": round(by[name] * cells / (t * 1e-3) / 1e9, 1),
+                              "frac": round(by[name] * cells / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                               "kernel": mixed_prof_name(sched[0], name, uniform)}
                        for name, t in zip(("bi_p", "bi_q", "bi_s", "bi_t", "bi_xr"), ms)}
         # per iteration: the coarse applications, and what the timing run puts between the stages
