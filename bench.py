@@ -191,7 +191,7 @@ def cpu_time_step_baseline(steps=4, threads=16):
 
 
 def oracle_c1(eps, threads):
-    """The oracle's state for the single-level flow past a sphere (C1 geometry, Re 100, dt 0.01,
+    """The oracle's state for the single-level flow past a sphere (8 x 4 x 4 blocks, Re 100, dt 0.01,
     explicit U) from the uniform flow u = (1, 0, 0): grid, fields, parameters (pressure by
     Jacobi-BiCGSTAB to eps) and the host arrays behind them."""
     import ctypes
@@ -241,7 +241,7 @@ def oracle_c1(eps, threads):
 
 def field_error_leg(steps=3, eps=1e-10, method=26, omega=1.6, threads=16):
     """Error against the FP64 oracle, per field: `steps` time steps of the single-level flow past a
-    sphere (C1 geometry from the uniform flow, explicit U, pressure to eps) on the oracle (the C
+    sphere (8 x 4 x 4 blocks, from the uniform flow, explicit U, pressure to eps) on the oracle (the C
     restatement of Solver::Update, FP64, Jacobi-BiCGSTAB), on the device's FP64 path (the same
     solver) and on the device's mixed-precision path (solve_mixed `method`); for each field the
     largest |device - oracle| over the interior cells, absolute and relative to the field's largest
@@ -264,7 +264,7 @@ def field_error_leg(steps=3, eps=1e-10, method=26, omega=1.6, threads=16):
         ref["v%d" % q] = lists["v"][q]
     shp = (nb, N + 2 * G, N + 2 * G, N + 2 * G)
     inner = (slice(None),) + (slice(G, G + N),) * 3
-    out = {"workload": "%d time steps of the single-level flow past a sphere (C1 geometry, uniform flow start, "
+    out = {"workload": "%d time steps of the single-level flow past a sphere (8 x 4 x 4 blocks, uniform flow start, "
                        "explicit U, pressure to %g), oracle on %d host threads" % (steps, eps, threads),
            "oracle_pressure_iterations": its_o, "fields": {}}
     tree, geo, params = st["tree"], st["geo"], st["params"]
@@ -602,11 +602,11 @@ def time_step_leg(case, steps, warmup, method, omega, fp64_steps=None):
 
 
 def flow_run_leg(case, steps, method=26, omega=1.6, every=25, fp64=False):
-    """C1 or C3 as specified, run for `steps` steps with solve_mixed `method`.  C1: the single-level
-    flow past a sphere (128 blocks, 524,288 cells, Re 100, dt 0.01, explicit U, pressure to 1e-8,
-    u = (1, 0, 0) plus seeded N(0, 0.01) noise, seed 1).  C3: the three-level flow (4,048 blocks, 16.6 M
-    cells, Re 300, dt at CFL 0.3 on the finest cells, implicit U, pressure to 1e-10) from the
-    uniform flow.  Every `every` steps the drag coefficient (pressure and viscous forces on the
+    """The single-level ("c1") or three-level ("c3") flow past a sphere, run for `steps` steps with
+    solve_mixed `method`.  Single level: 128 blocks, 524,288 cells, Re 100, dt 0.01, explicit U,
+    pressure to 1e-8, u = (1, 0, 0) plus seeded N(0, 0.01) noise, seed 1.  Three levels: 4,048
+    blocks, 16.6 M cells, Re 300, dt at CFL 0.3 on the finest cells, implicit U, pressure to 1e-10,
+    from the uniform flow.  Every `every` steps the drag coefficient (pressure and viscous forces on the
     sphere's cut faces, PrintForceCID's sum, over 0.5 rho U^2 pi D^2 / 4), max u_x and the step's
     pressure iterations are recorded, and a progress line is logged.  fp64: the reference's FP64
     Jacobi-BiCGSTAB pressure solve instead."""
@@ -618,7 +618,7 @@ def flow_run_leg(case, steps, method=26, omega=1.6, every=25, fp64=False):
         params = flows.flow_params(100.0, 0.01)
         kw = dict(solver=3, omega=omega, count_max=20000, eps=1e-8, mixed=True, mixed_method=method)
         init = uniform_flow_init(1, u0=1.0, noise=0.01)
-        desc = ("C1: flow past a sphere, single level, %d blocks of 16^3 (%d cells), Re 100, dt 0.01, explicit U, "
+        desc = ("flow past a sphere, single level, %d blocks of 16^3 (%d cells), Re 100, dt 0.01, explicit U, "
                 "pressure to 1e-8" % (tree.nblk, tree.nblk * N ** 3))
     else:
         tree, geo, counts = flows.three_level_sphere()
@@ -626,7 +626,7 @@ def flow_run_leg(case, steps, method=26, omega=1.6, every=25, fp64=False):
         kw = dict(solver=3, omega=omega, count_max=20000, eps=1e-10, mixed=True, mixed_method=method,
                   implicit_u=True, alpha_u=1.0, omega_u=1.0, count_max_u=1000, eps_u=1e-8)
         init = uniform_flow_init(0, u0=1.0, noise=0.0)
-        desc = ("C3: flow past a sphere, three levels (sphere and wake refinement), %d blocks of 16^3 (%d cells), "
+        desc = ("flow past a sphere, three levels (sphere and wake refinement), %d blocks of 16^3 (%d cells), "
                 "Re 300, dt %.5f (CFL 0.3 on the finest cells), implicit U, pressure to 1e-10"
                 % (tree.nblk, tree.nblk * N ** 3, params.dt))
     if fp64:
@@ -809,13 +809,13 @@ def main():
     ap.add_argument("--c4-only", action="store_true", help="only the batched sweep of 64 cases")
     ap.add_argument("--c4-cases", type=int, default=64)
     ap.add_argument("--c3-run", type=int, default=0,
-                    help="only C3 as specified: this many steps of the three-level flow (mixed solver), drag history")
+                    help="only this many steps of the three-level flow past a sphere (mixed solver), drag history")
     ap.add_argument("--c1-run", type=int, default=0,
-                    help="only C1 as specified: this many steps of the single-level flow (mixed solver), drag history")
+                    help="only this many steps of the single-level flow past a sphere (mixed solver), drag history")
     ap.add_argument("--run-fp64", action="store_true", help="--c1-run / --c3-run with the FP64 pressure solve")
     ap.add_argument("--field-errors", type=int, default=0,
                     help="only the per-field error of the FP64 and mixed device paths against the oracle over this "
-                         "many C1 steps")
+                         "many steps of the single-level flow past a sphere")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this benchmark runs on one GPU only; multi-GPU runs are not implemented (the "

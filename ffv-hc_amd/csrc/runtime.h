@@ -152,7 +152,7 @@ struct ffvhc_ctx {
   void* dctrec = nullptr;
   bool classes = true;
   // shadow residual of the compact-record methods: r0 = r (default, the reference's choice) or the
-  // seeded hash vector (FFVHC_OPTION_SHADOW 1: on C2 2-4 fewer iterations for methods 24-27 and no
+  // seeded hash vector (FFVHC_OPTION_SHADOW 1: on the 1,128-block Poisson benchmark 2-4 fewer iterations for methods 24-27 and no
   // r0 reads, on the flow past a sphere 15% more iterations a step)
   bool shadow_hash = false;
   // replay the BiCGSTAB iteration as a captured HIP graph (ffvhc_ctx_set_option; off by default:

@@ -106,7 +106,7 @@ long long ffvhc_ctx_device_bytes(const ffvhc_ctx* ctx);
 #define FFVHC_OPTION_CLASSES 2
 /* FFVHC_OPTION_SHADOW: 0 (default) the compact-record methods (11-28) take the reference's shadow
  * residual r0 = r; 1 the seeded hash vector of method 10, computed where it is read (no r0
- * vector is stored or read).  The iteration count changes (on the C2 Poisson system 2-4 fewer,
+ * vector is stored or read).  The iteration count changes (on the 1,128-block Poisson benchmark 2-4 fewer,
  * on the flow past a sphere more); the accepted answer meets the same FP64 residual. */
 #define FFVHC_OPTION_SHADOW 3
 /* FFVHC_OPTION_SETUP_REUSE: 1 (default) lets ffvhc_step keep the setup of its mixed-precision

@@ -88,7 +88,7 @@ def test_step_iteration_bytes():
 
 
 def test_cpu_time_step_baseline_runs_one_step():
-    """The bench's CPU baseline of the time step (the oracle's Solver::Update on the C1 flow) runs
+    """The bench's CPU baseline of the time step (the oracle's Solver::Update on the single-level flow past a sphere) runs
     without a GPU: one step on 4 threads, its pressure solve reaching 1e-8 in the layout of
     data-ils.txt, with the stage times of Solver::times[] filled."""
     sys.path.insert(0, ROOT)

@@ -1,4 +1,4 @@
-"""The flow past a sphere of C1 (8 x 4 x 4 blocks of 16^3, sphere D = 1, Re 100, dt 0.01, explicit
+"""The single-level flow past a sphere (8 x 4 x 4 blocks of 16^3, sphere D = 1, Re 100, dt 0.01, explicit
 U, pressure to 1e-8, seeded initial noise) over 30 time steps with the mixed-precision pressure
 solve and with the reference's FP64 Jacobi-BiCGSTAB: both solve every step to the same FP64
 residual, so the observables the reference prints agree to far below their own size -- the drag
@@ -28,7 +28,7 @@ def test_c1_drag_mixed_equals_fp64():
 
 
 def test_c1_fields_against_oracle():
-    """Two steps of the C1 flow from the uniform flow, pressure to 1e-10: the device's FP64 path and
+    """Two steps of the single-level flow from the uniform flow, pressure to 1e-10: the device's FP64 path and
     its mixed-precision path against the oracle's FP64 Solver::Update (C restatement, Jacobi-
     BiCGSTAB), field by field.  Velocities and face velocities agree to 1e-10 of their size; the
     pressure, which the solves fix only to their residual tolerance, to 1e-7, and its Laplacian lapp

@@ -1,4 +1,4 @@
-"""The benchmark's own system at full size (C2: 1,128 blocks of 16^3, two levels), checked through
+"""The benchmark's own system at full size (1,128 blocks of 16^3, two levels), checked through
 properties that hold at any size: every solver reaches the FP64 relative residual it reports
 (recomputed by ffvhc_residual), the fast solvers' answers equal the FP64 BiCGSTAB answer to the
 solver tolerance, and the solves are homogeneous (b -> 2b gives exactly 2x, bit for bit: scaling
