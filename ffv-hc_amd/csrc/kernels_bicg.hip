@@ -606,7 +606,7 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
                                                           size_t nslots, const double* __restrict__ ce = nullptr,
                                                           double* __restrict__ bsy = nullptr,
                                                           const float* __restrict__ tab = nullptr,
-                                                          const int* __restrict__ uni = nullptr) {
+                                                          const int* __restrict__ uni = nullptr, int early = 0) {
   __shared__ T s[NC + 4 * NF];
   double* const red = reinterpret_cast<double*>(s);  // after the last read of s (block_sum syncs first)
   static_assert(NC * sizeof(T) >= BI_NT / 64 * sizeof(double), "reduction scratch fits");
@@ -619,36 +619,44 @@ __global__ __launch_bounds__(BI_NT) void bi_matvec_kernel(const int* __restrict_
   const int o0 = tx + NB * ty;
   constexpr int KB = Batch<T>::K;
   double d0 = 0.0, d1 = 0.0, ys = 0.0;
-  // The operand's values and ghosts (and the class rows) depend on the slot alone: they are loaded
-  // before the case state is read, so the chain of dependent reads (block -> case -> state)
-  // overlaps the loads instead of preceding them (a finished case's workgroup then discards them;
-  // every address is inside its own block).  The empty asm with a memory clobber keeps the
-  // compiler from sinking the loads past the test; it waits for nothing.
+  // The operand's values and ghosts (and the class rows) depend on the slot alone.  early (a launch
+  // of one case, whose blocks finish together): they are loaded before the case state is read, so
+  // the chain of dependent reads (block -> case -> state) overlaps the loads instead of preceding
+  // them (every address is inside its own block; the empty asm with a memory clobber keeps the
+  // compiler from sinking the loads past the test and waits for nothing).  In a launch of several
+  // cases, which finish at different iterations, a finished case's workgroups would load for
+  // nothing: there they load after the test.
   const T* __restrict__ xo = (MODE == 0 ? V.pp : V.ss) + base;
   const T* __restrict__ h = (MODE == 0 ? V.hpp : V.hss) + (size_t)slot * NH;
-  T v[NB], gv[4];
-#pragma unroll
-  for (int k = 0; k < NB; k++) v[k] = xo[o0 + NF * k];
-#pragma unroll
-  for (int f = 0; f < 4; f++) gv[f] = h[f * NF + t];
-  T zm = h[halo_index(4, tx, ty)];  // ghost below k = 0
-  T zp = h[halo_index(5, tx, ty)];  // ghost above k = NB - 1
+  T v[NB], gv[4], zm, zp;  // zm / zp: the ghosts below k = 0 / above k = NB - 1
   // REG: the bands of the column's classes (k = 0, inner, NB - 1)
   const bool cls = REG == 1 || (REG == 2 && uni[slot] != 0);  // uniform over the workgroup
   TA al[7], ai[7], ah[7];
-  if (REG != 0 && cls) {
-    static_assert(REG == 0 || sizeof(TA) == 4, "class rows are FP32 bands");
-    const float* tr = tab + ((size_t)slot * 27 + stencil_class1(tx) + 3 * stencil_class1(ty)) * 8;
+  auto stage_loads = [&]() {
 #pragma unroll
-    for (int q = 0; q < 7; q++) {
-      al[q] = tr[q];
-      ai[q] = tr[9 * 8 + q];
-      ah[q] = tr[18 * 8 + q];
+    for (int k = 0; k < NB; k++) v[k] = xo[o0 + NF * k];
+#pragma unroll
+    for (int f = 0; f < 4; f++) gv[f] = h[f * NF + t];
+    zm = h[halo_index(4, tx, ty)];
+    zp = h[halo_index(5, tx, ty)];
+    if (REG != 0 && cls) {
+      static_assert(REG == 0 || sizeof(TA) == 4, "class rows are FP32 bands");
+      const float* tr = tab + ((size_t)slot * 27 + stencil_class1(tx) + 3 * stencil_class1(ty)) * 8;
+#pragma unroll
+      for (int q = 0; q < 7; q++) {
+        al[q] = tr[q];
+        ai[q] = tr[9 * 8 + q];
+        ah[q] = tr[18 * 8 + q];
+      }
     }
+  };
+  if (early) {
+    stage_loads();
+    asm volatile("" ::: "memory");
   }
-  asm volatile("" ::: "memory");
   const int c = blk[slot].case_id;
   if (st[c].done) return;
+  if (!early) stage_loads();
   const uint32_t local = (uint32_t)(slot - cs[c].slot_begin);
   T a[KB][7], wv[KB];
   auto load_batch = [&](int k0) {
@@ -733,7 +741,7 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
                                                       T* __restrict__ x, BiVecs<T> V, double* __restrict__ part,
                                                       size_t nslots, int count, const double* __restrict__ ce = nullptr,
                                                       double* __restrict__ bsr = nullptr,
-                                                      const double* __restrict__ tpart = nullptr) {
+                                                      const double* __restrict__ tpart = nullptr, int early = 0) {
   __shared__ double red[2 * (BI_NT / 64)];
   __shared__ double gshared;
   const int slot = active[blockIdx.x];
@@ -755,15 +763,19 @@ __global__ __launch_bounds__(BI_NT) void bi_xr_kernel(const int* __restrict__ ac
       if constexpr (!HR) r0v[kk] = V.r0[g];
     }
   };
-  // The first batch depends on the slot alone: it is issued before the case state is read, so the
-  // chain of dependent reads (block -> case -> state) overlaps the loads instead of preceding them
-  // (a finished case's workgroup then discards them; every address is inside its own block).  The
-  // empty asm with a memory clobber keeps the compiler from sinking the loads past the test
-  // below; it waits for nothing.
-  load_batch(0);
-  asm volatile("" ::: "memory");
+  // The first batch depends on the slot alone.  early (a launch of one case, whose blocks finish
+  // together): it is issued before the case state is read, so the chain of dependent reads (block
+  // -> case -> state) overlaps the loads instead of preceding them (every address is inside its own
+  // block; the empty asm with a memory clobber keeps the compiler from sinking the loads past the
+  // test and waits for nothing).  In a launch of several cases, which finish at different
+  // iterations, a finished case's workgroups would load for nothing: there it follows the test.
+  if (early) {
+    load_batch(0);
+    asm volatile("" ::: "memory");
+  }
   const int c = blk[slot].case_id;
   if (st[c].done) return;
+  if (!early) load_batch(0);
   // gamma: formed after the first batch of loads is issued (the partials' round trips then
   // overlap the block's own loads)
   auto form_gamma = [&]() -> double {
@@ -919,6 +931,8 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
   // the multiplicative coarse correction (methods 23-28's preconditioner)
   const bool gsc_mult = o.gs == BiOpts::GSC && o.cinv && o.mult;
   if (!(gsc_mult && count > 0) || (mask & 256)) mask &= ~(512 | 1024);
+  // XR and the matvecs load before reading the case state only in a launch of one case
+  const int early = ncase == 1 ? 1 : 0;
   BiVecs<double>& Vd = *reinterpret_cast<BiVecs<double>*>(&V);
   double* const part0 = part;
   auto precond = [&](auto mode_tag) {
@@ -964,7 +978,7 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
         constexpr int REG = decltype(reg_tag)::value;
         hipLaunchKernelGGL((bi_matvec_kernel<T, MODE, float, HR, CC, REG>), dim3(nact), dim3(BI_NT), 0, sm,
                            active, blk, cs, st, *o.A32, V, part, nslots, CC ? (const double*)o.ce : nullptr, bsy,
-                           o.tab, o.uni);
+                           o.tab, o.uni, early);
       };
       auto regs = [&](auto hr_tag, auto cc_tag) {
         if (reg == 1)
@@ -989,13 +1003,13 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
         ccs(std::false_type{});
     } else if (dbl && o.A32 && o.hash_r0)
       hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk,
-                         cs, st, *o.A32, Vd, part, nslots);
+                         cs, st, *o.A32, Vd, part, nslots, nullptr, nullptr, nullptr, nullptr, early);
     else if (dbl && o.A32)
       hipLaunchKernelGGL((bi_matvec_kernel<double, MODE, float>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs,
-                         st, *o.A32, Vd, part, nslots);
+                         st, *o.A32, Vd, part, nslots, nullptr, nullptr, nullptr, nullptr, early);
     else
       hipLaunchKernelGGL((bi_matvec_kernel<T, MODE>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, A, V,
-                         part, nslots);
+                         part, nslots, nullptr, nullptr, nullptr, nullptr, early);
   };
   using M0 = std::integral_constant<int, 0>;
   using M1 = std::integral_constant<int, 1>;
@@ -1032,13 +1046,13 @@ void launch_bicgstab_iter(hipStream_t sm, const int* active, int nact, const int
       const double* tpart = part + 2 * nslots;
       if (o.cinv && o.mult)  // p_ / s_ already hold P e; block sum of r kept
         hipLaunchKernelGGL((bi_xr_kernel<T, HR>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
-                           nslots, count, nullptr, o.ce + 2 * nslots, tpart);
+                           nslots, count, nullptr, o.ce + 2 * nslots, tpart, early);
       else if (o.cinv)
         hipLaunchKernelGGL((bi_xr_kernel<T, HR, true>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V,
-                           part, nslots, count, (const double*)o.ce, nullptr, tpart);
+                           part, nslots, count, (const double*)o.ce, nullptr, tpart, early);
       else
         hipLaunchKernelGGL((bi_xr_kernel<T, HR>), dim3(nact), dim3(BI_NT), 0, sm, active, blk, cs, st, x, V, part,
-                           nslots, count, nullptr, nullptr, tpart);
+                           nslots, count, nullptr, nullptr, tpart, early);
     };
     if (o.hash_r0)
       xr(std::true_type{});
